@@ -1,0 +1,226 @@
+"""Benchmark: batched DT15 Langevin bead-steps/s on one MI355X.
+
+Workload (the batched configuration): 512 independent DT15 hairpin-design
+chains with lengths lognormal(mu=ln 120, sigma=0.7) clipped to 20-600 nt, plus
+64 replicas of the 64-nt hairpin on a geometric 290-400 K ladder; FP32 pair
+arithmetic with FP64 accumulation and integration; Langevin dt = 0.05 tau,
+neighbour lists rebuilt every 10 steps; all systems advance in one
+hipGraph-captured launch sequence per step.  Synthetic seeded sequences and
+extended-helix starting coordinates (no external data).
+
+Prints one JSON line (see the contract in the repository documentation):
+value = whole-batch bead-steps/s; beside it the plain one-chain-per-launch
+GPU path on the same workload, the force error against the FP64 oracle on a
+sample of chains, roofline of the dominant kernel, and a CPU baseline (the
+oracle timed on this host's cores on a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from tis_amd import topology as T  # noqa: E402
+from tis_amd.engine import Engine, MODE_EAGER, MODE_GRAPH, MODE_PLAIN  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+SEED = 20261202
+
+
+def build_workload(nchains=512, nrep=64, seed=SEED, small=False):
+    rng = np.random.default_rng(seed)
+    lengths = T.config1_lengths(seed, nchains)
+    systems, coords, labels = [], [], []
+    for k, n in enumerate(lengths):
+        n = int(n)
+        stem = int(rng.integers(4, 11))
+        loop = int(rng.integers(3, 9))
+        while 2 * stem + loop + 2 > n:
+            stem = max(2, stem - 1)
+            loop = max(3, loop - 1)
+        seq, pairs = T.hairpin_design(rng, n, stem, loop)
+        s = T.build_rna_system([(seq, pairs)], tempk=310.0, ionic_strength=0.150, name=f"chain{k}")
+        systems.append(s)
+        coords.append(T.helix_coords(s, rng))
+        labels.append("chain")
+    seq0, pairs0 = T.config0_chain(seed)
+    for r in range(nrep):
+        tk = 290.0 * (400.0 / 290.0) ** (r / max(nrep - 1, 1))
+        s = T.build_rna_system([(seq0, pairs0)], tempk=tk, ionic_strength=0.150, name=f"rep{r}")
+        systems.append(s)
+        coords.append(T.helix_coords(s, rng))
+        labels.append("replica")
+    return systems, coords, labels
+
+
+def algorithmic_bytes(systems, eng):
+    """Algorithmic HBM bytes of one step: per bead 36 (position read + force
+    write) + 212 (integrator state), per list entry exv 8, DH 12, per term
+    bond 16, angle 20, stack 56, hbond 76 (counts at the current lists)."""
+    nb = sum(s.nmp for s in systems)
+    nexv = nele = 0
+    for k, s in enumerate(systems):
+        nexv += len(eng.pairs(k, 0))
+        nele += len(eng.pairs(k, 1)[0])
+    nbd = sum(len(s.bd_nat) for s in systems)
+    nba = sum(len(s.ba_nat) for s in systems)
+    nst = sum(len(s.st_mp) for s in systems)
+    nhb = sum(len(s.hb_mp) for s in systems)
+    main = nb * (36 + 212) + 8 * nexv + 16 * nbd + 20 * nba + 56 * nst + 76 * nhb
+    pre = 12 * nele
+    return {"beads": nb, "exv_pairs": nexv, "ele_pairs": nele, "main_bytes": main, "pre_bytes": pre}
+
+
+def force_check(systems, coords_now, eng, sample):
+    import oracle
+    worst = 0.0
+    for k in sample:
+        s = systems[k]
+        x = coords_now[k]
+        o = oracle.Oracle(s)
+        o.neighbor(x)
+        fo, _ = o.force(x, seed=0, sys_id=k, step=0)
+        fg = eng.get_state(k, 3)
+        nf = np.linalg.norm(fo, axis=1)
+        floor = 1e-3 * max(nf.max(), 1e-12)
+        err = np.linalg.norm(fg - fo, axis=1) / np.maximum(nf, floor)
+        worst = max(worst, float(err.max()))
+    return worst
+
+
+def cpu_baseline(systems, coords, budget_s=15.0):
+    """Oracle Langevin (FP64, O(N^2) list build, rebuild every 10 steps) on
+    this host, one core, over chains of the same workload until budget_s."""
+    import oracle
+    t0 = time.perf_counter()
+    bead_steps = 0
+    nch = 0
+    steps = 300
+    order = list(range(0, len(systems), max(1, len(systems) // 64)))
+    for k in order:
+        s = systems[k]
+        o = oracle.Oracle(s)
+        x = coords[k]
+        o.langevin(x, np.zeros_like(x), np.zeros_like(x), steps, tstep=0.05, rebuild_every=10, seed=1, sys_id=k)
+        bead_steps += s.nmp * steps
+        nch += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": bead_steps / dt, "unit": "bead-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{nch} systems of the workload x {steps} Langevin steps (FP64 oracle, 1 thread), {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--plain-steps", type=int, default=10)
+    ap.add_argument("--eager-steps", type=int, default=200)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the systems of this workload are independent and would be "
+              "sharded by bead budget across GPUs; multi-GPU runs are not part of this build.", file=sys.stderr)
+        sys.exit(2)
+
+    t_build = time.perf_counter()
+    systems, coords, labels = build_workload()
+    nbeads = sum(s.nmp for s in systems)
+    eng = Engine(systems, keys=list(range(len(systems))), seed=SEED)
+    for k, x in enumerate(coords):
+        eng.set_coords(k, x)
+    eng.langevin_init(tstep=0.05, rebuild_every=10, step0=0, init_accel=True)
+    print(f"[bench] {len(systems)} systems, {nbeads} beads, setup {time.perf_counter() - t_build:.1f} s",
+          file=sys.stderr, flush=True)
+
+    eng.langevin_steps(args.warmup, MODE_GRAPH)
+    eng.synchronize()
+    t0 = time.perf_counter()
+    eng.langevin_steps(args.steps, MODE_GRAPH)
+    eng.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = nbeads * args.steps / (t1 - t0)
+    dev_ms = eng.last_timing()["total_ms"]
+    print(f"[bench] batched graph: {ms_per_step:.4f} ms/step, {value:.4e} bead-steps/s "
+          f"(device {dev_ms / args.steps:.4f} ms/step)", file=sys.stderr, flush=True)
+
+    # per-kernel averages with HIP events on the context's stream (eager launches)
+    eng.langevin_steps(args.eager_steps, MODE_EAGER)
+    tm = eng.last_timing()
+    eager_ms = tm["total_ms"] / args.eager_steps
+    print(f"[bench] eager: {eager_ms:.4f} ms/step, k_main {tm['main_ms']:.4f} ms, k_pre {tm['pre_ms']:.4f} ms",
+          file=sys.stderr, flush=True)
+
+    # plain one-chain-per-launch path on the same workload
+    tp0 = time.perf_counter()
+    eng.langevin_steps(args.plain_steps, MODE_PLAIN)
+    eng.synchronize()
+    tp1 = time.perf_counter()
+    plain = nbeads * args.plain_steps / (tp1 - tp0)
+    print(f"[bench] plain: {(tp1 - tp0) * 1e3 / args.plain_steps:.3f} ms/step, {plain:.4e} bead-steps/s",
+          file=sys.stderr, flush=True)
+
+    # lists and forces at the final state; error against the FP64 oracle
+    eng.neighbor(-1)
+    eng.force_all()
+    ab = algorithmic_bytes(systems, eng)
+    ferr = None
+    if not args.no_check:
+        rng = np.random.default_rng(1)
+        sample = sorted(set(rng.choice(len(systems), size=6, replace=False).tolist()) | {int(np.argmax([s.nmp for s in systems]))})
+        now = [eng.coords(k) for k in range(len(systems))]
+        ferr = force_check(systems, now, eng, sample)
+        print(f"[bench] force max rel err vs FP64 oracle on {len(sample)} systems: {ferr:.3e}",
+              file=sys.stderr, flush=True)
+
+    main_ms, pre_ms = tm["main_ms"], tm["pre_ms"]
+    dom = "k_main" if main_ms >= pre_ms else "k_pre"
+    dom_bytes = ab["main_bytes"] if dom == "k_main" else ab["pre_bytes"]
+    dom_ms = max(main_ms, pre_ms)
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
+    roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "avg_launch_ms": round(dom_ms, 5), "bytes_per_launch": dom_bytes,
+            "k_main_ms": round(main_ms, 5), "k_pre_ms": round(pre_ms, 5)}
+
+    cpu = None if args.no_cpu else cpu_baseline(systems, coords, args.cpu_budget)
+    out = {
+        "metric": "batched bead-steps/s vs plain GPU @ force err <=1e-5 vs FP64; p50/p99 job latency",
+        "value": value,
+        "unit": "bead-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 pair / fp64 accumulate+integrate",
+        "data": "synthetic (seeded hairpin designs, extended-helix start)",
+        "config": {"workload": "512 lognormal(ln120,0.7) 20-600 nt DT15 chains + 64 replicas of the 64-nt "
+                               "hairpin on a 290-400 K ladder; Langevin dt 0.05 tau; rebuild every 10 steps",
+                   "systems": len(systems), "beads": nbeads, "parallelism": "single GPU, batched"},
+        "plain_gpu_bead_steps_per_s": plain,
+        "batched_over_plain": value / plain,
+        "force_max_rel_err_vs_fp64": ferr,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,117 @@
+/*
+ * tis_hip.h -- C-ABI of libtis_hip.so, the MI355X (gfx950) implementation of
+ * the DT15 TIS per-step force-and-integrate path.
+ *
+ * One context owns a batch of systems (one system = one replica of one
+ * molecule set, or one independent chain) resident in HBM.  Every entry point
+ * returns int32 status: 0 ok, < 0 error (message via tis_last_error).  Plain
+ * pointers and sizes only; no exceptions cross the ABI.  Calls on one context
+ * are stream-ordered and not thread-safe; separate contexts are independent.
+ *
+ * Reference interfaces replaced (paths in the reference's src/):
+ *   tis_neighbor       <- neighbor(irep)                 neighbor.F90:13
+ *   tis_force_sumup    <- force_sumup(force_mp, irep)    force_sumup.F90:15-32
+ *   tis_energy_sumup   <- energy_sumup(irep, velo_mp, energy, energy_unit)
+ *                                                        energy_sumup.F90:17-36
+ *   tis_langevin_*     <- the Langevin branch of time_integral
+ *                        (time_integral_pre.F90:160-180, time_integral.F90:57-189)
+ *   tis_set_system_T   <- simu_para2 / simu_ele_set / simu_set_dtrna refresh
+ *                        (simu_para2.F90:8-108) for one replica
+ */
+#ifndef TIS_HIP_H
+#define TIS_HIP_H
+
+#include <stdint.h>
+#include "tis_system.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tis_ctx tis_ctx;
+
+/* status codes */
+enum { TIS_OK = 0, TIS_ERR_ARG = -1, TIS_ERR_HIP = -2, TIS_ERR_STATE = -3,
+       TIS_ERR_OVERFLOW = -4, TIS_ERR_NOMEM = -5 };
+
+/* which list for tis_pairs_get */
+enum { TIS_LIST_EXV = 0, TIS_LIST_ELE = 1, TIS_LIST_HB = 2 };
+
+/* counters of tis_status_counters */
+enum { TIS_CNT_CLAMP_EXV = 0, TIS_CNT_CLAMP_ELE = 1, TIS_CNT_HB_DELETED = 2,
+       TIS_CNT_REBUILDS = 3, TIS_CNT_STEPS = 4, TIS_CNT_NB_OVERFLOW = 5,
+       TIS_CNT_MAX = 8 };
+
+/* Langevin launch modes for tis_langevin_steps */
+enum { TIS_MODE_GRAPH = 0,   /* batched, hipGraph-captured, one step = 2 launches */
+       TIS_MODE_EAGER = 1,   /* batched, launched eagerly                         */
+       TIS_MODE_PLAIN = 2 }; /* one system per launch (plain GPU path)            */
+
+/* Create a context on HIP device `device` (its own stream, default priority). */
+int32_t tis_ctx_create(int32_t device, tis_ctx **out);
+int32_t tis_ctx_destroy(tis_ctx *ctx);
+/* Last error message of the context (static storage owned by ctx). */
+const char *tis_last_error(const tis_ctx *ctx);
+
+/* Upload a batch of nsys systems (replaces any previous batch).  keys[s] is the
+ * stable identity of system s in the random streams (noise of a system depends
+ * only on (seed, key, step, bead), not on its slot in the batch); NULL means
+ * key = s.  Coordinates start at zero; set them with tis_coords_set. */
+int32_t tis_system_upload(tis_ctx *ctx, int32_t nsys, const tis_system *systems,
+                          const uint32_t *keys, uint64_t seed);
+
+/* Refresh the temperature-dependent coefficients of system isys (tempk,
+ * st_coef U0, charge, ele_coef, ele_cdist, beta_hb) from `sys`, which must
+ * have the same topology as the uploaded one.  Langevin constants follow. */
+int32_t tis_set_system_T(tis_ctx *ctx, int32_t isys, const tis_system *sys);
+
+/* State of system isys, (3,N) column-major FP64 like xyz_mp_rep(:,:,irep).
+ * which: 0 coordinates, 1 velocities, 2 accelerations. */
+int32_t tis_state_set(tis_ctx *ctx, int32_t isys, int32_t which, const double *buf);
+int32_t tis_state_get(tis_ctx *ctx, int32_t isys, int32_t which, double *buf);
+
+/* neighbor(irep): rebuild exv / ele / hb lists of system isys (-1: all) from
+ * the current coordinates. */
+int32_t tis_neighbor(tis_ctx *ctx, int32_t isys);
+
+/* Copy a list of system isys out (parity checks).  exv/ele: pairs[3*k] =
+ * (i, j, mirror) with i < j, 0-based, sorted by (i, j); coef[k] (ele only,
+ * may be NULL) = q_i q_j ele_coef.  hb: pairs[k] = candidate index.  *n gets
+ * the length; returns TIS_ERR_OVERFLOW if cap is too small. */
+int32_t tis_pairs_get(tis_ctx *ctx, int32_t isys, int32_t which, int32_t *pairs,
+                      double *coef, int64_t cap, int64_t *n);
+
+/* force_sumup: forces on every bead of system isys at the current coordinates
+ * and lists, out (3,N) FP64.  isys = -1 evaluates all systems in one launch
+ * and writes nothing (use tis_force_get). */
+int32_t tis_force_sumup(tis_ctx *ctx, int32_t isys, double *force_out);
+int32_t tis_force_get(tis_ctx *ctx, int32_t isys, double *force_out);
+
+/* energy_sumup: energy[TIS_E_MAX] (E_TYPE slots, TIS_E_TOTAL excludes VELO)
+ * of system isys at the current coordinates, lists and last hbond state;
+ * kinetic term from the stored velocities. */
+int32_t tis_energy_sumup(tis_ctx *ctx, int32_t isys, double *energy_out);
+
+/* Langevin dynamics for every system of the batch.
+ * tis_langevin_init: time step tstep (tau), neighbour rebuild every
+ * rebuild_every steps (> 0), first step index step0; if init_accel, the
+ * accelerations are set to F/m + c1*xi(step0) (time_integral_pre.F90:176-179),
+ * otherwise the stored accelerations are used. */
+int32_t tis_langevin_init(tis_ctx *ctx, double tstep, int32_t rebuild_every, uint64_t step0,
+                          int32_t init_accel);
+int32_t tis_langevin_steps(tis_ctx *ctx, int64_t nsteps, int32_t mode);
+
+/* Counters (TIS_CNT_*), accumulated since upload. */
+int32_t tis_status_counters(tis_ctx *ctx, int64_t *counters);
+int32_t tis_synchronize(tis_ctx *ctx);
+
+/* Device-side timing of the last tis_langevin_steps call: total milliseconds
+ * and the per-launch average of the force/integrate kernel (ms), from HIP
+ * events on the context's stream. */
+int32_t tis_last_timing(tis_ctx *ctx, double *total_ms, double *step_kernel_ms,
+                        double *pre_kernel_ms, int64_t *nlaunch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TIS_HIP_H */

@@ -1,0 +1,286 @@
+"""Python handles on the CPU checkers (test infrastructure only).
+
+``Oracle``  wraps liboracle.so, the FP64 C restatement (tis_oracle.c).
+``RefLib``  wraps _ref/libtisref.so, the reference's own Fortran routines
+            compiled from its sources (built only where they are present).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+import this package; it is the checker, never the thing measured or shipped.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REFLIB = os.path.join(HERE, "_ref", "libtisref.so")
+_f64p = C.POINTER(C.c_double)
+_i32p = C.POINTER(C.c_int32)
+
+
+def build(ref: bool = True) -> None:
+    """Compile liboracle.so (and _ref/libtisref.so when the reference is present)."""
+    subprocess.check_call(["make", "-s", "-C", HERE, "oracle"])
+    if ref and os.path.isdir("/root/reference/src"):
+        subprocess.check_call(["make", "-s", "-C", HERE, "ref"])
+
+
+def _lib():
+    if not os.path.exists(LIB):
+        build(ref=False)
+    lib = C.CDLL(LIB)
+    vp = C.c_void_p
+    lib.oracle_ws_new.restype = vp
+    lib.oracle_ws_new.argtypes = [vp]
+    lib.oracle_ws_free.argtypes = [vp]
+    lib.oracle_neighbor.argtypes = [vp, _f64p]
+    lib.oracle_ws_lists.restype = C.c_int64
+    lib.oracle_ws_lists.argtypes = [vp, C.c_int32, C.POINTER(_i32p), C.POINTER(_f64p)]
+    lib.oracle_force_sumup.restype = C.c_int32
+    lib.oracle_force_sumup.argtypes = [vp, _f64p, _f64p, C.c_uint64, C.c_uint32, C.c_uint64]
+    lib.oracle_energy_sumup.argtypes = [vp, _f64p, _f64p, _f64p]
+    lib.oracle_langevin_run.restype = C.c_int32
+    lib.oracle_langevin_run.argtypes = [vp, C.c_double, C.c_int64, C.c_int32, _f64p, _f64p, _f64p,
+                                        C.c_uint64, C.c_uint32, C.c_uint64, C.c_int32]
+    lib.oracle_langevin_consts.argtypes = [vp, C.c_double, _f64p]
+    lib.oracle_gauss3.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32, _f64p]
+    lib.oracle_uniform.restype = C.c_double
+    lib.oracle_uniform.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32]
+    lib.oracle_philox4x32.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+    for nm in ("oracle_force_bond", "oracle_force_bangle"):
+        getattr(lib, nm).argtypes = [vp, _f64p, _f64p]
+    lib.oracle_force_exv.restype = C.c_int32
+    lib.oracle_force_exv.argtypes = [vp, _f64p, _i32p, C.c_int64, _f64p]
+    lib.oracle_force_dh.restype = C.c_int32
+    lib.oracle_force_dh.argtypes = [vp, _f64p, _i32p, _f64p, C.c_int64, _f64p]
+    lib.oracle_force_stack.argtypes = [vp, _f64p, C.c_char_p, _f64p]
+    lib.oracle_force_tstack.argtypes = [vp, _f64p, _f64p, C.c_char_p, _f64p]
+    lib.oracle_hbond_eval.argtypes = [vp, _f64p, _f64p, _i32p, C.c_int32, _f64p, _f64p, C.c_char_p]
+    lib.oracle_hbond_valence.restype = C.c_int32
+    lib.oracle_hbond_valence.argtypes = [vp, _i32p, C.c_int32, _f64p, C.c_char_p,
+                                         C.c_uint64, C.c_uint32, C.c_uint64]
+    lib.oracle_hbond_scatter.argtypes = [vp, _i32p, C.c_int32, _f64p, C.c_char_p, _f64p]
+    return lib
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        _LIB = _lib()
+    return _LIB
+
+
+def fp(a):
+    return a.ctypes.data_as(_f64p)
+
+
+def ip(a):
+    return a.ctypes.data_as(_i32p)
+
+
+class Oracle:
+    """FP64 restatement for one System (tis_amd.sysdesc.System)."""
+
+    def __init__(self, system):
+        self.sys = system
+        self.cs = system.cstruct()
+        self.L = lib()
+        self.ws = self.L.oracle_ws_new(C.byref(self.cs))
+
+    def __del__(self):
+        try:
+            self.L.oracle_ws_free(self.ws)
+        except Exception:
+            pass
+
+    def neighbor(self, xyz):
+        x = np.ascontiguousarray(xyz, dtype=np.float64)
+        self.L.oracle_neighbor(self.ws, fp(x))
+
+    def lists(self, which):
+        p = _i32p()
+        c = _f64p()
+        n = self.L.oracle_ws_lists(self.ws, which, C.byref(p), C.byref(c))
+        if which == 2:
+            return np.ctypeslib.as_array(p, shape=(max(n, 1),))[:n].copy()
+        pairs = np.ctypeslib.as_array(p, shape=(max(n, 1) * 3,))[:3 * n].reshape(n, 3).copy()
+        if which == 1:
+            coef = np.ctypeslib.as_array(c, shape=(max(n, 1),))[:n].copy()
+            return pairs, coef
+        return pairs
+
+    def force(self, xyz, seed=0, sys_id=0, step=0):
+        x = np.ascontiguousarray(xyz, dtype=np.float64)
+        f = np.zeros_like(x)
+        ncl = self.L.oracle_force_sumup(self.ws, fp(x), fp(f), seed, sys_id, step)
+        return f, ncl
+
+    def energy(self, xyz, velo=None):
+        x = np.ascontiguousarray(xyz, dtype=np.float64)
+        e = np.zeros(25)
+        v = None if velo is None else np.ascontiguousarray(velo, dtype=np.float64)
+        self.L.oracle_energy_sumup(self.ws, fp(x), fp(v) if v is not None else None, fp(e))
+        return e
+
+    def langevin(self, xyz, velo, accel, nsteps, tstep=0.05, rebuild_every=10, seed=1,
+                 sys_id=0, step0=0, init_accel=True):
+        x = np.ascontiguousarray(xyz, dtype=np.float64).copy()
+        v = np.ascontiguousarray(velo, dtype=np.float64).copy()
+        a = np.ascontiguousarray(accel, dtype=np.float64).copy()
+        nr = self.L.oracle_langevin_run(self.ws, tstep, nsteps, rebuild_every, fp(x), fp(v), fp(a),
+                                        seed, sys_id, step0, 1 if init_accel else 0)
+        return x, v, a, nr
+
+    def langevin_consts(self, tstep=0.05):
+        r = np.zeros((self.sys.nmp, 4))
+        self.L.oracle_langevin_consts(C.byref(self.cs), tstep, fp(r))
+        return r
+
+    def term(self, name, xyz, pairs=None, coef=None):
+        """One force term on its own (lists from the last neighbor())."""
+        x = np.ascontiguousarray(xyz, dtype=np.float64)
+        f = np.zeros_like(x)
+        cs = C.byref(self.cs)
+        if name == "bond":
+            self.L.oracle_force_bond(cs, fp(x), fp(f))
+        elif name == "angle":
+            self.L.oracle_force_bangle(cs, fp(x), fp(f))
+        elif name == "exv":
+            pr = np.ascontiguousarray(self.lists(0) if pairs is None else pairs, dtype=np.int32)
+            self.L.oracle_force_exv(cs, fp(x), ip(pr), len(pr), fp(f))
+        elif name == "dh":
+            pr, cf = self.lists(1) if pairs is None else (pairs, coef)
+            pr = np.ascontiguousarray(pr, dtype=np.int32)
+            cf = np.ascontiguousarray(cf, dtype=np.float64)
+            self.L.oracle_force_dh(cs, fp(x), ip(pr), fp(cf), len(pr), fp(f))
+        elif name == "stack":
+            st = np.ones(max(len(self.sys.st_mp), 1), np.uint8)
+            self.L.oracle_force_tstack(cs, fp(x), fp(x), st.ctypes.data_as(C.c_char_p), fp(f))
+            self.L.oracle_force_stack(cs, fp(x), st.ctypes.data_as(C.c_char_p), fp(f))
+        elif name == "hbond":
+            hb = np.ascontiguousarray(self.lists(2), dtype=np.int32)
+            nh = max(len(self.sys.hb_mp), 1)
+            forh = np.zeros(18 * nh)
+            ene = np.zeros(nh)
+            stat = np.zeros(nh, np.uint8)
+            self.L.oracle_hbond_eval(cs, fp(x), fp(x), ip(hb), len(hb), fp(forh), fp(ene),
+                                     stat.ctypes.data_as(C.c_char_p))
+            self.L.oracle_hbond_valence(cs, ip(hb), len(hb), fp(ene),
+                                        stat.ctypes.data_as(C.c_char_p), 0, 0, 0)
+            self.L.oracle_hbond_scatter(cs, ip(hb), len(hb), fp(forh),
+                                        stat.ctypes.data_as(C.c_char_p), fp(f))
+            return f, stat, ene
+        else:
+            raise KeyError(name)
+        return f
+
+
+def gauss3(seed, sys_id, step, bead):
+    g = np.zeros(3)
+    lib().oracle_gauss3(seed, sys_id, step, bead, fp(g))
+    return g
+
+
+def uniform(seed, sys_id, step, stream, idx):
+    return lib().oracle_uniform(seed, sys_id, step, stream, idx)
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().oracle_philox4x32(c, k, o)
+    return list(o)
+
+
+# ---------------------------------------------------------------------------
+class RefLib:
+    """The reference's own Fortran routines (one replica, one thread)."""
+
+    TERMS = {"bond": 0, "angle": 1, "stack": 2, "stack_only": 3, "hbond": 4, "exv": 5, "dh": 6}
+
+    @staticmethod
+    def available() -> bool:
+        return os.path.exists(REFLIB)
+
+    def __init__(self):
+        self.L = C.CDLL(REFLIB)
+        L = self.L
+        L.tisref_load.argtypes = [C.c_void_p]
+        L.tisref_neighbor.argtypes = [_f64p]
+        L.tisref_move.argtypes = [_f64p]
+        L.tisref_list_count.restype = C.c_int64
+        L.tisref_list_count.argtypes = [C.c_int32]
+        L.tisref_list_get.argtypes = [C.c_int32, _i32p, _f64p]
+        L.tisref_force_term.argtypes = [C.c_int32, _f64p]
+        L.tisref_hb_state.argtypes = [_i32p, _f64p]
+        L.tisref_energy.argtypes = [_f64p, C.c_int32, _f64p]
+        L.tisref_set_mass.argtypes = [_f64p]
+        L.tisref_temperature.argtypes = [C.c_double, C.c_double, _f64p, _f64p, _f64p, _f64p,
+                                         _f64p, _f64p]
+        self.sys = None
+
+    def load(self, system):
+        self.sys = system
+        self.cs = system.cstruct()
+        self.L.tisref_load(C.byref(self.cs))
+        self._mass = np.ascontiguousarray(system.mass, dtype=np.float64)
+        self.L.tisref_set_mass(fp(self._mass))
+
+    def neighbor(self, xyz):
+        self._x = np.ascontiguousarray(xyz, dtype=np.float64)
+        self.L.tisref_neighbor(fp(self._x))
+
+    def move(self, xyz):
+        self._x = np.ascontiguousarray(xyz, dtype=np.float64)
+        self.L.tisref_move(fp(self._x))
+
+    def lists(self, which):
+        n = self.L.tisref_list_count(which)
+        if which == 2:
+            p = np.zeros(max(n, 1), np.int32)
+            self.L.tisref_list_get(which, ip(p), fp(np.zeros(1)))
+            return p[:n]
+        p = np.zeros(3 * max(n, 1), np.int32)
+        c = np.zeros(max(n, 1))
+        self.L.tisref_list_get(which, ip(p), fp(c))
+        pairs = p[:3 * n].reshape(n, 3)
+        return (pairs, c[:n]) if which == 1 else pairs
+
+    def force_term(self, name):
+        f = np.zeros((self.sys.nmp, 3))
+        self.L.tisref_force_term(self.TERMS[name], fp(f))
+        return f
+
+    def hb_state(self):
+        nh = max(len(self.sys.hb_mp), 1)
+        st = np.zeros(nh, np.int32)
+        ene = np.zeros(nh)
+        self.L.tisref_hb_state(ip(st), fp(ene))
+        return st[:len(self.sys.hb_mp)], ene[:len(self.sys.hb_mp)]
+
+    def energy(self, velo=None):
+        e = np.zeros(25)
+        if velo is None:
+            self.L.tisref_energy(fp(np.zeros(3)), 0, fp(e))
+        else:
+            v = np.ascontiguousarray(velo, dtype=np.float64)
+            self.L.tisref_energy(fp(v), 1, fp(e))
+        return e
+
+    def temperature(self, tempk, ionic):
+        hs = np.ascontiguousarray(self.sys.st_hsTm, dtype=np.float64)
+        if hs.size == 0:
+            hs = np.zeros((1, 3))
+        coef, cdist, zp, beta = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        u0 = np.zeros(max(len(self.sys.st_mp), 1))
+        self.L.tisref_temperature(tempk, ionic, fp(hs), C.byref(coef), C.byref(cdist),
+                                  C.byref(zp), fp(u0), C.byref(beta))
+        return coef.value, cdist.value, zp.value, u0[:len(self.sys.st_mp)], beta.value

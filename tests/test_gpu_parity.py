@@ -1,0 +1,121 @@
+"""GPU parity: HIP path (through the C-ABI) against the FP64 oracle."""
+import numpy as np
+import pytest
+
+import oracle
+from tis_amd.engine import Engine, LIST_ELE, LIST_EXV, LIST_HB, MODE_EAGER, MODE_GRAPH, MODE_PLAIN
+
+from systems import force_error, hairpin_design, hairpin64, parity_cases
+
+pytestmark = pytest.mark.gpu
+CASES = parity_cases()
+
+
+def _sorted(p):
+    p = np.asarray(p)
+    if p.ndim == 1:
+        return np.sort(p)
+    return p[np.lexsort(p.T[::-1])]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_lists_bit_exact(case):
+    name, s, x = case
+    print(f"[lists] {name} N={s.nmp}", flush=True)
+    eng = Engine([s])
+    eng.set_coords(0, x)
+    eng.neighbor(0)
+    o = oracle.Oracle(s)
+    o.neighbor(x)
+    ge = eng.pairs(0, LIST_EXV)
+    oe = o.lists(0)
+    assert np.array_equal(_sorted(ge), _sorted(oe)), (len(ge), len(oe))
+    gq, gc = eng.pairs(0, LIST_ELE)
+    oq, oc = o.lists(1)
+    assert np.array_equal(_sorted(gq), _sorted(oq))
+    assert np.array_equal(gc[np.lexsort(gq.T[::-1])], oc[np.lexsort(oq.T[::-1])])
+    assert np.array_equal(np.sort(eng.pairs(0, LIST_HB)), np.sort(o.lists(2)))
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_force_sumup(case):
+    name, s, x = case
+    print(f"[force] {name} N={s.nmp}", flush=True)
+    eng = Engine([s], keys=[0], seed=0)
+    eng.set_coords(0, x)
+    eng.neighbor(0)
+    f = eng.force_sumup(0)
+    o = oracle.Oracle(s)
+    o.neighbor(x)
+    fo, _ = o.force(x, seed=0, sys_id=0, step=0)
+    err = force_error(f, fo)
+    print(f"   max rel err {err.max():.3e} p99 {np.percentile(err, 99):.3e}", flush=True)
+    assert err.max() <= 1e-5
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_energy_sumup(case):
+    name, s, x = case
+    print(f"[energy] {name}", flush=True)
+    rng = np.random.default_rng(5)
+    v = rng.normal(size=x.shape)
+    eng = Engine([s], keys=[0], seed=0)
+    eng.set_coords(0, x)
+    eng.set_state(0, 1, v)
+    eng.neighbor(0)
+    eng.force_sumup(0)
+    e = eng.energy_sumup(0)
+    o = oracle.Oracle(s)
+    o.neighbor(x)
+    o.force(x, seed=0, sys_id=0, step=0)
+    eo = o.energy(x, v)
+    rel = np.abs(e - eo) / np.maximum(np.abs(eo), 1e-6)
+    print("   energy rel", rel.max(), flush=True)
+    assert rel.max() <= 1e-7
+
+
+def test_batch_equals_alone_bitwise():
+    """A system gives the same bits alone and inside a mixed batch."""
+    systems = [hairpin_design(100 + k, n) for k, n in enumerate([30, 77, 150, 64, 211])]
+    batch = Engine([s for s, _ in systems], keys=list(range(len(systems))), seed=3)
+    for k, (s, x) in enumerate(systems):
+        batch.set_coords(k, x)
+    batch.neighbor(-1)
+    fb = batch.force_all()
+    for k, (s, x) in enumerate(systems):
+        alone = Engine([s], keys=[k], seed=3)
+        alone.set_coords(0, x)
+        alone.neighbor(0)
+        fa = alone.force_sumup(0)
+        assert np.array_equal(fa, fb[k])
+
+
+def test_langevin_matches_oracle_short():
+    s, x = hairpin64(0, "helix")
+    v = np.zeros_like(x)
+    eng = Engine([s], keys=[9], seed=11)
+    eng.set_coords(0, x)
+    eng.set_state(0, 1, v)
+    eng.langevin_init(tstep=0.05, rebuild_every=10, step0=0, init_accel=True)
+    eng.langevin_steps(30, MODE_EAGER)
+    xg = eng.coords(0)
+    o = oracle.Oracle(s)
+    xo, vo, ao, nr = o.langevin(x, v, np.zeros_like(x), 30, tstep=0.05, rebuild_every=10, seed=11, sys_id=9)
+    d = np.abs(xg - xo).max()
+    print(f"   30-step max |dx| = {d:.3e} A", flush=True)
+    assert d < 1e-4
+
+
+def test_modes_bitwise_equal():
+    systems = [hairpin_design(200 + k, n) for k, n in enumerate([40, 90, 120])]
+    res = []
+    for mode in (MODE_EAGER, MODE_GRAPH, MODE_PLAIN):
+        eng = Engine([s for s, _ in systems], seed=5)
+        for k, (s, x) in enumerate(systems):
+            eng.set_coords(k, x)
+        eng.langevin_init(tstep=0.05, rebuild_every=10)
+        eng.langevin_steps(40, mode)
+        res.append([eng.coords(k) for k in range(len(systems))])
+    for k in range(len(systems)):
+        assert np.array_equal(res[0][k], res[1][k])
+        assert np.array_equal(res[0][k], res[2][k])

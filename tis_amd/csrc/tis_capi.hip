@@ -1,0 +1,882 @@
+// tis_capi.hip -- host side of libtis_hip: context, batch upload into the
+// HBM layout of tis_dev.h, and the C-ABI of include/tis_hip.h.  Kernel
+// launches are stream-ordered on the context's own stream (default
+// priority); the batched Langevin loop is captured once into a hipGraph and
+// replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/tis_hip.h"
+#include "tis_dev.h"
+
+namespace tis {
+__global__ void k_pre(DevPtrs, const double *, const double *, const int32_t *, int, const Tile *, int, int32_t, int);
+__global__ void k_main(DevPtrs, const double *, const double *, double *, double *, const Tile *, int, int32_t, int, int);
+__global__ void k_wrap(DevPtrs, const double *, double *, double *, const Tile *, int);
+__global__ void k_list_exv(DevPtrs, const double *, const double *, const Tile *);
+__global__ void k_list_ele(DevPtrs, const double *, const double *, const Tile *);
+__global__ void k_list_hb(DevPtrs, const double *, const double *, int, int);
+__global__ void k_advance(uint64_t *, uint64_t, unsigned long long *);
+__global__ void k_count_rebuild(unsigned long long *);
+__global__ void k_energy(DevPtrs, const double *, const double *, const int32_t *, double *);
+}  // namespace tis
+
+using namespace tis;
+
+namespace {
+
+constexpr double kBoltzKcalMol = 1.38064852e-23 * ((1.0 / 4184.0) * 6.022140857e23);
+
+struct HostSys {
+  DSys d;
+  int32_t bt_begin = 0, bt_count = 0;   // bead tiles
+  int32_t qt_begin = 0, qt_count = 0;   // charge tiles
+  bool has_task = false;                // hbonds or tertiary stacks
+  std::vector<double> mass, fric;
+  std::vector<int32_t> stack_ids;       // global stack indices
+};
+
+template <typename T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+};
+
+}  // namespace
+
+struct tis_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // batch
+  int32_t nsys = 0, ntot = 0, nqtot = 0, nhbtot = 0, ntsttot = 0, nsttot = 0;
+  std::vector<HostSys> hs;
+  std::vector<void *> allocs;
+  DevPtrs P{};
+  DSys *d_sys = nullptr;
+  Tile *d_btiles = nullptr, *d_qtiles = nullptr;
+  int32_t nbtile = 0, nqtile = 0;
+  int32_t *d_systask = nullptr, *d_sysidx = nullptr;
+  int32_t nsystask = 0;
+  std::vector<int32_t> systask_of;       // index into d_systask per system or -1
+  double *d_energy = nullptr;
+  unsigned long long *d_counters = nullptr;
+  uint64_t *d_stepbase = nullptr;
+  bool any_dh = false, any_periodic = false;
+  // state
+  int cur = 0;
+  bool lists_valid = false;
+  // Langevin
+  bool lv_init = false;
+  double tstep = 0.0;
+  int32_t rebuild_every = 10;
+  int64_t t_since_init = 0;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  int graph_len = 0;
+  // timing
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_total_ms = 0.0, last_main_ms = 0.0, last_pre_ms = 0.0;
+  int64_t last_nlaunch = 0;
+  // per-kernel events of eager runs: (before k_pre, between, after k_main)
+  bool timing = false;
+  std::vector<hipEvent_t> tev;
+  size_t tev_used = 0;
+};
+
+namespace {
+
+int fail(tis_ctx *c, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                               \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(ctx, TIS_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));     \
+  } while (0)
+
+template <typename T>
+int dalloc(tis_ctx *c, T **p, size_t n) {
+  *p = nullptr;
+  size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+  hipError_t e = hipMalloc((void **)p, bytes);
+  if (e != hipSuccess) return fail(c, TIS_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  c->allocs.push_back((void *)*p);
+  e = hipMemsetAsync(*p, 0, bytes, c->stream);
+  if (e != hipSuccess) return fail(c, TIS_ERR_HIP, "hipMemsetAsync failed: %s", hipGetErrorString(e));
+  return 0;
+}
+
+template <typename T>
+int dupload(tis_ctx *c, T **p, const std::vector<T> &h) {
+  int rc = dalloc(c, p, h.size());
+  if (rc) return rc;
+  if (!h.empty()) HIPCHK(c, hipMemcpyAsync(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  return 0;
+}
+
+void free_all(tis_ctx *c) {
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (int k = 0; k < 2; ++k)
+    if (c->graph[k]) { (void)hipGraphExecDestroy(c->graph[k]); c->graph[k] = nullptr; }
+  for (void *p : c->allocs) (void)hipFree(p);
+  c->allocs.clear();
+  c->hs.clear();
+  c->P = DevPtrs{};
+  c->nsys = c->ntot = 0;
+  c->lists_valid = false;
+  c->lv_init = false;
+}
+
+void fill_derived(DSys &d, const tis_system &s) {
+  const tis_params &p = s.p;
+  d.periodic = p.periodic;
+  for (int k = 0; k < 3; ++k) d.box[k] = p.box[k];
+  d.nsep[0] = p.n_sep_P; d.nsep[1] = p.n_sep_S; d.nsep[2] = p.n_sep_B;
+  d.ele_form = p.ele_form; d.dh_type = p.dh_cutoff_type;
+  d.exv_a = p.exv_adjust; d.exv_dist = p.exv_dist; d.exv_dist_PS = p.exv_dist_PS; d.exv_inf = p.exv_inf;
+  d.hb_cut = p.hb_cutoff_dist; d.margin = p.neigh_margin;
+  d.cutoff_ele = p.cutoff_ele; d.ele_coef = p.ele_coef; d.cdist = p.ele_cdist;
+  d.rcdist = 1.0 / p.ele_cdist;
+  // list radius, neighbor_list_ele.F90:73-90
+  double r;
+  if (p.ele_form == TIS_ELE_DH && p.dh_cutoff_type == 0) r = p.neigh_margin + p.cutoff_ele * p.ele_cdist;
+  else r = p.neigh_margin + p.cutoff_ele;
+  d.ele_rl2 = r * r;
+  double c = p.cutoff_ele * p.ele_cdist;
+  d.dh_cut2 = c * c;  // force cutoff ignores i_DH_cutoff_type (force_ele_DH.F90:44)
+  d.dh_ecut2 = p.dh_cutoff_type == 0 ? c * c : p.cutoff_ele * p.cutoff_ele;  // energy_ele_DH.F90:33-37
+  d.beta_hb = p.beta_hb;
+  d.tempk = s.tempk;
+}
+
+// Langevin constants, time_integral_pre.F90:160-170
+void langevin_consts(double tstep, double tempk, double fric, double mass, double r[4]) {
+  double h = 0.5 * tstep * fric;
+  double u1 = 1.0 - h, u2 = 1.0 - h + h * h;
+  r[0] = std::sqrt(2.0 * fric * kBoltzKcalMol * tempk / (tstep * mass));
+  r[1] = u1 * u2;
+  r[2] = (0.5 * tstep) * u1;
+  r[3] = tstep * u1;
+}
+
+int check_sys(tis_ctx *c, int32_t isys) {
+  if (c->nsys == 0) return fail(c, TIS_ERR_STATE, "no systems uploaded");
+  if (isys < -1 || isys >= c->nsys) return fail(c, TIS_ERR_ARG, "system index %d out of range", isys);
+  return 0;
+}
+
+double *Xc(tis_ctx *c) { return c->P.x[c->cur]; }
+double *PXc(tis_ctx *c) { return c->P.px[c->cur]; }
+
+// --- launch helpers on a system subset: isys = -1 all, else one system
+struct Range { int bt0, btn, qt0, qtn, task0, taskn; const int32_t *sysidx; int nsel; };
+
+Range range_of(tis_ctx *c, int32_t isys) {
+  Range r{};
+  if (isys < 0) {
+    r.bt0 = 0; r.btn = c->nbtile; r.qt0 = 0; r.qtn = c->nqtile; r.task0 = 0; r.taskn = c->nsystask;
+    r.sysidx = c->d_sysidx; r.nsel = c->nsys;
+  } else {
+    const HostSys &h = c->hs[isys];
+    r.bt0 = h.bt_begin; r.btn = h.bt_count; r.qt0 = h.qt_begin; r.qtn = h.qt_count;
+    r.task0 = c->systask_of[isys] < 0 ? 0 : c->systask_of[isys];
+    r.taskn = c->systask_of[isys] < 0 ? 0 : 1;
+    r.sysidx = c->d_sysidx + isys; r.nsel = 1;
+  }
+  return r;
+}
+
+int launch_lists(tis_ctx *c, const Range &r, const double *X, double *PXcur, double *PXpred, int with_pred) {
+  if (c->any_periodic && r.btn > 0)
+    hipLaunchKernelGGL(k_wrap, dim3(r.btn), dim3(kTile), 0, c->stream, c->P, X, PXcur, PXpred,
+                       c->d_btiles + r.bt0, with_pred);
+  if (r.btn > 0)
+    hipLaunchKernelGGL(k_list_exv, dim3(r.btn), dim3(kTile), 0, c->stream, c->P, X, PXcur, c->d_btiles + r.bt0);
+  if (r.qtn > 0)
+    hipLaunchKernelGGL(k_list_ele, dim3(r.qtn), dim3(kTile), 0, c->stream, c->P, X, PXcur, c->d_qtiles + r.qt0);
+  return 0;
+}
+
+}  // namespace
+
+// =====================================================================
+extern "C" {
+
+int32_t tis_ctx_create(int32_t device, tis_ctx **out) {
+  if (!out) return TIS_ERR_ARG;
+  *out = nullptr;
+  tis_ctx *c = new tis_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess) { delete c; return TIS_ERR_HIP; }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return TIS_ERR_HIP; }
+  (void)hipEventCreate(&c->ev0);
+  (void)hipEventCreate(&c->ev1);
+  *out = c;
+  return TIS_OK;
+}
+
+int32_t tis_ctx_destroy(tis_ctx *c) {
+  if (!c) return TIS_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  free_all(c);
+  for (hipEvent_t e : c->tev) (void)hipEventDestroy(e);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return TIS_OK;
+}
+
+const char *tis_last_error(const tis_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const uint32_t *keys,
+                          uint64_t seed) {
+  if (!c || nsys <= 0 || !sys) return fail(c, TIS_ERR_ARG, "bad arguments to tis_system_upload");
+  HIPCHK(c, hipSetDevice(c->device));
+  free_all(c);
+  c->nsys = nsys;
+  c->hs.resize(nsys);
+  // host staging
+  std::vector<double> rl2;
+  std::vector<int8_t> mptype, iclass;
+  std::vector<int32_t> unit, qidx, qbead, inc_off;
+  std::vector<uint32_t> inc;
+  std::vector<double> exv_rad, charge, rcm, mass;
+  std::vector<float> exv_eps;
+  std::vector<int32_t> bd_mp, ba_mp, st_mp, tst_mp, tst2st, tst_excl, hb_mp, hb_site, hb_tert, site_val,
+      hb_sys, tst_sys;
+  std::vector<double> bd_nat, bd_coef, ba_nat, ba_coef, st_nat, st_coef, tst_nat, tst_coef, hb_nat, hb_coef;
+  std::vector<Tile> btiles, qtiles;
+  std::vector<int32_t> systask;
+  c->systask_of.assign(nsys, -1);
+  int64_t moff = 0, mioff = 0;
+  int32_t off = 0, qoff = 0, nbd = 0, nba = 0, nst = 0, ntst = 0, nhb = 0, nsite = 0, uoff = 0;
+  c->any_dh = false;
+  c->any_periodic = false;
+  for (int is = 0; is < nsys; ++is) {
+    const tis_system &s = sys[is];
+    HostSys &h = c->hs[is];
+    if (s.nmp <= 0 || s.nunit <= 0) return fail(c, TIS_ERR_ARG, "system %d has no beads", is);
+    DSys &d = h.d;
+    std::memset(&d, 0, sizeof(d));
+    fill_derived(d, s);
+    d.key = keys ? keys[is] : (uint32_t)is;
+    d.off = off; d.nmp = s.nmp;
+    d.uoff = uoff; d.nunit = s.nunit;
+    for (int k = 0; k < s.nunit * s.nunit; ++k) rl2.push_back(s.rlist2_unit[k]);
+    uoff += s.nunit * s.nunit;
+    if (s.p.periodic) c->any_periodic = true;
+    if (s.p.ele_form == TIS_ELE_DH) c->any_dh = true;
+    // beads
+    d.qoff = qoff;
+    int nq = 0;
+    for (int i = 0; i < s.nmp; ++i) {
+      mptype.push_back((int8_t)s.mptype[i]);
+      iclass.push_back((int8_t)s.iclass[i]);
+      unit.push_back(s.unit[i]);
+      exv_rad.push_back(s.exv_rad[i]);
+      exv_eps.push_back((float)s.exv_eps[i]);
+      charge.push_back(s.charge[i]);
+      mass.push_back(s.mass[i]);
+      rcm.push_back(1.0 / s.mass[i]);
+      h.mass.push_back(s.mass[i]);
+      h.fric.push_back(s.fric[i]);
+      // charges of the ele list: |q| >= ZERO_JUDGE (neighbor_list_ele.F90:134,147)
+      if (s.p.ele_form != TIS_ELE_NONE && std::fabs(s.charge[i]) >= 1.0e-6) {
+        qidx.push_back(qoff + nq);
+        qbead.push_back(off + i);
+        ++nq;
+      } else {
+        qidx.push_back(-1);
+      }
+    }
+    d.nq = nq;
+    d.mwords = (nq + 31) / 32;
+    d.moff = moff;
+    moff += (int64_t)nq * d.mwords;
+    d.mioff = s.p.periodic ? mioff : -1;
+    if (s.p.periodic) mioff += (int64_t)nq * nq;
+    qoff += nq;
+    // terms with global bead indices
+    d.boff = nbd; d.nbd = s.nbd;
+    for (int b = 0; b < s.nbd; ++b) {
+      bd_mp.push_back(off + s.bd_mp[2 * b]); bd_mp.push_back(off + s.bd_mp[2 * b + 1]);
+      bd_nat.push_back(s.bd_nat[b]); bd_coef.push_back(s.bd_coef[b]);
+    }
+    d.aoff = nba; d.nba = s.nba;
+    for (int a = 0; a < s.nba; ++a) {
+      for (int k = 0; k < 3; ++k) ba_mp.push_back(off + s.ba_mp[3 * a + k]);
+      ba_nat.push_back(s.ba_nat[a]); ba_coef.push_back(s.ba_coef[a]);
+    }
+    d.soff = nst; d.nst = s.nst;
+    for (int t = 0; t < s.nst; ++t) {
+      for (int k = 0; k < 7; ++k) st_mp.push_back(off + s.st_mp[7 * t + k]);
+      for (int k = 0; k < 3; ++k) st_nat.push_back(s.st_nat[3 * t + k]);
+      for (int k = 0; k < 4; ++k) st_coef.push_back(s.st_coef[4 * t + k]);
+    }
+    d.toff = ntst; d.ntst = s.ntst;
+    for (int t = 0; t < s.ntst; ++t) {
+      for (int k = 0; k < 6; ++k) tst_mp.push_back(off + s.tst_mp[6 * t + k]);
+      for (int k = 0; k < 6; ++k) tst_nat.push_back(s.tst_nat[6 * t + k]);
+      for (int k = 0; k < 7; ++k) tst_coef.push_back(s.tst_coef[7 * t + k]);
+      for (int k = 0; k < 2; ++k) {
+        tst2st.push_back(s.tst2st[2 * t + k] >= 0 ? nst + s.tst2st[2 * t + k] : -1);
+        tst_excl.push_back(s.tst_excl[2 * t + k]);
+      }
+      tst_sys.push_back(is);
+    }
+    d.sitoff = nsite; d.nsite = s.nhbsite;
+    for (int k = 0; k < s.nhbsite; ++k) site_val.push_back(s.hbsite_valence[k]);
+    d.hoff = nhb; d.nhb = s.nhb;
+    std::vector<int> site_use(std::max(s.nhbsite, 1), 0);
+    for (int t = 0; t < s.nhb; ++t) {
+      for (int k = 0; k < 6; ++k) hb_mp.push_back(off + s.hb_mp[6 * t + k]);
+      for (int k = 0; k < 6; ++k) hb_nat.push_back(s.hb_nat[6 * t + k]);
+      for (int k = 0; k < 7; ++k) hb_coef.push_back(s.hb_coef[7 * t + k]);
+      for (int k = 0; k < 6; ++k) {
+        int st = s.hb_site[6 * t + k];
+        hb_site.push_back(st >= 0 ? nsite + st : -1);
+        if (st >= 0) site_use[st] += 1;
+      }
+      hb_tert.push_back(s.hb_tertiary ? s.hb_tertiary[t] : 0);
+      hb_sys.push_back(is);
+    }
+    d.hb_excess_possible = 0;
+    for (int k = 0; k < s.nhbsite; ++k)
+      if (site_use[k] > s.hbsite_valence[k]) d.hb_excess_possible = 1;
+    // incidence lists: bonds, angles, stacks, tertiary stacks, hbonds
+    std::vector<std::vector<uint32_t>> incb(s.nmp);
+    for (int b = 0; b < s.nbd; ++b)
+      for (int k = 0; k < 2; ++k) incb[s.bd_mp[2 * b + k]].push_back(inc_pack(INC_BOND, k, nbd + b));
+    for (int a = 0; a < s.nba; ++a)
+      for (int k = 0; k < 3; ++k) incb[s.ba_mp[3 * a + k]].push_back(inc_pack(INC_ANGLE, k, nba + a));
+    for (int t = 0; t < s.nst; ++t)
+      for (int k = 0; k < 7; ++k) incb[s.st_mp[7 * t + k]].push_back(inc_pack(INC_STACK, k, nst + t));
+    for (int t = 0; t < s.ntst; ++t)
+      for (int k = 0; k < 6; ++k) incb[s.tst_mp[6 * t + k]].push_back(inc_pack(INC_TSTACK, k, ntst + t));
+    for (int t = 0; t < s.nhb; ++t)
+      for (int k = 0; k < 6; ++k) incb[s.hb_mp[6 * t + k]].push_back(inc_pack(INC_HBOND, k, nhb + t));
+    for (int i = 0; i < s.nmp; ++i) {
+      inc_off.push_back((int32_t)inc.size());
+      inc.insert(inc.end(), incb[i].begin(), incb[i].end());
+    }
+    nbd += s.nbd; nba += s.nba; nst += s.nst; ntst += s.ntst; nhb += s.nhb; nsite += s.nhbsite;
+    // tiles
+    h.bt_begin = (int32_t)btiles.size();
+    for (int st0 = 0; st0 < s.nmp; st0 += kTile) btiles.push_back(Tile{is, st0});
+    h.bt_count = (int32_t)btiles.size() - h.bt_begin;
+    h.qt_begin = (int32_t)qtiles.size();
+    for (int st0 = 0; st0 < nq; st0 += kTile) qtiles.push_back(Tile{is, st0});
+    h.qt_count = (int32_t)qtiles.size() - h.qt_begin;
+    h.has_task = (s.nhb > 0 || s.ntst > 0);
+    if (h.has_task) { c->systask_of[is] = (int32_t)systask.size(); systask.push_back(is); }
+    off += s.nmp;
+  }
+  inc_off.push_back((int32_t)inc.size());
+  if (off >= (1 << 27)) return fail(c, TIS_ERR_ARG, "too many beads (%d)", off);
+  c->ntot = off; c->nqtot = qoff; c->nhbtot = nhb; c->ntsttot = ntst; c->nsttot = nst;
+  c->nbtile = (int32_t)btiles.size(); c->nqtile = (int32_t)qtiles.size(); c->nsystask = (int32_t)systask.size();
+
+  // device buffers
+  DevPtrs &P = c->P;
+  P = DevPtrs{};
+  P.nsys = nsys; P.ntot = off; P.nqtot = qoff;
+  int rc = 0;
+  std::vector<DSys> dsys(nsys);
+  for (int is = 0; is < nsys; ++is) dsys[is] = c->hs[is].d;
+#define UP(ptr, vec) do { rc = dupload(c, &ptr, vec); if (rc) return rc; } while (0)
+  UP(c->d_sys, dsys);
+  P.sys = c->d_sys;
+  for (int k = 0; k < 2; ++k) {
+    rc = dalloc(c, &P.x[k], 3 * (size_t)off); if (rc) return rc;
+    rc = dalloc(c, &P.px[k], c->any_periodic ? 3 * (size_t)off : 1); if (rc) return rc;
+  }
+  rc = dalloc(c, &P.v, 3 * (size_t)off); if (rc) return rc;
+  rc = dalloc(c, &P.a, 3 * (size_t)off); if (rc) return rc;
+  rc = dalloc(c, &P.f, 3 * (size_t)off); if (rc) return rc;
+  double *rlan = nullptr;
+  rc = dalloc(c, &rlan, 4 * (size_t)off); if (rc) return rc;
+  P.rlan = rlan;
+  double *d_rcm, *d_mass, *d_rad, *d_charge, *d_rl2;
+  int8_t *d_mpt, *d_icl;
+  int32_t *d_unit, *d_qidx, *d_qbead, *d_incoff;
+  uint32_t *d_inc;
+  float *d_eps;
+  UP(d_rcm, rcm); UP(d_mass, mass); UP(d_rad, exv_rad); UP(d_charge, charge); UP(d_rl2, rl2);
+  UP(d_mpt, mptype); UP(d_icl, iclass); UP(d_unit, unit); UP(d_qidx, qidx); UP(d_qbead, qbead);
+  UP(d_incoff, inc_off); UP(d_inc, inc); UP(d_eps, exv_eps);
+  P.rcm = d_rcm; P.mass = d_mass; P.exv_rad = d_rad; P.charge = d_charge; P.rl2 = d_rl2;
+  P.mptype = d_mpt; P.iclass = d_icl; P.unit = d_unit; P.qidx = d_qidx; P.qbead = d_qbead;
+  P.inc_off = d_incoff; P.inc = d_inc; P.exv_eps = d_eps;
+  int32_t *i32; double *f64;
+  UP(i32, bd_mp); P.bd_mp = i32; UP(f64, bd_nat); P.bd_nat = f64; UP(f64, bd_coef); P.bd_coef = f64; P.nbd = nbd;
+  UP(i32, ba_mp); P.ba_mp = i32; UP(f64, ba_nat); P.ba_nat = f64; UP(f64, ba_coef); P.ba_coef = f64; P.nba = nba;
+  UP(i32, st_mp); P.st_mp = i32; UP(f64, st_nat); P.st_nat = f64; UP(f64, st_coef); P.st_coef = f64; P.nst = nst;
+  UP(i32, tst_mp); P.tst_mp = i32; UP(f64, tst_nat); P.tst_nat = f64; UP(f64, tst_coef); P.tst_coef = f64;
+  UP(i32, tst2st); P.tst2st = i32; UP(i32, tst_excl); P.tst_excl = i32; P.ntst = ntst;
+  UP(i32, hb_mp); P.hb_mp = i32; UP(f64, hb_nat); P.hb_nat = f64; UP(f64, hb_coef); P.hb_coef = f64;
+  UP(i32, hb_site); P.hb_site = i32; UP(i32, hb_tert); P.hb_tert = i32; P.nhb = nhb;
+  UP(i32, hb_sys); P.hb_sys = i32; UP(i32, tst_sys); P.tst_sys = i32;
+  UP(i32, site_val); P.site_val = i32;
+  std::vector<uint8_t> ones(std::max(nst, 1), 1);
+  uint8_t *u8;
+  UP(u8, ones); P.st_status = u8;
+  rc = dalloc(c, &P.for_tst, 18 * (size_t)ntst); if (rc) return rc;
+  rc = dalloc(c, &P.hb_inlist, (size_t)nhb); if (rc) return rc;
+  rc = dalloc(c, &P.hb_status, (size_t)nhb); if (rc) return rc;
+  rc = dalloc(c, &P.hb_ene, (size_t)nhb); if (rc) return rc;
+  rc = dalloc(c, &P.for_hb, 18 * (size_t)nhb); if (rc) return rc;
+  rc = dalloc(c, &P.site_cnt, (size_t)nsite); if (rc) return rc;
+  rc = dalloc(c, &P.nb_cnt, (size_t)off); if (rc) return rc;
+  P.nb_cap = 64;
+  rc = dalloc(c, &P.nb, (size_t)P.nb_cap * off); if (rc) return rc;
+  rc = dalloc(c, &P.fdh, 3 * (size_t)qoff); if (rc) return rc;
+  rc = dalloc(c, &P.qmask, (size_t)moff); if (rc) return rc;
+  rc = dalloc(c, &P.qmirror, (size_t)std::max<int64_t>(mioff, 1)); if (rc) return rc;
+  rc = dalloc(c, &c->d_counters, TIS_CNT_MAX); if (rc) return rc;
+  rc = dalloc(c, &c->d_stepbase, 1); if (rc) return rc;
+  P.counters = c->d_counters;
+  P.stepbase = c->d_stepbase;
+  P.seed = seed;
+  UP(c->d_btiles, btiles);
+  UP(c->d_qtiles, qtiles);
+  UP(c->d_systask, systask);
+  std::vector<int32_t> sysidx(nsys);
+  for (int is = 0; is < nsys; ++is) sysidx[is] = is;
+  UP(c->d_sysidx, sysidx);
+  rc = dalloc(c, &c->d_energy, 25 * (size_t)nsys); if (rc) return rc;
+#undef UP
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->cur = 0;
+  return TIS_OK;
+}
+
+int32_t tis_set_system_T(tis_ctx *c, int32_t isys, const tis_system *s) {
+  if (!c || !s) return TIS_ERR_ARG;
+  int rc = check_sys(c, isys);
+  if (rc) return rc;
+  if (isys < 0) return fail(c, TIS_ERR_ARG, "tis_set_system_T needs one system");
+  HostSys &h = c->hs[isys];
+  if (s->nmp != h.d.nmp || s->nst != h.d.nst) return fail(c, TIS_ERR_ARG, "topology mismatch");
+  HIPCHK(c, hipSetDevice(c->device));
+  fill_derived(h.d, *s);
+  HIPCHK(c, hipMemcpyAsync(c->d_sys + isys, &h.d, sizeof(DSys), hipMemcpyHostToDevice, c->stream));
+  std::vector<double> q(s->charge, s->charge + s->nmp);
+  HIPCHK(c, hipMemcpyAsync((void *)(c->P.charge + h.d.off), q.data(), q.size() * sizeof(double),
+                           hipMemcpyHostToDevice, c->stream));
+  std::vector<double> stc(s->st_coef, s->st_coef + 4 * s->nst);
+  if (!stc.empty())
+    HIPCHK(c, hipMemcpyAsync((void *)(c->P.st_coef + 4 * (size_t)h.d.soff), stc.data(), stc.size() * sizeof(double),
+                             hipMemcpyHostToDevice, c->stream));
+  if (c->lv_init) {
+    const int n = c->ntot;
+    for (int k = 0; k < 4; ++k) {
+      std::vector<double> r(h.d.nmp);
+      for (int i = 0; i < h.d.nmp; ++i) {
+        double rr[4];
+        langevin_consts(c->tstep, s->tempk, h.fric[i], h.mass[i], rr);
+        r[i] = rr[k];
+      }
+      HIPCHK(c, hipMemcpyAsync((void *)(c->P.rlan + (size_t)k * n + h.d.off), r.data(), r.size() * sizeof(double),
+                               hipMemcpyHostToDevice, c->stream));
+    }
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TIS_OK;
+}
+
+int32_t tis_state_set(tis_ctx *c, int32_t isys, int32_t which, const double *buf) {
+  if (!c || !buf) return TIS_ERR_ARG;
+  int rc = check_sys(c, isys);
+  if (rc) return rc;
+  if (isys < 0) return fail(c, TIS_ERR_ARG, "tis_state_set needs one system");
+  HIPCHK(c, hipSetDevice(c->device));
+  const HostSys &h = c->hs[isys];
+  double *base = which == 0 ? Xc(c) : (which == 1 ? c->P.v : c->P.a);
+  if (which < 0 || which > 2) return fail(c, TIS_ERR_ARG, "bad state index %d", which);
+  std::vector<double> comp(h.d.nmp);
+  for (int d = 0; d < 3; ++d) {
+    for (int i = 0; i < h.d.nmp; ++i) comp[i] = buf[3 * i + d];
+    HIPCHK(c, hipMemcpyAsync(base + (size_t)d * c->ntot + h.d.off, comp.data(), comp.size() * sizeof(double),
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  if (which == 0) { c->lists_valid = false; c->lv_init = false; }
+  return TIS_OK;
+}
+
+int32_t tis_state_get(tis_ctx *c, int32_t isys, int32_t which, double *buf) {
+  if (!c || !buf) return TIS_ERR_ARG;
+  int rc = check_sys(c, isys);
+  if (rc) return rc;
+  if (isys < 0) return fail(c, TIS_ERR_ARG, "tis_state_get needs one system");
+  if (which < 0 || which > 3) return fail(c, TIS_ERR_ARG, "bad state index %d", which);
+  HIPCHK(c, hipSetDevice(c->device));
+  const HostSys &h = c->hs[isys];
+  const double *base = which == 0 ? Xc(c) : (which == 1 ? c->P.v : (which == 2 ? c->P.a : c->P.f));
+  std::vector<double> comp(h.d.nmp);
+  for (int d = 0; d < 3; ++d) {
+    HIPCHK(c, hipMemcpyAsync(comp.data(), base + (size_t)d * c->ntot + h.d.off, comp.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < h.d.nmp; ++i) buf[3 * i + d] = comp[i];
+  }
+  return TIS_OK;
+}
+
+static int rebuild_lists(tis_ctx *c, int32_t isys) {
+  Range r = range_of(c, isys);
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    HIPCHK(c, hipMemsetAsync(c->d_counters + TIS_CNT_NB_OVERFLOW, 0, sizeof(unsigned long long), c->stream));
+    launch_lists(c, r, Xc(c), PXc(c), nullptr, 0);
+    if (c->nhbtot > 0) {
+      int h0 = isys < 0 ? 0 : c->hs[isys].d.hoff, nh = isys < 0 ? c->nhbtot : c->hs[isys].d.nhb;
+      if (nh > 0)
+        hipLaunchKernelGGL(k_list_hb, dim3((nh + 255) / 256), dim3(256), 0, c->stream, c->P, Xc(c), PXc(c), h0, nh);
+    }
+    hipLaunchKernelGGL(k_count_rebuild, dim3(1), dim3(1), 0, c->stream, c->d_counters);
+    HIPCHK(c, hipGetLastError());
+    unsigned long long ovf = 0;
+    HIPCHK(c, hipMemcpyAsync(&ovf, c->d_counters + TIS_CNT_NB_OVERFLOW, sizeof(ovf), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (ovf == 0) { c->lists_valid = true; return 0; }
+    // grow the per-bead capacity and rebuild
+    int newcap = (int)(ovf + ovf / 2 + 16);
+    uint32_t *nb = nullptr;
+    int rc = dalloc(c, &nb, (size_t)newcap * c->ntot);
+    if (rc) return rc;
+    c->P.nb = nb;  // the old buffer stays in allocs until the batch is freed
+    c->P.nb_cap = newcap;
+    for (int k = 0; k < 2; ++k)
+      if (c->graph[k]) { (void)hipGraphExecDestroy(c->graph[k]); c->graph[k] = nullptr; }
+  }
+  return fail(c, TIS_ERR_OVERFLOW, "neighbour list capacity could not be grown");
+}
+
+int32_t tis_neighbor(tis_ctx *c, int32_t isys) {
+  if (!c) return TIS_ERR_ARG;
+  int rc = check_sys(c, isys);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  return rebuild_lists(c, isys);
+}
+
+int32_t tis_pairs_get(tis_ctx *c, int32_t isys, int32_t which, int32_t *pairs, double *coef, int64_t cap,
+                      int64_t *nout) {
+  if (!c || !nout) return TIS_ERR_ARG;
+  int rc = check_sys(c, isys);
+  if (rc) return rc;
+  if (isys < 0) return fail(c, TIS_ERR_ARG, "tis_pairs_get needs one system");
+  if (!c->lists_valid) return fail(c, TIS_ERR_STATE, "lists not built");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const DSys &d = c->hs[isys].d;
+  int64_t n = 0;
+  if (which == TIS_LIST_EXV) {
+    std::vector<int32_t> cnt(d.nmp);
+    std::vector<uint32_t> nb((size_t)c->P.nb_cap * d.nmp);
+    HIPCHK(c, hipMemcpy(cnt.data(), c->P.nb_cnt + d.off, d.nmp * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (int m = 0; m < c->P.nb_cap; ++m)
+      HIPCHK(c, hipMemcpy(nb.data() + (size_t)m * d.nmp, c->P.nb + (size_t)m * c->ntot + d.off,
+                          d.nmp * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (int i = 0; i < d.nmp; ++i)
+      for (int m = 0; m < std::min(cnt[i], c->P.nb_cap); ++m) {
+        uint32_t e = nb[(size_t)m * d.nmp + i];
+        int j = (int)(e & kIdxMask) - d.off;
+        if (j <= i) continue;
+        if (n < cap && pairs) { pairs[3 * n] = i; pairs[3 * n + 1] = j; pairs[3 * n + 2] = (int32_t)(e >> kMirrorShift); }
+        ++n;
+      }
+  } else if (which == TIS_LIST_ELE) {
+    std::vector<uint32_t> mask((size_t)d.nq * d.mwords);
+    std::vector<int32_t> qb(d.nq);
+    std::vector<double> q(d.nmp);
+    std::vector<uint8_t> mir(d.periodic ? (size_t)d.nq * d.nq : 0);
+    if (!mask.empty()) HIPCHK(c, hipMemcpy(mask.data(), c->P.qmask + d.moff, mask.size() * 4, hipMemcpyDeviceToHost));
+    if (d.nq) HIPCHK(c, hipMemcpy(qb.data(), c->P.qbead + d.qoff, d.nq * 4, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(q.data(), c->P.charge + d.off, d.nmp * 8, hipMemcpyDeviceToHost));
+    if (!mir.empty()) HIPCHK(c, hipMemcpy(mir.data(), c->P.qmirror + d.mioff, mir.size(), hipMemcpyDeviceToHost));
+    for (int qi = 0; qi < d.nq; ++qi)
+      for (int qj = qi + 1; qj < d.nq; ++qj) {
+        if (!((mask[(size_t)qi * d.mwords + (qj >> 5)] >> (qj & 31)) & 1u)) continue;
+        if (n < cap && pairs) {
+          int bi = qb[qi] - d.off, bj = qb[qj] - d.off;
+          pairs[3 * n] = bi; pairs[3 * n + 1] = bj;
+          pairs[3 * n + 2] = d.periodic ? mir[(size_t)qi * d.nq + qj] : 0;
+          if (coef) coef[n] = (q[bi] * q[bj]) * d.ele_coef;
+        }
+        ++n;
+      }
+  } else if (which == TIS_LIST_HB) {
+    std::vector<uint8_t> in(std::max(d.nhb, 1));
+    if (d.nhb) HIPCHK(c, hipMemcpy(in.data(), c->P.hb_inlist + d.hoff, d.nhb, hipMemcpyDeviceToHost));
+    for (int k = 0; k < d.nhb; ++k)
+      if (in[k]) { if (n < cap && pairs) pairs[n] = k; ++n; }
+  } else {
+    return fail(c, TIS_ERR_ARG, "unknown list %d", which);
+  }
+  *nout = n;
+  return n > cap ? fail(c, TIS_ERR_OVERFLOW, "list of %lld entries exceeds cap %lld", (long long)n, (long long)cap)
+                 : TIS_OK;
+}
+
+// k_pre then k_main.  Steps draw noise/valence numbers for step stepbase+t+1;
+// a stand-alone force evaluation or the initial acceleration uses stepbase.
+static void launch_force_pair(tis_ctx *c, const Range &r, const double *X, const double *PX, double *XN,
+                              double *PXN, int t_local, int mode) {
+  const int npre = r.taskn + r.qtn;
+  const int32_t pre_t = mode == 1 ? (int32_t)t_local : -1;
+  hipEvent_t e3[3] = {nullptr, nullptr, nullptr};
+  if (c->timing) {
+    while (c->tev.size() < c->tev_used + 3) {
+      hipEvent_t e;
+      (void)hipEventCreate(&e);
+      c->tev.push_back(e);
+    }
+    for (int k = 0; k < 3; ++k) e3[k] = c->tev[c->tev_used + k];
+    c->tev_used += 3;
+    (void)hipEventRecord(e3[0], c->stream);
+  }
+  if (npre > 0)
+    hipLaunchKernelGGL(k_pre, dim3(npre), dim3(kTile), 0, c->stream, c->P, X, PX, c->d_systask + r.task0,
+                       r.taskn, c->d_qtiles + r.qt0, r.qtn, pre_t, c->any_dh ? 1 : 0);
+  if (c->timing) (void)hipEventRecord(e3[1], c->stream);
+  if (r.btn > 0)
+    hipLaunchKernelGGL(k_main, dim3(r.btn), dim3(kTile), 0, c->stream, c->P, X, PX, XN, PXN,
+                       c->d_btiles + r.bt0, r.btn, (int32_t)t_local, mode, c->any_dh ? 1 : 0);
+  if (c->timing) (void)hipEventRecord(e3[2], c->stream);
+}
+
+int32_t tis_force_sumup(tis_ctx *c, int32_t isys, double *force_out) {
+  if (!c) return TIS_ERR_ARG;
+  int rc = check_sys(c, isys);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (!c->lists_valid) { rc = rebuild_lists(c, -1); if (rc) return rc; }
+  Range r = range_of(c, isys);
+  // force evaluation at the current coordinates; valence draws use step = stepbase
+  launch_force_pair(c, r, Xc(c), PXc(c), nullptr, nullptr, 0, 0);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (isys >= 0 && force_out) return tis_state_get(c, isys, 3, force_out);
+  return TIS_OK;
+}
+
+int32_t tis_force_get(tis_ctx *c, int32_t isys, double *force_out) {
+  if (!c || isys < 0 || !force_out) return TIS_ERR_ARG;
+  return tis_state_get(c, isys, 3, force_out);
+}
+
+int32_t tis_energy_sumup(tis_ctx *c, int32_t isys, double *energy_out) {
+  if (!c || !energy_out) return TIS_ERR_ARG;
+  int rc = check_sys(c, isys);
+  if (rc) return rc;
+  if (isys < 0) return fail(c, TIS_ERR_ARG, "tis_energy_sumup needs one system");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (!c->lists_valid) { rc = rebuild_lists(c, -1); if (rc) return rc; }
+  hipLaunchKernelGGL(k_energy, dim3(1), dim3(256), 0, c->stream, c->P, Xc(c), PXc(c), c->d_sysidx + isys,
+                     c->d_energy);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(energy_out, c->d_energy, 25 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TIS_OK;
+}
+
+int32_t tis_langevin_init(tis_ctx *c, double tstep, int32_t rebuild_every, uint64_t step0, int32_t init_accel) {
+  if (!c || tstep <= 0.0 || rebuild_every <= 0) return fail(c, TIS_ERR_ARG, "bad Langevin parameters");
+  int rc = check_sys(c, -1);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  c->tstep = tstep;
+  c->rebuild_every = rebuild_every;
+  c->P.tstep2 = 0.5 * tstep * tstep;
+  for (int k = 0; k < 2; ++k)
+    if (c->graph[k]) { (void)hipGraphExecDestroy(c->graph[k]); c->graph[k] = nullptr; }
+  // per-bead constants
+  const int n = c->ntot;
+  std::vector<double> rl(4 * (size_t)n);
+  for (int is = 0; is < c->nsys; ++is) {
+    const HostSys &h = c->hs[is];
+    for (int i = 0; i < h.d.nmp; ++i) {
+      double r[4];
+      langevin_consts(tstep, h.d.tempk, h.fric[i], h.mass[i], r);
+      for (int k = 0; k < 4; ++k) rl[(size_t)k * n + h.d.off + i] = r[k];
+    }
+  }
+  HIPCHK(c, hipMemcpyAsync((void *)c->P.rlan, rl.data(), rl.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_stepbase, &step0, sizeof(step0), hipMemcpyHostToDevice, c->stream));
+  rc = rebuild_lists(c, -1);
+  if (rc) return rc;
+  Range r = range_of(c, -1);
+  const int cu = c->cur;
+  if (init_accel) {
+    // a = F(x)/m + c1 xi(step0), then x_pred (mode 2)
+    launch_force_pair(c, r, c->P.x[cu], c->P.px[cu], c->P.x[cu ^ 1], c->P.px[cu ^ 1], 0, 2);
+  } else {
+    // keep the stored a; forces at x give the hbond state, then predict
+    launch_force_pair(c, r, c->P.x[cu], c->P.px[cu], c->P.x[cu ^ 1], c->P.px[cu ^ 1], 0, 3);
+  }
+  HIPCHK(c, hipGetLastError());
+  if (c->any_periodic)
+    hipLaunchKernelGGL(k_wrap, dim3(c->nbtile), dim3(kTile), 0, c->stream, c->P, c->P.x[cu], c->P.px[cu],
+                       c->P.px[cu ^ 1], c->d_btiles, 1);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->lv_init = true;
+  c->t_since_init = 0;
+  return TIS_OK;
+}
+
+// one Langevin step of the batch (or of system isys) in eager launches
+static int eager_step(tis_ctx *c, int32_t isys) {
+  const int cu = c->cur;
+  Range r = range_of(c, isys);
+  if (c->t_since_init % c->rebuild_every == 0) {
+    launch_lists(c, r, c->P.x[cu], c->P.px[cu], c->P.px[cu ^ 1], 1);
+    if (c->nhbtot > 0) {
+      int h0 = isys < 0 ? 0 : c->hs[isys].d.hoff, nh = isys < 0 ? c->nhbtot : c->hs[isys].d.nhb;
+      if (nh > 0)
+        hipLaunchKernelGGL(k_list_hb, dim3((nh + 255) / 256), dim3(256), 0, c->stream, c->P, c->P.x[cu],
+                           c->P.px[cu], h0, nh);
+    }
+    if (isys <= 0) hipLaunchKernelGGL(k_count_rebuild, dim3(1), dim3(1), 0, c->stream, c->d_counters);
+  }
+  launch_force_pair(c, r, c->P.x[cu ^ 1], c->P.px[cu ^ 1], c->P.x[cu], c->P.px[cu], 0, 1);
+  return 0;
+}
+
+static int capture_graph(tis_ctx *c, int parity) {
+  const int L = (c->rebuild_every % 2 == 0) ? c->rebuild_every : 2 * c->rebuild_every;
+  hipGraph_t g = nullptr;
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int cu = parity;
+  Range r = range_of(c, -1);
+  for (int t = 0; t < L; ++t) {
+    if (t % c->rebuild_every == 0) {
+      launch_lists(c, r, c->P.x[cu], c->P.px[cu], c->P.px[cu ^ 1], 1);
+      if (c->nhbtot > 0)
+        hipLaunchKernelGGL(k_list_hb, dim3((c->nhbtot + 255) / 256), dim3(256), 0, c->stream, c->P, c->P.x[cu],
+                           c->P.px[cu], 0, c->nhbtot);
+      hipLaunchKernelGGL(k_count_rebuild, dim3(1), dim3(1), 0, c->stream, c->d_counters);
+    }
+    launch_force_pair(c, r, c->P.x[cu ^ 1], c->P.px[cu ^ 1], c->P.x[cu], c->P.px[cu], t, 1);
+    cu ^= 1;
+  }
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, c->stream, c->d_stepbase, (uint64_t)L, c->d_counters);
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (e != hipSuccess) return fail(c, TIS_ERR_HIP, "stream capture failed: %s", hipGetErrorString(e));
+  e = hipGraphInstantiate(&c->graph[parity], g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return fail(c, TIS_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+  c->graph_len = L;
+  return 0;
+}
+
+int32_t tis_langevin_steps(tis_ctx *c, int64_t nsteps, int32_t mode) {
+  if (!c || nsteps < 0) return TIS_ERR_ARG;
+  if (!c->lv_init) return fail(c, TIS_ERR_STATE, "tis_langevin_init not called");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemsetAsync(c->d_counters + TIS_CNT_NB_OVERFLOW, 0, sizeof(unsigned long long), c->stream));
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  int64_t done = 0, nlaunch = 0;
+  const int L = (c->rebuild_every % 2 == 0) ? c->rebuild_every : 2 * c->rebuild_every;
+  if (mode == TIS_MODE_GRAPH) {
+    while (done < nsteps) {
+      if (c->t_since_init % L == 0 && nsteps - done >= L) {
+        if (!c->graph[c->cur]) { int rc = capture_graph(c, c->cur); if (rc) return rc; }
+        HIPCHK(c, hipGraphLaunch(c->graph[c->cur], c->stream));
+        done += L; c->t_since_init += L;  // L even: parity unchanged
+        nlaunch += 1;
+      } else {
+        eager_step(c, -1);
+        hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, c->stream, c->d_stepbase, (uint64_t)1, c->d_counters);
+        c->cur ^= 1; ++done; ++c->t_since_init; nlaunch += 2;
+      }
+    }
+  } else if (mode == TIS_MODE_EAGER) {
+    c->timing = true;
+    c->tev_used = 0;
+    for (; done < nsteps; ++done) {
+      eager_step(c, -1);
+      hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, c->stream, c->d_stepbase, (uint64_t)1, c->d_counters);
+      c->cur ^= 1; ++c->t_since_init; nlaunch += 2;
+    }
+  } else if (mode == TIS_MODE_PLAIN) {
+    for (; done < nsteps; ++done) {
+      for (int is = 0; is < c->nsys; ++is) eager_step(c, is);
+      hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, c->stream, c->d_stepbase, (uint64_t)1, c->d_counters);
+      c->cur ^= 1; ++c->t_since_init; nlaunch += 2 * c->nsys;
+    }
+  } else {
+    return fail(c, TIS_ERR_ARG, "unknown mode %d", mode);
+  }
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  HIPCHK(c, hipEventSynchronize(c->ev1));
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  c->last_total_ms = ms;
+  c->last_nlaunch = nlaunch;
+  c->last_main_ms = c->last_pre_ms = 0.0;
+  if (c->timing) {
+    double sp = 0.0, sm = 0.0;
+    size_t nstep = c->tev_used / 3;
+    for (size_t k = 0; k < nstep; ++k) {
+      float a = 0.0f, b = 0.0f;
+      (void)hipEventElapsedTime(&a, c->tev[3 * k], c->tev[3 * k + 1]);
+      (void)hipEventElapsedTime(&b, c->tev[3 * k + 1], c->tev[3 * k + 2]);
+      sp += a; sm += b;
+    }
+    if (nstep) { c->last_pre_ms = sp / nstep; c->last_main_ms = sm / nstep; }
+    c->timing = false;
+    c->tev_used = 0;
+  }
+  unsigned long long ovf = 0;
+  HIPCHK(c, hipMemcpy(&ovf, c->d_counters + TIS_CNT_NB_OVERFLOW, sizeof(ovf), hipMemcpyDeviceToHost));
+  if (ovf) return fail(c, TIS_ERR_OVERFLOW, "neighbour list overflow during the run (%llu > %d)", ovf, c->P.nb_cap);
+  return TIS_OK;
+}
+
+int32_t tis_status_counters(tis_ctx *c, int64_t *counters) {
+  if (!c || !counters) return TIS_ERR_ARG;
+  if (c->nsys == 0) { for (int k = 0; k < TIS_CNT_MAX; ++k) counters[k] = 0; return TIS_OK; }
+  HIPCHK(c, hipSetDevice(c->device));
+  unsigned long long h[TIS_CNT_MAX];
+  HIPCHK(c, hipMemcpyAsync(h, c->d_counters, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < TIS_CNT_MAX; ++k) counters[k] = (int64_t)h[k];
+  return TIS_OK;
+}
+
+int32_t tis_synchronize(tis_ctx *c) {
+  if (!c) return TIS_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TIS_OK;
+}
+
+int32_t tis_last_timing(tis_ctx *c, double *total_ms, double *step_kernel_ms, double *pre_kernel_ms,
+                        int64_t *nlaunch) {
+  if (!c) return TIS_ERR_ARG;
+  if (total_ms) *total_ms = c->last_total_ms;
+  if (step_kernel_ms) *step_kernel_ms = c->last_main_ms;
+  if (pre_kernel_ms) *pre_kernel_ms = c->last_pre_ms;
+  if (nlaunch) *nlaunch = c->last_nlaunch;
+  return TIS_OK;
+}
+
+}  // extern "C"

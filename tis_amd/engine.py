@@ -1,0 +1,188 @@
+"""Host-side mirror of the reference's per-replica call sites over the HIP
+C-ABI (include/tis_hip.h, libtis_hip.so).
+
+``Engine`` holds one context with a batch of systems resident in HBM and
+exposes the reference's entry points by name:
+
+  neighbor(isys)            neighbor(irep)                  src/neighbor.F90:13
+  force_sumup(isys)         force_sumup(force_mp, irep)     src/force_sumup.F90:15
+  energy_sumup(isys)        energy_sumup(irep, velo, E, ..) src/energy_sumup.F90:17
+  langevin_init / _steps    time_integral (Langevin branch) src/time_integral.F90:149-189
+
+The native library must be present: there is no CPU fallback on this path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .sysdesc import CSystem, System, E_MAX
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libtis_hip.so")
+
+MODE_GRAPH, MODE_EAGER, MODE_PLAIN = 0, 1, 2
+LIST_EXV, LIST_ELE, LIST_HB = 0, 1, 2
+CNT_NAMES = ("clamp_exv", "clamp_ele", "hb_deleted", "rebuilds", "steps", "nb_overflow")
+
+_f64p = C.POINTER(C.c_double)
+_i32p = C.POINTER(C.c_int32)
+_LIB = None
+
+
+class TisError(RuntimeError):
+    pass
+
+
+def load_library():
+    """Load libtis_hip.so; raise if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIBPATH):
+        raise TisError("libtis_hip.so not built: run __graft_entry__.build() (no CPU fallback)")
+    L = C.CDLL(LIBPATH)
+    vp = C.c_void_p
+    L.tis_ctx_create.argtypes = [C.c_int32, C.POINTER(vp)]
+    L.tis_ctx_destroy.argtypes = [vp]
+    L.tis_last_error.restype = C.c_char_p
+    L.tis_last_error.argtypes = [vp]
+    L.tis_system_upload.argtypes = [vp, C.c_int32, C.POINTER(CSystem), C.POINTER(C.c_uint32), C.c_uint64]
+    L.tis_set_system_T.argtypes = [vp, C.c_int32, C.POINTER(CSystem)]
+    L.tis_state_set.argtypes = [vp, C.c_int32, C.c_int32, _f64p]
+    L.tis_state_get.argtypes = [vp, C.c_int32, C.c_int32, _f64p]
+    L.tis_neighbor.argtypes = [vp, C.c_int32]
+    L.tis_pairs_get.argtypes = [vp, C.c_int32, C.c_int32, _i32p, _f64p, C.c_int64, C.POINTER(C.c_int64)]
+    L.tis_force_sumup.argtypes = [vp, C.c_int32, _f64p]
+    L.tis_force_get.argtypes = [vp, C.c_int32, _f64p]
+    L.tis_energy_sumup.argtypes = [vp, C.c_int32, _f64p]
+    L.tis_langevin_init.argtypes = [vp, C.c_double, C.c_int32, C.c_uint64, C.c_int32]
+    L.tis_langevin_steps.argtypes = [vp, C.c_int64, C.c_int32]
+    L.tis_status_counters.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.tis_synchronize.argtypes = [vp]
+    L.tis_last_timing.argtypes = [vp, _f64p, _f64p, _f64p, C.POINTER(C.c_int64)]
+    for fn in ("tis_ctx_create", "tis_ctx_destroy", "tis_system_upload", "tis_set_system_T", "tis_state_set",
+               "tis_state_get", "tis_neighbor", "tis_pairs_get", "tis_force_sumup", "tis_force_get",
+               "tis_energy_sumup", "tis_langevin_init", "tis_langevin_steps", "tis_status_counters",
+               "tis_synchronize", "tis_last_timing"):
+        getattr(L, fn).restype = C.c_int32
+    _LIB = L
+    return L
+
+
+def _fp(a):
+    return a.ctypes.data_as(_f64p)
+
+
+class Engine:
+    """A batch of DT15 systems on one GPU."""
+
+    def __init__(self, systems: list[System], keys=None, seed: int = 1, device: int = 0):
+        self.L = load_library()
+        self.ctx = C.c_void_p()
+        self._chk(self.L.tis_ctx_create(device, C.byref(self.ctx)), "tis_ctx_create")
+        self.systems = list(systems)
+        self._cs = (CSystem * len(self.systems))(*[s.cstruct() for s in self.systems])
+        k = None
+        if keys is not None:
+            self._keys = (C.c_uint32 * len(keys))(*[int(v) for v in keys])
+            k = self._keys
+        self._chk(self.L.tis_system_upload(self.ctx, len(self.systems), self._cs, k, seed), "upload")
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = self.L.tis_last_error(self.ctx).decode() if self.ctx else ""
+            raise TisError(f"{what} failed ({rc}): {msg}")
+
+    def close(self):
+        if self.ctx:
+            self.L.tis_ctx_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- state
+    def set_state(self, isys, which, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        if a.shape != (self.systems[isys].nmp, 3):
+            raise ValueError("state must be (N,3)")
+        self._chk(self.L.tis_state_set(self.ctx, isys, which, _fp(a)), "tis_state_set")
+
+    def get_state(self, isys, which):
+        a = np.zeros((self.systems[isys].nmp, 3))
+        self._chk(self.L.tis_state_get(self.ctx, isys, which, _fp(a)), "tis_state_get")
+        return a
+
+    def set_coords(self, isys, xyz):
+        self.set_state(isys, 0, xyz)
+
+    def coords(self, isys):
+        return self.get_state(isys, 0)
+
+    def set_temperature_system(self, isys, system: System):
+        cs = system.cstruct()
+        self._chk(self.L.tis_set_system_T(self.ctx, isys, C.byref(cs)), "tis_set_system_T")
+        self.systems[isys] = system
+
+    # ---- reference entry points
+    def neighbor(self, isys=-1):
+        self._chk(self.L.tis_neighbor(self.ctx, isys), "tis_neighbor")
+
+    def pairs(self, isys, which):
+        n = C.c_int64()
+        self.L.tis_pairs_get(self.ctx, isys, which, None, None, 0, C.byref(n))
+        cnt = n.value
+        if which == LIST_HB:
+            buf = np.zeros(max(cnt, 1), np.int32)
+            self._chk(self.L.tis_pairs_get(self.ctx, isys, which, buf.ctypes.data_as(_i32p), None, cnt,
+                                           C.byref(n)), "tis_pairs_get")
+            return buf[:cnt]
+        buf = np.zeros(3 * max(cnt, 1), np.int32)
+        coef = np.zeros(max(cnt, 1))
+        self._chk(self.L.tis_pairs_get(self.ctx, isys, which, buf.ctypes.data_as(_i32p), _fp(coef), cnt,
+                                       C.byref(n)), "tis_pairs_get")
+        pairs = buf[:3 * cnt].reshape(cnt, 3)
+        return (pairs, coef[:cnt]) if which == LIST_ELE else pairs
+
+    def force_sumup(self, isys):
+        f = np.zeros((self.systems[isys].nmp, 3))
+        self._chk(self.L.tis_force_sumup(self.ctx, isys, _fp(f)), "tis_force_sumup")
+        return f
+
+    def force_all(self):
+        self._chk(self.L.tis_force_sumup(self.ctx, -1, None), "tis_force_sumup")
+        return [self.get_state(i, 3) for i in range(len(self.systems))]
+
+    def energy_sumup(self, isys):
+        e = np.zeros(E_MAX)
+        self._chk(self.L.tis_energy_sumup(self.ctx, isys, _fp(e)), "tis_energy_sumup")
+        return e
+
+    # ---- Langevin
+    def langevin_init(self, tstep=0.05, rebuild_every=10, step0=0, init_accel=True):
+        self._chk(self.L.tis_langevin_init(self.ctx, tstep, rebuild_every, step0, 1 if init_accel else 0),
+                  "tis_langevin_init")
+
+    def langevin_steps(self, nsteps, mode=MODE_GRAPH):
+        self._chk(self.L.tis_langevin_steps(self.ctx, nsteps, mode), "tis_langevin_steps")
+
+    def synchronize(self):
+        self._chk(self.L.tis_synchronize(self.ctx), "tis_synchronize")
+
+    def counters(self):
+        a = (C.c_int64 * 8)()
+        self._chk(self.L.tis_status_counters(self.ctx, a), "tis_status_counters")
+        return dict(zip(CNT_NAMES, list(a)[:len(CNT_NAMES)]))
+
+    def last_timing(self):
+        t, m, p = C.c_double(), C.c_double(), C.c_double()
+        n = C.c_int64()
+        self._chk(self.L.tis_last_timing(self.ctx, C.byref(t), C.byref(m), C.byref(p), C.byref(n)),
+                  "tis_last_timing")
+        return {"total_ms": t.value, "main_ms": m.value, "pre_ms": p.value, "nlaunch": n.value}
