@@ -940,9 +940,10 @@ int64_t oracle_ws_lists(oracle_ws *w, int32_t which, const int32_t **pairs, cons
   *pairs = w->hb; if (coef) *coef = NULL; return w->nhbl;
 }
 
-/* force_sumup.F90:66-289 for a DT15 system.  Positions used by the pair terms
- * are xyz shifted by the wrap offsets of the last rebuild: the reference
- * advances xyz_mp_rep and pxyz_mp_rep by the same dxyz each step. */
+/* force_sumup.F90:66-289 for a DT15 system.  Non-periodic pair terms read
+ * xyz itself (pxyz_mp_rep equals xyz_mp_rep there); periodic ones read the
+ * wrapped copy, which the Langevin loop advances by the same dxyz as xyz
+ * (time_integral.F90:156-160). */
 int32_t oracle_force_sumup(oracle_ws *w, const double *xyz, double *f,
                            uint64_t seed, uint32_t sys, uint64_t step) {
   const tis_system *s = w->s;
@@ -955,9 +956,10 @@ int32_t oracle_force_sumup(oracle_ws *w, const double *xyz, double *f,
   oracle_hbond_eval(s, xyz, w->pxyz, w->hb, w->nhbl, w->for_hb, w->ene_hb, w->hb_status);
   oracle_hbond_valence(s, w->hb, w->nhbl, w->ene_hb, w->hb_status, seed, sys, step);
   oracle_hbond_scatter(s, w->hb, w->nhbl, w->for_hb, w->hb_status, f);
-  nclamp += oracle_force_exv(s, w->pxyz, w->exv, w->nexv, f);
+  const double *px = s->p.periodic ? w->pxyz : xyz;
+  nclamp += oracle_force_exv(s, px, w->exv, w->nexv, f);
   if (s->p.ele_form == TIS_ELE_DH)
-    nclamp += oracle_force_dh(s, w->pxyz, w->ele, w->ele_coef, w->nele, f);
+    nclamp += oracle_force_dh(s, px, w->ele, w->ele_coef, w->nele, f);
   return nclamp;
 }
 
@@ -979,9 +981,10 @@ void oracle_energy_sumup(oracle_ws *w, const double *xyz, const double *velo, do
   }
   energy[TIS_E_HBOND] = ehb;
   energy[TIS_E_THBOND] = ethb;
-  energy[TIS_E_EXV] = oracle_energy_exv(s, w->pxyz, w->exv, w->nexv);
+  const double *px = s->p.periodic ? w->pxyz : xyz;
+  energy[TIS_E_EXV] = oracle_energy_exv(s, px, w->exv, w->nexv);
   if (s->p.ele_form == TIS_ELE_DH)
-    energy[TIS_E_ELE] = oracle_energy_dh(s, w->pxyz, w->ele, w->ele_coef, w->nele);
+    energy[TIS_E_ELE] = oracle_energy_dh(s, px, w->ele, w->ele_coef, w->nele);
   double tot = 0.0;
   for (int k = 0; k < TIS_E_MAX; ++k)
     if (k != TIS_E_TOTAL && k != TIS_E_VELO) tot += energy[k];

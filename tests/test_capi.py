@@ -1,0 +1,46 @@
+"""CPU checks of the C-ABI boundary: the library loads, exports every entry
+point that include/tis_hip.h declares, and rejects bad arguments without a
+device call."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from tis_amd import engine
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    txt = open(os.path.join(ROOT, "include", "tis_hip.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(tis_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_declares_the_boundary():
+    names = declared_functions()
+    for must in ("tis_ctx_create", "tis_system_upload", "tis_neighbor", "tis_force_sumup",
+                 "tis_energy_sumup", "tis_langevin_init", "tis_langevin_steps", "tis_pairs_get"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = engine.load_library()
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_null_arguments_are_rejected():
+    lib = engine.load_library()
+    assert lib.tis_ctx_destroy(None) < 0
+    assert lib.tis_neighbor(None, 0) < 0
+    assert lib.tis_force_sumup(None, 0, None) < 0
+    assert lib.tis_langevin_steps(None, 1, 0) < 0
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(engine, "LIBPATH", str(tmp_path / "absent.so"))
+    monkeypatch.setattr(engine, "_LIB", None)
+    with pytest.raises(engine.TisError):
+        engine.load_library()
