@@ -17,11 +17,14 @@
 #include "tis_dev.h"
 
 namespace tis {
-__global__ void k_pre(DevPtrs, const double *, const double *, const int32_t *, int, const Tile *, int, int32_t, int);
+__global__ void k_terms(DevPtrs, const double *, const double *, const int32_t *, int32_t);
+__global__ void k_dh(DevPtrs, const double *, const double *, const Tile *);
+__global__ void k_bonded(DevPtrs, const double *, int, int, int, int, int, int);
+__global__ void k_bbox(DevPtrs, const double *, const double *, const Tile *, int, int, const Tile *, int);
 __global__ void k_main(DevPtrs, const double *, const double *, double *, double *, const Tile *, int, int32_t, int, int);
 __global__ void k_wrap(DevPtrs, const double *, double *, double *, const Tile *, int);
-__global__ void k_list_exv(DevPtrs, const double *, const double *, const Tile *);
-__global__ void k_list_ele(DevPtrs, const double *, const double *, const Tile *);
+__global__ void k_list_exv(DevPtrs, const double *, const double *, const Tile *, int);
+__global__ void k_list_ele(DevPtrs, const double *, const double *, const Tile *, int);
 __global__ void k_list_hb(DevPtrs, const double *, const double *, int, int);
 __global__ void k_advance(uint64_t *, uint64_t, unsigned long long *);
 __global__ void k_count_rebuild(unsigned long long *);
@@ -84,7 +87,7 @@ struct tis_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_total_ms = 0.0, last_main_ms = 0.0, last_pre_ms = 0.0;
   int64_t last_nlaunch = 0;
-  // per-kernel events of eager runs: (before k_pre, between, after k_main)
+  // per-kernel events of eager runs: (before k_terms/k_dh, between, after k_main)
   bool timing = false;
   std::vector<hipEvent_t> tev;
   size_t tev_used = 0;
@@ -184,19 +187,21 @@ double *Xc(tis_ctx *c) { return c->P.x[c->cur]; }
 double *PXc(tis_ctx *c) { return c->P.px[c->cur]; }
 
 // --- launch helpers on a system subset: isys = -1 all, else one system
-struct Range { int bt0, btn, qt0, qtn, task0, taskn; const int32_t *sysidx; int nsel; };
+struct Range { int bt0, btn, qt0, qtn, task0, taskn; const int32_t *sysidx; int nsel; int b0, nb, a0, na, s0, ns; };
 
 Range range_of(tis_ctx *c, int32_t isys) {
   Range r{};
   if (isys < 0) {
     r.bt0 = 0; r.btn = c->nbtile; r.qt0 = 0; r.qtn = c->nqtile; r.task0 = 0; r.taskn = c->nsystask;
     r.sysidx = c->d_sysidx; r.nsel = c->nsys;
+    r.b0 = 0; r.nb = c->P.nbd; r.a0 = 0; r.na = c->P.nba; r.s0 = 0; r.ns = c->P.nst;
   } else {
     const HostSys &h = c->hs[isys];
     r.bt0 = h.bt_begin; r.btn = h.bt_count; r.qt0 = h.qt_begin; r.qtn = h.qt_count;
     r.task0 = c->systask_of[isys] < 0 ? 0 : c->systask_of[isys];
     r.taskn = c->systask_of[isys] < 0 ? 0 : 1;
     r.sysidx = c->d_sysidx + isys; r.nsel = 1;
+    r.b0 = h.d.boff; r.nb = h.d.nbd; r.a0 = h.d.aoff; r.na = h.d.nba; r.s0 = h.d.soff; r.ns = h.d.nst;
   }
   return r;
 }
@@ -205,10 +210,13 @@ int launch_lists(tis_ctx *c, const Range &r, const double *X, double *PXcur, dou
   if (c->any_periodic && r.btn > 0)
     hipLaunchKernelGGL(k_wrap, dim3(r.btn), dim3(kTile), 0, c->stream, c->P, X, PXcur, PXpred,
                        c->d_btiles + r.bt0, with_pred);
+  if (r.btn + r.qtn > 0)
+    hipLaunchKernelGGL(k_bbox, dim3(r.btn + r.qtn), dim3(kTile), 0, c->stream, c->P, X, PXcur, c->d_btiles,
+                       r.bt0, r.btn, c->d_qtiles, r.qt0);
   if (r.btn > 0)
-    hipLaunchKernelGGL(k_list_exv, dim3(r.btn), dim3(kTile), 0, c->stream, c->P, X, PXcur, c->d_btiles + r.bt0);
+    hipLaunchKernelGGL(k_list_exv, dim3(r.btn), dim3(kTile), 0, c->stream, c->P, X, PXcur, c->d_btiles, r.bt0);
   if (r.qtn > 0)
-    hipLaunchKernelGGL(k_list_ele, dim3(r.qtn), dim3(kTile), 0, c->stream, c->P, X, PXcur, c->d_qtiles + r.qt0);
+    hipLaunchKernelGGL(k_list_ele, dim3(r.qtn), dim3(kTile), 0, c->stream, c->P, X, PXcur, c->d_qtiles, r.qt0);
   return 0;
 }
 
@@ -263,6 +271,16 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
   std::vector<double> bd_nat, bd_coef, ba_nat, ba_coef, st_nat, st_coef, tst_nat, tst_coef, hb_nat, hb_coef;
   std::vector<Tile> btiles, qtiles;
   std::vector<int32_t> systask;
+  // excluded-volume species: distinct (mptype, class, radius, epsilon)
+  struct Spec { int8_t t, c; double r, e; };
+  std::vector<Spec> specs;
+  std::vector<uint8_t> species;
+  auto spec_of = [&](int8_t t, int8_t cl, double r, double e) -> int {
+    for (size_t k = 0; k < specs.size(); ++k)
+      if (specs[k].t == t && specs[k].c == cl && specs[k].r == r && specs[k].e == e) return (int)k;
+    specs.push_back(Spec{t, cl, r, e});
+    return (int)specs.size() - 1;
+  };
   c->systask_of.assign(nsys, -1);
   int64_t moff = 0, mioff = 0;
   int32_t off = 0, qoff = 0, nbd = 0, nba = 0, nst = 0, ntst = 0, nhb = 0, nsite = 0, uoff = 0;
@@ -278,7 +296,11 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
     d.key = keys ? keys[is] : (uint32_t)is;
     d.off = off; d.nmp = s.nmp;
     d.uoff = uoff; d.nunit = s.nunit;
-    for (int k = 0; k < s.nunit * s.nunit; ++k) rl2.push_back(s.rlist2_unit[k]);
+    d.rl2max = 0.0;
+    for (int k = 0; k < s.nunit * s.nunit; ++k) {
+      rl2.push_back(s.rlist2_unit[k]);
+      d.rl2max = std::max(d.rl2max, s.rlist2_unit[k]);
+    }
     uoff += s.nunit * s.nunit;
     if (s.p.periodic) c->any_periodic = true;
     if (s.p.ele_form == TIS_ELE_DH) c->any_dh = true;
@@ -291,6 +313,7 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
       unit.push_back(s.unit[i]);
       exv_rad.push_back(s.exv_rad[i]);
       exv_eps.push_back((float)s.exv_eps[i]);
+      species.push_back((uint8_t)spec_of((int8_t)s.mptype[i], (int8_t)s.iclass[i], s.exv_rad[i], s.exv_eps[i]));
       charge.push_back(s.charge[i]);
       mass.push_back(s.mass[i]);
       rcm.push_back(1.0 / s.mass[i]);
@@ -383,12 +406,55 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
     h.qt_begin = (int32_t)qtiles.size();
     for (int st0 = 0; st0 < nq; st0 += kTile) qtiles.push_back(Tile{is, st0});
     h.qt_count = (int32_t)qtiles.size() - h.qt_begin;
+    d.bt0 = h.bt_begin; d.nbt = h.bt_count; d.qt0 = h.qt_begin; d.nqt = h.qt_count;
     h.has_task = (s.nhb > 0 || s.ntst > 0);
     if (h.has_task) { c->systask_of[is] = (int32_t)systask.size(); systask.push_back(is); }
     off += s.nmp;
   }
   inc_off.push_back((int32_t)inc.size());
   if (off >= (1 << 27)) return fail(c, TIS_ERR_ARG, "too many beads (%d)", off);
+  if (specs.size() > 255) return fail(c, TIS_ERR_ARG, "too many bead species (%zu)", specs.size());
+  // incidence index of every (term, slot): the term passes write there
+  std::vector<int32_t> bd_pos(2 * (size_t)nbd), ba_pos(3 * (size_t)nba), st_pos(7 * (size_t)nst),
+      tst_pos(6 * (size_t)ntst), hb_pos(6 * (size_t)nhb);
+  for (size_t k = 0; k < inc.size(); ++k) {
+    const uint32_t e = inc[k], type = e >> 29, slot = (e >> 26) & 7u, term = e & ((1u << 26) - 1u);
+    switch (type) {
+      case INC_BOND: bd_pos[(size_t)slot * nbd + term] = (int32_t)k; break;
+      case INC_ANGLE: ba_pos[(size_t)slot * nba + term] = (int32_t)k; break;
+      case INC_STACK: st_pos[(size_t)slot * nst + term] = (int32_t)k; break;
+      case INC_TSTACK: tst_pos[(size_t)slot * ntst + term] = (int32_t)k; break;
+      default: hb_pos[(size_t)slot * nhb + term] = (int32_t)k; break;
+    }
+  }
+  // species-pair tables (force_exv_dt15.F90:60-80): D_ij and eps_i eps_j,
+  // one per distinct (exv_dist, exv_dist_PS) of the batch
+  const int NS = (int)specs.size();
+  std::vector<SpPair> sptab;
+  std::vector<std::pair<double, double>> tabkeys;
+  for (int is = 0; is < nsys; ++is) {
+    DSys &d = c->hs[is].d;
+    int found = -1;
+    for (size_t k = 0; k < tabkeys.size(); ++k)
+      if (tabkeys[k].first == d.exv_dist && tabkeys[k].second == d.exv_dist_PS) found = (int)k;
+    if (found < 0) {
+      found = (int)tabkeys.size();
+      tabkeys.push_back({d.exv_dist, d.exv_dist_PS});
+      for (int a = 0; a < NS; ++a)
+        for (int b = 0; b < NS; ++b) {
+          const Spec &A = specs[a], &B = specs[b];
+          double dij;
+          if ((A.t == TIS_MP_RNA_PHOS && B.t == TIS_MP_RNA_SUGAR) || (A.t == TIS_MP_RNA_SUGAR && B.t == TIS_MP_RNA_PHOS))
+            dij = d.exv_dist_PS;
+          else if (A.c == TIS_CLASS_RNA && B.c == TIS_CLASS_RNA)
+            dij = d.exv_dist;
+          else
+            dij = A.r + B.r;
+          sptab.push_back(SpPair{dij, dij * dij, A.e * B.e});
+        }
+    }
+    d.sptab = found * NS * NS;
+  }
   c->ntot = off; c->nqtot = qoff; c->nhbtot = nhb; c->ntsttot = ntst; c->nsttot = nst;
   c->nbtile = (int32_t)btiles.size(); c->nqtile = (int32_t)qtiles.size(); c->nsystask = (int32_t)systask.size();
 
@@ -420,6 +486,17 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
   UP(d_rcm, rcm); UP(d_mass, mass); UP(d_rad, exv_rad); UP(d_charge, charge); UP(d_rl2, rl2);
   UP(d_mpt, mptype); UP(d_icl, iclass); UP(d_unit, unit); UP(d_qidx, qidx); UP(d_qbead, qbead);
   UP(d_incoff, inc_off); UP(d_inc, inc); UP(d_eps, exv_eps);
+  int32_t *d_pos;
+  UP(d_pos, bd_pos); P.bd_pos = d_pos; UP(d_pos, ba_pos); P.ba_pos = d_pos; UP(d_pos, st_pos); P.st_pos = d_pos;
+  UP(d_pos, tst_pos); P.tst_pos = d_pos; UP(d_pos, hb_pos); P.hb_pos = d_pos;
+  P.ninc = (int32_t)inc.size();
+  rc = dalloc(c, &P.fslot, 3 * std::max<size_t>(inc.size(), 1)); if (rc) return rc;
+  uint8_t *d_species;
+  SpPair *d_sptab;
+  UP(d_species, species); UP(d_sptab, sptab);
+  P.species = d_species; P.sptab = d_sptab; P.nspecies = NS;
+  rc = dalloc(c, &P.bbox, (size_t)c->nbtile); if (rc) return rc;
+  rc = dalloc(c, &P.qbbox, (size_t)c->nqtile); if (rc) return rc;
   P.rcm = d_rcm; P.mass = d_mass; P.exv_rad = d_rad; P.charge = d_charge; P.rl2 = d_rl2;
   P.mptype = d_mpt; P.iclass = d_icl; P.unit = d_unit; P.qidx = d_qidx; P.qbead = d_qbead;
   P.inc_off = d_incoff; P.inc = d_inc; P.exv_eps = d_eps;
@@ -436,7 +513,6 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
   std::vector<uint8_t> ones(std::max(nst, 1), 1);
   uint8_t *u8;
   UP(u8, ones); P.st_status = u8;
-  rc = dalloc(c, &P.for_tst, 18 * (size_t)ntst); if (rc) return rc;
   rc = dalloc(c, &P.hb_inlist, (size_t)nhb); if (rc) return rc;
   rc = dalloc(c, &P.hb_status, (size_t)nhb); if (rc) return rc;
   rc = dalloc(c, &P.hb_ene, (size_t)nhb); if (rc) return rc;
@@ -635,11 +711,11 @@ int32_t tis_pairs_get(tis_ctx *c, int32_t isys, int32_t which, int32_t *pairs, d
                  : TIS_OK;
 }
 
-// k_pre then k_main.  Steps draw noise/valence numbers for step stepbase+t+1;
-// a stand-alone force evaluation or the initial acceleration uses stepbase.
+// k_terms, k_bonded, k_dh, then k_main.  Steps draw noise/valence numbers for step
+// stepbase+t+1; a stand-alone force evaluation or the initial acceleration
+// uses stepbase.
 static void launch_force_pair(tis_ctx *c, const Range &r, const double *X, const double *PX, double *XN,
                               double *PXN, int t_local, int mode) {
-  const int npre = r.taskn + r.qtn;
   const int32_t pre_t = mode == 1 ? (int32_t)t_local : -1;
   hipEvent_t e3[3] = {nullptr, nullptr, nullptr};
   if (c->timing) {
@@ -652,12 +728,17 @@ static void launch_force_pair(tis_ctx *c, const Range &r, const double *X, const
     c->tev_used += 3;
     (void)hipEventRecord(e3[0], c->stream);
   }
-  if (npre > 0)
-    hipLaunchKernelGGL(k_pre, dim3(npre), dim3(kTile), 0, c->stream, c->P, X, PX, c->d_systask + r.task0,
-                       r.taskn, c->d_qtiles + r.qt0, r.qtn, pre_t, c->any_dh ? 1 : 0);
+  if (r.taskn > 0)
+    hipLaunchKernelGGL(k_terms, dim3(r.taskn), dim3(kTile), 0, c->stream, c->P, X, PX, c->d_systask + r.task0, pre_t);
+  const int nterm = r.nb + r.na + r.ns;
+  if (nterm > 0)
+    hipLaunchKernelGGL(k_bonded, dim3((nterm + 255) / 256), dim3(256), 0, c->stream, c->P, X, r.b0, r.nb, r.a0,
+                       r.na, r.s0, r.ns);
+  if (r.qtn > 0 && c->any_dh)
+    hipLaunchKernelGGL(k_dh, dim3(r.qtn), dim3(kTile * kWaves), 0, c->stream, c->P, X, PX, c->d_qtiles + r.qt0);
   if (c->timing) (void)hipEventRecord(e3[1], c->stream);
   if (r.btn > 0)
-    hipLaunchKernelGGL(k_main, dim3(r.btn), dim3(kTile), 0, c->stream, c->P, X, PX, XN, PXN,
+    hipLaunchKernelGGL(k_main, dim3(r.btn), dim3(kTile * 2), 0, c->stream, c->P, X, PX, XN, PXN,
                        c->d_btiles + r.bt0, r.btn, (int32_t)t_local, mode, c->any_dh ? 1 : 0);
   if (c->timing) (void)hipEventRecord(e3[2], c->stream);
 }

@@ -20,6 +20,7 @@
 namespace tis {
 
 constexpr int kTile = 64;            // beads (or charges) per work tile = one wave
+constexpr int kWaves = 4;            // waves sharing one tile in k_main / k_dh
 constexpr int kMirrorShift = 27;     // nb entry: (mirror << 27) | j
 constexpr uint32_t kIdxMask = (1u << 27) - 1u;
 constexpr double kDeMax = 50.0;      // const_physical.F90:35
@@ -42,6 +43,9 @@ struct DSys {
   int32_t aoff, nba;         // angles
   int32_t sitoff, nsite;     // hbond sites
   int32_t hb_excess_possible; // some site has more candidates than its valence
+  int32_t sptab;             // offset of this system's species-pair table
+  int32_t bt0, nbt, qt0, nqt; // its bead tiles and charge tiles
+  double rl2max;             // largest exv list radius^2 over its unit pairs
   int32_t uoff, nunit;       // unit-pair list radius table rl2[uoff + a*nunit + b]
   int64_t moff;              // ele bit-matrix words
   int32_t mwords;            // words per row = ceil(nq/32)
@@ -59,6 +63,13 @@ struct DSys {
 
 struct Tile { int32_t sys, start; };
 
+// excluded-volume parameters of a species pair: D_ij (FP64 and its square)
+// and eps_i*eps_j (force_exv_dt15.F90:60-80,92)
+struct SpPair { double dij, dij2, coef; };
+
+// axis-aligned bounding box of a tile of beads (or charges)
+struct BBox { double lo[3], hi[3]; };
+
 // All device pointers of a context, passed by value to kernels.
 struct DevPtrs {
   const DSys *sys;
@@ -70,13 +81,19 @@ struct DevPtrs {
   const double *rcm, *mass;
   const int8_t *mptype, *iclass;
   const int32_t *unit, *qidx;   // qidx: charge index (global) or -1
+  const uint8_t *species;       // excluded-volume species of each bead
+  const SpPair *sptab; int32_t nspecies;
+  BBox *bbox, *qbbox;          // per bead tile / per charge tile
   const double *exv_rad;
   const float *exv_eps;
   const double *charge;
   const double *rl2;
-  // incidence
+  // incidence: per bead the slots of the bonded terms it takes part in
   const int32_t *inc_off;
   const uint32_t *inc;
+  int32_t ninc;
+  double *fslot;             // [3][ninc] slot forces written by the term passes
+  const int32_t *bd_pos, *ba_pos, *st_pos, *tst_pos, *hb_pos;  // slot k of term t -> incidence index
   // terms (global bead indices)
   const int32_t *bd_mp; const double *bd_nat, *bd_coef; int32_t nbd;
   const int32_t *ba_mp; const double *ba_nat, *ba_coef; int32_t nba;
@@ -87,7 +104,6 @@ struct DevPtrs {
   const int32_t *site_val;
   // per-step state of terms
   uint8_t *st_status;        // [nst]
-  double *for_tst;           // [18][ntst]
   uint8_t *hb_inlist, *hb_status; // [nhb]
   double *hb_ene;            // [nhb]
   double *for_hb;            // [18][nhb]

@@ -37,20 +37,23 @@ __device__ inline void vsub(const double *X, int n, int a, int b, double v[3]) {
 }
 
 // ------------------------------------------------------------------ bonded
-// force on slot `slot` of bond b (force_bond.F90:40-58)
-__device__ inline void bond_slot(const DevPtrs &P, const double *X, int b, int slot, double acc[3]) {
+// Each bonded term is evaluated once (one lane per term) and its per-bead
+// forces are written to the slots of the beads' incidence lists.
+
+// force_bond.F90:40-58: fo[0..2] on bead 1, fo[3..5] on bead 2
+__device__ inline void bond_eval(const DevPtrs &P, const double *X, int b, double fo[6]) {
   int i = P.bd_mp[2 * b], j = P.bd_mp[2 * b + 1];
   double v[3];
   vsub(X, P.ntot, j, i, v);
   double dist = sqrt(dot3(v, v));
   double dd = dist - P.bd_nat[b];
   double fr = P.bd_coef[b] * (-2.0 * dd / dist);
-  double sg = slot == 0 ? -1.0 : 1.0;
-  acc[0] += sg * (fr * v[0]); acc[1] += sg * (fr * v[1]); acc[2] += sg * (fr * v[2]);
+  for (int d = 0; d < 3; ++d) { fo[d] = -(fr * v[d]); fo[3 + d] = fr * v[d]; }
 }
 
 // force_bangle.F90:40-84
-__device__ inline void angle_slot(const DevPtrs &P, const double *X, int a, int slot, double acc[3]) {
+__device__ inline void angle_eval(const DevPtrs &P, const double *X, int a, double fo[9]) {
+  for (int k = 0; k < 9; ++k) fo[k] = 0.0;
   double coef = P.ba_coef[a];
   if (coef < 1.0e-6) return;
   int i1 = P.ba_mp[3 * a], i2 = P.ba_mp[3 * a + 1], i3 = P.ba_mp[3 * a + 2];
@@ -64,21 +67,19 @@ __device__ inline void angle_slot(const DevPtrs &P, const double *X, int a, int 
   double t3 = c11 * c22 - c21 * c21;
   if (t3 <= 1.0) t3 = 1.0;
   double fr = 2.0 * coef * dba / sqrt(t3);
-#pragma unroll
   for (int d = 0; d < 3; ++d) {
     double f21 = fr * (v21[d] * (c21 / c11) - v32[d]);
     double f32 = fr * (v32[d] * (c21 / c22) - v21[d]);
-    acc[d] += slot == 0 ? -f21 : (slot == 1 ? f21 - f32 : f32);
+    fo[d] = -f21; fo[3 + d] = f21 - f32; fo[6 + d] = f32;
   }
 }
 
-// force_dtrna_stack.F90:53-165, force on one of its seven beads
-__device__ inline void stack_slot(const DevPtrs &P, const double *X, int st, int slot, double acc[3]) {
-  if (!P.st_status[st]) return;
+// force_dtrna_stack.F90:53-165: fo[3*k + d] on bead k of B1 B2 P1 S1 P2 S2 P3
+__device__ inline void stack_eval(const DevPtrs &P, const double *X, int st, double fo[21]) {
   const int n = P.ntot;
   const int32_t *m7 = P.st_mp + 7 * st;
   const double *nat = P.st_nat + 3 * st, *cf = P.st_coef + 4 * st;
-  double v21[3], v34[3], v54[3], v56[3], v76[3], m[3], nn[3], fi[3], fl[3], out[3] = {0, 0, 0};
+  double v21[3], v34[3], v54[3], v56[3], v76[3], m[3], nn[3], fi[3], fl[3];
   vsub(X, n, m7[1], m7[0], v21);
   vsub(X, n, m7[2], m7[3], v34);
   vsub(X, n, m7[4], m7[3], v54);
@@ -88,9 +89,9 @@ __device__ inline void stack_slot(const DevPtrs &P, const double *X, int st, int
   double dist = sqrt(dot3(v21, v21));
   double dd = dist - nat[0];
   ediv += cf[1] * dd * dd;
-  if (slot <= 1) {
-    double sg = slot == 0 ? -1.0 : 1.0;
-    for (int d = 0; d < 3; ++d) out[d] = sg * (2.0 * cf[1] * dd / dist * v21[d]);
+  for (int d = 0; d < 3; ++d) {
+    fi[d] = 2.0 * cf[1] * dd / dist * v21[d];
+    fo[d] = -fi[d]; fo[3 + d] = fi[d];
   }
   cross3(v34, v54, m);
   cross3(v54, v56, nn);
@@ -102,11 +103,11 @@ __device__ inline void stack_slot(const DevPtrs &P, const double *X, int st, int
   for (int d = 0; d < 3; ++d) {
     fi[d] = 2.0 * cf[2] * dph * abs54 / dmm * m[d];
     fl[d] = -2.0 * cf[2] * dph * abs54 / dnn * nn[d];
+    fo[6 + d] = fi[d];
+    fo[9 + d] = (-1.0 + d3454o) * fi[d] - d5654o * fl[d];
+    fo[12 + d] = (-1.0 + d5654o) * fl[d] - d3454o * fi[d];
+    fo[15 + d] = fl[d];
   }
-  if (slot == 2) for (int d = 0; d < 3; ++d) out[d] += fi[d];
-  if (slot == 3) for (int d = 0; d < 3; ++d) out[d] += (-1.0 + d3454o) * fi[d] - d5654o * fl[d];
-  if (slot == 4) for (int d = 0; d < 3; ++d) out[d] += (-1.0 + d5654o) * fl[d] - d3454o * fi[d];
-  if (slot == 5) for (int d = 0; d < 3; ++d) out[d] += fl[d];
   cross3(v76, v56, m);
   for (int d = 0; d < 3; ++d) nn[d] = -nn[d];
   dmm = dot3(m, m);
@@ -117,13 +118,50 @@ __device__ inline void stack_slot(const DevPtrs &P, const double *X, int st, int
   for (int d = 0; d < 3; ++d) {
     fi[d] = 2.0 * cf[3] * dph * abs56 / dmm * m[d];
     fl[d] = -2.0 * cf[3] * dph * abs56 / dnn * nn[d];
+    fo[18 + d] = fi[d];
+    fo[15 + d] += (-1.0 + d7656o) * fi[d] - d5456o * fl[d];
+    fo[12 + d] += -d7656o * fi[d] + (-1.0 + d5456o) * fl[d];
+    fo[9 + d] += fl[d];
   }
-  if (slot == 6) for (int d = 0; d < 3; ++d) out[d] += fi[d];
-  if (slot == 5) for (int d = 0; d < 3; ++d) out[d] += (-1.0 + d7656o) * fi[d] - d5456o * fl[d];
-  if (slot == 4) for (int d = 0; d < 3; ++d) out[d] += -d7656o * fi[d] + (-1.0 + d5456o) * fl[d];
-  if (slot == 3) for (int d = 0; d < 3; ++d) out[d] += fl[d];
   double sc = cf[0] / (ediv * ediv);
-  for (int d = 0; d < 3; ++d) acc[d] += sc * out[d];
+  for (int k = 0; k < 21; ++k) fo[k] *= sc;
+}
+
+__device__ inline void put_slots(const DevPtrs &P, const int32_t *pos, int nterm, int term, int nslot,
+                                 const double *fo) {
+  for (int k = 0; k < nslot; ++k) {
+    const int q = pos[(size_t)k * nterm + term];
+    P.fslot[q] = fo[3 * k];
+    P.fslot[(size_t)P.ninc + q] = fo[3 * k + 1];
+    P.fslot[2 * (size_t)P.ninc + q] = fo[3 * k + 2];
+  }
+}
+
+// One lane per bond, angle or consecutive stack of the launch range.
+__global__ __launch_bounds__(256) void k_bonded(DevPtrs P, const double *X, int b0, int nb, int a0, int na,
+                                                int s0, int ns) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nb) {
+    double fo[6];
+    bond_eval(P, X, b0 + t, fo);
+    put_slots(P, P.bd_pos, P.nbd, b0 + t, 2, fo);
+    return;
+  }
+  t -= nb;
+  if (t < na) {
+    double fo[9];
+    angle_eval(P, X, a0 + t, fo);
+    put_slots(P, P.ba_pos, P.nba, a0 + t, 3, fo);
+    return;
+  }
+  t -= na;
+  if (t < ns) {
+    const int st = s0 + t;
+    double fo[21];
+    if (P.st_status[st]) stack_eval(P, X, st, fo);
+    else for (int k = 0; k < 21; ++k) fo[k] = 0.0;
+    put_slots(P, P.st_pos, P.nst, st, 7, fo);
+  }
 }
 
 // ------------------------------------------------------------------ tertiary stack
@@ -385,124 +423,155 @@ __device__ int hbond_valence_serial(const DevPtrs &P, const DSys &s, uint64_t st
   return ndel;
 }
 
-// ------------------------------------------------------------------ pre-pass
-// Work items: [0, nsystask) = one system each (tertiary stacks, hbonds and
-// their valence pass); [nsystask, nsystask+nqtile) = DH charge tiles.
-__global__ __launch_bounds__(kTile) void k_pre(DevPtrs P, const double *X, const double *PX,
-                                               const int32_t *systask, int nsystask,
-                                               const Tile *qtiles, int nqtile, int32_t t_local,
-                                               int do_dh) {
+// ------------------------------------------------------------------ terms pass
+// One wave per system that has tertiary stacks or hbond candidates: tertiary
+// stack forces and st_status, hbond geometry of listed candidates and the
+// valence pass.  Forces go to per-term slots gathered by k_main.
+__global__ __launch_bounds__(kTile) void k_terms(DevPtrs P, const double *X, const double *PX,
+                                                 const int32_t *systask, int32_t t_local) {
   const int lane = threadIdx.x;
-  const int w = blockIdx.x;
   const uint64_t step = *P.stepbase + (uint64_t)t_local + 1u;
-  if (w < nsystask) {
-    const DSys s = P.sys[systask[w]];
-    // tertiary stacks
-    for (int k = lane; k < s.nst; k += kTile) P.st_status[s.soff + k] = 1;
-    __syncthreads();
-    for (int k = lane; k < s.ntst; k += kTile) {
-      double fo[18];
-      tstack_eval(P, s, X, PX, s.toff + k, fo);
-      for (int q = 0; q < 18; ++q) P.for_tst[(size_t)q * P.ntst + s.toff + k] = fo[q];
-    }
-    // hbond candidates in the list
-    if (s.hb_excess_possible)
-      for (int k = lane; k < s.nsite; k += kTile) P.site_cnt[s.sitoff + k] = 0;
-    __syncthreads();
-    for (int k = lane; k < s.nhb; k += kTile) {
-      int h = s.hoff + k;
-      int st = 0;
-      double e = 0.0, fo[18];
-      if (P.hb_inlist[h]) st = hbond_eval(P, s, X, PX, h, fo, &e);
-      P.hb_status[h] = (uint8_t)st;
-      P.hb_ene[h] = e;
-      if (st) {
-        for (int q = 0; q < 18; ++q) P.for_hb[(size_t)q * P.nhb + h] = fo[q];
-        if (s.hb_excess_possible)
-          for (int q = 0; q < 6; ++q) {
-            int site = P.hb_site[6 * h + q];
-            if (site >= 0) atomicAdd(&P.site_cnt[site], 1);
-          }
-      }
-    }
-    __syncthreads();
-    if (s.hb_excess_possible) {
-      int any = 0;
-      for (int k = lane; k < s.nsite; k += kTile)
-        if (__hip_atomic_load(&P.site_cnt[s.sitoff + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) -
-                P.site_val[s.sitoff + k] > 0)
-          any = 1;
-      any = __syncthreads_or(any);
-      if (any && lane == 0) {
-        int nd = hbond_valence_serial(P, s, step);
-        if (nd) atomicAdd(&P.counters[2], (unsigned long long)nd);
-      }
-    }
-    return;
+  const DSys s = P.sys[systask[blockIdx.x]];
+  for (int k = lane; k < s.nst; k += kTile) P.st_status[s.soff + k] = 1;
+  __syncthreads();
+  for (int k = lane; k < s.ntst; k += kTile) {
+    double fo[18];
+    tstack_eval(P, s, X, PX, s.toff + k, fo);
+    put_slots(P, P.tst_pos, P.ntst, s.toff + k, 6, fo);
   }
-  if (!do_dh) return;
-  // ---- Debye-Hueckel charge tile (force_ele_DH.F90:68-118), owner-computes
-  __shared__ double sx[kTile], sy[kTile], sz[kTile], sq[kTile];
-  const Tile tl = qtiles[w - nsystask];
+  if (s.hb_excess_possible)
+    for (int k = lane; k < s.nsite; k += kTile) P.site_cnt[s.sitoff + k] = 0;
+  __syncthreads();
+  for (int k = lane; k < s.nhb; k += kTile) {
+    int h = s.hoff + k;
+    int st = 0;
+    double e = 0.0, fo[18];
+    if (P.hb_inlist[h]) st = hbond_eval(P, s, X, PX, h, fo, &e);
+    P.hb_status[h] = (uint8_t)st;
+    P.hb_ene[h] = e;
+    if (st) {
+      for (int q = 0; q < 18; ++q) P.for_hb[(size_t)q * P.nhb + h] = fo[q];
+      if (s.hb_excess_possible)
+        for (int q = 0; q < 6; ++q) {
+          int site = P.hb_site[6 * h + q];
+          if (site >= 0) atomicAdd(&P.site_cnt[site], 1);
+        }
+    }
+  }
+  __syncthreads();
+  if (s.hb_excess_possible) {
+    int any = 0;
+    for (int k = lane; k < s.nsite; k += kTile)
+      if (site_count(P, s.sitoff + k) - P.site_val[s.sitoff + k] > 0) any = 1;
+    any = __syncthreads_or(any);
+    if (any && lane == 0) {
+      int nd = hbond_valence_serial(P, s, step);
+      if (nd) atomicAdd(&P.counters[2], (unsigned long long)nd);
+    }
+  }
+  __syncthreads();
+  // forces of the hbonds that survive into their beads' slots
+  for (int k = lane; k < s.nhb; k += kTile) {
+    const int h = s.hoff + k;
+    double fo[18];
+    const bool on = P.hb_status[h] != 0;
+    for (int q = 0; q < 18; ++q) fo[q] = on ? P.for_hb[(size_t)q * P.nhb + h] : 0.0;
+    put_slots(P, P.hb_pos, P.nhb, h, 6, fo);
+  }
+}
+
+// ------------------------------------------------------------------ Debye-Hueckel
+// force_ele_DH.F90:68-118, owner-computes over one charge tile: each lane of
+// wave w owns a charge of the tile and sweeps the system's charge chunks
+// w, w+4, ... staged through LDS; list membership from the bit matrix,
+// cutoff test in FP64, pair arithmetic in FP32, accumulation in FP64; the
+// four partial sums are added in wave order.
+__global__ __launch_bounds__(kTile * kWaves) void k_dh(DevPtrs P, const double *X, const double *PX,
+                                                       const Tile *qtiles) {
+  __shared__ double sx[kWaves][kTile], sy[kWaves][kTile], sz[kWaves][kTile];
+  __shared__ float sq[kWaves][kTile];
+  __shared__ double red[3][kWaves][kTile];
+  const int lane = threadIdx.x & (kTile - 1), w = threadIdx.x / kTile;
+  const Tile tl = qtiles[blockIdx.x];
   const DSys s = P.sys[tl.sys];
   if (s.ele_form != 0) return;
   const int qi = tl.start + lane;
   const bool valid = qi < s.nq;
   const double *Y = s.periodic ? PX : X;
   const int n = P.ntot;
-  double xi[3] = {0, 0, 0}, chi = 0.0;
+  double xi0 = 0.0, xi1 = 0.0, xi2 = 0.0, chi = 0.0;
   if (valid) {
     int gi = P.qbead[s.qoff + qi];
-    ld3(Y, n, gi, xi);
+    xi0 = Y[gi]; xi1 = Y[n + gi]; xi2 = Y[2 * n + gi];
     chi = P.charge[gi];
   }
-  double F[3] = {0.0, 0.0, 0.0};
+  const float qiC = (float)(chi * s.ele_coef);
   const float rcd = (float)s.rcdist;
+  const double cut2 = s.dh_cut2;
+  double F0 = 0.0, F1 = 0.0, F2 = 0.0;
   unsigned nclamp = 0;
   const uint32_t *row = P.qmask + s.moff + (int64_t)(valid ? qi : 0) * s.mwords;
-  for (int jt = 0; jt < s.nq; jt += kTile) {
-    int qj = jt + lane;
-    if (qj < s.nq) {
-      int gj = P.qbead[s.qoff + qj];
-      sx[lane] = Y[gj]; sy[lane] = Y[n + gj]; sz[lane] = Y[2 * n + gj];
-      sq[lane] = P.charge[gj];
+  const int nch = (s.nq + kTile - 1) / kTile;
+  for (int it = 0; it < nch; it += kWaves) {
+    const int jc = it + w;
+    const int jt = jc * kTile;
+    if (jc < nch && jt + lane < s.nq) {
+      int gj = P.qbead[s.qoff + jt + lane];
+      sx[w][lane] = Y[gj]; sy[w][lane] = Y[n + gj]; sz[w][lane] = Y[2 * n + gj];
+      sq[w][lane] = (float)P.charge[gj];
     }
     __syncthreads();
-    if (valid) {
+    if (valid && jc < nch) {
       const int wbase = jt >> 5;
-      uint32_t w0 = row[wbase];
-      uint32_t w1 = (wbase + 1 < s.mwords) ? row[wbase + 1] : 0u;
-      uint64_t bits = (uint64_t)w0 | ((uint64_t)w1 << 32);
-      while (bits) {
-        int k = __ffsll((unsigned long long)bits) - 1;
-        bits &= bits - 1;
-        double v0 = sx[k] - xi[0], v1 = sy[k] - xi[1], v2 = sz[k] - xi[2];
-        if (s.periodic) {
-          int code = P.qmirror[s.mioff + (int64_t)qi * s.nq + jt + k];
-          v0 += mirror_shift(s, code, 0); v1 += mirror_shift(s, code, 1); v2 += mirror_shift(s, code, 2);
+      const uint32_t w0 = row[wbase];
+      const uint32_t w1 = (wbase + 1 < s.mwords) ? row[wbase + 1] : 0u;
+      const int kend = min(kTile, s.nq - jt);
+      const double *SX = sx[w], *SY = sy[w], *SZ = sz[w];
+      const float *SQ = sq[w];
+      if (!s.periodic) {
+#pragma unroll 4
+        for (int k = 0; k < kend; ++k) {
+          const uint32_t bit = k < 32 ? (w0 >> k) & 1u : (w1 >> (k - 32)) & 1u;
+          const double v0 = SX[k] - xi0, v1 = SY[k] - xi1, v2 = SZ[k] - xi2;
+          const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+          if (!bit || d2 > cut2) continue;
+          const float f0 = (float)v0, f1 = (float)v1, f2 = (float)v2;
+          const float rinv = rsqrtf((float)d2);
+          const float d1 = (float)d2 * rinv;
+          float dv = qiC * SQ[k] * rinv * rinv * (rinv + rcd) * __expf(-d1 * rcd);
+          if (dv > (float)kDeMax) { dv = (float)kDeMax; if (qi < jt + k) ++nclamp; }
+          F0 -= (double)(dv * f0); F1 -= (double)(dv * f1); F2 -= (double)(dv * f2);
         }
-        double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-        if (d2 > s.dh_cut2) continue;
-        float coef = (float)((chi * sq[k]) * s.ele_coef);
-        float f0 = (float)v0, f1 = (float)v1, f2 = (float)v2;
-        float d1 = sqrtf((float)d2);
-        float rd = 1.0f / d1;
-        float dv = coef * rd * rd * (rd + rcd) * __expf(-d1 * rcd);
-        if (dv > (float)kDeMax) { dv = (float)kDeMax; if (qi < jt + k) ++nclamp; }
-        F[0] -= (double)(dv * f0); F[1] -= (double)(dv * f1); F[2] -= (double)(dv * f2);
+      } else {
+        for (int k = 0; k < kend; ++k) {
+          const uint32_t bit = k < 32 ? (w0 >> k) & 1u : (w1 >> (k - 32)) & 1u;
+          if (!bit) continue;
+          const int code = P.qmirror[s.mioff + (int64_t)qi * s.nq + jt + k];
+          const double v0 = SX[k] - xi0 + mirror_shift(s, code, 0);
+          const double v1 = SY[k] - xi1 + mirror_shift(s, code, 1);
+          const double v2 = SZ[k] - xi2 + mirror_shift(s, code, 2);
+          const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+          if (d2 > cut2) continue;
+          const float f0 = (float)v0, f1 = (float)v1, f2 = (float)v2;
+          const float rinv = rsqrtf((float)d2);
+          const float d1 = (float)d2 * rinv;
+          float dv = qiC * SQ[k] * rinv * rinv * (rinv + rcd) * __expf(-d1 * rcd);
+          if (dv > (float)kDeMax) { dv = (float)kDeMax; if (qi < jt + k) ++nclamp; }
+          F0 -= (double)(dv * f0); F1 -= (double)(dv * f1); F2 -= (double)(dv * f2);
+        }
       }
     }
     __syncthreads();
   }
-  if (valid) {
+  red[0][w][lane] = F0; red[1][w][lane] = F1; red[2][w][lane] = F2;
+  __syncthreads();
+  if (w == 0 && valid) {
     const int q = s.qoff + qi;
-    P.fdh[q] = F[0];
-    P.fdh[(size_t)P.nqtot + q] = F[1];
-    P.fdh[2 * (size_t)P.nqtot + q] = F[2];
+    for (int d = 0; d < 3; ++d)
+      P.fdh[(size_t)d * P.nqtot + q] = ((red[d][0][lane] + red[d][1][lane]) + red[d][2][lane]) + red[d][3][lane];
   }
   if (nclamp) atomicAdd(&P.counters[1], (unsigned long long)nclamp);
 }
-
 
 // ------------------------------------------------------------------ main pass
 // mode 0: store F; mode 1: Langevin step; mode 2: initial acceleration;
@@ -533,82 +602,91 @@ __device__ inline void integrate(const DevPtrs &P, const DSys &s, int g, int li,
   }
 }
 
-__global__ __launch_bounds__(kTile) void k_main(DevPtrs P, const double *X, const double *PX,
-                                                double *XN, double *PXN, const Tile *tiles,
-                                                int ntile, int32_t t_local, int mode, int do_dh) {
-  const int lane = threadIdx.x;
+// Four waves share a tile of 64 beads: waves 0-1 take alternate bonded
+// entries of each bead's incidence list, waves 2-3 alternate groups of four
+// excluded-volume neighbours; the partial sums are added in wave order (a
+// fixed order: results are bitwise reproducible), then wave 0 adds the DH
+// sum and integrates.
+__device__ inline void exv_group(const DevPtrs &P, const DSys &s, const double *Y, int g, int m0, int cnt,
+                                 double xi0, double xi1, double xi2, const SpPair *tab, double F[3],
+                                 unsigned &nclamp) {
+  const int n = P.ntot;
+  uint32_t e[4];
+  double y0[4], y1[4], y2[4];
+  int sp[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) e[u] = (m0 + u < cnt) ? P.nb[(size_t)(m0 + u) * n + g] : (uint32_t)g;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int j = (int)(e[u] & kIdxMask);
+    y0[u] = Y[j]; y1[u] = Y[n + j]; y2[u] = Y[2 * n + j];
+    sp[u] = P.species[j];
+  }
+  const double a = s.exv_a;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    if (m0 + u >= cnt) continue;
+    double v0 = y0[u] - xi0, v1 = y1[u] - xi1, v2 = y2[u] - xi2;
+    if (s.periodic) {
+      const int code = (int)(e[u] >> kMirrorShift);
+      v0 += mirror_shift(s, code, 0); v1 += mirror_shift(s, code, 1); v2 += mirror_shift(s, code, 2);
+    }
+    const SpPair t = tab[sp[u]];
+    const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+    if (d2 > t.dij2) continue;
+    // contact pair: the mWCA factor (a/dr)^14 amplifies rounding 14-fold, so
+    // this formula runs in FP64 (only pairs within D_ij reach it)
+    const double dist = sqrt(d2);
+    const double dr = dist + a - t.dij;
+    const double id = 1.0 / dr, ar = a * id, r2 = ar * ar, r6 = r2 * r2 * r2, r12 = r6 * r6;
+    double dv = fabs(12.0 * t.coef * (r12 - r6) * id / dist);
+    if (dv > kDeMax) { dv = kDeMax; if (g < (int)(e[u] & kIdxMask)) ++nclamp; }
+    F[0] -= dv * v0; F[1] -= dv * v1; F[2] -= dv * v2;
+  }
+}
+
+__global__ __launch_bounds__(kTile * 2) void k_main(DevPtrs P, const double *X, const double *PX,
+                                                    double *XN, double *PXN, const Tile *tiles,
+                                                    int ntile, int32_t t_local, int mode, int do_dh) {
+  __shared__ double red[3][2][kTile];
+  const int lane = threadIdx.x & (kTile - 1), w = threadIdx.x / kTile;
   const Tile tl = tiles[blockIdx.x];
   const DSys s = P.sys[tl.sys];
   const int li = tl.start + lane;
-  if (li >= s.nmp) return;
-  const int g = s.off + li;
+  const bool valid = li < s.nmp;
+  const int g = s.off + (valid ? li : 0);
   const int n = P.ntot;
-  const uint64_t step = *P.stepbase + (uint64_t)t_local + (mode == 2 ? 0u : 1u);
   double F[3] = {0.0, 0.0, 0.0};
-  // bonded terms in incidence order
-  const int ib = P.inc_off[g], ie = P.inc_off[g + 1];
-  for (int k = ib; k < ie; ++k) {
-    uint32_t e = P.inc[k];
-    uint32_t type = e >> 29, slot = (e >> 26) & 7u, term = e & ((1u << 26) - 1u);
-    switch (type) {
-      case INC_BOND: bond_slot(P, X, (int)term, (int)slot, F); break;
-      case INC_ANGLE: angle_slot(P, X, (int)term, (int)slot, F); break;
-      case INC_STACK: stack_slot(P, X, (int)term, (int)slot, F); break;
-      case INC_TSTACK:
-        for (int d = 0; d < 3; ++d) F[d] += P.for_tst[(size_t)(slot * 3 + d) * P.ntst + term];
-        break;
-      default:
-        if (P.hb_status[term])
-          for (int d = 0; d < 3; ++d) F[d] += P.for_hb[(size_t)(slot * 3 + d) * P.nhb + term];
-        break;
+  if (valid) {
+    if (w == 0) {
+      // bonded slots in incidence order
+      const int ib = P.inc_off[g], ie = P.inc_off[g + 1];
+      for (int k = ib; k < ie; ++k) {
+        F[0] += P.fslot[k]; F[1] += P.fslot[(size_t)P.ninc + k]; F[2] += P.fslot[2 * (size_t)P.ninc + k];
+      }
     }
-  }
-  // excluded volume (force_exv_dt15.F90:53-115), owner view of each pair
-  {
+    // excluded volume (force_exv_dt15.F90:53-115), owner view of each pair;
+    // the two waves take alternate groups of four neighbours
     const double *Y = s.periodic ? PX : X;
-    double xi[3];
-    ld3(Y, n, g, xi);
-    const int ti = P.mptype[g], ci = P.iclass[g];
-    const double ri = P.exv_rad[g];
-    const float epi = P.exv_eps[g];
-    const float a = (float)s.exv_a, a2 = a * a;
+    const double xi0 = Y[g], xi1 = Y[n + g], xi2 = Y[2 * n + g];
+    const SpPair *tab = P.sptab + s.sptab + (int)P.species[g] * P.nspecies;
     const int cnt = min(P.nb_cnt[g], P.nb_cap);
     unsigned nclamp = 0;
-    for (int m = 0; m < cnt; ++m) {
-      uint32_t e = P.nb[(size_t)m * n + g];
-      int j = (int)(e & kIdxMask);
-      double v0 = Y[j] - xi[0], v1 = Y[n + j] - xi[1], v2 = Y[2 * n + j] - xi[2];
-      if (s.periodic) {
-        int code = (int)(e >> kMirrorShift);
-        v0 += mirror_shift(s, code, 0); v1 += mirror_shift(s, code, 1); v2 += mirror_shift(s, code, 2);
-      }
-      const int tj = P.mptype[j];
-      double dij;
-      if ((ti == 2 && tj == 4) || (ti == 4 && tj == 2)) dij = s.exv_dist_PS;
-      else if (ci == 2 && P.iclass[j] == 2) dij = s.exv_dist;
-      else dij = ri + P.exv_rad[j];
-      double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-      if (d2 > dij * dij) continue;
-      float dist = sqrtf((float)d2);
-      float coef = epi * P.exv_eps[j];
-      float dr = dist + a - (float)dij;
-      float r2 = a2 / (dr * dr), r4 = r2 * r2, r8 = r4 * r4, r14 = r2 * r4 * r8;
-      float dv = fabsf(12.0f * coef * (r14 - r8) * dr / a2 / dist);
-      if (dv > (float)kDeMax) { dv = (float)kDeMax; if (g < j) ++nclamp; }
-      F[0] -= (double)(dv * (float)v0); F[1] -= (double)(dv * (float)v1); F[2] -= (double)(dv * (float)v2);
-    }
+    for (int m0 = 4 * w; m0 < cnt; m0 += 8) exv_group(P, s, Y, g, m0, cnt, xi0, xi1, xi2, tab, F, nclamp);
     if (nclamp) atomicAdd(&P.counters[0], (unsigned long long)nclamp);
   }
-  // Debye-Hueckel sums of the pre-pass
+  for (int d = 0; d < 3; ++d) red[d][w][lane] = F[d];
+  __syncthreads();
+  if (w != 0 || !valid) return;
+  for (int d = 0; d < 3; ++d) F[d] = red[d][0][lane] + red[d][1][lane];
+  // Debye-Hueckel sums of the DH pass
   if (do_dh && s.ele_form == 0) {
     const int q = P.qidx[g];
     if (q >= 0) { F[0] += P.fdh[q]; F[1] += P.fdh[(size_t)P.nqtot + q]; F[2] += P.fdh[2 * (size_t)P.nqtot + q]; }
   }
-  if (mode == 0) {
-    P.f[g] = F[0]; P.f[n + g] = F[1]; P.f[2 * n + g] = F[2];
-    return;
-  }
   P.f[g] = F[0]; P.f[n + g] = F[1]; P.f[2 * n + g] = F[2];
+  if (mode == 0) return;
+  const uint64_t step = *P.stepbase + (uint64_t)t_local + (mode == 2 ? 0u : 1u);
   integrate(P, s, g, li, F, X, PX, XN, PXN, step, mode);
 }
 
@@ -645,14 +723,67 @@ __device__ inline bool exv_keep(const DSys &s, int ulo, int uhi, int clo, int tl
   return hi >= lo + sep;
 }
 
+// Bounding boxes of bead tiles (blocks [0, nbt)) and charge tiles (blocks
+// [nbt, nbt + nqt)), from the coordinates the lists are built from.
+__global__ __launch_bounds__(kTile) void k_bbox(DevPtrs P, const double *X, const double *PX,
+                                                const Tile *btiles, int bt0, int nbt, const Tile *qtiles,
+                                                int qt0) {
+  const int lane = threadIdx.x;
+  const bool charge = (int)blockIdx.x >= nbt;
+  const int it = charge ? qt0 + (int)blockIdx.x - nbt : bt0 + (int)blockIdx.x;
+  const Tile tl = charge ? qtiles[it] : btiles[it];
+  const DSys s = P.sys[tl.sys];
+  const double *Y = s.periodic ? PX : X;
+  const int n = P.ntot;
+  const int li = tl.start + lane;
+  const int lim = charge ? s.nq : s.nmp;
+  const int first = charge ? P.qbead[s.qoff + tl.start] : s.off + tl.start;
+  int g = first;
+  if (li < lim) g = charge ? P.qbead[s.qoff + li] : s.off + li;
+  double lo[3] = {Y[g], Y[n + g], Y[2 * n + g]}, hi[3] = {lo[0], lo[1], lo[2]};
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = fmin(lo[d], __shfl_xor(lo[d], off));
+      hi[d] = fmax(hi[d], __shfl_xor(hi[d], off));
+    }
+  if (lane == 0) {
+    BBox b;
+    for (int d = 0; d < 3; ++d) { b.lo[d] = lo[d]; b.hi[d] = hi[d]; }
+    if (charge) P.qbbox[it] = b; else P.bbox[it] = b;
+  }
+}
+
+// squared distance bounds between two boxes (FP64); conservative culling
+// margins of 1e-9 relative cover the rounding of the per-pair test.
+__device__ inline double box_dmin2(const BBox &a, const BBox &b) {
+  double s2 = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    double g = fmax(0.0, fmax(b.lo[d] - a.hi[d], a.lo[d] - b.hi[d]));
+    s2 += g * g;
+  }
+  return s2;
+}
+__device__ inline double box_dmax2(const BBox &a, const BBox &b) {
+  double s2 = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    double g = fmax(fabs(b.hi[d] - a.lo[d]), fabs(a.hi[d] - b.lo[d]));
+    s2 += g * g;
+  }
+  return s2;
+}
+
 // exv lists (neighbor_list.F90:56-87 + neighbor_assign.F90): every bead of
-// the tile scans the system's beads staged through LDS, FP64 test
-// d^2 < (cutoff + margin)^2 exactly as the reference rounds it.
-__global__ __launch_bounds__(kTile) void k_list_exv(DevPtrs P, const double *X, const double *PX, const Tile *tiles) {
+// the tile scans the system's bead tiles that its tile's box can reach,
+// staged through LDS, with the FP64 test d^2 < (cutoff + margin)^2 rounded as
+// the reference rounds it.
+__global__ __launch_bounds__(kTile) void k_list_exv(DevPtrs P, const double *X, const double *PX, const Tile *tiles,
+                                                    int tile0) {
   __shared__ double sx[kTile], sy[kTile], sz[kTile];
   __shared__ int su[kTile], stc[kTile];
   const int lane = threadIdx.x;
-  const Tile tl = tiles[blockIdx.x];
+  const int it = tile0 + blockIdx.x;
+  const Tile tl = tiles[it];
   const DSys s = P.sys[tl.sys];
   const int li = tl.start + lane;
   const bool valid = li < s.nmp;
@@ -662,8 +793,11 @@ __global__ __launch_bounds__(kTile) void k_list_exv(DevPtrs P, const double *X, 
   double xi[3] = {0, 0, 0};
   int ui = 0, ti = 0, ci = 0;
   if (valid) { ld3(Y, n, g, xi); ui = P.unit[g]; ti = P.mptype[g]; ci = P.iclass[g]; }
+  const BBox bi = P.bbox[it];
   int cnt = 0;
-  for (int jt = 0; jt < s.nmp; jt += kTile) {
+  for (int jtile = 0; jtile < s.nbt; ++jtile) {
+    const int jt = jtile * kTile;
+    if (!s.periodic && box_dmin2(bi, P.bbox[s.bt0 + jtile]) > s.rl2max * (1.0 + 1e-9)) continue;
     int lj = jt + lane;
     if (lj < s.nmp) {
       int gj = s.off + lj;
@@ -703,11 +837,15 @@ __global__ __launch_bounds__(kTile) void k_list_exv(DevPtrs P, const double *X, 
 }
 
 // ele bit matrix (neighbor_list_ele.F90:123-210): bit (qi, qj) set when the
-// pair is in the list; radius (margin + cutoff)^2 per i_DH_cutoff_type.
-__global__ __launch_bounds__(kTile) void k_list_ele(DevPtrs P, const double *X, const double *PX, const Tile *qtiles) {
+// pair is in the list; radius (margin + cutoff)^2 per i_DH_cutoff_type.  Tile
+// pairs whose boxes lie entirely inside (outside) the radius are filled with
+// ones (zeros) without per-pair tests; the rest are tested pair by pair.
+__global__ __launch_bounds__(kTile) void k_list_ele(DevPtrs P, const double *X, const double *PX, const Tile *qtiles,
+                                                    int tile0) {
   __shared__ double sx[kTile], sy[kTile], sz[kTile];
   const int lane = threadIdx.x;
-  const Tile tl = qtiles[blockIdx.x];
+  const int it = tile0 + blockIdx.x;
+  const Tile tl = qtiles[it];
   const DSys s = P.sys[tl.sys];
   const int qi = tl.start + lane;
   const bool valid = qi < s.nq;
@@ -716,34 +854,48 @@ __global__ __launch_bounds__(kTile) void k_list_ele(DevPtrs P, const double *X, 
   double xi[3] = {0, 0, 0};
   if (valid) ld3(Y, n, P.qbead[s.qoff + qi], xi);
   uint32_t *row = P.qmask + s.moff + (int64_t)(valid ? qi : 0) * s.mwords;
-  for (int jt = 0; jt < s.nq; jt += kTile) {
-    int qj = jt + lane;
-    if (qj < s.nq) {
-      int gj = P.qbead[s.qoff + qj];
-      sx[lane] = Y[gj]; sy[lane] = Y[n + gj]; sz[lane] = Y[2 * n + gj];
-    }
-    __syncthreads();
-    uint32_t w[2] = {0u, 0u};
+  const BBox bi = P.qbbox[it];
+  for (int jtile = 0; jtile < s.nqt; ++jtile) {
+    const int jt = jtile * kTile;
     const int kend = min(kTile, s.nq - jt);
-    if (valid)
-      for (int k = 0; k < kend; ++k) {
-        int qj2 = jt + k;
-        if (qj2 == qi) continue;
-        const bool own_lo = qi < qj2;
-        double v[3];
-        if (own_lo) { v[0] = __dadd_rn(sx[k], -xi[0]); v[1] = __dadd_rn(sy[k], -xi[1]); v[2] = __dadd_rn(sz[k], -xi[2]); }
-        else { v[0] = __dadd_rn(xi[0], -sx[k]); v[1] = __dadd_rn(xi[1], -sy[k]); v[2] = __dadd_rn(xi[2], -sz[k]); }
-        int code = s.periodic ? pbneighbor(s, v) : 0;
-        double d2 = sq3_exact(v[0], v[1], v[2]);
-        if (!(d2 < s.ele_rl2)) continue;
-        w[k >> 5] |= 1u << (k & 31);
-        if (s.periodic) P.qmirror[s.mioff + (int64_t)qi * s.nq + qj2] = (uint8_t)(own_lo ? code : mirror_flip(code));
+    int mode = 2;  // 0 none, 1 all, 2 test
+    if (!s.periodic) {
+      const BBox bj = P.qbbox[s.qt0 + jtile];
+      if (box_dmin2(bi, bj) > s.ele_rl2 * (1.0 + 1e-9)) mode = 0;
+      else if (box_dmax2(bi, bj) < s.ele_rl2 * (1.0 - 1e-9)) mode = 1;
+    }
+    uint32_t w[2] = {0u, 0u};
+    if (mode == 1) {
+      w[0] = kend >= 32 ? 0xFFFFFFFFu : ((1u << kend) - 1u);
+      w[1] = kend >= 64 ? 0xFFFFFFFFu : (kend > 32 ? ((1u << (kend - 32)) - 1u) : 0u);
+      if (qi >= jt && qi < jt + kend) w[(qi - jt) >> 5] &= ~(1u << ((qi - jt) & 31));
+    } else if (mode == 2) {
+      int qj = jt + lane;
+      if (qj < s.nq) {
+        int gj = P.qbead[s.qoff + qj];
+        sx[lane] = Y[gj]; sy[lane] = Y[n + gj]; sz[lane] = Y[2 * n + gj];
       }
+      __syncthreads();
+      if (valid)
+        for (int k = 0; k < kend; ++k) {
+          int qj2 = jt + k;
+          if (qj2 == qi) continue;
+          const bool own_lo = qi < qj2;
+          double v[3];
+          if (own_lo) { v[0] = __dadd_rn(sx[k], -xi[0]); v[1] = __dadd_rn(sy[k], -xi[1]); v[2] = __dadd_rn(sz[k], -xi[2]); }
+          else { v[0] = __dadd_rn(xi[0], -sx[k]); v[1] = __dadd_rn(xi[1], -sy[k]); v[2] = __dadd_rn(xi[2], -sz[k]); }
+          int code = s.periodic ? pbneighbor(s, v) : 0;
+          double d2 = sq3_exact(v[0], v[1], v[2]);
+          if (!(d2 < s.ele_rl2)) continue;
+          w[k >> 5] |= 1u << (k & 31);
+          if (s.periodic) P.qmirror[s.mioff + (int64_t)qi * s.nq + qj2] = (uint8_t)(own_lo ? code : mirror_flip(code));
+        }
+      __syncthreads();
+    }
     if (valid) {
       row[jt >> 5] = w[0];
       if ((jt >> 5) + 1 < s.mwords) row[(jt >> 5) + 1] = w[1];
     }
-    __syncthreads();
   }
 }
 
