@@ -96,6 +96,52 @@ def force_check(systems, coords_now, eng, sample):
     return worst
 
 
+def serving_stream(njobs=100, scale=0.01, inflight=5.0, round_steps=500, seed=SEED + 4):
+    """Serving stream: jobs arrive as a Poisson process; each is a seeded
+    hairpin-design chain, lognormal(ln 200, 1.0) clipped to 20-2000 nt, run
+    for U[1e4, 1e6] x scale Langevin steps (a multiple of 10) at 150 mM and
+    310 K.  The arrival rate is set from a calibration run so that about
+    `inflight` jobs are on the GPU on average (Little's law)."""
+    from tis_amd.serve import Job, Server, poisson_stream
+    rng = np.random.default_rng(seed)
+
+    def make(k, r):
+        n = int(np.clip(np.round(r.lognormal(math.log(200.0), 1.0)), 20, 2000))
+        stem = int(r.integers(4, 11))
+        loop = int(r.integers(3, 9))
+        while 2 * stem + loop + 2 > n:
+            stem, loop = max(2, stem - 1), max(3, loop - 1)
+        seq, pairs = T.hairpin_design(r, n, stem, loop)
+        s = T.build_rna_system([(seq, pairs)], tempk=310.0, ionic_strength=0.150, name=f"job{k}")
+        steps = int(r.uniform(1e4, 1e6) * scale) // 10 * 10
+        return Job(key=100000 + k, system=s, x0=T.helix_coords(s, r), nsteps=max(steps, 10))
+
+    # calibration: ms/step of a batch of `inflight` jobs of the same distribution
+    cal = [make(k, np.random.default_rng(seed + 1 + k)) for k in range(int(round(inflight)))]
+    eng = Engine([j.system for j in cal], keys=[j.key for j in cal], seed=SEED)
+    for k, j in enumerate(cal):
+        eng.set_coords(k, j.x0)
+    eng.langevin_init(tstep=0.05, rebuild_every=10, step0=0, init_accel=True)
+    eng.langevin_steps(100, MODE_GRAPH)
+    eng.synchronize()
+    t0 = time.perf_counter()
+    eng.langevin_steps(500, MODE_GRAPH)
+    eng.synchronize()
+    step_s = (time.perf_counter() - t0) / 500
+    eng.close()
+    mean_steps = 0.5 * (1e4 + 1e6) * scale
+    rate = inflight / (mean_steps * step_s)
+    jobs = poisson_stream(rng, njobs, rate, make)
+    srv = Server(hbm_budget=64 << 30, round_steps=round_steps, clock="wall", seed=SEED)
+    st = srv.run(jobs)
+    out = st.summary()
+    out.update({"arrival_rate_per_s": rate, "calibration_ms_per_step": step_s * 1e3, "round_steps": round_steps,
+                "steps_scale": scale,
+                "workload": f"{njobs} Poisson jobs, lognormal(ln200,1.0) 20-2000 nt hairpin designs, "
+                            f"U[1e4,1e6]x{scale:g} steps, 150 mM, 310 K, target {inflight:g} in flight"})
+    return out
+
+
 def cpu_baseline(systems, coords, budget_s=15.0):
     """Oracle Langevin (FP64, O(N^2) list build, rebuild every 10 steps) on
     this host, one core, over chains of the same workload until budget_s."""
@@ -129,6 +175,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=100)
+    ap.add_argument("--serve-scale", type=float, default=0.01)
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the systems of this workload are independent and would be "
@@ -196,6 +245,13 @@ def main():
             "avg_launch_ms": round(dom_ms, 5), "bytes_per_launch": dom_bytes,
             "k_main_ms": round(main_ms, 5), "k_pre_ms": round(pre_ms, 5)}
 
+    eng.close()
+    serve = None
+    if not args.no_serve:
+        serve = serving_stream(njobs=args.serve_jobs, scale=args.serve_scale)
+        print(f"[bench] serving: {serve['jobs']} jobs, p50 {serve.get('p50_latency', 0):.3f} s, "
+              f"p99 {serve.get('p99_latency', 0):.3f} s, {serve['bead_steps_per_s_busy']:.4e} bead-steps/s, "
+              f"mean in flight {serve.get('mean_inflight', 0):.2f}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(systems, coords, args.cpu_budget)
     out = {
         "metric": "batched bead-steps/s vs plain GPU @ force err <=1e-5 vs FP64; p50/p99 job latency",
@@ -216,6 +272,9 @@ def main():
         "plain_gpu_bead_steps_per_s": plain,
         "batched_over_plain": value / plain,
         "force_max_rel_err_vs_fp64": ferr,
+        "p50_job_latency_s": serve.get("p50_latency") if serve else None,
+        "p99_job_latency_s": serve.get("p99_latency") if serve else None,
+        "serving": serve,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

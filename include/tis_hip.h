@@ -96,10 +96,20 @@ int32_t tis_energy_sumup(tis_ctx *ctx, int32_t isys, double *energy_out);
  * tis_langevin_init: time step tstep (tau), neighbour rebuild every
  * rebuild_every steps (> 0), first step index step0; if init_accel, the
  * accelerations are set to F/m + c1*xi(step0) (time_integral_pre.F90:176-179),
- * otherwise the stored accelerations are used. */
+ * if 0 the stored accelerations are used, if -1 per system (tis_system_origin). */
 int32_t tis_langevin_init(tis_ctx *ctx, double tstep, int32_t rebuild_every, uint64_t step0,
                           int32_t init_accel);
 int32_t tis_langevin_steps(tis_ctx *ctx, int64_t nsteps, int32_t mode);
+
+/* Step origin of system isys: its own step count is (global step - origin),
+ * which keys its noise and valence draws, so a system started at any point of
+ * a run draws exactly what it would draw alone.  init_accel: when
+ * tis_langevin_init is called with init_accel = -1, systems flagged here get
+ * new accelerations and the others keep their stored ones. */
+int32_t tis_system_origin(tis_ctx *ctx, int32_t isys, uint64_t step_origin, int32_t init_accel);
+
+/* Device memory held by the context's batch, in bytes (admission control). */
+int32_t tis_device_bytes(tis_ctx *ctx, int64_t *bytes);
 
 /* Counters (TIS_CNT_*), accumulated since upload. */
 int32_t tis_status_counters(tis_ctx *ctx, int64_t *counters);

@@ -46,7 +46,7 @@ enum {
 };
 
 /* Electrostatics function form, inele%i_function_form (src/force_sumup.F90:235-252) */
-enum { TIS_ELE_NONE = -1, TIS_ELE_DH = 0, TIS_ELE_EWALD = 2 };
+enum { TIS_ELE_NONE = -1, TIS_ELE_DH = 0, TIS_ELE_COULOMB = 1, TIS_ELE_EWALD = 2 };
 
 typedef struct tis_params {
   /* DT15 excluded volume (indtrna15, para/rna.para:343-351) */

@@ -64,6 +64,14 @@ def _lib():
     lib.oracle_hbond_valence.argtypes = [vp, _i32p, C.c_int32, _f64p, C.c_char_p,
                                          C.c_uint64, C.c_uint32, C.c_uint64]
     lib.oracle_hbond_scatter.argtypes = [vp, _i32p, C.c_int32, _f64p, C.c_char_p, _f64p]
+    lib.oracle_force_coulomb.argtypes = [vp, _f64p, _i32p, _f64p, C.c_int64, _f64p]
+    lib.oracle_force_ewald.argtypes = [vp, _f64p, _i32p, _f64p, C.c_int64, _f64p]
+    lib.oracle_energy_coulomb.restype = C.c_double
+    lib.oracle_energy_coulomb.argtypes = [vp, _f64p, _i32p, _f64p, C.c_int64]
+    lib.oracle_energy_ewald.restype = C.c_double
+    lib.oracle_energy_ewald.argtypes = [vp, _f64p, _i32p, _f64p, C.c_int64, _f64p]
+    lib.oracle_ewald_kvec.restype = C.c_int32
+    lib.oracle_ewald_kvec.argtypes = [vp, _i32p, _f64p]
     return lib
 
 
@@ -160,6 +168,12 @@ class Oracle:
             pr = np.ascontiguousarray(pr, dtype=np.int32)
             cf = np.ascontiguousarray(cf, dtype=np.float64)
             self.L.oracle_force_dh(cs, fp(x), ip(pr), fp(cf), len(pr), fp(f))
+        elif name in ("coulomb", "ewald"):
+            pr, cf = self.lists(1) if pairs is None else (pairs, coef)
+            pr = np.ascontiguousarray(pr, dtype=np.int32)
+            cf = np.ascontiguousarray(cf, dtype=np.float64)
+            fn = self.L.oracle_force_coulomb if name == "coulomb" else self.L.oracle_force_ewald
+            fn(cs, fp(x), ip(pr), fp(cf), len(pr), fp(f))
         elif name == "stack":
             st = np.ones(max(len(self.sys.st_mp), 1), np.uint8)
             self.L.oracle_force_tstack(cs, fp(x), fp(x), st.ctypes.data_as(C.c_char_p), fp(f))
@@ -182,6 +196,27 @@ class Oracle:
         return f
 
 
+    def ewald_kvec(self):
+        """Reciprocal vectors h (nk, 3) and coefficients f(h) of the system."""
+        cs = C.byref(self.cs)
+        nk = self.L.oracle_ewald_kvec(cs, None, None)
+        h = np.zeros((max(nk, 1), 3), np.int32)
+        c = np.zeros(max(nk, 1))
+        self.L.oracle_ewald_kvec(cs, ip(h), fp(c))
+        return h[:nk], c[:nk]
+
+    def ewald_energy_parts(self, pxyz):
+        """(Fourier, real, self) Ewald energies at wrapped coordinates pxyz
+        with the ele list of the last neighbor()."""
+        x = np.ascontiguousarray(pxyz, dtype=np.float64)
+        pr, cf = self.lists(1)
+        pr = np.ascontiguousarray(pr, dtype=np.int32)
+        cf = np.ascontiguousarray(cf, dtype=np.float64)
+        parts = np.zeros(3)
+        self.L.oracle_energy_ewald(C.byref(self.cs), fp(x), ip(pr), fp(cf), len(pr), fp(parts))
+        return parts
+
+
 def gauss3(seed, sys_id, step, bead):
     g = np.zeros(3)
     lib().oracle_gauss3(seed, sys_id, step, bead, fp(g))
@@ -201,10 +236,35 @@ def philox(ctr, key):
 
 
 # ---------------------------------------------------------------------------
+def _on_big_stack(fn, *args):
+    """Run fn(*args) on a thread with a 1 GiB stack: the reference's list
+    builders keep MXMPELE * ncharge pair buffers as automatic (stack) arrays,
+    which overflow the default 8 MiB stack for a few hundred charges."""
+    import threading
+    out, err = [], []
+
+    def run():
+        try:
+            out.append(fn(*args))
+        except BaseException as e:  # noqa: BLE001 - re-raised below
+            err.append(e)
+    old = threading.stack_size(1 << 30)
+    try:
+        t = threading.Thread(target=run)
+        t.start()
+        t.join()
+    finally:
+        threading.stack_size(old)
+    if err:
+        raise err[0]
+    return out[0] if out else None
+
+
 class RefLib:
     """The reference's own Fortran routines (one replica, one thread)."""
 
-    TERMS = {"bond": 0, "angle": 1, "stack": 2, "stack_only": 3, "hbond": 4, "exv": 5, "dh": 6}
+    TERMS = {"bond": 0, "angle": 1, "stack": 2, "stack_only": 3, "hbond": 4, "exv": 5, "dh": 6,
+             "coulomb": 7, "ewald": 8}
 
     @staticmethod
     def available() -> bool:
@@ -236,7 +296,7 @@ class RefLib:
 
     def neighbor(self, xyz):
         self._x = np.ascontiguousarray(xyz, dtype=np.float64)
-        self.L.tisref_neighbor(fp(self._x))
+        _on_big_stack(self.L.tisref_neighbor, fp(self._x))
 
     def move(self, xyz):
         self._x = np.ascontiguousarray(xyz, dtype=np.float64)
@@ -256,7 +316,7 @@ class RefLib:
 
     def force_term(self, name):
         f = np.zeros((self.sys.nmp, 3))
-        self.L.tisref_force_term(self.TERMS[name], fp(f))
+        _on_big_stack(self.L.tisref_force_term, self.TERMS[name], fp(f))
         return f
 
     def hb_state(self):
@@ -269,10 +329,10 @@ class RefLib:
     def energy(self, velo=None):
         e = np.zeros(25)
         if velo is None:
-            self.L.tisref_energy(fp(np.zeros(3)), 0, fp(e))
+            _on_big_stack(self.L.tisref_energy, fp(np.zeros(3)), 0, fp(e))
         else:
             v = np.ascontiguousarray(velo, dtype=np.float64)
-            self.L.tisref_energy(fp(v), 1, fp(e))
+            _on_big_stack(self.L.tisref_energy, fp(v), 1, fp(e))
         return e
 
     def temperature(self, tempk, ionic):

@@ -145,6 +145,7 @@ contains
     inele%coef(1) = s%p%ele_coef
     inele%i_sopsc_consec_res = 1
     inele%n_charge_change = 0
+    inele%ewld_dipole = -1
     beta_hbond15 = s%p%beta_hb
 
     ! charges: every bead with a charge value, in bead order
@@ -263,6 +264,15 @@ contains
     if (allocated(lele)) deallocate(lele, iele2mp, coef_ele)
     allocate(lele(1), iele2mp(4, max(MXMPELE*ncharge,1), 1), coef_ele(max(MXMPELE*ncharge,1), 1))
     lele(:) = 0
+    ! Ewald tables (mloop_ewld.F90), cubic box
+    if (s%p%ele_form == 2) then
+       inele%ewld_alpha = s%p%ewld_alpha
+       inele%ewld_hmax = real(s%p%ewld_hmax, PREC)
+       if (allocated(ewld_h)) deallocate(ewld_h)
+       if (allocated(ewld_f_coef)) deallocate(ewld_f_coef)
+       if (allocated(ewld_f_rlv)) deallocate(ewld_f_rlv)
+       call mloop_ewld()
+    end if
     if (allocated(lmp2con)) deallocate(lmp2con, lmp2LJ, lmp2wca)
     allocate(lmp2con(n), lmp2LJ(n), lmp2wca(n))
     lmp2con(:) = 0; lmp2LJ(:) = 0; lmp2wca(:) = 0
@@ -353,7 +363,7 @@ contains
 
   ! One force term into f(3,N) (zeroed first).  term: 0 bond, 1 angle,
   ! 2 tstack+stack (st_status from the tstack pass), 3 stack only (all
-  ! st_status true), 4 hbond15, 5 exv, 6 DH.
+  ! st_status true), 4 hbond15, 5 exv, 6 DH, 7 plain Coulomb, 8 Ewald.
   subroutine tisref_force_term(term, f) bind(C, name="tisref_force_term")
     integer(c_int32_t), value :: term
     real(c_double), intent(out) :: f(3, nmp_all)
@@ -377,6 +387,10 @@ contains
        call force_exv_dt15(1, f)
     case (6)
        call force_ele_DH(1, f)
+    case (7)
+       call force_ele_coulomb(1, f)
+    case (8)
+       call force_ele_coulomb_ewld(1, f)
     end select
   end subroutine tisref_force_term
 
@@ -411,6 +425,10 @@ contains
     call energy_exv_dt15(1, eu, e)
     if (inmisc%force_flag(INTERACT%ELE) .and. inele%i_function_form == 0) then
        call energy_ele_DH(1, e, eu)
+    else if (inmisc%force_flag(INTERACT%ELE) .and. inele%i_function_form == 1) then
+       call energy_ele_coulomb(1, e, eu)
+    else if (inmisc%force_flag(INTERACT%ELE) .and. inele%i_function_form == 2) then
+       call energy_ele_coulomb_ewld(1, e, eu)
     end if
     if (nvelo > 0) then
        do i = 1, nmp_all

@@ -343,6 +343,167 @@ int32_t oracle_force_dh(const tis_system *s, const double *pxyz, const int32_t *
 static inline double dot3(const double a[3], const double b[3]) {
   return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
 }
+
+/* ---- plain Coulomb and Ewald ---- */
+
+/* force_ele_coulomb.F90:63-84: dv = coef_ij / r^3 inside cutoff_ele, no clamp */
+void oracle_force_coulomb(const tis_system *s, const double *pxyz, const int32_t *pairs,
+                          const double *coef, int64_t n, double *f) {
+  const double cutoff2 = s->p.cutoff_ele * s->p.cutoff_ele;
+  for (int64_t k = 0; k < n; ++k) {
+    int i = pairs[3 * k], j = pairs[3 * k + 1];
+    double sh[3], v[3];
+    oracle_mirror_shift(s, pairs[3 * k + 2], sh);
+    for (int d = 0; d < 3; ++d) v[d] = pxyz[3 * j + d] - pxyz[3 * i + d] + sh[d];
+    double d2 = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
+    if (d2 > cutoff2) continue;
+    double d1 = sqrt(d2);
+    double dv = coef[k] / (d1 * d2);
+    for (int d = 0; d < 3; ++d) {
+      double fd = dv * v[d];
+      f[3 * i + d] -= fd;
+      f[3 * j + d] += fd;
+    }
+  }
+}
+
+/* energy_ele_coulomb.F90:47-80: coef_ij (1/r - 1/r_c) */
+double oracle_energy_coulomb(const tis_system *s, const double *pxyz, const int32_t *pairs,
+                             const double *coef, int64_t n) {
+  const double cutoff2 = s->p.cutoff_ele * s->p.cutoff_ele, cut_inv = 1.0 / s->p.cutoff_ele;
+  double e = 0.0;
+  for (int64_t k = 0; k < n; ++k) {
+    int i = pairs[3 * k], j = pairs[3 * k + 1];
+    double sh[3], v[3];
+    oracle_mirror_shift(s, pairs[3 * k + 2], sh);
+    for (int d = 0; d < 3; ++d) v[d] = pxyz[3 * j + d] - pxyz[3 * i + d] + sh[d];
+    double d2 = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
+    if (d2 > cutoff2) continue;
+    e += coef[k] * (1.0 / sqrt(d2) - cut_inv);
+  }
+  return e;
+}
+
+/* mloop_ewld.F90:52-136: integer vectors h != 0 with |h|^2 <= hmax^2, ordered
+ * by |h|^2 (ties in generation order: the reference's comb sort orders ties
+ * differently, which changes only the FP64 summation order), coefficient
+ * f(h) = 2 pi / L^3 exp(-k^2 / 4 alpha^2) / k^2 with k = 2 pi h / L and L the
+ * first box edge (cubic box).  Returns the count (h, coef may be NULL). */
+int32_t oracle_ewald_kvec(const tis_system *s, int32_t *h, double *coef) {
+  const int hmax = s->p.ewld_hmax, lmax = hmax;
+  const double hmax2 = (double)hmax * (double)hmax;
+  int32_t nk = 0;
+  for (int r2 = 1; r2 <= hmax * hmax; ++r2)
+    for (int ix = -lmax; ix <= lmax; ++ix)
+      for (int iy = -lmax; iy <= lmax; ++iy)
+        for (int iz = -lmax; iz <= lmax; ++iz) {
+          if (ix * ix + iy * iy + iz * iz != r2 || (double)r2 > hmax2) continue;
+          if (h) { h[3 * nk] = ix; h[3 * nk + 1] = iy; h[3 * nk + 2] = iz; }
+          if (coef) {
+            const double L = s->p.box[0], a = s->p.ewld_alpha;
+            const double factor = 2.0 * F_PI / L;
+            coef[nk] = 2.0 * F_PI / (L * L * L) * exp(-factor * factor * (double)r2 / (4.0 * a * a)) /
+                       (factor * factor * (double)r2);
+          }
+          ++nk;
+        }
+  return nk;
+}
+
+/* force_ele_coulomb_ewld.F90:56-137.  Reciprocal part, per h:
+ * S = sum_q q (cos, sin)(k.r_q); F_q += C 2 f(h) (q sin_q Sc - q cos_q Ss) k.
+ * Real part over the list: dv = coef_ij (erfc(a r)/r + 2a/sqrt(pi)
+ * exp(-a^2 r^2)) / r^2 inside cutoff_ele, no clamp. */
+void oracle_force_ewald(const tis_system *s, const double *pxyz, const int32_t *pairs,
+                        const double *coef, int64_t n, double *f) {
+  const int32_t nk = oracle_ewald_kvec(s, NULL, NULL);
+  int32_t *h = (int32_t *)malloc(sizeof(int32_t) * 3 * (size_t)(nk + 1));
+  double *fk = (double *)malloc(sizeof(double) * (size_t)(nk + 1));
+  oracle_ewald_kvec(s, h, fk);
+  const double factor = 2.0 * F_PI / s->p.box[0], C = s->p.ele_coef;
+  int nq = 0;
+  for (int i = 0; i < s->nmp; ++i) if (fabs(s->charge[i]) >= ZERO_JUDGE) ++nq;
+  int *qi = (int *)malloc(sizeof(int) * (size_t)(nq + 1));
+  double *qc = (double *)malloc(sizeof(double) * (size_t)(nq + 1));
+  double *qs = (double *)malloc(sizeof(double) * (size_t)(nq + 1));
+  nq = 0;
+  for (int i = 0; i < s->nmp; ++i) if (fabs(s->charge[i]) >= ZERO_JUDGE) qi[nq++] = i;
+  for (int g = 0; g < nk; ++g) {
+    const double kv[3] = {factor * h[3 * g], factor * h[3 * g + 1], factor * h[3 * g + 2]};
+    double scos = 0.0, ssin = 0.0;
+    for (int q = 0; q < nq; ++q) {
+      const double *r = pxyz + 3 * qi[q];
+      double dp = (kv[0] * r[0] + kv[1] * r[1]) + kv[2] * r[2];
+      double qq = s->charge[qi[q]];
+      qc[q] = qq * cos(dp); qs[q] = qq * sin(dp);
+    }
+    for (int q = 0; q < nq; ++q) { scos += qc[q]; ssin += qs[q]; }
+    for (int q = 0; q < nq; ++q) {
+      double dv = C * 2.0 * fk[g] * (qs[q] * scos - qc[q] * ssin);
+      for (int d = 0; d < 3; ++d) f[3 * qi[q] + d] += dv * kv[d];
+    }
+  }
+  free(h); free(fk); free(qi); free(qc); free(qs);
+  const double a = s->p.ewld_alpha, a2 = a * a, beta = 2.0 * a / sqrt(F_PI);
+  const double cutoff2 = s->p.cutoff_ele * s->p.cutoff_ele;
+  for (int64_t k = 0; k < n; ++k) {
+    int i = pairs[3 * k], j = pairs[3 * k + 1];
+    double sh[3], v[3];
+    oracle_mirror_shift(s, pairs[3 * k + 2], sh);
+    for (int d = 0; d < 3; ++d) v[d] = pxyz[3 * j + d] - pxyz[3 * i + d] + sh[d];
+    double d2 = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
+    if (d2 > cutoff2) continue;
+    double d1 = sqrt(d2);
+    double dv = coef[k] * (erfc(a * d1) / d1 + beta * exp(-a2 * d2)) / d2;
+    for (int d = 0; d < 3; ++d) {
+      double fd = dv * v[d];
+      f[3 * i + d] -= fd;
+      f[3 * j + d] += fd;
+    }
+  }
+}
+
+/* energy_ele_coulomb_ewld.F90:56-212 (no dipole term): Fourier
+ * sum_h C f(h) (Sc^2 + Ss^2) + real sum coef_ij erfc(a r)/r + self
+ * C (-a/sqrt(pi)) sum_q q^2 (mloop_ewld.F90:146-154). */
+double oracle_energy_ewald(const tis_system *s, const double *pxyz, const int32_t *pairs,
+                           const double *coef, int64_t n, double parts[3]) {
+  const int32_t nk = oracle_ewald_kvec(s, NULL, NULL);
+  int32_t *h = (int32_t *)malloc(sizeof(int32_t) * 3 * (size_t)(nk + 1));
+  double *fk = (double *)malloc(sizeof(double) * (size_t)(nk + 1));
+  oracle_ewald_kvec(s, h, fk);
+  const double factor = 2.0 * F_PI / s->p.box[0], C = s->p.ele_coef;
+  double efour = 0.0, ereal = 0.0, eself = 0.0, q2 = 0.0;
+  for (int g = 0; g < nk; ++g) {
+    const double kv[3] = {factor * h[3 * g], factor * h[3 * g + 1], factor * h[3 * g + 2]};
+    double scos = 0.0, ssin = 0.0;
+    for (int i = 0; i < s->nmp; ++i) {
+      if (fabs(s->charge[i]) < ZERO_JUDGE) continue;
+      const double *r = pxyz + 3 * i;
+      double dp = (kv[0] * r[0] + kv[1] * r[1]) + kv[2] * r[2];
+      scos += s->charge[i] * cos(dp);
+      ssin += s->charge[i] * sin(dp);
+    }
+    efour += C * fk[g] * (scos * scos + ssin * ssin);
+  }
+  free(h); free(fk);
+  const double a = s->p.ewld_alpha, cutoff2 = s->p.cutoff_ele * s->p.cutoff_ele;
+  for (int64_t k = 0; k < n; ++k) {
+    int i = pairs[3 * k], j = pairs[3 * k + 1];
+    double sh[3], v[3];
+    oracle_mirror_shift(s, pairs[3 * k + 2], sh);
+    for (int d = 0; d < 3; ++d) v[d] = pxyz[3 * j + d] - pxyz[3 * i + d] + sh[d];
+    double d2 = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
+    if (d2 > cutoff2) continue;
+    double d1 = sqrt(d2);
+    ereal += coef[k] * erfc(a * d1) / d1;
+  }
+  for (int i = 0; i < s->nmp; ++i)
+    if (fabs(s->charge[i]) >= ZERO_JUDGE) q2 += s->charge[i] * s->charge[i];
+  eself = C * (-a / sqrt(F_PI)) * q2;
+  if (parts) { parts[0] = efour; parts[1] = ereal; parts[2] = eself; }
+  return (efour + ereal) + eself;
+}
 static inline void cross3(const double a[3], const double b[3], double c[3]) {
   c[0] = a[1] * b[2] - a[2] * b[1];
   c[1] = a[2] * b[0] - a[0] * b[2];
@@ -960,6 +1121,10 @@ int32_t oracle_force_sumup(oracle_ws *w, const double *xyz, double *f,
   nclamp += oracle_force_exv(s, px, w->exv, w->nexv, f);
   if (s->p.ele_form == TIS_ELE_DH)
     nclamp += oracle_force_dh(s, px, w->ele, w->ele_coef, w->nele, f);
+  else if (s->p.ele_form == TIS_ELE_COULOMB)
+    oracle_force_coulomb(s, px, w->ele, w->ele_coef, w->nele, f);
+  else if (s->p.ele_form == TIS_ELE_EWALD)
+    oracle_force_ewald(s, px, w->ele, w->ele_coef, w->nele, f);
   return nclamp;
 }
 
@@ -985,6 +1150,10 @@ void oracle_energy_sumup(oracle_ws *w, const double *xyz, const double *velo, do
   energy[TIS_E_EXV] = oracle_energy_exv(s, px, w->exv, w->nexv);
   if (s->p.ele_form == TIS_ELE_DH)
     energy[TIS_E_ELE] = oracle_energy_dh(s, px, w->ele, w->ele_coef, w->nele);
+  else if (s->p.ele_form == TIS_ELE_COULOMB)
+    energy[TIS_E_ELE] = oracle_energy_coulomb(s, px, w->ele, w->ele_coef, w->nele);
+  else if (s->p.ele_form == TIS_ELE_EWALD)
+    energy[TIS_E_ELE] = oracle_energy_ewald(s, px, w->ele, w->ele_coef, w->nele, NULL);
   double tot = 0.0;
   for (int k = 0; k < TIS_E_MAX; ++k)
     if (k != TIS_E_TOTAL && k != TIS_E_VELO) tot += energy[k];

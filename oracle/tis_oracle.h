@@ -78,6 +78,20 @@ double oracle_energy_stack(const tis_system *s, const double *xyz, const uint8_t
 double oracle_energy_tstack(const tis_system *s, const double *xyz, const double *pxyz);
 double oracle_energy_kinetic(const tis_system *s, const double *velo);
 
+/* ---- plain Coulomb (force_ele_coulomb.F90, energy_ele_coulomb.F90) and
+ *      Ewald (mloop_ewld.F90:52-154, force_ele_coulomb_ewld.F90:56-137,
+ *      energy_ele_coulomb_ewld.F90:56-212 without the dipole term) ---- */
+void oracle_force_coulomb(const tis_system *s, const double *pxyz, const int32_t *pairs,
+                          const double *coef, int64_t n, double *f);
+double oracle_energy_coulomb(const tis_system *s, const double *pxyz, const int32_t *pairs,
+                             const double *coef, int64_t n);
+int32_t oracle_ewald_kvec(const tis_system *s, int32_t *h, double *coef);
+void oracle_force_ewald(const tis_system *s, const double *pxyz, const int32_t *pairs,
+                        const double *coef, int64_t n, double *f);
+/* parts (may be NULL): Fourier, real, self */
+double oracle_energy_ewald(const tis_system *s, const double *pxyz, const int32_t *pairs,
+                           const double *coef, int64_t n, double parts[3]);
+
 /* ---- full force_sumup / energy_sumup for one system (DT15, DH or none) ----
  * Lists are rebuilt from xyz first (neighbor.F90), as the parity harness
  * does.  energy[TIS_E_MAX]. */

@@ -2,6 +2,7 @@
 import numpy as np
 
 from tis_amd import topology as T
+from tis_amd.sysdesc import ELE_COULOMB, ELE_EWALD
 
 
 def hairpin64(seed=0, coords="helix"):
@@ -33,11 +34,27 @@ def hairpin_design(seed, n):
     return s, x
 
 
-def parity_cases():
+def small_ions_box(seed=5, n=20, box=60.0, alpha=0.2, rcut=25.0, hmax=6, form=ELE_EWALD):
+    """A 20-nt chain (phosphate charge -1) with 24 ions in a 60 A periodic
+    cube: Ewald (alpha 0.2, r_c 25 A, |h| <= 6) or plain Coulomb."""
+    rng = np.random.default_rng(seed)
+    seq = T.random_sequence(rng, n)
+    ions = ["Mg"] * 2 + ["K"] * 12 + ["Cl"] * 10
+    s = T.build_rna_system([(seq, [])], ions=ions, box=(box, box, box), ele_form=form, cutoff_ele=rcut,
+                           ewld_alpha=alpha, ewld_hmax=hmax, phos_charge=-1.0, name="ions_box")
+    x = T.compact_coords(s, rng, box=box * 0.9, min_dist=3.0)
+    return s, x
+
+
+def parity_cases(ewald=True):
     out = [("hairpin64_helix",) + hairpin64(0, "helix"),
            ("hairpin64_compact",) + hairpin64(3, "compact")]
     for k, kw in enumerate([{}, {"cutoff_ele": 2.0}, {"ions": True}, {"ions": True, "periodic": True}]):
         out.append((f"compact{k}",) + compact_allpairs(seed=10 + k, **kw))
+    if ewald:
+        out.append(("ions_box_ewald",) + small_ions_box())
+        out.append(("ions_box_coulomb",) + small_ions_box(form=ELE_COULOMB))
+        out.append(("c2_ewald_565ions",) + T.config2_ewald())
     return out
 
 

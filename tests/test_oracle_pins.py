@@ -21,7 +21,7 @@ from tis_amd.sysdesc import (E_BANGLE, E_BOND, E_ELE, E_EXV, E_HBOND, E_STACK, E
 
 from systems import compact_allpairs, hairpin64, parity_cases
 
-CASES = parity_cases()
+CASES = parity_cases(ewald=False)
 needs_ref = pytest.mark.skipif(not RefLib.available(), reason="oracle/_ref not built (reference absent)")
 
 

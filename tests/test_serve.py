@@ -1,0 +1,146 @@
+"""Continuous-batching request loop: scheduling logic on the CPU (with a
+recording stand-in for the device batch), and on the GPU the property that a
+job's result is the same alone and in the mix, whatever its admission step
+and batch neighbours."""
+import numpy as np
+import pytest
+
+from tis_amd import topology as T
+from tis_amd.serve import Job, Server, estimate_bytes, poisson_stream
+
+
+def _chain(n, seed):
+    rng = np.random.default_rng(seed)
+    stem = max(2, min(6, (n - 5) // 2))
+    seq, pairs = T.hairpin_design(rng, n, stem, 3 if n < 30 else 5)
+    s = T.build_rna_system([(seq, pairs)], tempk=310.0, ionic_strength=0.150, name=f"job{seed}")
+    return s, T.helix_coords(s, rng)
+
+
+class RecordingBatch:
+    """Stands in for Engine in the CPU tests: every step adds (key + 1) * 1e-3
+    to each coordinate of a system, so a job's final coordinates tell how
+    many steps it ran, and whether its state survived re-batching."""
+
+    def __init__(self):
+        self.uploads = []
+        self.sys, self.keys = [], []
+
+    def upload(self, systems, keys=None, seed=1):
+        self.sys, self.keys = list(systems), list(keys)
+        self.x = [np.zeros((s.nmp, 3)) for s in systems]
+        self.v = [np.zeros((s.nmp, 3)) for s in systems]
+        self.a = [np.zeros((s.nmp, 3)) for s in systems]
+        self.origin = [None] * len(systems)
+        self.uploads.append(list(keys))
+
+    def set_state(self, k, which, arr):
+        [self.x, self.v, self.a][which][k] = np.array(arr)
+
+    def get_state(self, k, which):
+        return np.array([self.x, self.v, self.a][which][k])
+
+    def system_origin(self, k, origin, init_accel):
+        self.origin[k] = (origin, init_accel)
+
+    def langevin_init(self, **kw):
+        assert kw["init_accel"] is None
+        self.step0 = kw["step0"]
+
+    def device_bytes(self):
+        return sum(estimate_bytes(s) for s in self.sys)
+
+    def langevin_steps(self, n, mode):
+        for k, key in enumerate(self.keys):
+            self.x[k] = self.x[k] + n * (key + 1) * 1e-3
+
+
+def _jobs(nsteps, arrivals, sizes):
+    out = []
+    for k, (ns, t, n) in enumerate(zip(nsteps, arrivals, sizes)):
+        s, x = _chain(n, 100 + k)
+        out.append(Job(key=k, system=s, x0=x, nsteps=ns, arrival=t))
+    return out
+
+
+def test_schedule_step_clock():
+    jobs = _jobs([60, 120, 40, 100, 80], [0, 0, 20, 40, 45], [24, 40, 31, 64, 20])
+    rb = RecordingBatch()
+    srv = Server(round_steps=20, clock="steps", engine=rb)
+    st = srv.run(jobs)
+    assert st.jobs == 5 and len(srv.done) == 5
+    for j in jobs:
+        assert j.steps_done == j.nsteps
+        np.testing.assert_allclose(j.x, j.x0 + j.nsteps * (j.key + 1) * 1e-3, rtol=0, atol=1e-12)
+        # admitted at the first round boundary at or after arrival
+        assert j.origin == 20 * int(np.ceil(j.arrival / 20))
+        assert j.finished >= j.origin + j.nsteps
+    # jobs 3 (arrival 40) and 4 (arrival 45, admitted at 60) change the batch
+    assert [3] in [[k for k in u if k == 3] for u in rb.uploads]
+    assert st.max_inflight >= 3
+
+
+def test_budget_admission_fifo():
+    jobs = _jobs([40, 40, 40, 40], [0, 0, 0, 0], [64, 64, 64, 64])
+    est = estimate_bytes(jobs[0].system)
+    rb = RecordingBatch()
+    srv = Server(hbm_budget=2 * est + est // 2, round_steps=20, clock="steps", engine=rb)
+    st = srv.run(jobs)
+    assert st.jobs == 4
+    assert max(len(u) for u in rb.uploads) == 2          # never more than the budget allows
+    assert [j.origin for j in jobs] == [0, 0, 40, 40]    # first come, first served
+    for j in jobs:
+        np.testing.assert_allclose(j.x, j.x0 + 40 * (j.key + 1) * 1e-3, rtol=0, atol=1e-12)
+
+
+def test_submit_rejects():
+    (j,) = _jobs([35], [0], [20])
+    srv = Server(round_steps=20, clock="steps", engine=RecordingBatch())
+    with pytest.raises(ValueError):
+        srv.submit(j)
+    (j,) = _jobs([40], [0], [20])
+    srv = Server(hbm_budget=1000, round_steps=20, clock="steps", engine=RecordingBatch())
+    with pytest.raises(ValueError):
+        srv.submit(j)
+    with pytest.raises(ValueError):
+        Server(round_steps=25, rebuild_every=10)
+
+
+def test_poisson_stream_and_latency_stats():
+    rng = np.random.default_rng(3)
+    sysx = [_chain(20, 7)]
+    jobs = poisson_stream(rng, 200, 4.0, lambda k, r: Job(key=k, system=sysx[0][0], x0=sysx[0][1], nsteps=10))
+    gaps = np.diff([0.0] + [j.arrival for j in jobs])
+    assert abs(gaps.mean() - 0.25) < 0.05
+    srv = Server(round_steps=10, clock="wall", engine=RecordingBatch())
+    st = srv.run(jobs[:30])
+    s = st.summary()
+    assert s["jobs"] == 30 and s["p99_latency"] >= s["p50_latency"] >= 0.0
+
+
+@pytest.mark.gpu
+def test_alone_equals_mix_gpu():
+    """Each job's final x, v, a in the mix are bitwise those of the job run
+    alone, for two schedules (all admitted vs budget-limited queueing)."""
+    from tis_amd.engine import Engine, MODE_GRAPH
+    seed = 77
+    spec = ([60, 120, 40, 100, 80], [0, 0, 20, 40, 45], [24, 40, 31, 64, 20])
+    alone = []
+    for j in _jobs(*spec):
+        e = Engine([j.system], keys=[j.key], seed=seed)
+        e.set_coords(0, j.x0)
+        e.langevin_init(tstep=0.05, rebuild_every=10, step0=0, init_accel=True)
+        e.langevin_steps(j.nsteps, MODE_GRAPH)
+        alone.append([e.get_state(0, w) for w in range(3)])
+        e.close()
+    for budget in (1 << 34, None):
+        jobs = _jobs(*spec)
+        if budget is None:
+            budget = 2 * max(estimate_bytes(j.system) for j in jobs)
+        srv = Server(hbm_budget=budget, round_steps=20, clock="steps", seed=seed)
+        srv.run(jobs)
+        for j, ref in zip(jobs, alone):
+            assert j.steps_done == j.nsteps
+            for got, want in zip((j.x, j.v, j.a), ref):
+                assert np.array_equal(got, want), f"job {j.key} differs (budget {budget})"
+        srv._eng.close()

@@ -18,7 +18,9 @@
 
 namespace tis {
 __global__ void k_terms(DevPtrs, const double *, const double *, const int32_t *, int32_t);
-__global__ void k_dh(DevPtrs, const double *, const double *, const Tile *);
+__global__ void k_ele(DevPtrs, const double *, const double *, const Tile *);
+__global__ void k_ewald_sk(DevPtrs, const double *, const EwTask *, int);
+__global__ void k_ewald_f(DevPtrs, const double *, const EwTask *);
 __global__ void k_bonded(DevPtrs, const double *, int, int, int, int, int, int);
 __global__ void k_bbox(DevPtrs, const double *, const double *, const Tile *, int, int, const Tile *, int);
 __global__ void k_main(DevPtrs, const double *, const double *, double *, double *, const Tile *, int, int32_t, int, int);
@@ -42,6 +44,8 @@ struct HostSys {
   int32_t bt_begin = 0, bt_count = 0;   // bead tiles
   int32_t qt_begin = 0, qt_count = 0;   // charge tiles
   bool has_task = false;                // hbonds or tertiary stacks
+  int32_t esk_begin = 0, esk_count = 0;  // Ewald structure-factor tasks
+  int32_t ef_begin = 0, ef_count = 0;    // Ewald force tasks
   std::vector<double> mass, fric;
   std::vector<int32_t> stack_ids;       // global stack indices
 };
@@ -62,6 +66,7 @@ struct tis_ctx {
   int32_t nsys = 0, ntot = 0, nqtot = 0, nhbtot = 0, ntsttot = 0, nsttot = 0;
   std::vector<HostSys> hs;
   std::vector<void *> allocs;
+  size_t dev_bytes = 0;                  // bytes held in allocs
   DevPtrs P{};
   DSys *d_sys = nullptr;
   Tile *d_btiles = nullptr, *d_qtiles = nullptr;
@@ -72,7 +77,9 @@ struct tis_ctx {
   double *d_energy = nullptr;
   unsigned long long *d_counters = nullptr;
   uint64_t *d_stepbase = nullptr;
-  bool any_dh = false, any_periodic = false;
+  bool any_ele = false, any_periodic = false;
+  EwTask *d_ewsk = nullptr, *d_ewf = nullptr;
+  int32_t n_ewsk = 0, n_ewf = 0;
   // state
   int cur = 0;
   bool lists_valid = false;
@@ -87,7 +94,7 @@ struct tis_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_total_ms = 0.0, last_main_ms = 0.0, last_pre_ms = 0.0;
   int64_t last_nlaunch = 0;
-  // per-kernel events of eager runs: (before k_terms/k_dh, between, after k_main)
+  // per-kernel events of eager runs: (before k_terms, before k_main, after k_main)
   bool timing = false;
   std::vector<hipEvent_t> tev;
   size_t tev_used = 0;
@@ -119,6 +126,7 @@ int dalloc(tis_ctx *c, T **p, size_t n) {
   hipError_t e = hipMalloc((void **)p, bytes);
   if (e != hipSuccess) return fail(c, TIS_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
   c->allocs.push_back((void *)*p);
+  c->dev_bytes += bytes;
   e = hipMemsetAsync(*p, 0, bytes, c->stream);
   if (e != hipSuccess) return fail(c, TIS_ERR_HIP, "hipMemsetAsync failed: %s", hipGetErrorString(e));
   return 0;
@@ -138,6 +146,7 @@ void free_all(tis_ctx *c) {
     if (c->graph[k]) { (void)hipGraphExecDestroy(c->graph[k]); c->graph[k] = nullptr; }
   for (void *p : c->allocs) (void)hipFree(p);
   c->allocs.clear();
+  c->dev_bytes = 0;
   c->hs.clear();
   c->P = DevPtrs{};
   c->nsys = c->ntot = 0;
@@ -165,6 +174,39 @@ void fill_derived(DSys &d, const tis_system &s) {
   d.dh_ecut2 = p.dh_cutoff_type == 0 ? c * c : p.cutoff_ele * p.cutoff_ele;  // energy_ele_DH.F90:33-37
   d.beta_hb = p.beta_hb;
   d.tempk = s.tempk;
+  d.cl_cut2 = p.cutoff_ele * p.cutoff_ele;
+  d.cl_rcinv = p.cutoff_ele > 0.0 ? 1.0 / p.cutoff_ele : 0.0;
+  if (p.ele_form == TIS_ELE_EWALD) {
+    const double a = p.ewld_alpha, L = p.box[0];
+    d.ew_alpha = a;
+    d.ew_beta = 2.0 * a / std::sqrt(kPi);
+    d.ew_a2 = a * a;
+    d.ew_invL = 1.0 / L;
+    d.ew_fscale = 2.0 * p.ele_coef * (2.0 * kPi / L);
+    // self term, mloop_ewld.F90:146-154 (charges of this system)
+    double q2 = 0.0;
+    for (int i = 0; i < s.nmp; ++i)
+      if (std::fabs(s.charge[i]) >= 1.0e-6) q2 += s.charge[i] * s.charge[i];
+    d.ew_self = p.ele_coef * (-a / std::sqrt(kPi)) * q2;
+  }
+}
+
+// Reciprocal vectors of mloop_ewld.F90:52-136: integer h != 0 with
+// |h|^2 <= hmax^2 ordered by |h|^2 (ties in generation order), and
+// f(h) = 2 pi / L^3 exp(-k^2 / 4 alpha^2) / k^2, k = 2 pi h / L.
+void ewald_kvectors(const tis_params &p, std::vector<double> &hx, std::vector<double> &hy,
+                    std::vector<double> &hz, std::vector<double> &coef) {
+  const int hmax = p.ewld_hmax;
+  const double L = p.box[0], a = p.ewld_alpha, factor = 2.0 * kPi / L;
+  for (int r2 = 1; r2 <= hmax * hmax; ++r2)
+    for (int ix = -hmax; ix <= hmax; ++ix)
+      for (int iy = -hmax; iy <= hmax; ++iy)
+        for (int iz = -hmax; iz <= hmax; ++iz) {
+          if (ix * ix + iy * iy + iz * iz != r2) continue;
+          hx.push_back(ix); hy.push_back(iy); hz.push_back(iz);
+          coef.push_back(2.0 * kPi / (L * L * L) * std::exp(-factor * factor * (double)r2 / (4.0 * a * a)) /
+                         (factor * factor * (double)r2));
+        }
 }
 
 // Langevin constants, time_integral_pre.F90:160-170
@@ -187,7 +229,12 @@ double *Xc(tis_ctx *c) { return c->P.x[c->cur]; }
 double *PXc(tis_ctx *c) { return c->P.px[c->cur]; }
 
 // --- launch helpers on a system subset: isys = -1 all, else one system
-struct Range { int bt0, btn, qt0, qtn, task0, taskn; const int32_t *sysidx; int nsel; int b0, nb, a0, na, s0, ns; };
+struct Range {
+  int bt0, btn, qt0, qtn, task0, taskn;
+  const int32_t *sysidx; int nsel;
+  int b0, nb, a0, na, s0, ns;
+  int ek0, ekn, ef0, efn;   // Ewald tasks
+};
 
 Range range_of(tis_ctx *c, int32_t isys) {
   Range r{};
@@ -195,6 +242,7 @@ Range range_of(tis_ctx *c, int32_t isys) {
     r.bt0 = 0; r.btn = c->nbtile; r.qt0 = 0; r.qtn = c->nqtile; r.task0 = 0; r.taskn = c->nsystask;
     r.sysidx = c->d_sysidx; r.nsel = c->nsys;
     r.b0 = 0; r.nb = c->P.nbd; r.a0 = 0; r.na = c->P.nba; r.s0 = 0; r.ns = c->P.nst;
+    r.ek0 = 0; r.ekn = c->n_ewsk; r.ef0 = 0; r.efn = c->n_ewf;
   } else {
     const HostSys &h = c->hs[isys];
     r.bt0 = h.bt_begin; r.btn = h.bt_count; r.qt0 = h.qt_begin; r.qtn = h.qt_count;
@@ -202,6 +250,7 @@ Range range_of(tis_ctx *c, int32_t isys) {
     r.taskn = c->systask_of[isys] < 0 ? 0 : 1;
     r.sysidx = c->d_sysidx + isys; r.nsel = 1;
     r.b0 = h.d.boff; r.nb = h.d.nbd; r.a0 = h.d.aoff; r.na = h.d.nba; r.s0 = h.d.soff; r.ns = h.d.nst;
+    r.ek0 = h.esk_begin; r.ekn = h.esk_count; r.ef0 = h.ef_begin; r.efn = h.ef_count;
   }
   return r;
 }
@@ -271,6 +320,9 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
   std::vector<double> bd_nat, bd_coef, ba_nat, ba_coef, st_nat, st_coef, tst_nat, tst_coef, hb_nat, hb_coef;
   std::vector<Tile> btiles, qtiles;
   std::vector<int32_t> systask;
+  std::vector<double> ewhx, ewhy, ewhz, ewc;
+  std::vector<EwTask> ewsk, ewf;
+  int64_t psk_off = 0, pf_off = 0;
   // excluded-volume species: distinct (mptype, class, radius, epsilon)
   struct Spec { int8_t t, c; double r, e; };
   std::vector<Spec> specs;
@@ -284,7 +336,7 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
   c->systask_of.assign(nsys, -1);
   int64_t moff = 0, mioff = 0;
   int32_t off = 0, qoff = 0, nbd = 0, nba = 0, nst = 0, ntst = 0, nhb = 0, nsite = 0, uoff = 0;
-  c->any_dh = false;
+  c->any_ele = false;
   c->any_periodic = false;
   for (int is = 0; is < nsys; ++is) {
     const tis_system &s = sys[is];
@@ -303,7 +355,16 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
     }
     uoff += s.nunit * s.nunit;
     if (s.p.periodic) c->any_periodic = true;
-    if (s.p.ele_form == TIS_ELE_DH) c->any_dh = true;
+    if (s.p.ele_form >= 0) c->any_ele = true;
+    if (s.p.ele_form == TIS_ELE_EWALD) {
+      if (!s.p.periodic || s.p.box[0] != s.p.box[1] || s.p.box[0] != s.p.box[2])
+        return fail(c, TIS_ERR_ARG, "system %d: Ewald needs a periodic cubic box", is);
+      if (!(s.p.ewld_alpha > 0.0) || s.p.ewld_hmax <= 0)
+        return fail(c, TIS_ERR_ARG, "system %d: Ewald needs ewld_alpha > 0 and ewld_hmax > 0", is);
+    }
+    if (s.p.ele_form != TIS_ELE_NONE && s.p.ele_form != TIS_ELE_DH && s.p.ele_form != TIS_ELE_COULOMB &&
+        s.p.ele_form != TIS_ELE_EWALD)
+      return fail(c, TIS_ERR_ARG, "system %d: unknown ele_form %d", is, s.p.ele_form);
     // beads
     d.qoff = qoff;
     int nq = 0;
@@ -409,6 +470,24 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
     d.bt0 = h.bt_begin; d.nbt = h.bt_count; d.qt0 = h.qt_begin; d.nqt = h.qt_count;
     h.has_task = (s.nhb > 0 || s.ntst > 0);
     if (h.has_task) { c->systask_of[is] = (int32_t)systask.size(); systask.push_back(is); }
+    // Ewald reciprocal vectors and work items
+    if (s.p.ele_form == TIS_ELE_EWALD && nq > 0) {
+      d.ewk0 = (int32_t)ewc.size();
+      ewald_kvectors(s.p, ewhx, ewhy, ewhz, ewc);
+      d.nk = (int32_t)ewc.size() - d.ewk0;
+      d.nqc = (nq + kTile - 1) / kTile;
+      d.nkc = (d.nk + kEwKChunk - 1) / kEwKChunk;
+      d.psk_off = psk_off; psk_off += 2 * (int64_t)d.nqc * d.nk;
+      d.pf_off = pf_off; pf_off += 3 * (int64_t)d.nkc * nq;
+      h.esk_begin = (int32_t)ewsk.size();
+      for (int kt = 0; kt < (d.nk + kTile - 1) / kTile; ++kt)
+        for (int qc = 0; qc < d.nqc; ++qc) ewsk.push_back(EwTask{is, kt, qc});
+      h.esk_count = (int32_t)ewsk.size() - h.esk_begin;
+      h.ef_begin = (int32_t)ewf.size();
+      for (int qt = 0; qt < d.nqc; ++qt)
+        for (int kc = 0; kc < d.nkc; ++kc) ewf.push_back(EwTask{is, qt, kc});
+      h.ef_count = (int32_t)ewf.size() - h.ef_begin;
+    }
     off += s.nmp;
   }
   inc_off.push_back((int32_t)inc.size());
@@ -531,6 +610,21 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
   P.seed = seed;
   UP(c->d_btiles, btiles);
   UP(c->d_qtiles, qtiles);
+  {
+    std::vector<double> h3;
+    h3.insert(h3.end(), ewhx.begin(), ewhx.end());
+    h3.insert(h3.end(), ewhy.begin(), ewhy.end());
+    h3.insert(h3.end(), ewhz.begin(), ewhz.end());
+    double *dh = nullptr, *dc = nullptr;
+    UP(dh, h3); UP(dc, ewc);
+    P.ewh = dh; P.ewc = dc; P.nktot = (int32_t)ewc.size();
+    rc = dalloc(c, &P.psk, (size_t)std::max<int64_t>(psk_off, 1)); if (rc) return rc;
+    rc = dalloc(c, &P.pf, (size_t)std::max<int64_t>(pf_off, 1)); if (rc) return rc;
+    UP(c->d_ewsk, ewsk);
+    UP(c->d_ewf, ewf);
+    c->n_ewsk = (int32_t)ewsk.size();
+    c->n_ewf = (int32_t)ewf.size();
+  }
   UP(c->d_systask, systask);
   std::vector<int32_t> sysidx(nsys);
   for (int is = 0; is < nsys; ++is) sysidx[is] = is;
@@ -711,12 +805,12 @@ int32_t tis_pairs_get(tis_ctx *c, int32_t isys, int32_t which, int32_t *pairs, d
                  : TIS_OK;
 }
 
-// k_terms, k_bonded, k_dh, then k_main.  Steps draw noise/valence numbers for step
+// k_terms, k_bonded, k_ele, the Ewald passes, then k_main.  Steps draw noise/valence numbers for step
 // stepbase+t+1; a stand-alone force evaluation or the initial acceleration
 // uses stepbase.
 static void launch_force_pair(tis_ctx *c, const Range &r, const double *X, const double *PX, double *XN,
                               double *PXN, int t_local, int mode) {
-  const int32_t pre_t = mode == 1 ? (int32_t)t_local : -1;
+  const int32_t pre_t = mode == 1 ? (int32_t)t_local : -1;  // k_terms draws at the force step
   hipEvent_t e3[3] = {nullptr, nullptr, nullptr};
   if (c->timing) {
     while (c->tev.size() < c->tev_used + 3) {
@@ -734,12 +828,16 @@ static void launch_force_pair(tis_ctx *c, const Range &r, const double *X, const
   if (nterm > 0)
     hipLaunchKernelGGL(k_bonded, dim3((nterm + 255) / 256), dim3(256), 0, c->stream, c->P, X, r.b0, r.nb, r.a0,
                        r.na, r.s0, r.ns);
-  if (r.qtn > 0 && c->any_dh)
-    hipLaunchKernelGGL(k_dh, dim3(r.qtn), dim3(kTile * kWaves), 0, c->stream, c->P, X, PX, c->d_qtiles + r.qt0);
+  if (r.qtn > 0 && c->any_ele)
+    hipLaunchKernelGGL(k_ele, dim3(r.qtn), dim3(kTile * kWaves), 0, c->stream, c->P, X, PX, c->d_qtiles + r.qt0);
+  if (r.ekn > 0) {
+    hipLaunchKernelGGL(k_ewald_sk, dim3(r.ekn), dim3(kTile), 0, c->stream, c->P, PX, c->d_ewsk + r.ek0, 0);
+    hipLaunchKernelGGL(k_ewald_f, dim3(r.efn), dim3(kTile * kWaves), 0, c->stream, c->P, PX, c->d_ewf + r.ef0);
+  }
   if (c->timing) (void)hipEventRecord(e3[1], c->stream);
   if (r.btn > 0)
     hipLaunchKernelGGL(k_main, dim3(r.btn), dim3(kTile * 2), 0, c->stream, c->P, X, PX, XN, PXN,
-                       c->d_btiles + r.bt0, r.btn, (int32_t)t_local, mode, c->any_dh ? 1 : 0);
+                       c->d_btiles + r.bt0, r.btn, (int32_t)t_local, mode, c->any_ele ? 1 : 0);
   if (c->timing) (void)hipEventRecord(e3[2], c->stream);
 }
 
@@ -770,11 +868,38 @@ int32_t tis_energy_sumup(tis_ctx *c, int32_t isys, double *energy_out) {
   if (isys < 0) return fail(c, TIS_ERR_ARG, "tis_energy_sumup needs one system");
   HIPCHK(c, hipSetDevice(c->device));
   if (!c->lists_valid) { rc = rebuild_lists(c, -1); if (rc) return rc; }
+  {
+    const HostSys &h = c->hs[isys];
+    if (h.esk_count > 0)  // structure factors at the current coordinates
+      hipLaunchKernelGGL(k_ewald_sk, dim3(h.esk_count), dim3(kTile), 0, c->stream, c->P, PXc(c),
+                         c->d_ewsk + h.esk_begin, 1);
+  }
   hipLaunchKernelGGL(k_energy, dim3(1), dim3(256), 0, c->stream, c->P, Xc(c), PXc(c), c->d_sysidx + isys,
                      c->d_energy);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(energy_out, c->d_energy, 25 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TIS_OK;
+}
+
+int32_t tis_system_origin(tis_ctx *c, int32_t isys, uint64_t step_origin, int32_t init_accel) {
+  if (!c) return TIS_ERR_ARG;
+  int rc = check_sys(c, isys);
+  if (rc) return rc;
+  if (isys < 0) return fail(c, TIS_ERR_ARG, "tis_system_origin needs one system");
+  HIPCHK(c, hipSetDevice(c->device));
+  HostSys &h = c->hs[isys];
+  h.d.step_origin = step_origin;
+  h.d.init_accel = init_accel ? 1 : 0;
+  // kernels (and captured graphs) read DSys from device memory
+  HIPCHK(c, hipMemcpyAsync(c->d_sys + isys, &h.d, sizeof(DSys), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TIS_OK;
+}
+
+int32_t tis_device_bytes(tis_ctx *c, int64_t *bytes) {
+  if (!c || !bytes) return TIS_ERR_ARG;
+  *bytes = (int64_t)c->dev_bytes;
   return TIS_OK;
 }
 
@@ -805,7 +930,10 @@ int32_t tis_langevin_init(tis_ctx *c, double tstep, int32_t rebuild_every, uint6
   if (rc) return rc;
   Range r = range_of(c, -1);
   const int cu = c->cur;
-  if (init_accel) {
+  if (init_accel < 0) {
+    // per system: new accelerations (mode 2) or kept ones (mode 3)
+    launch_force_pair(c, r, c->P.x[cu], c->P.px[cu], c->P.x[cu ^ 1], c->P.px[cu ^ 1], 0, 4);
+  } else if (init_accel) {
     // a = F(x)/m + c1 xi(step0), then x_pred (mode 2)
     launch_force_pair(c, r, c->P.x[cu], c->P.px[cu], c->P.x[cu ^ 1], c->P.px[cu ^ 1], 0, 2);
   } else {

@@ -52,6 +52,8 @@ struct DSys {
   int64_t mioff;             // periodic: ele mirror bytes (nq*nq), else -1
   int32_t periodic;
   uint32_t key;              // identity in the random streams
+  int32_t init_accel;        // mode-4 initialisation: 1 = new accelerations, 0 = keep
+  uint64_t step_origin;      // global step at which this system's step count is zero
   int32_t nsep[3];           // P, S, B
   int32_t ele_form, dh_type;
   double box[3];
@@ -59,7 +61,18 @@ struct DSys {
   double hb_cut, margin;
   double cutoff_ele, ele_coef, cdist, rcdist, ele_rl2, dh_cut2, dh_ecut2;
   double beta_hb, tempk;
+  // plain Coulomb / Ewald (ele_form 1 / 2): cutoff_ele^2, 1/cutoff_ele
+  double cl_cut2, cl_rcinv;
+  // Ewald: alpha, 2 alpha / sqrt(pi), alpha^2, 1/L, 2 C (2 pi / L), self energy
+  double ew_alpha, ew_beta, ew_a2, ew_invL, ew_fscale, ew_self;
+  int32_t ewk0, nk;          // reciprocal vectors [ewk0, ewk0 + nk)
+  int32_t nqc, nkc;          // charge chunks of the S pass, k chunks of the force pass
+  int64_t psk_off, pf_off;   // partial structure factors [nqc][nk][2], partial forces [nkc][3][nq]
 };
+
+// Ewald work item: (system, tile a, chunk b)
+struct EwTask { int32_t sys, a, b; };
+constexpr int kEwKChunk = 256;   // reciprocal vectors per block of the force pass
 
 struct Tile { int32_t sys, start; };
 
@@ -114,6 +127,12 @@ struct DevPtrs {
   double *fdh;               // [3][nq]
   uint32_t *qmask;           // bit matrix
   uint8_t *qmirror;          // periodic ele mirror codes
+  // Ewald reciprocal space
+  const double *ewh;         // [3][nktot] integer h as FP64
+  const double *ewc;         // [nktot] f(h)
+  int32_t nktot;
+  double *psk;               // partial structure factors
+  double *pf;                // partial reciprocal forces
   // counters / timing
   unsigned long long *counters;  // TIS_CNT_*
   uint64_t *stepbase;

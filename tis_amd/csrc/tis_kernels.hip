@@ -430,8 +430,8 @@ __device__ int hbond_valence_serial(const DevPtrs &P, const DSys &s, uint64_t st
 __global__ __launch_bounds__(kTile) void k_terms(DevPtrs P, const double *X, const double *PX,
                                                  const int32_t *systask, int32_t t_local) {
   const int lane = threadIdx.x;
-  const uint64_t step = *P.stepbase + (uint64_t)t_local + 1u;
   const DSys s = P.sys[systask[blockIdx.x]];
+  const uint64_t step = *P.stepbase + (uint64_t)t_local + 1u - s.step_origin;
   for (int k = lane; k < s.nst; k += kTile) P.st_status[s.soff + k] = 1;
   __syncthreads();
   for (int k = lane; k < s.ntst; k += kTile) {
@@ -480,22 +480,45 @@ __global__ __launch_bounds__(kTile) void k_terms(DevPtrs P, const double *X, con
   }
 }
 
-// ------------------------------------------------------------------ Debye-Hueckel
-// force_ele_DH.F90:68-118, owner-computes over one charge tile: each lane of
-// wave w owns a charge of the tile and sweeps the system's charge chunks
-// w, w+4, ... staged through LDS; list membership from the bit matrix,
-// cutoff test in FP64, pair arithmetic in FP32, accumulation in FP64; the
-// four partial sums are added in wave order.
-__global__ __launch_bounds__(kTile * kWaves) void k_dh(DevPtrs P, const double *X, const double *PX,
-                                                       const Tile *qtiles) {
-  __shared__ double sx[kWaves][kTile], sy[kWaves][kTile], sz[kWaves][kTile];
-  __shared__ float sq[kWaves][kTile];
-  __shared__ double red[3][kWaves][kTile];
+// ------------------------------------------------------------------ electrostatics
+// Pair magnitude dv (force on i = -dv * (x_j - x_i)) in FP32 for the three
+// forms: Debye-Hueckel (force_ele_DH.F90:100-112, clamped at DE_MAX), plain
+// Coulomb (force_ele_coulomb.F90:75-79) and the Ewald real-space term
+// (force_ele_coulomb_ewld.F90:128-132).
+template <int FORM>
+__device__ inline float ele_dv(const DSys &s, float qq, float d2, unsigned &nclamp, bool count) {
+  const float rinv = rsqrtf(d2);
+  if (FORM == 0) {
+    const float rcd = (float)s.rcdist;
+    const float d1 = d2 * rinv;
+    float dv = qq * rinv * rinv * (rinv + rcd) * __expf(-d1 * rcd);
+    if (dv > (float)kDeMax) { dv = (float)kDeMax; if (count) ++nclamp; }
+    return dv;
+  } else if (FORM == 1) {
+    return qq * rinv * rinv * rinv;
+  } else {
+    const float d1 = d2 * rinv;
+    return qq * (erfcf((float)s.ew_alpha * d1) * rinv + (float)s.ew_beta * __expf(-(float)s.ew_a2 * d2)) * rinv * rinv;
+  }
+}
+
+// Owner-computes over one charge tile: each lane of wave w owns a charge of
+// the tile and sweeps the system's charge chunks w, w+4, ... staged through
+// LDS; list membership from the bit matrix, cutoff test in FP64, pair
+// arithmetic in FP32, accumulation in FP64; the four partial sums are added
+// in wave order.
+struct EleLds {
+  double sx[kWaves][kTile], sy[kWaves][kTile], sz[kWaves][kTile];
+  float sq[kWaves][kTile];
+  double red[3][kWaves][kTile];
+};
+
+template <int FORM>
+__device__ inline void ele_tile(const DevPtrs &P, const DSys &s, const double *X, const double *PX, int qi0,
+                                EleLds &L) {
+  auto &sx = L.sx; auto &sy = L.sy; auto &sz = L.sz; auto &sq = L.sq; auto &red = L.red;
   const int lane = threadIdx.x & (kTile - 1), w = threadIdx.x / kTile;
-  const Tile tl = qtiles[blockIdx.x];
-  const DSys s = P.sys[tl.sys];
-  if (s.ele_form != 0) return;
-  const int qi = tl.start + lane;
+  const int qi = qi0 + lane;
   const bool valid = qi < s.nq;
   const double *Y = s.periodic ? PX : X;
   const int n = P.ntot;
@@ -506,8 +529,7 @@ __global__ __launch_bounds__(kTile * kWaves) void k_dh(DevPtrs P, const double *
     chi = P.charge[gi];
   }
   const float qiC = (float)(chi * s.ele_coef);
-  const float rcd = (float)s.rcdist;
-  const double cut2 = s.dh_cut2;
+  const double cut2 = FORM == 0 ? s.dh_cut2 : s.cl_cut2;
   double F0 = 0.0, F1 = 0.0, F2 = 0.0;
   unsigned nclamp = 0;
   const uint32_t *row = P.qmask + s.moff + (int64_t)(valid ? qi : 0) * s.mwords;
@@ -535,12 +557,8 @@ __global__ __launch_bounds__(kTile * kWaves) void k_dh(DevPtrs P, const double *
           const double v0 = SX[k] - xi0, v1 = SY[k] - xi1, v2 = SZ[k] - xi2;
           const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
           if (!bit || d2 > cut2) continue;
-          const float f0 = (float)v0, f1 = (float)v1, f2 = (float)v2;
-          const float rinv = rsqrtf((float)d2);
-          const float d1 = (float)d2 * rinv;
-          float dv = qiC * SQ[k] * rinv * rinv * (rinv + rcd) * __expf(-d1 * rcd);
-          if (dv > (float)kDeMax) { dv = (float)kDeMax; if (qi < jt + k) ++nclamp; }
-          F0 -= (double)(dv * f0); F1 -= (double)(dv * f1); F2 -= (double)(dv * f2);
+          const float dv = ele_dv<FORM>(s, qiC * SQ[k], (float)d2, nclamp, qi < jt + k);
+          F0 -= (double)(dv * (float)v0); F1 -= (double)(dv * (float)v1); F2 -= (double)(dv * (float)v2);
         }
       } else {
         for (int k = 0; k < kend; ++k) {
@@ -552,12 +570,8 @@ __global__ __launch_bounds__(kTile * kWaves) void k_dh(DevPtrs P, const double *
           const double v2 = SZ[k] - xi2 + mirror_shift(s, code, 2);
           const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
           if (d2 > cut2) continue;
-          const float f0 = (float)v0, f1 = (float)v1, f2 = (float)v2;
-          const float rinv = rsqrtf((float)d2);
-          const float d1 = (float)d2 * rinv;
-          float dv = qiC * SQ[k] * rinv * rinv * (rinv + rcd) * __expf(-d1 * rcd);
-          if (dv > (float)kDeMax) { dv = (float)kDeMax; if (qi < jt + k) ++nclamp; }
-          F0 -= (double)(dv * f0); F1 -= (double)(dv * f1); F2 -= (double)(dv * f2);
+          const float dv = ele_dv<FORM>(s, qiC * SQ[k], (float)d2, nclamp, qi < jt + k);
+          F0 -= (double)(dv * (float)v0); F1 -= (double)(dv * (float)v1); F2 -= (double)(dv * (float)v2);
         }
       }
     }
@@ -573,9 +587,123 @@ __global__ __launch_bounds__(kTile * kWaves) void k_dh(DevPtrs P, const double *
   if (nclamp) atomicAdd(&P.counters[1], (unsigned long long)nclamp);
 }
 
+// force_ele_DH.F90:68-118, force_ele_coulomb.F90:63-84 and the real-space
+// loop of force_ele_coulomb_ewld.F90:118-137, one block per charge tile.
+__global__ __launch_bounds__(kTile * kWaves) void k_ele(DevPtrs P, const double *X, const double *PX,
+                                                        const Tile *qtiles) {
+  __shared__ EleLds lds;
+  const Tile tl = qtiles[blockIdx.x];
+  const DSys s = P.sys[tl.sys];
+  if (s.ele_form == 0) ele_tile<0>(P, s, X, PX, tl.start, lds);
+  else if (s.ele_form == 1) ele_tile<1>(P, s, X, PX, tl.start, lds);
+  else if (s.ele_form == 2) ele_tile<2>(P, s, X, PX, tl.start, lds);
+}
+
+// ------------------------------------------------------------------ Ewald reciprocal space
+// force_ele_coulomb_ewld.F90:62-84.  The phase k.r = 2 pi (h . r/L) is
+// formed in FP64 from fractional coordinates and reduced to (-1/2, 1/2]
+// turns before the FP32 sincospi, so its rounding does not grow with |h| or
+// the box; structure factors and forces accumulate in FP64.
+//
+// Pass 1: partial structure factors.  One lane per reciprocal vector of a
+// 64-vector tile, over one 64-charge chunk staged in LDS (broadcast reads).
+// exact = 1 (energy evaluation) takes sin/cos in FP64.
+__global__ __launch_bounds__(kTile) void k_ewald_sk(DevPtrs P, const double *PX, const EwTask *tasks, int exact) {
+  __shared__ double fx[kTile], fy[kTile], fz[kTile], fq[kTile];
+  const EwTask tk = tasks[blockIdx.x];
+  const DSys s = P.sys[tk.sys];
+  const int lane = threadIdx.x, n = P.ntot;
+  const int qi = tk.b * kTile + lane;
+  if (qi < s.nq) {
+    const int g = P.qbead[s.qoff + qi];
+    fx[lane] = PX[g] * s.ew_invL; fy[lane] = PX[n + g] * s.ew_invL; fz[lane] = PX[2 * n + g] * s.ew_invL;
+    fq[lane] = P.charge[g];
+  } else {
+    fx[lane] = fy[lane] = fz[lane] = 0.0; fq[lane] = 0.0;
+  }
+  __syncthreads();
+  const int k = tk.a * kTile + lane;
+  if (k >= s.nk) return;
+  const int kg = s.ewk0 + k;
+  const double hx = P.ewh[kg], hy = P.ewh[P.nktot + kg], hz = P.ewh[2 * P.nktot + kg];
+  const int jn = min(kTile, s.nq - tk.b * kTile);
+  double sc = 0.0, ss = 0.0;
+  if (exact) {
+    for (int j = 0; j < jn; ++j) {
+      double t = hx * fx[j] + hy * fy[j] + hz * fz[j];
+      t -= rint(t);
+      double sn, cs;
+      sincospi(2.0 * t, &sn, &cs);
+      sc += fq[j] * cs;
+      ss += fq[j] * sn;
+    }
+  } else {
+    for (int j = 0; j < jn; ++j) {
+      double t = hx * fx[j] + hy * fy[j] + hz * fz[j];
+      t -= rint(t);
+      float sn, cs;
+      sincospif((float)(2.0 * t), &sn, &cs);
+      sc += fq[j] * (double)cs;
+      ss += fq[j] * (double)sn;
+    }
+  }
+  double *o = P.psk + s.psk_off + 2 * ((int64_t)tk.b * s.nk + k);
+  o[0] = sc; o[1] = ss;
+}
+
+// Pass 2: forces.  A block takes a 64-charge tile and a chunk of 256
+// reciprocal vectors: it first sums the partial structure factors of its
+// vectors (fixed chunk order) into LDS, then each wave sweeps 64 of them for
+// its lane's charge; partial forces per chunk go to pf (summed by k_main).
+__global__ __launch_bounds__(kTile * kWaves) void k_ewald_f(DevPtrs P, const double *PX, const EwTask *tasks) {
+  __shared__ double shx[kEwKChunk], shy[kEwKChunk], shz[kEwKChunk], sfc[kEwKChunk], ssc[kEwKChunk], sss[kEwKChunk];
+  __shared__ double red[3][kWaves][kTile];
+  const EwTask tk = tasks[blockIdx.x];
+  const DSys s = P.sys[tk.sys];
+  const int tid = threadIdx.x, n = P.ntot;
+  {
+    const int k = tk.b * kEwKChunk + tid;
+    double sc = 0.0, ss = 0.0, fc = 0.0, hx = 0.0, hy = 0.0, hz = 0.0;
+    if (k < s.nk) {
+      const double *ps = P.psk + s.psk_off + 2 * (int64_t)k;
+      for (int c = 0; c < s.nqc; ++c) { sc += ps[2 * (int64_t)c * s.nk]; ss += ps[2 * (int64_t)c * s.nk + 1]; }
+      const int kg = s.ewk0 + k;
+      fc = P.ewc[kg]; hx = P.ewh[kg]; hy = P.ewh[P.nktot + kg]; hz = P.ewh[2 * P.nktot + kg];
+    }
+    shx[tid] = hx; shy[tid] = hy; shz[tid] = hz; sfc[tid] = fc; ssc[tid] = sc; sss[tid] = ss;
+  }
+  __syncthreads();
+  const int lane = tid & (kTile - 1), w = tid / kTile;
+  const int qi = tk.a * kTile + lane;
+  const bool valid = qi < s.nq;
+  double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+  if (valid) {
+    const int g = P.qbead[s.qoff + qi];
+    const double x0 = PX[g] * s.ew_invL, x1 = PX[n + g] * s.ew_invL, x2 = PX[2 * n + g] * s.ew_invL;
+    const int kb = w * kTile, ke = min(kb + kTile, s.nk - tk.b * kEwKChunk);
+    for (int k = kb; k < ke; ++k) {
+      double t = shx[k] * x0 + shy[k] * x1 + shz[k] * x2;
+      t -= rint(t);
+      float sn, cs;
+      sincospif((float)(2.0 * t), &sn, &cs);
+      const double term = sfc[k] * ((double)sn * ssc[k] - (double)cs * sss[k]);
+      f0 += term * shx[k]; f1 += term * shy[k]; f2 += term * shz[k];
+    }
+  }
+  red[0][w][lane] = f0; red[1][w][lane] = f1; red[2][w][lane] = f2;
+  __syncthreads();
+  if (w == 0 && valid) {
+    const double scale = s.ew_fscale * P.charge[P.qbead[s.qoff + qi]];
+    for (int d = 0; d < 3; ++d)
+      P.pf[s.pf_off + ((int64_t)tk.b * 3 + d) * s.nq + qi] =
+          scale * (((red[d][0][lane] + red[d][1][lane]) + red[d][2][lane]) + red[d][3][lane]);
+  }
+}
+
 // ------------------------------------------------------------------ main pass
 // mode 0: store F; mode 1: Langevin step; mode 2: initial acceleration;
-// mode 3: store F and predict positions from the stored v, a.
+// mode 3: store F and predict positions from the stored v, a; mode 4: mode
+// 2 or 3 per system (DSys::init_accel).
 // Langevin (time_integral.F90:169-183, then the next step's position update
 // :152-160): a' = F/m + c1 xi; v = c2 v + c3 (a + a'); a = a';
 // x_next = x + c4 v + (dt^2/2) a.
@@ -679,14 +807,23 @@ __global__ __launch_bounds__(kTile * 2) void k_main(DevPtrs P, const double *X, 
   __syncthreads();
   if (w != 0 || !valid) return;
   for (int d = 0; d < 3; ++d) F[d] = red[d][0][lane] + red[d][1][lane];
-  // Debye-Hueckel sums of the DH pass
-  if (do_dh && s.ele_form == 0) {
+  // electrostatic pair sums of the ele pass, then the Ewald reciprocal
+  // partial forces in chunk order
+  if (do_dh && s.ele_form >= 0) {
     const int q = P.qidx[g];
-    if (q >= 0) { F[0] += P.fdh[q]; F[1] += P.fdh[(size_t)P.nqtot + q]; F[2] += P.fdh[2 * (size_t)P.nqtot + q]; }
+    if (q >= 0) {
+      F[0] += P.fdh[q]; F[1] += P.fdh[(size_t)P.nqtot + q]; F[2] += P.fdh[2 * (size_t)P.nqtot + q];
+      if (s.ele_form == 2) {
+        const double *pf = P.pf + s.pf_off + (q - s.qoff);
+        for (int c = 0; c < s.nkc; ++c)
+          for (int d = 0; d < 3; ++d) F[d] += pf[((int64_t)c * 3 + d) * s.nq];
+      }
+    }
   }
   P.f[g] = F[0]; P.f[n + g] = F[1]; P.f[2 * n + g] = F[2];
   if (mode == 0) return;
-  const uint64_t step = *P.stepbase + (uint64_t)t_local + (mode == 2 ? 0u : 1u);
+  if (mode == 4) mode = s.init_accel ? 2 : 3;
+  const uint64_t step = *P.stepbase + (uint64_t)t_local + (mode == 2 ? 0u : 1u) - s.step_origin;
   integrate(P, s, g, li, F, X, PX, XN, PXN, step, mode);
 }
 
@@ -1074,6 +1211,41 @@ __global__ __launch_bounds__(kEBlock) void k_energy(DevPtrs P, const double *X, 
         e[7] += (P.charge[gi] * P.charge[gj]) * s.ele_coef / d1 * exp(-d1 * s.rcdist);
       }
     }
+  }
+  // plain Coulomb (energy_ele_coulomb.F90:55-66) and the Ewald real-space
+  // term (energy_ele_coulomb_ewld.F90:126-138), each pair once
+  if (s.ele_form == 1 || s.ele_form == 2) {
+    const double *Y = s.periodic ? PX : X;
+    for (int qi = t; qi < s.nq; qi += kEBlock) {
+      int gi = P.qbead[s.qoff + qi];
+      double xi[3];
+      ld3(Y, n, gi, xi);
+      const uint32_t *row = P.qmask + s.moff + (int64_t)qi * s.mwords;
+      for (int qj = qi + 1; qj < s.nq; ++qj) {
+        if (!((row[qj >> 5] >> (qj & 31)) & 1u)) continue;
+        int gj = P.qbead[s.qoff + qj];
+        double v[3] = {Y[gj] - xi[0], Y[n + gj] - xi[1], Y[2 * n + gj] - xi[2]};
+        if (s.periodic) { int c = P.qmirror[s.mioff + (int64_t)qi * s.nq + qj]; for (int d = 0; d < 3; ++d) v[d] += mirror_shift(s, c, d); }
+        double d2 = dot3(v, v);
+        if (d2 > s.cl_cut2) continue;
+        double d1 = sqrt(d2);
+        const double cij = (P.charge[gi] * P.charge[gj]) * s.ele_coef;
+        e[7] += s.ele_form == 1 ? cij * (1.0 / d1 - s.cl_rcinv) : cij * erfc(s.ew_alpha * d1) / d1;
+      }
+    }
+  }
+  // Ewald Fourier sum (energy_ele_coulomb_ewld.F90:70-90) from the partial
+  // structure factors of the last k_ewald_sk pass, plus the self term
+  if (s.ele_form == 2) {
+    for (int k = t; k < s.nk; k += kEBlock) {
+      double sc = 0.0, ss = 0.0;
+      for (int c = 0; c < s.nqc; ++c) {
+        const double *ps = P.psk + s.psk_off + 2 * ((int64_t)c * s.nk + k);
+        sc += ps[0]; ss += ps[1];
+      }
+      e[7] += s.ele_coef * P.ewc[s.ewk0 + k] * (sc * sc + ss * ss);
+    }
+    if (t == 0) e[7] += s.ew_self;
   }
   // kinetic (energy_kinetic.F90:33-44)
   for (int li = t; li < s.nmp; li += kEBlock) {

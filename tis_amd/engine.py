@@ -60,12 +60,15 @@ def load_library():
     L.tis_energy_sumup.argtypes = [vp, C.c_int32, _f64p]
     L.tis_langevin_init.argtypes = [vp, C.c_double, C.c_int32, C.c_uint64, C.c_int32]
     L.tis_langevin_steps.argtypes = [vp, C.c_int64, C.c_int32]
+    L.tis_system_origin.argtypes = [vp, C.c_int32, C.c_uint64, C.c_int32]
+    L.tis_device_bytes.argtypes = [vp, C.POINTER(C.c_int64)]
     L.tis_status_counters.argtypes = [vp, C.POINTER(C.c_int64)]
     L.tis_synchronize.argtypes = [vp]
     L.tis_last_timing.argtypes = [vp, _f64p, _f64p, _f64p, C.POINTER(C.c_int64)]
     for fn in ("tis_ctx_create", "tis_ctx_destroy", "tis_system_upload", "tis_set_system_T", "tis_state_set",
                "tis_state_get", "tis_neighbor", "tis_pairs_get", "tis_force_sumup", "tis_force_get",
                "tis_energy_sumup", "tis_langevin_init", "tis_langevin_steps", "tis_status_counters",
+               "tis_system_origin", "tis_device_bytes",
                "tis_synchronize", "tis_last_timing"):
         getattr(L, fn).restype = C.c_int32
     _LIB = L
@@ -83,10 +86,16 @@ class Engine:
         self.L = load_library()
         self.ctx = C.c_void_p()
         self._chk(self.L.tis_ctx_create(device, C.byref(self.ctx)), "tis_ctx_create")
+        self.upload(systems, keys, seed)
+
+    def upload(self, systems: list[System], keys=None, seed: int = 1):
+        """Replace the batch (the context, its stream and library state stay)."""
         self.systems = list(systems)
         self._cs = (CSystem * len(self.systems))(*[s.cstruct() for s in self.systems])
         k = None
         if keys is not None:
+            if len(keys) != len(self.systems):
+                raise ValueError("one key per system")
             self._keys = (C.c_uint32 * len(keys))(*[int(v) for v in keys])
             k = self._keys
         self._chk(self.L.tis_system_upload(self.ctx, len(self.systems), self._cs, k, seed), "upload")
@@ -165,15 +174,25 @@ class Engine:
         return e
 
     # ---- Langevin
+    def system_origin(self, isys, step_origin, init_accel):
+        self._chk(self.L.tis_system_origin(self.ctx, isys, int(step_origin), 1 if init_accel else 0),
+                  "tis_system_origin")
+
     def langevin_init(self, tstep=0.05, rebuild_every=10, step0=0, init_accel=True):
-        self._chk(self.L.tis_langevin_init(self.ctx, tstep, rebuild_every, step0, 1 if init_accel else 0),
-                  "tis_langevin_init")
+        """init_accel: True, False, or None for the per-system flags of system_origin."""
+        flag = -1 if init_accel is None else (1 if init_accel else 0)
+        self._chk(self.L.tis_langevin_init(self.ctx, tstep, rebuild_every, step0, flag), "tis_langevin_init")
 
     def langevin_steps(self, nsteps, mode=MODE_GRAPH):
         self._chk(self.L.tis_langevin_steps(self.ctx, nsteps, mode), "tis_langevin_steps")
 
     def synchronize(self):
         self._chk(self.L.tis_synchronize(self.ctx), "tis_synchronize")
+
+    def device_bytes(self):
+        n = C.c_int64()
+        self._chk(self.L.tis_device_bytes(self.ctx, C.byref(n)), "tis_device_bytes")
+        return n.value
 
     def counters(self):
         a = (C.c_int64 * 8)()

@@ -22,7 +22,7 @@ CLASS_RNA, CLASS_ION = 2, 4
 E_TOTAL, E_VELO, E_BOND, E_BANGLE, E_ELE = 0, 1, 2, 3, 7
 E_STACK, E_HBOND, E_EXV, E_TSTACK, E_THBOND = 15, 16, 17, 18, 19
 E_MAX = 25
-ELE_NONE, ELE_DH, ELE_EWALD = -1, 0, 2
+ELE_NONE, ELE_DH, ELE_COULOMB, ELE_EWALD = -1, 0, 1, 2
 
 _i32p = C.POINTER(C.c_int32)
 _f64p = C.POINTER(C.c_double)

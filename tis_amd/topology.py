@@ -176,7 +176,7 @@ def hairpin_pairs(n: int, stem5: int, stem_len: int, loop: int):
 def build_rna_system(chains, tempk=310.0, ionic_strength=0.150, viscosity=1.0e-5,
                      ele_form=ELE_DH, cutoff_ele=99.0, dh_cutoff_type=0,
                      neigh_margin=NEIGH_MARGIN, ions=None, box=None,
-                     ewld_alpha=0.0, ewld_hmax=0, all_pair_hbonds=False, name=""):
+                     ewld_alpha=0.0, ewld_hmax=0, all_pair_hbonds=False, phos_charge=None, name=""):
     """Build one DT15 system.
 
     chains: list of (sequence, base_pairs) with base_pairs [(i, j), ...]
@@ -185,6 +185,9 @@ def build_rna_system(chains, tempk=310.0, ionic_strength=0.150, viscosity=1.0e-5
     all_pair_hbonds: make every canonical (i, j) with j - i >= 4 a candidate
       (a tertiary-rich candidate set, as in ninfo/azo.hb-dist.ninfo), instead
       of the designed stem pairs only.
+    phos_charge: None for the reduced phosphate charge -Zp(T, I) of
+      simu_ele_set.F90:155-185 (i_charge = 1), else the fixed value (e.g.
+      -1.0 with explicit ions).
     """
     coef, cdist, zp = ele_params(tempk, ionic_strength)
     visc = langevin_visc(viscosity)
@@ -208,7 +211,7 @@ def build_rna_system(chains, tempk=310.0, ionic_strength=0.150, viscosity=1.0e-5
             iclass.append(CLASS_RNA)
             r, e = EXV_PARAM[nm]
             exv_rad.append(r); exv_eps.append(math.sqrt(e))
-            charge.append(-zp if nm == "P" else 0.0)
+            charge.append((-zp if phos_charge is None else float(phos_charge)) if nm == "P" else 0.0)
             mass.append(MASS[nm]); radius.append(STOKES_R[nm])
             names.append(nm)
             del is_base
@@ -426,6 +429,37 @@ def config0_chain(seed=20261202):
     seq = (random_sequence(rng, 16) + stem + random_sequence(rng, 8)
            + "".join(COMPLEMENT[c] for c in reversed(stem)) + random_sequence(rng, 16))
     return seq, hairpin_pairs(64, 16, 12, 8)
+
+
+def config2_ewald(seed=20261202, n=76, box=150.0, n_mg=10, n_k=305, n_cl=250, alpha=0.1,
+                  rcut=30.0, hmax=10):
+    """Explicit ions with Ewald: a random n-nt chain (phosphate charge -1) and
+    n_mg Mg2+, n_k K+, n_cl Cl- placed uniformly at random without overlap in a
+    periodic cube; full Coulomb by Ewald (alpha, real-space cutoff rcut,
+    |h| <= hmax).  Returns (system, coordinates)."""
+    rng = np.random.default_rng(seed)
+    seq = random_sequence(rng, n)
+    ions = ["Mg"] * n_mg + ["K"] * n_k + ["Cl"] * n_cl
+    s = build_rna_system([(seq, [])], ions=ions, box=(box, box, box), ele_form=ELE_EWALD,
+                         cutoff_ele=rcut, ewld_alpha=alpha, ewld_hmax=hmax, phos_charge=-1.0,
+                         name=f"ewald{n}")
+    x = compact_coords(s, rng, box=50.0, min_dist=3.0)
+    rna = s.iclass == CLASS_RNA
+    x[rna] -= x[rna].mean(axis=0)
+    placed = [p for p in x[rna]]
+    rad = s.exv_rad
+    for i in np.nonzero(~rna)[0]:
+        for _ in range(10000):
+            c = rng.uniform(-box / 2, box / 2, 3)
+            d = np.asarray(placed) - c
+            d -= box * np.rint(d / box)
+            if np.min(np.einsum("ij,ij->i", d, d)) >= (2.0 * rad[i] + 1.0) ** 2:
+                break
+        else:
+            raise RuntimeError("could not place ion")
+        x[i] = c
+        placed.append(c)
+    return s, x
 
 
 def config1_lengths(seed=20261202, nchains=512):
