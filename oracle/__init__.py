@@ -72,6 +72,13 @@ def _lib():
     lib.oracle_energy_ewald.argtypes = [vp, _f64p, _i32p, _f64p, C.c_int64, _f64p]
     lib.oracle_ewald_kvec.restype = C.c_int32
     lib.oracle_ewald_kvec.argtypes = [vp, _i32p, _f64p]
+    lib.oracle_rpy_fill.argtypes = [C.c_int32, _f64p, _f64p, C.c_double, C.c_double, C.c_int32, _f64p]
+    lib.oracle_cholesky.restype = C.c_int32
+    lib.oracle_cholesky.argtypes = [C.c_int32, _f64p, _f64p]
+    lib.oracle_force_fene.argtypes = [C.c_int32, _i32p, _f64p, _f64p, _f64p, _f64p, _f64p]
+    lib.oracle_force_wca.argtypes = [C.c_int32, _i32p, _f64p, _f64p, C.c_double, _f64p, _f64p]
+    lib.oracle_bdhi_step.argtypes = [C.c_int32, _f64p, _f64p, _f64p, C.c_double, C.c_double, C.c_uint64,
+                                     C.c_uint32, C.c_uint64, _f64p]
     return lib
 
 
@@ -236,6 +243,56 @@ def philox(ctr, key):
 
 
 # ---------------------------------------------------------------------------
+# Brownian dynamics with hydrodynamic interaction (BDSystem of tis_amd.bdhi)
+def rpy_tensor(bd, xyz):
+    """D (3n, 3n), simu_hydro_tensors.F90:17-302."""
+    n = bd.nmp
+    x = np.ascontiguousarray(xyz, dtype=np.float64)
+    D = np.zeros((3 * n, 3 * n), order="F")
+    lib().oracle_rpy_fill(n, fp(x), fp(bd.radius), bd.tempk, bd.viscosity, bd.hydro_tensor,
+                          D.ctypes.data_as(_f64p))
+    return D
+
+
+def cholesky(D):
+    """Lower L with D = L L^T (simu_hydro_tensors.F90:304-324); raises on a
+    non-positive pivot."""
+    Df = np.asfortranarray(D, dtype=np.float64)
+    L = np.zeros_like(Df, order="F")
+    rc = lib().oracle_cholesky(Df.shape[0], Df.ctypes.data_as(_f64p), L.ctypes.data_as(_f64p))
+    if rc:
+        raise np.linalg.LinAlgError(f"non-positive pivot at row {rc - 1}")
+    return L
+
+
+def bd_forces(bd, xyz):
+    """force_fene + force_wca (non-periodic)."""
+    x = np.ascontiguousarray(xyz, dtype=np.float64)
+    f = np.zeros_like(x)
+    L = lib()
+    L.oracle_force_fene(len(bd.fene_nat), ip(bd.fene_mp), fp(bd.fene_nat), fp(bd.fene_coef),
+                        fp(bd.fene_dist2), fp(x), fp(f))
+    L.oracle_force_wca(len(bd.wca_nat2), ip(bd.wca_mp), fp(bd.wca_nat2), fp(bd.wca_coef), bd.cutoff_wca,
+                       fp(x), fp(f))
+    return f
+
+
+def bdhi_run(bd, xyz, nsteps, tstep=0.05, refactor_every=1, seed=1, key=0, step0=0):
+    """Ermak-McCammon steps (time_integral.F90:429-457); the tensors are
+    rebuilt before the first step and every refactor_every steps."""
+    x = np.ascontiguousarray(xyz, dtype=np.float64).copy()
+    D = L = None
+    for t in range(nsteps):
+        if t % refactor_every == 0:
+            D = rpy_tensor(bd, x)
+            L = cholesky(D)
+        f = bd_forces(bd, x)
+        lib().oracle_bdhi_step(bd.nmp, D.ctypes.data_as(_f64p), L.ctypes.data_as(_f64p), fp(f), bd.tempk, tstep,
+                               seed, key, step0 + t + 1, fp(x))
+    return x
+
+
+# ---------------------------------------------------------------------------
 def _on_big_stack(fn, *args):
     """Run fn(*args) on a thread with a 1 GiB stack: the reference's list
     builders keep MXMPELE * ncharge pair buffers as automatic (stack) arrays,
@@ -285,6 +342,9 @@ class RefLib:
         L.tisref_set_mass.argtypes = [_f64p]
         L.tisref_temperature.argtypes = [C.c_double, C.c_double, _f64p, _f64p, _f64p, _f64p,
                                          _f64p, _f64p]
+        L.tisref_hydro.argtypes = [C.c_int32, _f64p, _f64p, C.c_double, C.c_double, C.c_int32, _f64p, _f64p]
+        L.tisref_bd_forces.argtypes = [C.c_int32, _f64p, C.c_int32, _i32p, _f64p, _f64p, _f64p, C.c_int32,
+                                       _i32p, _f64p, _f64p, C.c_double, _f64p]
         self.sys = None
 
     def load(self, system):
@@ -334,6 +394,24 @@ class RefLib:
             v = np.ascontiguousarray(velo, dtype=np.float64)
             _on_big_stack(self.L.tisref_energy, fp(v), 1, fp(e))
         return e
+
+    def hydro(self, bd, xyz):
+        """(D, L) of the reference's simu_hydro_tensors."""
+        n = bd.nmp
+        x = np.ascontiguousarray(xyz, dtype=np.float64)
+        D = np.zeros((3 * n, 3 * n), order="F")
+        Lm = np.zeros((3 * n, 3 * n), order="F")
+        _on_big_stack(self.L.tisref_hydro, n, fp(x), fp(bd.radius), bd.tempk, bd.viscosity, bd.hydro_tensor,
+                      D.ctypes.data_as(_f64p), Lm.ctypes.data_as(_f64p))
+        return D, Lm
+
+    def bd_forces(self, bd, xyz):
+        x = np.ascontiguousarray(xyz, dtype=np.float64)
+        f = np.zeros_like(x)
+        _on_big_stack(self.L.tisref_bd_forces, bd.nmp, fp(x), len(bd.fene_nat), ip(bd.fene_mp), fp(bd.fene_nat),
+                      fp(bd.fene_coef), fp(bd.fene_dist2), len(bd.wca_nat2), ip(bd.wca_mp), fp(bd.wca_nat2),
+                      fp(bd.wca_coef), bd.cutoff_wca, fp(f))
+        return f
 
     def temperature(self, tempk, ionic):
         hs = np.ascontiguousarray(self.sys.st_hsTm, dtype=np.float64)

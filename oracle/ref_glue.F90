@@ -394,6 +394,70 @@ contains
     end select
   end subroutine tisref_force_term
 
+  ! Diffusion tensor and its Cholesky factor by simu_hydro_tensors for n
+  ! beads (n <= MXMP): D and L (3n,3n) column-major.
+  subroutine tisref_hydro(n, xyz, rad, tempk, visc, htype, D, L) bind(C, name="tisref_hydro")
+    integer(c_int32_t), value :: n, htype
+    real(c_double), intent(in) :: xyz(3, n), rad(n)
+    real(c_double), value :: tempk, visc
+    real(c_double), intent(out) :: D(3*n, 3*n), L(3*n, 3*n)
+    nmp_real = n
+    if (allocated(xyz_mp_rep)) deallocate(xyz_mp_rep, pxyz_mp_rep)
+    allocate(xyz_mp_rep(3, n, 1), pxyz_mp_rep(3, n, 1))
+    xyz_mp_rep(:,:,1) = xyz
+    radius(1:n) = rad
+    inpara%viscosity = visc
+    inmisc%i_hydro_tensor = htype
+    if (allocated(diffuse_tensor)) deallocate(diffuse_tensor)
+    if (allocated(random_tensor)) deallocate(random_tensor)
+    allocate(diffuse_tensor(3*n, 3*n), random_tensor(3*n, 3*n))
+    call simu_hydro_tensors(1, tempk)
+    D = diffuse_tensor
+    L = random_tensor
+  end subroutine tisref_hydro
+
+  ! FENE + WCA forces (force_fene, force_wca) of a non-periodic bead set.
+  subroutine tisref_bd_forces(n, xyz, nf, fmp, fnat, fcoef, fdist2, nw, wmp, wnat2, wcoef, wcut, f) &
+       bind(C, name="tisref_bd_forces")
+    integer(c_int32_t), value :: n, nf, nw
+    real(c_double), intent(in) :: xyz(3, n)
+    integer(c_int32_t), intent(in) :: fmp(2, *), wmp(2, *)
+    real(c_double), intent(in) :: fnat(*), fcoef(*), fdist2(*), wnat2(*), wcoef(2, *)
+    real(c_double), value :: wcut
+    real(c_double), intent(out) :: f(3, n)
+    real(c_double), allocatable :: fl(:,:)
+    integer :: k
+    nmp_all = n
+    if (allocated(xyz_mp_rep)) deallocate(xyz_mp_rep, pxyz_mp_rep)
+    allocate(xyz_mp_rep(3, n, 1), pxyz_mp_rep(3, n, 1))
+    xyz_mp_rep(:,:,1) = xyz
+    pxyz_mp_rep(:,:,1) = xyz
+    inperi%i_periodic = 0
+    nfene = nf
+    if (allocated(ifene2mp)) deallocate(ifene2mp, fene_nat, coef_fene, dist2_fene)
+    allocate(ifene2mp(2, max(nf,1)), fene_nat(max(nf,1)), coef_fene(max(nf,1)), dist2_fene(max(nf,1)))
+    do k = 1, nf
+       ifene2mp(:, k) = fmp(:, k) + 1
+       fene_nat(k) = fnat(k); coef_fene(k) = fcoef(k); dist2_fene(k) = fdist2(k)
+    end do
+    nwca = nw
+    if (allocated(iwca2mp)) deallocate(iwca2mp, wca_nat2, coef_wca)
+    allocate(iwca2mp(2, max(nw,1)), wca_nat2(max(nw,1)), coef_wca(max(nw,1), 2))
+    do k = 1, nw
+       iwca2mp(:, k) = wmp(:, k) + 1
+       wca_nat2(k) = wnat2(k); coef_wca(k, 1) = wcoef(1, k); coef_wca(k, 2) = wcoef(2, k)
+    end do
+    inpro%cutoff_wca = wcut
+    allocate(fl(3, n))
+    fl(:,:) = 0.0e0_PREC
+    call force_fene(1, fl)
+    f = fl
+    fl(:,:) = 0.0e0_PREC
+    call force_wca(1, fl)
+    f = f + fl
+    deallocate(fl)
+  end subroutine tisref_bd_forces
+
   subroutine tisref_hb_state(status, ene) bind(C, name="tisref_hb_state")
     integer(c_int32_t), intent(out) :: status(*)
     real(c_double), intent(out) :: ene(*)

@@ -1240,3 +1240,141 @@ int32_t oracle_langevin_run(oracle_ws *w, double tstep, int64_t nsteps, int32_t 
   free(rlan); free(f); free(dxyz);
   return nrebuild;
 }
+
+/* ------------------------------------------------------------------ */
+/* Brownian dynamics with hydrodynamic interaction (Ermak-McCammon)    */
+
+/* simu_hydro_tensors.F90:17-302: D (3n x 3n, column-major, FP64).
+ * i_hydro_tensor (const_index.F90:39-45): 4 Zuk et al. RPY with overlap and
+ * unequal radii (:50-134), 1 RPY (:139-182), 2 RPY with overlap (:187-241),
+ * 3 Ermak with overlap (:246-302); types 1-3 use radius[0] for every bead.  c6 = kT/(6 pi eta),
+ * c8 = kT/(8 pi eta). */
+static void rpy_block(double *D, int n3, int i, int j, const double r[3], double pre1, double pre2) {
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      double v = pre2 * r[a] * r[b] + (a == b ? pre1 : 0.0);
+      D[(3 * i + a) + (size_t)(3 * j + b) * n3] = v;
+      D[(3 * j + b) + (size_t)(3 * i + a) * n3] = v;
+    }
+}
+
+void oracle_rpy_fill(int32_t n, const double *xyz, const double *radius, double tempk, double visc,
+                     int32_t type, double *D) {
+  const int n3 = 3 * n;
+  const double c6 = BOLTZ_KCAL_MOL * tempk / (6.0 * F_PI * visc);
+  const double c8 = BOLTZ_KCAL_MOL * tempk / (8.0 * F_PI * visc);
+  memset(D, 0, sizeof(double) * (size_t)n3 * n3);
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) {
+      double r[3];
+      for (int d = 0; d < 3; ++d) r[d] = xyz[3 * j + d] - xyz[3 * i + d];
+      const double r_sq = (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2];
+      const double r_abs = sqrt(r_sq);
+      double pre1, pre2;
+      if (type == 4) {
+        const double ri = radius[i], rj = radius[j];
+        if (ri + rj < r_abs) {
+          pre1 = c8 / r_abs * (1.0 + (ri * ri + rj * rj) / (3.0 * r_sq));
+          pre2 = c8 / r_abs * (1.0 - (ri * ri + rj * rj) / r_sq) / r_sq;
+        } else {
+          const double rs = ri < rj ? ri : rj, rl = ri < rj ? rj : ri;
+          if (rl - rs < r_abs) {
+            const double r_cb = r_sq * r_abs, dd = (ri - rj) * (ri - rj);
+            pre1 = c6 / ri / rj * (16.0 * r_cb * (ri + rj) - (dd + 3.0 * r_sq) * (dd + 3.0 * r_sq)) / 32.0 / r_cb;
+            pre2 = c6 / ri / rj * 3.0 * (dd - r_sq) * (dd - r_sq) / 32.0 / r_cb / r_sq;
+          } else {
+            pre1 = c6 / rl;
+            pre2 = 0.0;
+          }
+        }
+      } else {
+        const double a = radius[0], a2 = a * a;
+        if (type == 1 || 2.0 * a < r_abs) {
+          pre1 = c8 / r_abs * (1.0 + 2.0 * a2 / (3.0 * r_sq));
+          pre2 = c8 / r_abs * (1.0 - 2.0 * a2 / r_sq) / r_sq;
+        } else if (type == 2) {
+          pre1 = c6 / a * (1.0 - 9.0 / 32.0 * r_abs / a);
+          pre2 = c6 / a * 3.0 / 32.0 / a / r_abs;
+        } else {
+          pre1 = c8 / 2.0 / a * 7.0 / 6.0;
+          pre2 = c8 / 2.0 / 2.0 / r_sq;
+        }
+      }
+      rpy_block(D, n3, i, j, r, pre1, pre2);
+    }
+  for (int k = 0; k < n3; ++k) D[k + (size_t)k * n3] = c6 / (type == 4 ? radius[k / 3] : radius[0]);
+}
+
+/* simu_hydro_tensors.F90:304-324: left-looking Cholesky, L lower
+ * (column-major, upper part zero).  Returns 0, or k+1 if the pivot of
+ * row k is not positive. */
+int32_t oracle_cholesky(int32_t n3, const double *D, double *L) {
+  memset(L, 0, sizeof(double) * (size_t)n3 * n3);
+  for (int i = 0; i < n3; ++i) {
+    double x = D[i + (size_t)i * n3];
+    for (int k = 0; k < i; ++k) x -= L[i + (size_t)k * n3] * L[i + (size_t)k * n3];
+    if (x <= 0.0) return i + 1;
+    x = sqrt(x);
+    L[i + (size_t)i * n3] = x;
+    for (int r = i + 1; r < n3; ++r) {
+      double y = D[i + (size_t)r * n3];
+      for (int k = 0; k < i; ++k) y -= L[r + (size_t)k * n3] * L[i + (size_t)k * n3];
+      L[r + (size_t)i * n3] = y / x;
+    }
+  }
+  return 0;
+}
+
+/* force_fene.F90:40-57: F = -k (r - r0) / (1 - (r - r0)^2 / R0^2) / r along v21 */
+void oracle_force_fene(int32_t nfene, const int32_t *mp, const double *nat, const double *coef,
+                       const double *dist2, const double *xyz, double *f) {
+  for (int b = 0; b < nfene; ++b) {
+    int i = mp[2 * b], j = mp[2 * b + 1];
+    double v[3];
+    for (int d = 0; d < 3; ++d) v[d] = xyz[3 * j + d] - xyz[3 * i + d];
+    double dist = sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+    double dd = dist - nat[b];
+    double fr = -coef[b] * dd / (1.0 - dd * dd / dist2[b]) / dist;
+    for (int d = 0; d < 3; ++d) { f[3 * i + d] -= fr * v[d]; f[3 * j + d] += fr * v[d]; }
+  }
+}
+
+/* force_wca.F90:64-96 (non-periodic): (s/r)^2 >= 1/cutoff^2 pairs;
+ * coef rep (r < s) or att; dv = 12 coef / s^2 ((s/r)^14 - (s/r)^8), clamped at DE_MAX */
+void oracle_force_wca(int32_t nwca, const int32_t *mp, const double *nat2, const double *coef,
+                      double cutoff, const double *xyz, double *f) {
+  const double rc2 = 1.0 / (cutoff * cutoff);
+  for (int k = 0; k < nwca; ++k) {
+    int i = mp[2 * k], j = mp[2 * k + 1];
+    double v[3];
+    for (int d = 0; d < 3; ++d) v[d] = xyz[3 * j + d] - xyz[3 * i + d];
+    double d2 = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
+    double ro2 = nat2[k] / d2;
+    if (ro2 < rc2) continue;
+    double ro14 = ro2 * ro2 * ro2 * ro2 * ro2 * ro2 * ro2, ro8 = ro2 * ro2 * ro2 * ro2;
+    double c = ro2 > 1.0 ? coef[2 * k] : coef[2 * k + 1];
+    double dv = 12.0 * c / nat2[k] * (ro14 - ro8);
+    if (dv > DE_MAX) dv = DE_MAX;
+    for (int d = 0; d < 3; ++d) { f[3 * i + d] -= dv * v[d]; f[3 * j + d] += dv * v[d]; }
+  }
+}
+
+/* time_integral.F90:429-457 with time_integral_pre.F90:313-317: one
+ * Ermak-McCammon step dx = (dt/kT) D F + sqrt(2 dt) L xi, xi ~ N(0,1)
+ * from the counter-based stream of (seed, key, step, bead). */
+void oracle_bdhi_step(int32_t n, const double *D, const double *L, const double *F, double tempk,
+                      double tstep, uint64_t seed, uint32_t key, uint64_t step, double *xyz) {
+  const int n3 = 3 * n;
+  const double c1 = tstep / (BOLTZ_KCAL_MOL * tempk), c2 = sqrt(2.0 * tstep);
+  double *xi = (double *)malloc(sizeof(double) * (size_t)n3);
+  double *dx = (double *)malloc(sizeof(double) * (size_t)n3);
+  for (int i = 0; i < n; ++i) oracle_gauss3(seed, key, step, (uint32_t)i, xi + 3 * i);
+  for (int k = 0; k < n3; ++k) {
+    double a = 0.0, b = 0.0;
+    for (int l = 0; l < n3; ++l) a += D[l + (size_t)k * n3] * F[l];
+    for (int l = 0; l <= k; ++l) b += L[k + (size_t)l * n3] * xi[l];
+    dx[k] = c1 * a + c2 * b;
+  }
+  for (int k = 0; k < n3; ++k) xyz[k] += dx[k];
+  free(xi); free(dx);
+}

@@ -104,6 +104,19 @@ int32_t oracle_force_sumup(oracle_ws *w, const double *xyz, double *f,
 void oracle_energy_sumup(oracle_ws *w, const double *xyz, const double *velo, double *energy);
 int64_t oracle_ws_lists(oracle_ws *w, int32_t which, const int32_t **pairs, const double **coef);
 
+/* ---- Brownian dynamics with hydrodynamic interaction ----
+ * simu_hydro_tensors.F90:17-324, force_fene.F90, force_wca.F90,
+ * time_integral.F90:429-457.  Coordinates (3,n); D, L (3n,3n) column-major. */
+void oracle_rpy_fill(int32_t n, const double *xyz, const double *radius, double tempk, double visc,
+                     int32_t type, double *D);
+int32_t oracle_cholesky(int32_t n3, const double *D, double *L);
+void oracle_force_fene(int32_t nfene, const int32_t *mp, const double *nat, const double *coef,
+                       const double *dist2, const double *xyz, double *f);
+void oracle_force_wca(int32_t nwca, const int32_t *mp, const double *nat2, const double *coef,
+                      double cutoff, const double *xyz, double *f);
+void oracle_bdhi_step(int32_t n, const double *D, const double *L, const double *F, double tempk,
+                      double tstep, uint64_t seed, uint32_t key, uint64_t step, double *xyz);
+
 /* ---- Langevin (time_integral_pre.F90:160-180, time_integral.F90:149-189) ---- */
 void oracle_langevin_consts(const tis_system *s, double tstep, double *rlan /* [4*nmp] */);
 /* Runs nsteps; state arrays (3,N) are updated in place.  The neighbour list is

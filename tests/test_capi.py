@@ -12,16 +12,24 @@ from tis_amd import engine
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
+HEADERS = ("tis_hip.h", "tis_bdhi.h")
+
+
 def declared_functions():
-    txt = open(os.path.join(ROOT, "include", "tis_hip.h")).read()
-    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(tis_[a-z0-9_]+)\s*\(", txt)))
+    names = set()
+    for h in HEADERS:
+        txt = open(os.path.join(ROOT, "include", h)).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        names |= set(re.findall(r"\b(tis_[a-z0-9_]+)\s*\(", txt))
+    return sorted(names)
 
 
 def test_header_declares_the_boundary():
     names = declared_functions()
     for must in ("tis_ctx_create", "tis_system_upload", "tis_neighbor", "tis_force_sumup",
                  "tis_energy_sumup", "tis_langevin_init", "tis_langevin_steps", "tis_pairs_get"):
+        assert must in names
+    for must in ("tis_bdhi_setup", "tis_bdhi_tensors", "tis_bdhi_steps", "tis_bdhi_factor_error"):
         assert must in names
 
 
@@ -37,6 +45,9 @@ def test_null_arguments_are_rejected():
     assert lib.tis_neighbor(None, 0) < 0
     assert lib.tis_force_sumup(None, 0, None) < 0
     assert lib.tis_langevin_steps(None, 1, 0) < 0
+    assert lib.tis_bdhi_tensors(None) < 0
+    assert lib.tis_bdhi_steps(None, C.c_double(0.05), 1, 1, 0) < 0
+    assert lib.tis_system_origin(None, 0, 0, 0) < 0
 
 
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):

@@ -6,7 +6,7 @@ import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["tis_kernels.hip", "tis_capi.hip"]
+SOURCES = ["tis_kernels.hip", "tis_capi.hip", "tis_bdhi.hip"]
 OUT = os.path.join(HERE, "libtis_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

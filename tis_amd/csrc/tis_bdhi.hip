@@ -1,0 +1,702 @@
+// tis_bdhi.hip -- Brownian dynamics with hydrodynamic interaction on one
+// MI355X: RPY diffusion tensor fill, blocked Cholesky (FP64 diagonal blocks
+// and triangular solves, FP32 matrix-core trailing updates), and the
+// Ermak-McCammon step (two matrix-vector products per step, HBM-bound).
+//
+// Layout: coordinates, forces and noise are (3,n) column-major vectors of
+// length 3n (the reshape of time_integral.F90:433-434); D and L are (3n,3n)
+// column-major FP64, L lower with a zero upper part.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/tis_bdhi.h"
+#include "tis_dev.h"
+#include "tis_internal.h"
+
+namespace tis {
+
+constexpr int kCb = 64;         // Cholesky block size
+constexpr int kGemvRows = 256;  // rows per block of the step products
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+struct BdDev {
+  int n, n3;
+  double *x, *f, *xi;            // [3n]
+  double *D, *L;                 // [n3 * n3]
+  const double *radius;
+  const int32_t *fb_off, *fb;    // per-bead FENE incidence (bond | side << 31)
+  const int32_t *fene_mp;
+  const double *fene_nat, *fene_coef, *fene_dist2;
+  const int32_t *wb_off, *wb;    // per-bead WCA incidence (pair | side << 31)
+  const int32_t *wca_mp;
+  const double *wca_nat2, *wca_coef;
+  double wca_rc2;                // 1 / cutoff_wca^2
+  double c6, c8;
+  int type;
+  uint64_t seed;
+  uint32_t key;
+  int *info;                     // first failing pivot + 1, or 0
+};
+
+// ---------------------------------------------------------------- forces
+// force_fene.F90:40-57 and force_wca.F90:64-96, owner-computes: each bead
+// sums its own bonds and pairs in list order (no atomics).  with_noise also
+// draws the bead's Gaussian triple for step `step`.
+__global__ __launch_bounds__(256) void k_bd_force(BdDev B, uint64_t step, int with_noise) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= B.n) return;
+  double F[3] = {0.0, 0.0, 0.0};
+  for (int e = B.fb_off[i]; e < B.fb_off[i + 1]; ++e) {
+    const uint32_t ent = (uint32_t)B.fb[e];
+    const int b = (int)(ent & 0x7fffffffu);
+    const int i0 = B.fene_mp[2 * b], j0 = B.fene_mp[2 * b + 1];
+    double v[3];
+    for (int d = 0; d < 3; ++d) v[d] = B.x[3 * j0 + d] - B.x[3 * i0 + d];
+    const double dist = sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+    const double dd = dist - B.fene_nat[b];
+    const double fr = -B.fene_coef[b] * dd / (1.0 - dd * dd / B.fene_dist2[b]) / dist;
+    const double sg = (ent >> 31) ? 1.0 : -1.0;
+    for (int d = 0; d < 3; ++d) F[d] += sg * (fr * v[d]);
+  }
+  for (int e = B.wb_off[i]; e < B.wb_off[i + 1]; ++e) {
+    const uint32_t ent = (uint32_t)B.wb[e];
+    const int k = (int)(ent & 0x7fffffffu);
+    const int i0 = B.wca_mp[2 * k], j0 = B.wca_mp[2 * k + 1];
+    double v[3];
+    for (int d = 0; d < 3; ++d) v[d] = B.x[3 * j0 + d] - B.x[3 * i0 + d];
+    const double d2 = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
+    const double ro2 = B.wca_nat2[k] / d2;
+    if (ro2 < B.wca_rc2) continue;
+    const double ro8 = ro2 * ro2 * ro2 * ro2;
+    const double ro14 = ro2 * ro2 * ro2 * ro2 * ro2 * ro2 * ro2;
+    const double c = ro2 > 1.0 ? B.wca_coef[2 * k] : B.wca_coef[2 * k + 1];
+    double dv = 12.0 * c / B.wca_nat2[k] * (ro14 - ro8);
+    if (dv > kDeMax) dv = kDeMax;
+    const double sg = (ent >> 31) ? 1.0 : -1.0;
+    for (int d = 0; d < 3; ++d) F[d] += sg * (dv * v[d]);
+  }
+  for (int d = 0; d < 3; ++d) B.f[3 * i + d] = F[d];
+  if (with_noise) {
+    double g[3];
+    gauss3(B.seed, B.key, step, (uint32_t)i, g);
+    for (int d = 0; d < 3; ++d) B.xi[3 * i + d] = g[d];
+  }
+}
+
+// ---------------------------------------------------------------- tensor
+// simu_hydro_tensors.F90:50-302.  Thread (i, j), i < j, writes the 3x3 block
+// and its transpose into D and the lower copy into L (upper zero); i == j
+// writes the diagonal block.
+__global__ __launch_bounds__(256) void k_rpy_fill(BdDev B) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * 16 + threadIdx.x, j = blockIdx.y * 16 + threadIdx.y;
+  if (i >= B.n || j >= B.n || i > j) return;
+  const int n3 = B.n3;
+  if (i == j) {
+    const double dg = B.c6 / (B.type == 4 ? B.radius[i] : B.radius[0]);
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) {
+        const size_t o = (size_t)(3 * i + a) + (size_t)(3 * i + b) * n3;
+        B.D[o] = a == b ? dg : 0.0;
+        B.L[o] = (a == b) ? dg : 0.0;
+      }
+    return;
+  }
+  double r[3];
+  for (int d = 0; d < 3; ++d) r[d] = B.x[3 * j + d] - B.x[3 * i + d];
+  const double r_sq = (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2];
+  const double r_abs = sqrt(r_sq);
+  double pre1, pre2;
+  if (B.type == 4) {   // Zuk et al.
+    const double ri = B.radius[i], rj = B.radius[j];
+    if (ri + rj < r_abs) {
+      pre1 = B.c8 / r_abs * (1.0 + (ri * ri + rj * rj) / (3.0 * r_sq));
+      pre2 = B.c8 / r_abs * (1.0 - (ri * ri + rj * rj) / r_sq) / r_sq;
+    } else {
+      const double rs = ri < rj ? ri : rj, rl = ri < rj ? rj : ri;
+      if (rl - rs < r_abs) {
+        const double r_cb = r_sq * r_abs, dd = (ri - rj) * (ri - rj);
+        pre1 = B.c6 / ri / rj * (16.0 * r_cb * (ri + rj) - (dd + 3.0 * r_sq) * (dd + 3.0 * r_sq)) / 32.0 / r_cb;
+        pre2 = B.c6 / ri / rj * 3.0 * (dd - r_sq) * (dd - r_sq) / 32.0 / r_cb / r_sq;
+      } else {
+        pre1 = B.c6 / rl;
+        pre2 = 0.0;
+      }
+    }
+  } else {
+    const double a = B.radius[0], a2 = a * a;
+    if (B.type == 1 || 2.0 * a < r_abs) {
+      pre1 = B.c8 / r_abs * (1.0 + 2.0 * a2 / (3.0 * r_sq));
+      pre2 = B.c8 / r_abs * (1.0 - 2.0 * a2 / r_sq) / r_sq;
+    } else if (B.type == 2) {
+      pre1 = B.c6 / a * (1.0 - 9.0 / 32.0 * r_abs / a);
+      pre2 = B.c6 / a * 3.0 / 32.0 / a / r_abs;
+    } else {
+      pre1 = B.c8 / 2.0 / a * 7.0 / 6.0;
+      pre2 = B.c8 / 2.0 / 2.0 / r_sq;
+    }
+  }
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      const double v = pre2 * r[a] * r[b] + (a == b ? pre1 : 0.0);
+      const size_t up = (size_t)(3 * i + a) + (size_t)(3 * j + b) * n3;   // row < col
+      const size_t lo = (size_t)(3 * j + b) + (size_t)(3 * i + a) * n3;   // row > col
+      B.D[up] = v;
+      B.D[lo] = v;
+      B.L[lo] = v;
+      B.L[up] = 0.0;
+    }
+}
+
+__global__ void k_bd_reset(BdDev B) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) atomicExch(B.info, 0);
+}
+
+// ---------------------------------------------------------------- Cholesky
+// Right-looking blocked factorisation of L (holding the lower part of D) in
+// place.  Per 64-column panel starting at k: the diagonal block in FP64 in
+// LDS, the panel below by an FP64 triangular solve (also storing an FP32 copy
+// P, column-major [64][mpad]), then the trailing lower triangle minus P P^T
+// on FP32 matrix cores, subtracted from the FP64 matrix.
+__global__ __launch_bounds__(256) void k_chol_diag(BdDev B, int k) {
+  __shared__ double A[kCb][kCb + 1];
+  const int tid = threadIdx.x, n3 = B.n3;
+  const int kb = min(kCb, n3 - k);
+  for (int idx = tid; idx < kCb * kCb; idx += 256) {
+    const int r = idx % kCb, c = idx / kCb;
+    A[r][c] = (r < kb && c < kb && r >= c) ? B.L[(size_t)(k + r) + (size_t)(k + c) * n3] : 0.0;
+  }
+  __syncthreads();
+  for (int j = 0; j < kb; ++j) {
+    if (tid == 0) {
+      const double p = A[j][j];
+      if (!(p > 0.0)) atomicCAS(B.info, 0, k + j + 1);
+      A[j][j] = sqrt(p);
+    }
+    __syncthreads();
+    const double dj = A[j][j];
+    for (int r = j + 1 + tid; r < kb; r += 256) A[r][j] /= dj;
+    __syncthreads();
+    for (int idx = tid; idx < kCb * kCb; idx += 256) {
+      const int r = idx % kCb, c = idx / kCb;
+      if (c > j && r >= c && r < kb) A[r][c] -= A[r][j] * A[c][j];
+    }
+    __syncthreads();
+  }
+  for (int idx = tid; idx < kb * kb; idx += 256) {
+    const int r = idx % kb, c = idx / kb;
+    B.L[(size_t)(k + r) + (size_t)(k + c) * n3] = r >= c ? A[r][c] : 0.0;
+  }
+}
+
+// rows r >= k + 64: x L11^T = a  (forward substitution per row, FP64)
+__global__ __launch_bounds__(kCb) void k_chol_trsm(BdDev B, int k, float *P, int mpad) {
+  __shared__ double T[kCb][kCb + 1];
+  const int lane = threadIdx.x, n3 = B.n3;
+  for (int c = 0; c < kCb; ++c) T[lane][c] = (lane >= c) ? B.L[(size_t)(k + lane) + (size_t)(k + c) * n3] : 0.0;
+  __syncthreads();
+  const int rl = blockIdx.x * kCb + lane;     // row within the trailing part
+  const int r = k + kCb + rl;
+  if (rl >= mpad) return;
+  if (r >= n3) {
+    for (int c = 0; c < kCb; ++c) P[(size_t)c * mpad + rl] = 0.0f;
+    return;
+  }
+  double x[kCb];
+#pragma unroll
+  for (int c = 0; c < kCb; ++c) x[c] = B.L[(size_t)r + (size_t)(k + c) * n3];
+#pragma unroll
+  for (int c = 0; c < kCb; ++c) {
+    double s = x[c];
+#pragma unroll
+    for (int l = 0; l < c; ++l) s -= x[l] * T[c][l];
+    x[c] = s / T[c][c];
+  }
+#pragma unroll
+  for (int c = 0; c < kCb; ++c) {
+    B.L[(size_t)r + (size_t)(k + c) * n3] = x[c];
+    P[(size_t)c * mpad + rl] = (float)x[c];
+  }
+}
+
+// trailing update of the lower triangle by 64x64 tiles; 4 waves, each a
+// 32x32 quadrant with v_mfma_f32_32x32x2f32 over K = 64
+__global__ __launch_bounds__(256) void k_chol_syrk(BdDev B, int k, const float *P, int mpad) {
+  __shared__ float Pa[kCb][kCb + 1], Pb[kCb][kCb + 1];
+  __shared__ float C[kCb][kCb + 1];
+  const int tid = threadIdx.x, n3 = B.n3;
+  const int bidx = blockIdx.x;
+  int ti = (int)((sqrt(8.0 * (double)bidx + 1.0) - 1.0) * 0.5);
+  while ((ti + 1) * (ti + 2) / 2 <= bidx) ++ti;
+  while (ti * (ti + 1) / 2 > bidx) --ti;
+  const int tj = bidx - ti * (ti + 1) / 2;
+  for (int idx = tid; idx < kCb * kCb; idx += 256) {
+    const int r = idx % kCb, c = idx / kCb;
+    Pa[r][c] = P[(size_t)c * mpad + ti * kCb + r];
+    Pb[r][c] = P[(size_t)c * mpad + tj * kCb + r];
+  }
+  __syncthreads();
+  const int w = tid / 64, lane = tid & 63;
+  const int qi = w >> 1, qj = w & 1;
+  floatx16 acc;
+  for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
+  const int ar = 32 * qi + (lane & 31), br = 32 * qj + (lane & 31), kk = lane >> 5;
+  for (int s = 0; s < kCb / 2; ++s) {
+    const float a = Pa[ar][2 * s + kk];
+    const float b = Pb[br][2 * s + kk];
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+  }
+  for (int v = 0; v < 16; ++v) {
+    const int row = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+    C[32 * qi + row][32 * qj + (lane & 31)] = acc[v];
+  }
+  __syncthreads();
+  const int r0 = k + kCb + ti * kCb, c0 = k + kCb + tj * kCb;
+  for (int idx = tid; idx < kCb * kCb; idx += 256) {
+    const int r = idx % kCb, c = idx / kCb;
+    const int gr = r0 + r, gc = c0 + c;
+    if (gr < n3 && gc < n3 && gr >= gc) B.L[(size_t)gr + (size_t)gc * n3] -= (double)C[r][c];
+  }
+}
+
+// ---------------------------------------------------------------- step
+// Partial products over a column chunk: yD = D[:, chunk] F[chunk],
+// yL = L[:, chunk] xi[chunk] (lower part only); one thread per row.
+__global__ __launch_bounds__(kGemvRows) void k_bd_gemv(BdDev B, double *ypart, int chunk) {
+  const int k = blockIdx.x * kGemvRows + threadIdx.x;
+  const int c = blockIdx.y, n3 = B.n3;
+  const int l0 = c * chunk, l1 = min(n3, l0 + chunk);
+  if (k >= n3) return;
+  double aD = 0.0, aL = 0.0;
+  for (int l = l0; l < l1; ++l) aD += B.D[(size_t)l * n3 + k] * B.f[l];
+  const int lL = min(l1, k + 1);
+  for (int l = l0; l < lL; ++l) aL += B.L[(size_t)l * n3 + k] * B.xi[l];
+  ypart[(size_t)(2 * c) * n3 + k] = aD;
+  ypart[(size_t)(2 * c + 1) * n3 + k] = aL;
+}
+
+// time_integral.F90:440-446: dx = (dt/kT) D F + sqrt(2 dt) L xi
+__global__ __launch_bounds__(kGemvRows) void k_bd_update(BdDev B, const double *ypart, int nchunk, double c1,
+                                                          double c2) {
+#pragma clang fp contract(off)
+  const int k = blockIdx.x * kGemvRows + threadIdx.x;
+  if (k >= B.n3) return;
+  double a = 0.0, b = 0.0;
+  for (int c = 0; c < nchunk; ++c) { a += ypart[(size_t)(2 * c) * B.n3 + k]; b += ypart[(size_t)(2 * c + 1) * B.n3 + k]; }
+  B.x[k] += c1 * a + c2 * b;
+}
+
+// ||L L^T - D||_F^2 and ||D||_F^2 over 64x64 lower tiles (off-diagonal
+// tiles counted twice); FP64, K staged through LDS in chunks of 16.
+__global__ __launch_bounds__(256) void k_bd_llt_err(BdDev B, double *part) {
+  __shared__ double Li[kCb][17], Lj[kCb][17];
+  __shared__ double red[2][256];
+  const int tid = threadIdx.x, n3 = B.n3;
+  const int bidx = blockIdx.x;
+  int ti = (int)((sqrt(8.0 * (double)bidx + 1.0) - 1.0) * 0.5);
+  while ((ti + 1) * (ti + 2) / 2 <= bidx) ++ti;
+  while (ti * (ti + 1) / 2 > bidx) --ti;
+  const int tj = bidx - ti * (ti + 1) / 2;
+  const int r0 = ti * kCb, c0 = tj * kCb;
+  // thread owns a 4x4 sub-block
+  const int sr = (tid % 16) * 4, sc = (tid / 16) * 4;
+  double acc[4][4] = {};
+  const int kend = min(n3, c0 + kCb);  // L[j, l] = 0 for l > j
+  for (int l0 = 0; l0 < kend; l0 += 16) {
+    for (int idx = tid; idx < kCb * 16; idx += 256) {
+      const int r = idx % kCb, l = idx / kCb;
+      const int gl = l0 + l;
+      Li[r][l] = (r0 + r < n3 && gl < n3) ? B.L[(size_t)(r0 + r) + (size_t)gl * n3] : 0.0;
+      Lj[r][l] = (c0 + r < n3 && gl < n3) ? B.L[(size_t)(c0 + r) + (size_t)gl * n3] : 0.0;
+    }
+    __syncthreads();
+    for (int l = 0; l < 16; ++l)
+      for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b) acc[a][b] += Li[sr + a][l] * Lj[sc + b][l];
+    __syncthreads();
+  }
+  double e2 = 0.0, d2 = 0.0;
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) {
+      const int gr = r0 + sr + a, gc = c0 + sc + b;
+      if (gr >= n3 || gc >= n3 || gr < gc) continue;
+      const double dv = B.D[(size_t)gr + (size_t)gc * n3];
+      const double e = acc[a][b] - dv;
+      const double wgt = gr == gc ? 1.0 : 2.0;
+      e2 += wgt * e * e;
+      d2 += wgt * dv * dv;
+    }
+  red[0][tid] = e2; red[1][tid] = d2;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (tid < s) { red[0][tid] += red[0][tid + s]; red[1][tid] += red[1][tid + s]; }
+    __syncthreads();
+  }
+  if (tid == 0) { part[2 * bidx] = red[0][0]; part[2 * bidx + 1] = red[1][0]; }
+}
+
+// ---------------------------------------------------------------- host state
+struct BdState {
+  BdDev d{};
+  std::vector<void *> allocs;
+  float *P = nullptr;
+  int mpad = 0;
+  double *ypart = nullptr;
+  int nchunk = 1, chunk = 1;
+  double *llt_part = nullptr;
+  double tempk = 0.0, visc = 0.0;
+  hipGraphExec_t fill_graph = nullptr, factor_graph = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+  double last_total_ms = 0.0, last_fill_ms = 0.0, last_factor_ms = 0.0;
+};
+
+void bd_free(tis_ctx *c) {
+  BdState *&b = ctx_bd(c);
+  if (!b) return;
+  hipStream_t st = ctx_stream(c);
+  if (st) (void)hipStreamSynchronize(st);
+  if (b->fill_graph) (void)hipGraphExecDestroy(b->fill_graph);
+  if (b->factor_graph) (void)hipGraphExecDestroy(b->factor_graph);
+  for (void *p : b->allocs) (void)hipFree(p);
+  if (b->e0) (void)hipEventDestroy(b->e0);
+  if (b->e1) (void)hipEventDestroy(b->e1);
+  if (b->e2) (void)hipEventDestroy(b->e2);
+  delete b;
+  b = nullptr;
+}
+
+namespace {
+
+int bfail(tis_ctx *c, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  return ctx_fail(c, code, buf);
+}
+
+#define BCHK(ctx, expr)                                                                  \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) return bfail(ctx, TIS_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+int balloc(tis_ctx *c, BdState *b, T **p, size_t n) {
+  *p = nullptr;
+  const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+  hipError_t e = hipMalloc((void **)p, bytes);
+  if (e != hipSuccess) return bfail(c, TIS_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  b->allocs.push_back((void *)*p);
+  e = hipMemsetAsync(*p, 0, bytes, ctx_stream(c));
+  if (e != hipSuccess) return bfail(c, TIS_ERR_HIP, "hipMemsetAsync failed: %s", hipGetErrorString(e));
+  return 0;
+}
+
+template <typename T>
+int bupload(tis_ctx *c, BdState *b, T **p, const std::vector<T> &h) {
+  int rc = balloc(c, b, p, h.size());
+  if (rc) return rc;
+  if (!h.empty()) BCHK(c, hipMemcpyAsync(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, ctx_stream(c)));
+  return 0;
+}
+
+BdState *need(tis_ctx *c) {
+  BdState *b = ctx_bd(c);
+  if (!b) ctx_fail(c, TIS_ERR_STATE, "tis_bdhi_setup not called");
+  return b;
+}
+
+// fill, and the factorisation, each captured once as a graph (the launch
+// sequence depends only on n)
+void launch_fill(tis_ctx *c, BdState *b) {
+  hipStream_t st = ctx_stream(c);
+  const BdDev &d = b->d;
+  hipLaunchKernelGGL(k_bd_reset, dim3(1), dim3(64), 0, st, d);
+  const int gb = (d.n + 15) / 16;
+  hipLaunchKernelGGL(k_rpy_fill, dim3(gb, gb), dim3(16, 16), 0, st, d);
+}
+
+void launch_factor(tis_ctx *c, BdState *b) {
+  hipStream_t st = ctx_stream(c);
+  const BdDev &d = b->d;
+  for (int k = 0; k < d.n3; k += kCb) {
+    hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(256), 0, st, d, k);
+    const int m = d.n3 - k - kCb;
+    if (m <= 0) break;
+    const int mt = (m + kCb - 1) / kCb;
+    hipLaunchKernelGGL(k_chol_trsm, dim3(mt), dim3(kCb), 0, st, d, k, b->P, mt * kCb);
+    hipLaunchKernelGGL(k_chol_syrk, dim3(mt * (mt + 1) / 2), dim3(256), 0, st, d, k, (const float *)b->P, mt * kCb);
+  }
+}
+
+int capture(tis_ctx *c, BdState *b, void (*fn)(tis_ctx *, BdState *), hipGraphExec_t *out) {
+  hipStream_t st = ctx_stream(c);
+  hipGraph_t g = nullptr;
+  BCHK(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  fn(c, b);
+  hipError_t e = hipStreamEndCapture(st, &g);
+  if (e != hipSuccess) return bfail(c, TIS_ERR_HIP, "capture failed: %s", hipGetErrorString(e));
+  e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return bfail(c, TIS_ERR_HIP, "instantiate failed: %s", hipGetErrorString(e));
+  return 0;
+}
+
+// events e0 | fill | e1 | factorisation | e2 on the context's stream
+int run_tensors(tis_ctx *c, BdState *b) {
+  hipStream_t st = ctx_stream(c);
+  int rc = 0;
+  if (!b->fill_graph && (rc = capture(c, b, launch_fill, &b->fill_graph))) return rc;
+  if (!b->factor_graph && (rc = capture(c, b, launch_factor, &b->factor_graph))) return rc;
+  BCHK(c, hipEventRecord(b->e0, st));
+  BCHK(c, hipGraphLaunch(b->fill_graph, st));
+  BCHK(c, hipEventRecord(b->e1, st));
+  BCHK(c, hipGraphLaunch(b->factor_graph, st));
+  BCHK(c, hipEventRecord(b->e2, st));
+  return 0;
+}
+
+int check_info(tis_ctx *c, BdState *b) {
+  int info = 0;
+  BCHK(c, hipMemcpyAsync(&info, b->d.info, sizeof(int), hipMemcpyDeviceToHost, ctx_stream(c)));
+  BCHK(c, hipStreamSynchronize(ctx_stream(c)));
+  if (info) return bfail(c, TIS_ERR_STATE, "Cholesky failed: non-positive pivot at row %d", info - 1);
+  return 0;
+}
+
+}  // namespace
+}  // namespace tis
+
+using namespace tis;
+
+extern "C" {
+
+int32_t tis_bdhi_setup(tis_ctx *c, const tis_bd_system *s, uint32_t key, uint64_t seed) {
+  if (!c || !s || s->nmp <= 1 || !s->radius) return TIS_ERR_ARG;
+  if (s->hydro_tensor < 1 || s->hydro_tensor > 4) return bfail(c, TIS_ERR_ARG, "hydro_tensor must be 1-4");
+  if (!(s->viscosity > 0.0) || !(s->tempk > 0.0)) return bfail(c, TIS_ERR_ARG, "bad temperature or viscosity");
+  if (s->nwca > 0 && !(s->cutoff_wca > 0.0)) return bfail(c, TIS_ERR_ARG, "cutoff_wca must be > 0");
+  const int64_t n3 = 3 * (int64_t)s->nmp;
+  if (n3 > 46000) return bfail(c, TIS_ERR_ARG, "too many beads for a dense tensor (%d)", s->nmp);
+  BCHK(c, hipSetDevice(ctx_device(c)));
+  bd_free(c);
+  BdState *b = new BdState();
+  ctx_bd(c) = b;
+  BdDev &d = b->d;
+  const int n = s->nmp;
+  d.n = n; d.n3 = (int)n3;
+  for (int k = 0; k < s->nfene; ++k)
+    for (int e = 0; e < 2; ++e)
+      if (s->fene_mp[2 * k + e] < 0 || s->fene_mp[2 * k + e] >= n) return bfail(c, TIS_ERR_ARG, "bad FENE bead index");
+  for (int k = 0; k < s->nwca; ++k)
+    for (int e = 0; e < 2; ++e)
+      if (s->wca_mp[2 * k + e] < 0 || s->wca_mp[2 * k + e] >= n) return bfail(c, TIS_ERR_ARG, "bad WCA bead index");
+  // per-bead incidence lists (bond order, pair order)
+  auto csr = [n](int m, const int32_t *mp, std::vector<int32_t> &off, std::vector<int32_t> &ent) {
+    std::vector<std::vector<int32_t>> inc(n);
+    for (int k = 0; k < m; ++k) {
+      inc[mp[2 * k]].push_back(k);
+      inc[mp[2 * k + 1]].push_back((int32_t)((uint32_t)k | 0x80000000u));
+    }
+    off.assign(1, 0);
+    for (int i = 0; i < n; ++i) {
+      ent.insert(ent.end(), inc[i].begin(), inc[i].end());
+      off.push_back((int32_t)ent.size());
+    }
+  };
+  std::vector<int32_t> fb_off, fb, wb_off, wb;
+  csr(s->nfene, s->fene_mp, fb_off, fb);
+  csr(s->nwca, s->wca_mp, wb_off, wb);
+  std::vector<double> rad(s->radius, s->radius + n);
+  std::vector<int32_t> fmp(s->fene_mp, s->fene_mp + 2 * s->nfene), wmp(s->wca_mp, s->wca_mp + 2 * s->nwca);
+  std::vector<double> fnat(s->fene_nat, s->fene_nat + s->nfene), fcoef(s->fene_coef, s->fene_coef + s->nfene),
+      fd2(s->fene_dist2, s->fene_dist2 + s->nfene), wn2(s->wca_nat2, s->wca_nat2 + s->nwca),
+      wcf(s->wca_coef, s->wca_coef + 2 * s->nwca);
+  int rc = 0;
+#define BUP(ptr, vec) do { rc = bupload(c, b, &ptr, vec); if (rc) return rc; } while (0)
+  { int32_t *p; BUP(p, fb_off); d.fb_off = p; BUP(p, fb); d.fb = p; BUP(p, fmp); d.fene_mp = p;
+    BUP(p, wb_off); d.wb_off = p; BUP(p, wb); d.wb = p; BUP(p, wmp); d.wca_mp = p; }
+  { double *p; BUP(p, rad); d.radius = p; BUP(p, fnat); d.fene_nat = p; BUP(p, fcoef); d.fene_coef = p;
+    BUP(p, fd2); d.fene_dist2 = p; BUP(p, wn2); d.wca_nat2 = p; BUP(p, wcf); d.wca_coef = p; }
+#undef BUP
+  if ((rc = balloc(c, b, &d.x, (size_t)n3))) return rc;
+  if ((rc = balloc(c, b, &d.f, (size_t)n3))) return rc;
+  if ((rc = balloc(c, b, &d.xi, (size_t)n3))) return rc;
+  if ((rc = balloc(c, b, &d.D, (size_t)n3 * n3))) return rc;
+  if ((rc = balloc(c, b, &d.L, (size_t)n3 * n3))) return rc;
+  if ((rc = balloc(c, b, &d.info, 1))) return rc;
+  const int mt = (int)((n3 + kCb - 1) / kCb);
+  b->mpad = mt * kCb;
+  if ((rc = balloc(c, b, &b->P, (size_t)kCb * b->mpad))) return rc;
+  // step products: ~1024+ blocks
+  const int rows = (int)((n3 + kGemvRows - 1) / kGemvRows);
+  b->nchunk = std::max(1, std::min((int)((n3 + 127) / 128), (1024 + rows - 1) / rows));
+  b->chunk = (int)((n3 + b->nchunk - 1) / b->nchunk);
+  if ((rc = balloc(c, b, &b->ypart, (size_t)2 * b->nchunk * n3))) return rc;
+  if ((rc = balloc(c, b, &b->llt_part, (size_t)2 * mt * (mt + 1) / 2))) return rc;
+  d.wca_rc2 = s->nwca > 0 ? 1.0 / (s->cutoff_wca * s->cutoff_wca) : 0.0;
+  d.c6 = kBoltzKcalMol * s->tempk / (6.0 * kPi * s->viscosity);
+  d.c8 = kBoltzKcalMol * s->tempk / (8.0 * kPi * s->viscosity);
+  d.type = s->hydro_tensor;
+  d.seed = seed;
+  d.key = key;
+  b->tempk = s->tempk;
+  b->visc = s->viscosity;
+  BCHK(c, hipEventCreate(&b->e0));
+  BCHK(c, hipEventCreate(&b->e1));
+  BCHK(c, hipEventCreate(&b->e2));
+  BCHK(c, hipStreamSynchronize(ctx_stream(c)));
+  return TIS_OK;
+}
+
+int32_t tis_bdhi_state_set(tis_ctx *c, int32_t which, const double *buf) {
+  if (!c || !buf) return TIS_ERR_ARG;
+  BdState *b = need(c);
+  if (!b) return TIS_ERR_STATE;
+  if (which != 0) return bfail(c, TIS_ERR_ARG, "only coordinates can be set");
+  BCHK(c, hipSetDevice(ctx_device(c)));
+  BCHK(c, hipMemcpyAsync(b->d.x, buf, sizeof(double) * 3 * (size_t)b->d.n, hipMemcpyHostToDevice, ctx_stream(c)));
+  BCHK(c, hipStreamSynchronize(ctx_stream(c)));
+  return TIS_OK;
+}
+
+int32_t tis_bdhi_state_get(tis_ctx *c, int32_t which, double *buf) {
+  if (!c || !buf) return TIS_ERR_ARG;
+  BdState *b = need(c);
+  if (!b) return TIS_ERR_STATE;
+  if (which != 0 && which != 1) return bfail(c, TIS_ERR_ARG, "which must be 0 or 1");
+  BCHK(c, hipSetDevice(ctx_device(c)));
+  BCHK(c, hipMemcpyAsync(buf, which == 0 ? b->d.x : b->d.f, sizeof(double) * 3 * (size_t)b->d.n,
+                         hipMemcpyDeviceToHost, ctx_stream(c)));
+  BCHK(c, hipStreamSynchronize(ctx_stream(c)));
+  return TIS_OK;
+}
+
+int32_t tis_bdhi_forces(tis_ctx *c, double *force_out) {
+  if (!c) return TIS_ERR_ARG;
+  BdState *b = need(c);
+  if (!b) return TIS_ERR_STATE;
+  BCHK(c, hipSetDevice(ctx_device(c)));
+  hipLaunchKernelGGL(k_bd_force, dim3((b->d.n + 255) / 256), dim3(256), 0, ctx_stream(c), b->d, (uint64_t)0, 0);
+  BCHK(c, hipGetLastError());
+  if (force_out) return tis_bdhi_state_get(c, 1, force_out);
+  BCHK(c, hipStreamSynchronize(ctx_stream(c)));
+  return TIS_OK;
+}
+
+int32_t tis_bdhi_tensors(tis_ctx *c) {
+  if (!c) return TIS_ERR_ARG;
+  BdState *b = need(c);
+  if (!b) return TIS_ERR_STATE;
+  BCHK(c, hipSetDevice(ctx_device(c)));
+  int rc = run_tensors(c, b);
+  if (rc) return rc;
+  BCHK(c, hipGetLastError());
+  rc = check_info(c, b);
+  float a = 0.0f, f = 0.0f;
+  (void)hipEventElapsedTime(&a, b->e0, b->e1);
+  (void)hipEventElapsedTime(&f, b->e1, b->e2);
+  b->last_fill_ms = a;
+  b->last_factor_ms = f;
+  b->last_total_ms = (double)a + (double)f;
+  return rc;
+}
+
+int32_t tis_bdhi_tensor_get(tis_ctx *c, int32_t which, double *buf) {
+  if (!c || !buf) return TIS_ERR_ARG;
+  BdState *b = need(c);
+  if (!b) return TIS_ERR_STATE;
+  if (which != 0 && which != 1) return bfail(c, TIS_ERR_ARG, "which must be 0 (D) or 1 (L)");
+  BCHK(c, hipSetDevice(ctx_device(c)));
+  BCHK(c, hipMemcpyAsync(buf, which == 0 ? b->d.D : b->d.L, sizeof(double) * (size_t)b->d.n3 * b->d.n3,
+                         hipMemcpyDeviceToHost, ctx_stream(c)));
+  BCHK(c, hipStreamSynchronize(ctx_stream(c)));
+  return TIS_OK;
+}
+
+int32_t tis_bdhi_factor_error(tis_ctx *c, double *rel_err) {
+  if (!c || !rel_err) return TIS_ERR_ARG;
+  BdState *b = need(c);
+  if (!b) return TIS_ERR_STATE;
+  BCHK(c, hipSetDevice(ctx_device(c)));
+  const int mt = (b->d.n3 + kCb - 1) / kCb;
+  const int nt = mt * (mt + 1) / 2;
+  hipLaunchKernelGGL(k_bd_llt_err, dim3(nt), dim3(256), 0, ctx_stream(c), b->d, b->llt_part);
+  BCHK(c, hipGetLastError());
+  std::vector<double> part(2 * (size_t)nt);
+  BCHK(c, hipMemcpyAsync(part.data(), b->llt_part, part.size() * sizeof(double), hipMemcpyDeviceToHost,
+                         ctx_stream(c)));
+  BCHK(c, hipStreamSynchronize(ctx_stream(c)));
+  double e2 = 0.0, d2 = 0.0;
+  for (int k = 0; k < nt; ++k) { e2 += part[2 * k]; d2 += part[2 * k + 1]; }
+  *rel_err = d2 > 0.0 ? std::sqrt(e2 / d2) : 0.0;
+  return TIS_OK;
+}
+
+int32_t tis_bdhi_steps(tis_ctx *c, double tstep, int64_t nsteps, int32_t refactor_every, uint64_t step0) {
+  if (!c || !(tstep > 0.0) || nsteps < 0 || refactor_every <= 0) return TIS_ERR_ARG;
+  BdState *b = need(c);
+  if (!b) return TIS_ERR_STATE;
+  BCHK(c, hipSetDevice(ctx_device(c)));
+  hipStream_t st = ctx_stream(c);
+  const BdDev &d = b->d;
+  // time_integral_pre.F90:313-317
+  const double c1 = tstep / (kBoltzKcalMol * b->tempk), c2 = std::sqrt(2.0 * tstep);
+  hipEvent_t t0, t1;
+  BCHK(c, hipEventCreate(&t0));
+  BCHK(c, hipEventCreate(&t1));
+  BCHK(c, hipEventRecord(t0, st));
+  double fact_ms = 0.0, fill_ms = 0.0;
+  const int rows = (d.n3 + kGemvRows - 1) / kGemvRows;
+  for (int64_t t = 0; t < nsteps; ++t) {
+    if (t % refactor_every == 0) {
+      int rc = run_tensors(c, b);
+      if (rc) return rc;
+      if (nsteps <= 64 * refactor_every || t == 0) {   // pivots checked on the first refactor and short runs
+        rc = check_info(c, b);
+        if (rc) { (void)hipEventDestroy(t0); (void)hipEventDestroy(t1); return rc; }
+        float a = 0.0f, f = 0.0f;
+        (void)hipEventElapsedTime(&a, b->e0, b->e1);
+        (void)hipEventElapsedTime(&f, b->e1, b->e2);
+        fill_ms += a; fact_ms += f;
+      }
+    }
+    hipLaunchKernelGGL(k_bd_force, dim3((d.n + 255) / 256), dim3(256), 0, st, d, step0 + (uint64_t)t + 1u, 1);
+    hipLaunchKernelGGL(k_bd_gemv, dim3(rows, b->nchunk), dim3(kGemvRows), 0, st, d, b->ypart, b->chunk);
+    hipLaunchKernelGGL(k_bd_update, dim3(rows), dim3(kGemvRows), 0, st, d, (const double *)b->ypart, b->nchunk,
+                       c1, c2);
+  }
+  BCHK(c, hipGetLastError());
+  BCHK(c, hipEventRecord(t1, st));
+  BCHK(c, hipEventSynchronize(t1));
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, t0, t1);
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  b->last_total_ms = ms;
+  b->last_fill_ms = fill_ms;
+  b->last_factor_ms = fact_ms;
+  return check_info(c, b);
+}
+
+int32_t tis_bdhi_timing(tis_ctx *c, double *total_ms, double *fill_ms, double *factor_ms) {
+  if (!c) return TIS_ERR_ARG;
+  BdState *b = need(c);
+  if (!b) return TIS_ERR_STATE;
+  if (total_ms) *total_ms = b->last_total_ms;
+  if (fill_ms) *fill_ms = b->last_fill_ms;
+  if (factor_ms) *factor_ms = b->last_factor_ms;
+  return TIS_OK;
+}
+
+}  // extern "C"

@@ -15,6 +15,7 @@
 
 #include "../../include/tis_hip.h"
 #include "tis_dev.h"
+#include "tis_internal.h"
 
 namespace tis {
 __global__ void k_terms(DevPtrs, const double *, const double *, const int32_t *, int32_t);
@@ -37,7 +38,6 @@ using namespace tis;
 
 namespace {
 
-constexpr double kBoltzKcalMol = 1.38064852e-23 * ((1.0 / 4184.0) * 6.022140857e23);
 
 struct HostSys {
   DSys d;
@@ -98,7 +98,19 @@ struct tis_ctx {
   bool timing = false;
   std::vector<hipEvent_t> tev;
   size_t tev_used = 0;
+  // Brownian dynamics with hydrodynamic interaction (tis_bdhi.hip)
+  tis::BdState *bd = nullptr;
 };
+
+namespace tis {
+hipStream_t ctx_stream(tis_ctx *c) { return c->stream; }
+int ctx_device(tis_ctx *c) { return c->device; }
+BdState *&ctx_bd(tis_ctx *c) { return c->bd; }
+int ctx_fail(tis_ctx *c, int code, const char *msg) {
+  if (c) c->err = msg;
+  return code;
+}
+}  // namespace tis
 
 namespace {
 
@@ -291,6 +303,7 @@ int32_t tis_ctx_destroy(tis_ctx *c) {
   if (!c) return TIS_ERR_ARG;
   (void)hipSetDevice(c->device);
   free_all(c);
+  bd_free(c);
   for (hipEvent_t e : c->tev) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);

@@ -25,6 +25,8 @@ constexpr int kMirrorShift = 27;     // nb entry: (mirror << 27) | j
 constexpr uint32_t kIdxMask = (1u << 27) - 1u;
 constexpr double kDeMax = 50.0;      // const_physical.F90:35
 constexpr double kPi = 3.14159265358979323846264338;
+// k_B in kcal/mol/K: BOLTZ_J * JOUL2KCAL_MOL (const_physical.F90:21)
+constexpr double kBoltzKcalMol = 1.38064852e-23 * ((1.0 / 4184.0) * 6.022140857e23);
 constexpr double kHighEnergy = 1.0e10;
 
 // incidence entry: type(3) | slot(3) | term(26)

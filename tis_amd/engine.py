@@ -205,3 +205,77 @@ class Engine:
         self._chk(self.L.tis_last_timing(self.ctx, C.byref(t), C.byref(m), C.byref(p), C.byref(n)),
                   "tis_last_timing")
         return {"total_ms": t.value, "main_ms": m.value, "pre_ms": p.value, "nlaunch": n.value}
+
+
+class BDEngine:
+    """Brownian dynamics with hydrodynamic interaction for one bead system
+    (include/tis_bdhi.h) on its own context."""
+
+    def __init__(self, system, key: int = 0, seed: int = 1, device: int = 0):
+        self.L = load_library()
+        L = self.L
+        vp = C.c_void_p
+        L.tis_bdhi_setup.argtypes = [vp, C.c_void_p, C.c_uint32, C.c_uint64]
+        L.tis_bdhi_state_set.argtypes = [vp, C.c_int32, _f64p]
+        L.tis_bdhi_state_get.argtypes = [vp, C.c_int32, _f64p]
+        L.tis_bdhi_forces.argtypes = [vp, _f64p]
+        L.tis_bdhi_tensors.argtypes = [vp]
+        L.tis_bdhi_tensor_get.argtypes = [vp, C.c_int32, _f64p]
+        L.tis_bdhi_factor_error.argtypes = [vp, _f64p]
+        L.tis_bdhi_steps.argtypes = [vp, C.c_double, C.c_int64, C.c_int32, C.c_uint64]
+        L.tis_bdhi_timing.argtypes = [vp, _f64p, _f64p, _f64p]
+        for fn in ("tis_bdhi_setup", "tis_bdhi_state_set", "tis_bdhi_state_get", "tis_bdhi_forces",
+                   "tis_bdhi_tensors", "tis_bdhi_tensor_get", "tis_bdhi_factor_error", "tis_bdhi_steps",
+                   "tis_bdhi_timing"):
+            getattr(L, fn).restype = C.c_int32
+        self.ctx = C.c_void_p()
+        self._chk(L.tis_ctx_create(device, C.byref(self.ctx)), "tis_ctx_create")
+        self.system = system
+        self._cs = system.cstruct()
+        self._chk(L.tis_bdhi_setup(self.ctx, C.cast(C.pointer(self._cs), C.c_void_p), key, seed), "tis_bdhi_setup")
+
+    _chk = Engine._chk
+    close = Engine.close
+    __del__ = Engine.__del__
+
+    @property
+    def n3(self):
+        return 3 * self.system.nmp
+
+    def set_coords(self, xyz):
+        a = np.ascontiguousarray(xyz, dtype=np.float64)
+        if a.shape != (self.system.nmp, 3):
+            raise ValueError("coordinates must be (N,3)")
+        self._chk(self.L.tis_bdhi_state_set(self.ctx, 0, _fp(a)), "tis_bdhi_state_set")
+
+    def coords(self):
+        a = np.zeros((self.system.nmp, 3))
+        self._chk(self.L.tis_bdhi_state_get(self.ctx, 0, _fp(a)), "tis_bdhi_state_get")
+        return a
+
+    def forces(self):
+        a = np.zeros((self.system.nmp, 3))
+        self._chk(self.L.tis_bdhi_forces(self.ctx, _fp(a)), "tis_bdhi_forces")
+        return a
+
+    def tensors(self):
+        self._chk(self.L.tis_bdhi_tensors(self.ctx), "tis_bdhi_tensors")
+
+    def tensor(self, which):
+        """0: D, 1: L, as (3n, 3n) arrays (column-major on the device)."""
+        a = np.zeros((self.n3, self.n3), order="F")
+        self._chk(self.L.tis_bdhi_tensor_get(self.ctx, which, a.ctypes.data_as(_f64p)), "tis_bdhi_tensor_get")
+        return a
+
+    def factor_error(self):
+        e = C.c_double()
+        self._chk(self.L.tis_bdhi_factor_error(self.ctx, C.byref(e)), "tis_bdhi_factor_error")
+        return e.value
+
+    def steps(self, nsteps, tstep=0.05, refactor_every=10, step0=0):
+        self._chk(self.L.tis_bdhi_steps(self.ctx, tstep, nsteps, refactor_every, step0), "tis_bdhi_steps")
+
+    def timing(self):
+        t, f, k = C.c_double(), C.c_double(), C.c_double()
+        self._chk(self.L.tis_bdhi_timing(self.ctx, C.byref(t), C.byref(f), C.byref(k)), "tis_bdhi_timing")
+        return {"total_ms": t.value, "fill_ms": f.value, "factor_ms": k.value}
