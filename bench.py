@@ -142,6 +142,77 @@ def serving_stream(njobs=100, scale=0.01, inflight=5.0, round_steps=500, seed=SE
     return out
 
 
+def ewald_leg(steps=400, warmup=40):
+    """Explicit ions with Ewald: the 76-nt chain + 565 ions in a 150 A cube
+    (alpha 0.1, r_c 30 A, |h| <= 10), Langevin in hipGraph mode."""
+    import oracle
+    s, x = T.config2_ewald()
+    eng = Engine([s], keys=[0], seed=SEED)
+    eng.set_coords(0, x)
+    eng.langevin_init(tstep=0.05, rebuild_every=10, step0=0, init_accel=True)
+    eng.langevin_steps(warmup, MODE_GRAPH)
+    eng.synchronize()
+    t0 = time.perf_counter()
+    eng.langevin_steps(steps, MODE_GRAPH)
+    eng.synchronize()
+    dt = time.perf_counter() - t0
+    eng.langevin_steps(50, MODE_EAGER)
+    tm = eng.last_timing()
+    eng.neighbor(0)
+    fg = eng.force_sumup(0)
+    xn = eng.coords(0)
+    o = oracle.Oracle(s)
+    o.neighbor(xn)
+    fo, _ = o.force(xn, seed=0, sys_id=0, step=0)
+    nf = np.linalg.norm(fo, axis=1)
+    err = float((np.linalg.norm(fg - fo, axis=1) / np.maximum(nf, 1e-3 * nf.max())).max())
+    eo = o.energy(xn)[7]
+    eg = eng.energy_sumup(0)[7]
+    eng.close()
+    return {"workload": "76-nt chain + 10 Mg2+ 305 K+ 250 Cl- in a 150 A periodic cube; Ewald alpha 0.1, "
+                        "r_c 30 A, |h| <= 10 (4168 vectors); Langevin dt 0.05 tau",
+            "particles": s.nmp, "ms_per_step": dt * 1e3 / steps, "bead_steps_per_s": s.nmp * steps / dt,
+            "force_max_rel_err_vs_fp64": err, "ele_energy_rel_err_vs_fp64": abs(eg - eo) / abs(eo),
+            "eager_pre_ms": tm["pre_ms"], "eager_main_ms": tm["main_ms"]}
+
+
+def bdhi_leg(sizes=(300, 1000, 3000), steps=50, refactor=10):
+    """Brownian dynamics with HI: bead-spring chains (radius 3.5 A, FENE +
+    WCA), RPY (Zuk et al.) tensor refactored every `refactor` steps."""
+    from tis_amd.bdhi import bead_spring_chain
+    from tis_amd.engine import BDEngine
+    out = []
+    for n in sizes:
+        s, x = bead_spring_chain(n, seed=SEED % 1000 + n)
+        e = BDEngine(s, key=n, seed=SEED)
+        e.set_coords(x)
+        e.tensors()                       # capture + first run
+        e.tensors()
+        tt = e.timing()
+        rel = e.factor_error()
+        n3 = 3 * n
+        flop = n3 ** 3 / 3.0
+        rec = {"beads": n, "dim": n3, "fill_ms": tt["fill_ms"], "factor_ms": tt["factor_ms"],
+               "factor_tflops": flop / (tt["factor_ms"] * 1e-3) / 1e12, "llt_rel_err": rel}
+        if n <= 1000:
+            D = e.tensor(0)
+            L = e.tensor(1)
+            L64 = np.linalg.cholesky(D)
+            rec["l_rel_err_vs_fp64"] = float(np.abs(L - L64).max() / np.abs(L64).max())
+        e.steps(2, tstep=0.05, refactor_every=refactor)
+        t0 = time.perf_counter()
+        e.steps(steps, tstep=0.05, refactor_every=refactor)
+        dt = time.perf_counter() - t0
+        rec.update({"steps": steps, "refactor_every": refactor, "ms_per_step": dt * 1e3 / steps,
+                    "bead_steps_per_s": n * steps / dt})
+        e.close()
+        out.append(rec)
+        print(f"[bench] bdhi n={n}: fill {rec['fill_ms']:.3f} ms, factor {rec['factor_ms']:.3f} ms "
+              f"({rec['factor_tflops']:.2f} TFLOP/s), ||LL^T-D||/||D|| {rel:.2e}, "
+              f"{rec['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+    return out
+
+
 def cpu_baseline(systems, coords, budget_s=15.0):
     """Oracle Langevin (FP64, O(N^2) list build, rebuild every 10 steps) on
     this host, one core, over chains of the same workload until budget_s."""
@@ -178,6 +249,9 @@ def main():
     ap.add_argument("--serve-jobs", type=int, default=100)
     ap.add_argument("--serve-scale", type=float, default=0.01)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-ewald", action="store_true")
+    ap.add_argument("--no-bdhi", action="store_true")
+    ap.add_argument("--swap-every", type=int, default=1000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the systems of this workload are independent and would be "
@@ -194,10 +268,21 @@ def main():
     print(f"[bench] {len(systems)} systems, {nbeads} beads, setup {time.perf_counter() - t_build:.1f} s",
           file=sys.stderr, flush=True)
 
+    # the 64-replica ladder: temperature labels swap every swap_every steps
+    from tis_amd.remd import ReplicaLadder
+    rep_slots = [k for k, lab in enumerate(labels) if lab == "replica"]
+    ladder = ReplicaLadder(eng, rep_slots, systems[rep_slots[0]], [systems[k].tempk for k in rep_slots],
+                           seed=SEED, key=1)
     eng.langevin_steps(args.warmup, MODE_GRAPH)
     eng.synchronize()
     t0 = time.perf_counter()
-    eng.langevin_steps(args.steps, MODE_GRAPH)
+    done = 0
+    while done < args.steps:
+        chunk = min(args.swap_every - (done % args.swap_every), args.steps - done)
+        eng.langevin_steps(chunk, MODE_GRAPH)
+        done += chunk
+        if done % args.swap_every == 0:
+            ladder.attempt()
     eng.synchronize()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
@@ -252,6 +337,11 @@ def main():
         print(f"[bench] serving: {serve['jobs']} jobs, p50 {serve.get('p50_latency', 0):.3f} s, "
               f"p99 {serve.get('p99_latency', 0):.3f} s, {serve['bead_steps_per_s_busy']:.4e} bead-steps/s, "
               f"mean in flight {serve.get('mean_inflight', 0):.2f}", file=sys.stderr, flush=True)
+    ewald = None if args.no_ewald else ewald_leg()
+    if ewald:
+        print(f"[bench] ewald: {ewald['ms_per_step']:.4f} ms/step, force err {ewald['force_max_rel_err_vs_fp64']:.2e}",
+              file=sys.stderr, flush=True)
+    bdhi = None if args.no_bdhi else bdhi_leg()
     cpu = None if args.no_cpu else cpu_baseline(systems, coords, args.cpu_budget)
     out = {
         "metric": "batched bead-steps/s vs plain GPU @ force err <=1e-5 vs FP64; p50/p99 job latency",
@@ -267,7 +357,8 @@ def main():
         "dtype": "fp32 pair / fp64 accumulate+integrate",
         "data": "synthetic (seeded hairpin designs, extended-helix start)",
         "config": {"workload": "512 lognormal(ln120,0.7) 20-600 nt DT15 chains + 64 replicas of the 64-nt "
-                               "hairpin on a 290-400 K ladder; Langevin dt 0.05 tau; rebuild every 10 steps",
+                               "hairpin on a 290-400 K ladder with swap attempts every 1000 steps; "
+                               "Langevin dt 0.05 tau; rebuild every 10 steps",
                    "systems": len(systems), "beads": nbeads, "parallelism": "single GPU, batched"},
         "plain_gpu_bead_steps_per_s": plain,
         "batched_over_plain": value / plain,
@@ -275,6 +366,10 @@ def main():
         "p50_job_latency_s": serve.get("p50_latency") if serve else None,
         "p99_job_latency_s": serve.get("p99_latency") if serve else None,
         "serving": serve,
+        "remd": {"replicas": len(rep_slots), "swap_every": args.swap_every, "attempts": ladder.attempts,
+                 "acceptance": ladder.acceptance},
+        "ewald": ewald,
+        "bdhi": bdhi,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

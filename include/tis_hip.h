@@ -89,7 +89,9 @@ int32_t tis_force_get(tis_ctx *ctx, int32_t isys, double *force_out);
 
 /* energy_sumup: energy[TIS_E_MAX] (E_TYPE slots, TIS_E_TOTAL excludes VELO)
  * of system isys at the current coordinates, lists and last hbond state;
- * kinetic term from the stored velocities. */
+ * kinetic term from the stored velocities.  isys = -1: every system in one
+ * launch, energy_out[nsys * TIS_E_MAX] (the replica loop of
+ * energy_allrep.F90:123-125). */
 int32_t tis_energy_sumup(tis_ctx *ctx, int32_t isys, double *energy_out);
 
 /* Langevin dynamics for every system of the batch.

@@ -878,19 +878,18 @@ int32_t tis_energy_sumup(tis_ctx *c, int32_t isys, double *energy_out) {
   if (!c || !energy_out) return TIS_ERR_ARG;
   int rc = check_sys(c, isys);
   if (rc) return rc;
-  if (isys < 0) return fail(c, TIS_ERR_ARG, "tis_energy_sumup needs one system");
   HIPCHK(c, hipSetDevice(c->device));
   if (!c->lists_valid) { rc = rebuild_lists(c, -1); if (rc) return rc; }
-  {
-    const HostSys &h = c->hs[isys];
-    if (h.esk_count > 0)  // structure factors at the current coordinates
-      hipLaunchKernelGGL(k_ewald_sk, dim3(h.esk_count), dim3(kTile), 0, c->stream, c->P, PXc(c),
-                         c->d_ewsk + h.esk_begin, 1);
-  }
-  hipLaunchKernelGGL(k_energy, dim3(1), dim3(256), 0, c->stream, c->P, Xc(c), PXc(c), c->d_sysidx + isys,
-                     c->d_energy);
+  // structure factors at the current coordinates (FP64 sin/cos)
+  const int ek0 = isys < 0 ? 0 : c->hs[isys].esk_begin, ekn = isys < 0 ? c->n_ewsk : c->hs[isys].esk_count;
+  if (ekn > 0)
+    hipLaunchKernelGGL(k_ewald_sk, dim3(ekn), dim3(kTile), 0, c->stream, c->P, PXc(c), c->d_ewsk + ek0, 1);
+  const int nsel = isys < 0 ? c->nsys : 1;
+  hipLaunchKernelGGL(k_energy, dim3(nsel), dim3(256), 0, c->stream, c->P, Xc(c), PXc(c),
+                     c->d_sysidx + (isys < 0 ? 0 : isys), c->d_energy);
   HIPCHK(c, hipGetLastError());
-  HIPCHK(c, hipMemcpyAsync(energy_out, c->d_energy, 25 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(energy_out, c->d_energy, 25 * sizeof(double) * (size_t)nsel, hipMemcpyDeviceToHost,
+                           c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return TIS_OK;
 }

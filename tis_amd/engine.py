@@ -173,6 +173,12 @@ class Engine:
         self._chk(self.L.tis_energy_sumup(self.ctx, isys, _fp(e)), "tis_energy_sumup")
         return e
 
+    def energy_all(self):
+        """(nsys, E_MAX) energies of every system in one launch."""
+        e = np.zeros((len(self.systems), E_MAX))
+        self._chk(self.L.tis_energy_sumup(self.ctx, -1, _fp(e)), "tis_energy_sumup")
+        return e
+
     # ---- Langevin
     def system_origin(self, isys, step_origin, init_accel):
         self._chk(self.L.tis_system_origin(self.ctx, isys, int(step_origin), 1 if init_accel else 0),
