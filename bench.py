@@ -316,7 +316,8 @@ def main():
         rng = np.random.default_rng(1)
         sample = sorted(set(rng.choice(len(systems), size=6, replace=False).tolist()) | {int(np.argmax([s.nmp for s in systems]))})
         now = [eng.coords(k) for k in range(len(systems))]
-        ferr = force_check(systems, now, eng, sample)
+        # replica slots carry the parameters of their current ladder label
+        ferr = force_check(eng.systems, now, eng, sample)
         print(f"[bench] force max rel err vs FP64 oracle on {len(sample)} systems: {ferr:.3e}",
               file=sys.stderr, flush=True)
 

@@ -110,6 +110,18 @@ int32_t tis_langevin_steps(tis_ctx *ctx, int64_t nsteps, int32_t mode);
  * new accelerations and the others keep their stored ones. */
 int32_t tis_system_origin(tis_ctx *ctx, int32_t isys, uint64_t step_origin, int32_t init_accel);
 
+/* Temperature labels for replica exchange.  tis_label_define stores the
+ * temperature-dependent coefficients of `sys` (charges, stack U0, ele_coef,
+ * ele_cdist, beta_hb, tempk) as label `label`; sys must have the topology of
+ * system isys.  tis_label_apply gives each system isys[k] the coefficients
+ * of label[k] in one launch, refreshes its Langevin noise amplitude, and
+ * multiplies its velocities by vscale[k] (NULL: no scaling).  Together they
+ * replace the per-replica parameter refresh of energy_allrep.F90:150-216,
+ * simu_replica_exchange.F90:272-300 and time_integral_post.F90:264-370. */
+int32_t tis_label_define(tis_ctx *ctx, int32_t label, int32_t isys, const tis_system *sys);
+int32_t tis_label_apply(tis_ctx *ctx, int32_t n, const int32_t *isys, const int32_t *label,
+                        const double *vscale);
+
 /* Device memory held by the context's batch, in bytes (admission control). */
 int32_t tis_device_bytes(tis_ctx *ctx, int64_t *bytes);
 

@@ -25,7 +25,7 @@ def _observables(s, x, v, e):
 
 def test_trajectory_statistics_match_oracle():
     from scipy import stats
-    nrep, burn, chunk, nsamp = 16, 2000, 200, 50
+    nrep, burn, chunk, nsamp = 32, 2000, 200, 40
     systems, x0 = [], []
     for k in range(nrep):
         rng = np.random.default_rng(40 + k)
@@ -61,25 +61,28 @@ def test_trajectory_statistics_match_oracle():
     g = np.asarray(gpu)   # (nrep, nsamp, 4)
     o = np.asarray(ora)
     names = ["T_kin", "Rg", "E_stack", "E_hbond"]
+    # samples of one replica are correlated in time; the replicas are
+    # independent, so the tests run on one time average per replica
     for q, name in enumerate(names):
-        gm, om = g[:, :, q].mean(axis=1), o[:, :, q].mean(axis=1)   # one mean per independent replica
+        gm, om = g[:, :, q].mean(axis=1), o[:, :, q].mean(axis=1)
         se = np.sqrt(gm.var(ddof=1) / nrep + om.var(ddof=1) / nrep)
-        ks = stats.ks_2samp(g[:, :, q].ravel(), o[:, :, q].ravel())
+        ks = stats.ks_2samp(gm, om)
         print(f"   {name}: gpu {gm.mean():.4f} oracle {om.mean():.4f} (se {se:.4f}), KS p {ks.pvalue:.3g}",
               flush=True)
         assert abs(gm.mean() - om.mean()) <= 4.0 * se + 1e-12
-        assert ks.pvalue > 1e-4
+        assert ks.pvalue > 1e-3
 
 
 def test_kinetic_temperature_of_the_integrator():
     """<T_kin> of the Langevin integrator at T = 300 K, eta = 1e-5 Pa s for a
-    4-nt chain, dt = 0.05 and 0.01 tau.  The reference's temperature check
-    (integrator_test/Temperature_check, DT13 'uuuu', same integrator) reports
-    284.2-284.3 K and 296.7-297.1 K; the deficit comes from the integrator
-    and shrinks with dt."""
-    ncopy, burn, steps = 512, 2000, 20000
+    4-nt DT15 chain, dt = 0.05 and 0.01 tau.  The reference's own DT15 check
+    (integrator_test/Temperature_check, 'azo' at 310 K, dt 0.05) reports
+    309.2-309.7 K, within 0.3 % of the set temperature; its DT13 'uuuu'
+    runs (284 K at dt 0.05) use a model that is not on this path."""
+    ncopy, burn_tau, run_tau = 512, 300.0, 600.0
     out = {}
     for dt in (0.05, 0.01):
+        burn, steps = int(burn_tau / dt), int(run_tau / dt)
         systems = []
         for k in range(ncopy):
             s = T.build_rna_system([("UUUU", [])], tempk=300.0, ionic_strength=0.1)
@@ -98,5 +101,5 @@ def test_kinetic_temperature_of_the_integrator():
         out[dt] = float(np.mean(tks))
         eng.close()
         print(f"   dt {dt}: <T_kin> = {out[dt]:.2f} K", flush=True)
-    assert out[0.01] > out[0.05]
-    assert 270.0 < out[0.05] < 300.0 and 290.0 < out[0.01] < 302.0
+    for dt, tk in out.items():
+        assert abs(tk - 300.0) < 3.0, (dt, tk)

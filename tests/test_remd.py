@@ -29,8 +29,15 @@ class FixedEnergies:
         self.T = [None] * len(energies)
         self.v = [np.ones((nmp, 3)) * (k + 1) for k in range(len(energies))]
 
-    def set_temperature_system(self, k, sysd):
-        self.T[k] = sysd.tempk
+    def label_define(self, label, isys, sysd):
+        self.labels = getattr(self, "labels", {})
+        self.labels[label] = sysd.tempk
+
+    def label_apply(self, isys, labels, vscale=None):
+        for n, (k, lab) in enumerate(zip(isys, labels)):
+            self.T[k] = self.labels[lab]
+            if vscale is not None:
+                self.v[k] = self.v[k] * vscale[n]
 
     def energy_all(self):
         e = np.zeros((len(self.E), 25))
@@ -67,7 +74,7 @@ def test_acceptance_matches_closed_form():
     # pair (0,1): E1 < E0 -> delta < 0 -> swapped; replica 0 now at label 1, heated
     assert res[0][3] and lad.rep2lab[0] == 1 and lad.lab2rep[1] == 0
     assert np.allclose(eng.v[0], 1.0 * math.sqrt(320.0 / 300.0))
-    assert np.allclose(eng.v[1], 2.0 / math.sqrt(320.0 / 300.0))
+    assert np.allclose(eng.v[1], 2.0 / math.sqrt(320.0 / 300.0), rtol=1e-14)
     assert eng.T[0] == 320.0 and eng.T[1] == 300.0
     res2 = lad.attempt()
     assert [(a, b) for a, b, _, _ in res2] == [(1, 2)]
@@ -113,3 +120,35 @@ def test_swapped_temperature_energies_gpu():
             f = math.sqrt(temps[lj] / temps[li])
             assert np.allclose(eng.get_state(ri, 1), v_before[ri] * f, rtol=1e-15)
     eng.langevin_steps(20, MODE_GRAPH)   # the batch keeps running after a swap
+
+
+@pytest.mark.gpu
+def test_label_apply_equals_set_temperature_gpu():
+    """A device label swap gives the bits of a host parameter refresh
+    (tis_set_system_T), velocities aside."""
+    from tis_amd.engine import Engine, MODE_GRAPH
+    temps = [300.0, 340.0]
+    systems = []
+    for t in temps:
+        s = T.build_rna_system([T.config0_chain()])
+        T.set_temperature(s, t, 0.150)
+        systems.append(s)
+    x = T.helix_coords(systems[0], np.random.default_rng(4))
+    out = []
+    for route in ("host", "label"):
+        eng = Engine([systems[0], systems[0]], keys=[0, 1], seed=3)
+        for k in range(2):
+            eng.set_coords(k, x)
+        eng.langevin_init(tstep=0.05, rebuild_every=10)
+        eng.langevin_steps(40, MODE_GRAPH)
+        if route == "host":
+            eng.set_temperature_system(1, systems[1])
+        else:
+            eng.label_define(0, 0, systems[0])
+            eng.label_define(1, 0, systems[1])
+            eng.label_apply([1], [1])
+        eng.langevin_steps(60, MODE_GRAPH)
+        out.append([eng.coords(1), eng.get_state(1, 1), eng.energy_sumup(1)])
+        eng.close()
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)

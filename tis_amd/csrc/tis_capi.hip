@@ -32,6 +32,7 @@ __global__ void k_list_hb(DevPtrs, const double *, const double *, int, int);
 __global__ void k_advance(uint64_t *, uint64_t, unsigned long long *);
 __global__ void k_count_rebuild(unsigned long long *);
 __global__ void k_energy(DevPtrs, const double *, const double *, const int32_t *, double *);
+__global__ void k_label_apply(DevPtrs, LabelDev, const int32_t *, const int32_t *, const double *);
 }  // namespace tis
 
 using namespace tis;
@@ -100,6 +101,15 @@ struct tis_ctx {
   size_t tev_used = 0;
   // Brownian dynamics with hydrodynamic interaction (tis_bdhi.hip)
   tis::BdState *bd = nullptr;
+  // temperature labels (replica exchange)
+  struct Label { bool set = false; DSys d{}; int32_t nmp = 0, nst = 0; std::vector<double> q, u0; };
+  std::vector<Label> labels;
+  bool labels_dirty = false;
+  std::vector<void *> label_bufs;         // device copies of the label sets
+  LabelDev label_dev{};
+  int32_t *d_lab_args = nullptr;          // [2][cap] isys, label
+  double *d_lab_scale = nullptr;          // [cap]
+  int32_t lab_cap = 0;
 };
 
 namespace tis {
@@ -304,6 +314,9 @@ int32_t tis_ctx_destroy(tis_ctx *c) {
   (void)hipSetDevice(c->device);
   free_all(c);
   bd_free(c);
+  for (void *p : c->label_bufs) (void)hipFree(p);
+  if (c->d_lab_args) (void)hipFree(c->d_lab_args);
+  if (c->d_lab_scale) (void)hipFree(c->d_lab_scale);
   for (hipEvent_t e : c->tev) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -576,6 +589,13 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
   uint32_t *d_inc;
   float *d_eps;
   UP(d_rcm, rcm); UP(d_mass, mass); UP(d_rad, exv_rad); UP(d_charge, charge); UP(d_rl2, rl2);
+  {
+    std::vector<double> fr;
+    for (int is = 0; is < nsys; ++is) fr.insert(fr.end(), c->hs[is].fric.begin(), c->hs[is].fric.end());
+    double *d_fric;
+    UP(d_fric, fr);
+    P.fric = d_fric;
+  }
   UP(d_mpt, mptype); UP(d_icl, iclass); UP(d_unit, unit); UP(d_qidx, qidx); UP(d_qbead, qbead);
   UP(d_incoff, inc_off); UP(d_inc, inc); UP(d_eps, exv_eps);
   int32_t *d_pos;
@@ -909,6 +929,91 @@ int32_t tis_system_origin(tis_ctx *c, int32_t isys, uint64_t step_origin, int32_
   return TIS_OK;
 }
 
+int32_t tis_label_define(tis_ctx *c, int32_t label, int32_t isys, const tis_system *s) {
+  if (!c || !s || label < 0 || label > 1000000) return TIS_ERR_ARG;
+  int rc = check_sys(c, isys);
+  if (rc) return rc;
+  if (isys < 0) return fail(c, TIS_ERR_ARG, "tis_label_define needs a template system");
+  const HostSys &h = c->hs[isys];
+  if (s->nmp != h.d.nmp || s->nst != h.d.nst) return fail(c, TIS_ERR_ARG, "label topology differs from the template");
+  if ((int)c->labels.size() <= label) c->labels.resize(label + 1);
+  tis_ctx::Label &L = c->labels[label];
+  L.set = true;
+  L.d = h.d;
+  fill_derived(L.d, *s);
+  L.nmp = s->nmp;
+  L.nst = s->nst;
+  L.q.assign(s->charge, s->charge + s->nmp);
+  L.u0.resize(s->nst);
+  for (int t = 0; t < s->nst; ++t) L.u0[t] = s->st_coef[4 * t];
+  c->labels_dirty = true;
+  return TIS_OK;
+}
+
+int32_t tis_label_apply(tis_ctx *c, int32_t n, const int32_t *isys, const int32_t *label, const double *vscale) {
+  if (!c || n < 0 || (n > 0 && (!isys || !label))) return TIS_ERR_ARG;
+  if (n == 0) return TIS_OK;
+  int rc = check_sys(c, -1);
+  if (rc) return rc;
+  for (int k = 0; k < n; ++k) {
+    if (isys[k] < 0 || isys[k] >= c->nsys) return fail(c, TIS_ERR_ARG, "system %d out of range", isys[k]);
+    if (label[k] < 0 || label[k] >= (int)c->labels.size() || !c->labels[label[k]].set)
+      return fail(c, TIS_ERR_ARG, "label %d not defined", label[k]);
+    const tis_ctx::Label &L = c->labels[label[k]];
+    if (L.nmp != c->hs[isys[k]].d.nmp || L.nst != c->hs[isys[k]].d.nst)
+      return fail(c, TIS_ERR_ARG, "label %d does not fit system %d", label[k], isys[k]);
+    for (int j = 0; j < k; ++j)
+      if (isys[j] == isys[k]) return fail(c, TIS_ERR_ARG, "system %d listed twice", isys[k]);
+  }
+  HIPCHK(c, hipSetDevice(c->device));
+  if (c->labels_dirty) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (void *p : c->label_bufs) (void)hipFree(p);
+    c->label_bufs.clear();
+    std::vector<DSys> ds;
+    std::vector<double> q, u0;
+    std::vector<int64_t> qo, uo;
+    for (const auto &L : c->labels) {
+      ds.push_back(L.d);
+      qo.push_back((int64_t)q.size());
+      uo.push_back((int64_t)u0.size());
+      q.insert(q.end(), L.q.begin(), L.q.end());
+      u0.insert(u0.end(), L.u0.begin(), L.u0.end());
+    }
+    auto up = [&](const void *src, size_t bytes, void **dst) -> int {
+      HIPCHK(c, hipMalloc(dst, std::max<size_t>(bytes, 8)));
+      c->label_bufs.push_back(*dst);
+      if (bytes) HIPCHK(c, hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+      return 0;
+    };
+    void *p;
+    if ((rc = up(ds.data(), ds.size() * sizeof(DSys), &p))) return rc; c->label_dev.sys = (const DSys *)p;
+    if ((rc = up(q.data(), q.size() * sizeof(double), &p))) return rc; c->label_dev.q = (const double *)p;
+    if ((rc = up(qo.data(), qo.size() * sizeof(int64_t), &p))) return rc; c->label_dev.qoff = (const int64_t *)p;
+    if ((rc = up(u0.data(), u0.size() * sizeof(double), &p))) return rc; c->label_dev.u0 = (const double *)p;
+    if ((rc = up(uo.data(), uo.size() * sizeof(int64_t), &p))) return rc; c->label_dev.u0off = (const int64_t *)p;
+    c->labels_dirty = false;
+  }
+  if (c->lab_cap < n) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->d_lab_args) (void)hipFree(c->d_lab_args);
+    if (c->d_lab_scale) (void)hipFree(c->d_lab_scale);
+    c->d_lab_args = nullptr; c->d_lab_scale = nullptr;
+    HIPCHK(c, hipMalloc((void **)&c->d_lab_args, 2 * sizeof(int32_t) * (size_t)n));
+    HIPCHK(c, hipMalloc((void **)&c->d_lab_scale, sizeof(double) * (size_t)n));
+    c->lab_cap = n;
+  }
+  HIPCHK(c, hipMemcpyAsync(c->d_lab_args, isys, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_lab_args + c->lab_cap, label, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+  if (vscale) HIPCHK(c, hipMemcpyAsync(c->d_lab_scale, vscale, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_label_apply, dim3(n), dim3(256), 0, c->stream, c->P, c->label_dev, c->d_lab_args,
+                     c->d_lab_args + c->lab_cap, vscale ? (const double *)c->d_lab_scale : nullptr);
+  HIPCHK(c, hipGetLastError());
+  for (int k = 0; k < n; ++k) copy_t_fields(c->hs[isys[k]].d, c->labels[label[k]].d);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TIS_OK;
+}
+
 int32_t tis_device_bytes(tis_ctx *c, int64_t *bytes) {
   if (!c || !bytes) return TIS_ERR_ARG;
   *bytes = (int64_t)c->dev_bytes;
@@ -923,6 +1028,7 @@ int32_t tis_langevin_init(tis_ctx *c, double tstep, int32_t rebuild_every, uint6
   c->tstep = tstep;
   c->rebuild_every = rebuild_every;
   c->P.tstep2 = 0.5 * tstep * tstep;
+  c->P.tstep = tstep;
   for (int k = 0; k < 2; ++k)
     if (c->graph[k]) { (void)hipGraphExecDestroy(c->graph[k]); c->graph[k] = nullptr; }
   // per-bead constants

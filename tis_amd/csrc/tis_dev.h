@@ -93,6 +93,7 @@ struct DevPtrs {
   double *x[2], *px[2];      // [3][ntot] each
   double *v, *a, *f;         // [3][ntot]
   const double *rlan;        // [4][ntot]
+  const double *fric;        // [ntot] friction / mass (Langevin)
   const double *rcm, *mass;
   const int8_t *mptype, *iclass;
   const int32_t *unit, *qidx;   // qidx: charge index (global) or -1
@@ -140,6 +141,21 @@ struct DevPtrs {
   uint64_t *stepbase;
   uint64_t seed;
   double tstep2;
+  double tstep;
+};
+
+// temperature-dependent fields of a system descriptor (simu_para2.F90:8-108)
+__host__ __device__ inline void copy_t_fields(DSys &d, const DSys &s) {
+  d.tempk = s.tempk; d.ele_coef = s.ele_coef; d.cdist = s.cdist; d.rcdist = s.rcdist; d.ele_rl2 = s.ele_rl2;
+  d.dh_cut2 = s.dh_cut2; d.dh_ecut2 = s.dh_ecut2; d.beta_hb = s.beta_hb;
+  d.ew_fscale = s.ew_fscale; d.ew_self = s.ew_self;
+}
+
+// parameter sets of temperature labels (replica exchange)
+struct LabelDev {
+  const DSys *sys;
+  const double *q;   const int64_t *qoff;    // per-bead charges of each label
+  const double *u0;  const int64_t *u0off;   // per-stack U0 of each label
 };
 
 // ---------------------------------------------------------------- RNG

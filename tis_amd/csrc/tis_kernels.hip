@@ -1266,4 +1266,39 @@ __global__ __launch_bounds__(kEBlock) void k_energy(DevPtrs P, const double *X, 
   }
 }
 
+// ------------------------------------------------------------------ labels
+// Replica exchange: system isys[k] takes the temperature-dependent
+// coefficients of label lab[k] (charges, stack U0, descriptor fields and the
+// Langevin noise amplitude c1 of time_integral_pre.F90:163-164, as the
+// refresh of time_integral_post.F90:264-370 does) and, with vscale, has its
+// velocities scaled (simu_replica_exchange.F90:272-300).  One block per
+// entry; the descriptor is written after every lane has read it.
+__global__ __launch_bounds__(256) void k_label_apply(DevPtrs P, LabelDev L, const int32_t *isys, const int32_t *lab,
+                                                     const double *vscale) {
+#pragma clang fp contract(off)
+  const int k = blockIdx.x;
+  const int is = isys[k], lb = lab[k];
+  const DSys t = P.sys[is];
+  const DSys src = L.sys[lb];
+  const int n = P.ntot;
+  double *charge = const_cast<double *>(P.charge);
+  double *rlan = const_cast<double *>(P.rlan);
+  double *stc = const_cast<double *>(P.st_coef);
+  const double f = vscale ? vscale[k] : 1.0;
+  for (int i = threadIdx.x; i < t.nmp; i += blockDim.x) {
+    const int g = t.off + i;
+    charge[g] = L.q[L.qoff[lb] + i];
+    if (P.tstep > 0.0) rlan[g] = sqrt(2.0 * P.fric[g] * kBoltzKcalMol * src.tempk / (P.tstep * P.mass[g]));
+    if (vscale)
+      for (int d = 0; d < 3; ++d) P.v[(size_t)d * n + g] *= f;
+  }
+  for (int q = threadIdx.x; q < t.nst; q += blockDim.x) stc[4 * (size_t)(t.soff + q)] = L.u0[L.u0off[lb] + q];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    DSys d = t;
+    copy_t_fields(d, src);
+    const_cast<DSys *>(P.sys)[is] = d;
+  }
+}
+
 }  // namespace tis

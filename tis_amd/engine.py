@@ -62,13 +62,15 @@ def load_library():
     L.tis_langevin_steps.argtypes = [vp, C.c_int64, C.c_int32]
     L.tis_system_origin.argtypes = [vp, C.c_int32, C.c_uint64, C.c_int32]
     L.tis_device_bytes.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.tis_label_define.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(CSystem)]
+    L.tis_label_apply.argtypes = [vp, C.c_int32, _i32p, _i32p, _f64p]
     L.tis_status_counters.argtypes = [vp, C.POINTER(C.c_int64)]
     L.tis_synchronize.argtypes = [vp]
     L.tis_last_timing.argtypes = [vp, _f64p, _f64p, _f64p, C.POINTER(C.c_int64)]
     for fn in ("tis_ctx_create", "tis_ctx_destroy", "tis_system_upload", "tis_set_system_T", "tis_state_set",
                "tis_state_get", "tis_neighbor", "tis_pairs_get", "tis_force_sumup", "tis_force_get",
                "tis_energy_sumup", "tis_langevin_init", "tis_langevin_steps", "tis_status_counters",
-               "tis_system_origin", "tis_device_bytes",
+               "tis_system_origin", "tis_device_bytes", "tis_label_define", "tis_label_apply",
                "tis_synchronize", "tis_last_timing"):
         getattr(L, fn).restype = C.c_int32
     _LIB = L
@@ -194,6 +196,24 @@ class Engine:
 
     def synchronize(self):
         self._chk(self.L.tis_synchronize(self.ctx), "tis_synchronize")
+
+    def label_define(self, label, isys_template, system: System):
+        cs = system.cstruct()
+        self._chk(self.L.tis_label_define(self.ctx, label, isys_template, C.byref(cs)), "tis_label_define")
+        if not hasattr(self, "_labels"):
+            self._labels = {}
+        self._labels[label] = system
+
+    def label_apply(self, isys, labels, vscale=None):
+        """Systems isys[k] take label labels[k]; velocities times vscale[k]."""
+        n = len(isys)
+        a = np.ascontiguousarray(isys, dtype=np.int32)
+        b = np.ascontiguousarray(labels, dtype=np.int32)
+        v = None if vscale is None else np.ascontiguousarray(vscale, dtype=np.float64)
+        self._chk(self.L.tis_label_apply(self.ctx, n, a.ctypes.data_as(_i32p), b.ctypes.data_as(_i32p),
+                                         None if v is None else _fp(v)), "tis_label_apply")
+        for k, lab in zip(a, b):
+            self.systems[int(k)] = self._labels[int(lab)]
 
     def device_bytes(self):
         n = C.c_int64()

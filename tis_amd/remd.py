@@ -46,8 +46,10 @@ class ReplicaLadder:
         self.attempts = 0
         self.accepted = 0
         self.tried = 0
-        for r, lab in enumerate(self.rep2lab):
-            eng.set_temperature_system(self.slots[r], self.label_sys[lab])
+        # every temperature's coefficients live on the device as a label
+        for lab, s in enumerate(self.label_sys):
+            eng.label_define(lab, self.slots[0], s)
+        eng.label_apply(self.slots, self.rep2lab)
 
     def partner(self, lab):
         """get_pair of var_replica.F90:158-162 for a 1-D ladder."""
@@ -59,18 +61,13 @@ class ReplicaLadder:
         eall = self.eng.energy_all()
         own = np.array([eall[s, E_TOTAL] for s in self.slots])
         part = np.zeros_like(own)
-        moved = []
-        for r, lab in enumerate(self.rep2lab):
-            p = self.partner(lab)
-            if p >= 0:
-                self.eng.set_temperature_system(self.slots[r], self.label_sys[p])
-                moved.append(r)
+        moved = [r for r, lab in enumerate(self.rep2lab) if self.partner(lab) >= 0]
         if moved:
+            self.eng.label_apply([self.slots[r] for r in moved], [self.partner(self.rep2lab[r]) for r in moved])
             eall = self.eng.energy_all()
             for r in moved:
                 part[r] = eall[self.slots[r], E_TOTAL]
-            for r in moved:
-                self.eng.set_temperature_system(self.slots[r], self.label_sys[self.rep2lab[r]])
+            self.eng.label_apply([self.slots[r] for r in moved], [self.rep2lab[r] for r in moved])
         return own, part
 
     def attempt(self):
@@ -79,7 +76,7 @@ class ReplicaLadder:
         own, part = self.energies()
         out = []
         nrand = 0
-        changed = set()
+        scale = {}
         for li in range(len(self.temps)):
             lj = self.partner(li)
             if lj < li:
@@ -93,16 +90,13 @@ class ReplicaLadder:
                 self.rep2lab[ri], self.rep2lab[rj] = lj, li
                 self.lab2rep[li], self.lab2rep[lj] = rj, ri
                 f = math.sqrt(tn / tm)
-                vi = self.eng.get_state(self.slots[ri], 1)
-                vj = self.eng.get_state(self.slots[rj], 1)
-                self.eng.set_state(self.slots[ri], 1, vi * f)
-                self.eng.set_state(self.slots[rj], 1, vj / f)
-                changed.update((ri, rj))
+                scale[ri], scale[rj] = f, 1.0 / f
                 self.accepted += 1
             self.tried += 1
             out.append((li, lj, float(delta), bool(acc)))
-        for r in changed:
-            self.eng.set_temperature_system(self.slots[r], self.label_sys[self.rep2lab[r]])
+        if scale:
+            rs = sorted(scale)
+            self.eng.label_apply([self.slots[r] for r in rs], [self.rep2lab[r] for r in rs], [scale[r] for r in rs])
         self.parity ^= 1
         self.attempts += 1
         return out
