@@ -252,11 +252,30 @@ def main():
     ap.add_argument("--no-ewald", action="store_true")
     ap.add_argument("--no-bdhi", action="store_true")
     ap.add_argument("--swap-every", type=int, default=1000)
+    ap.add_argument("--only", choices=["ewald", "bdhi", "copy"], default=None,
+                    help="run one side leg alone (profiling); 'copy' is a plain 1 GiB device copy "
+                         "that calibrates the HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the systems of this workload are independent and would be "
               "sharded by bead budget across GPUs; multi-GPU runs are not part of this build.", file=sys.stderr)
         sys.exit(2)
+    if args.only == "ewald":
+        print(json.dumps({"ewald": ewald_leg()}))
+        return
+    if args.only == "bdhi":
+        print(json.dumps({"bdhi": bdhi_leg(sizes=(300, 3000), steps=20)}))
+        return
+    if args.only == "copy":
+        import torch
+        nbytes = 1 << 30
+        a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": nbytes, "launches": 3}))
+        return
 
     t_build = time.perf_counter()
     systems, coords, labels = build_workload()

@@ -162,69 +162,88 @@ __global__ void k_bd_reset(BdDev B) {
 
 // ---------------------------------------------------------------- Cholesky
 // Right-looking blocked factorisation of L (holding the lower part of D) in
-// place.  Per 64-column panel starting at k: the diagonal block in FP64 in
-// LDS, the panel below by an FP64 triangular solve (also storing an FP32 copy
-// P, column-major [64][mpad]), then the trailing lower triangle minus P P^T
-// on FP32 matrix cores, subtracted from the FP64 matrix.
-__global__ __launch_bounds__(256) void k_chol_diag(BdDev B, int k) {
-  __shared__ double A[kCb][kCb + 1];
-  const int tid = threadIdx.x, n3 = B.n3;
-  const int kb = min(kCb, n3 - k);
-  for (int idx = tid; idx < kCb * kCb; idx += 256) {
-    const int r = idx % kCb, c = idx / kCb;
-    A[r][c] = (r < kb && c < kb && r >= c) ? B.L[(size_t)(k + r) + (size_t)(k + c) * n3] : 0.0;
+// place.  Per 64-column panel starting at k: the diagonal block in FP64, the
+// panel below by an FP64 triangular solve (also storing an FP32 copy P,
+// column-major [64][mpad]), then the trailing lower triangle minus P P^T on
+// FP32 matrix cores, subtracted from the FP64 matrix.
+// Dot-product step shared by the diagonal factorisation and the panel
+// solve: a - sum_{l<c} x[l*64] y[l*64] with four partial sums.  Both
+// operands live in LDS column-major ([column][row], row = lane), so a lane's
+// own row is read conflict-free and another row is a broadcast.
+__device__ inline double chol_dot(double a, const double *x, const double *y, int c) {
+  double s0 = a, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int l = 0;
+  for (; l + 4 <= c; l += 4) {
+    s0 -= x[l * kCb] * y[l * kCb];
+    s1 -= x[(l + 1) * kCb] * y[(l + 1) * kCb];
+    s2 -= x[(l + 2) * kCb] * y[(l + 2) * kCb];
+    s3 -= x[(l + 3) * kCb] * y[(l + 3) * kCb];
   }
-  __syncthreads();
-  for (int j = 0; j < kb; ++j) {
-    if (tid == 0) {
-      const double p = A[j][j];
-      if (!(p > 0.0)) atomicCAS(B.info, 0, k + j + 1);
-      A[j][j] = sqrt(p);
-    }
-    __syncthreads();
-    const double dj = A[j][j];
-    for (int r = j + 1 + tid; r < kb; r += 256) A[r][j] /= dj;
-    __syncthreads();
-    for (int idx = tid; idx < kCb * kCb; idx += 256) {
-      const int r = idx % kCb, c = idx / kCb;
-      if (c > j && r >= c && r < kb) A[r][c] -= A[r][j] * A[c][j];
-    }
-    __syncthreads();
-  }
-  for (int idx = tid; idx < kb * kb; idx += 256) {
-    const int r = idx % kb, c = idx / kb;
-    B.L[(size_t)(k + r) + (size_t)(k + c) * n3] = r >= c ? A[r][c] : 0.0;
-  }
+  for (; l < c; ++l) s0 -= x[l * kCb] * y[l * kCb];
+  return (s0 + s1) + (s2 + s3);
 }
 
-// rows r >= k + 64: x L11^T = a  (forward substitution per row, FP64)
-__global__ __launch_bounds__(kCb) void k_chol_trsm(BdDev B, int k, float *P, int mpad) {
-  __shared__ double T[kCb][kCb + 1];
+// One panel: every block (one wave) factors the 64x64 diagonal block
+// column by column in the Crout order (lane r owns row r; a column needs
+// only earlier columns, so each step is one dot product per lane and one
+// barrier); block 0 stores it, blocks 1.. each solve 64 rows of the panel
+// below against it (forward substitution, FP64) and store them with their
+// FP32 copy P.  Global loads are unrolled so that all 64 are in flight.
+__global__ __launch_bounds__(kCb) void k_chol_panel(BdDev B, int k, float *P, int mpad) {
+  __shared__ double A[kCb * kCb];   // A[c * 64 + r]
+  __shared__ double X[kCb * kCb];   // X[c * 64 + lane]
   const int lane = threadIdx.x, n3 = B.n3;
-  for (int c = 0; c < kCb; ++c) T[lane][c] = (lane >= c) ? B.L[(size_t)(k + lane) + (size_t)(k + c) * n3] : 0.0;
+  const int kb = min(kCb, n3 - k);
+  {
+    // unconditional loads from clamped (in-bounds) addresses, selected after:
+    // no branch separates them, so all 64 are in flight together
+    double t[kCb];
+    const size_t row = (size_t)min(k + lane, n3 - 1);
+#pragma unroll
+    for (int c = 0; c < kCb; ++c) t[c] = B.L[row + (size_t)min(k + c, n3 - 1) * n3];
+#pragma unroll
+    for (int c = 0; c < kCb; ++c)
+      A[c * kCb + lane] = (lane < kb && c < kb && c <= lane) ? t[c] : (c == lane ? 1.0 : 0.0);
+  }
   __syncthreads();
-  const int rl = blockIdx.x * kCb + lane;     // row within the trailing part
-  const int r = k + kCb + rl;
-  if (rl >= mpad) return;
-  if (r >= n3) {
-    for (int c = 0; c < kCb; ++c) P[(size_t)c * mpad + rl] = 0.0f;
+  for (int c = 0; c < kCb; ++c) {
+    // column c: the pivot row first, then every row below it
+    const double v = chol_dot(A[c * kCb + lane], A + lane, A + c, c);
+    if (lane == c) {
+      if (blockIdx.x == 0 && c < kb && !(v > 0.0)) atomicCAS(B.info, 0, k + c + 1);
+      A[c * kCb + c] = sqrt(v);
+    }
+    __syncthreads();
+    if (lane > c) A[c * kCb + lane] = v / A[c * kCb + c];
+    __syncthreads();
+  }
+  if (blockIdx.x == 0) {
+    if (lane < kb)
+#pragma unroll 8
+      for (int c = 0; c < kCb; ++c)
+        if (c < kb) B.L[(size_t)(k + lane) + (size_t)(k + c) * n3] = c <= lane ? A[c * kCb + lane] : 0.0;
     return;
   }
-  double x[kCb];
+  const int rl = (blockIdx.x - 1) * kCb + lane;   // row within the trailing part
+  const int r = k + kCb + rl;
+  const bool live = r < n3;
+  {
+    double t[kCb];
+    const size_t row = (size_t)min(r, n3 - 1);
 #pragma unroll
-  for (int c = 0; c < kCb; ++c) x[c] = B.L[(size_t)r + (size_t)(k + c) * n3];
+    for (int c = 0; c < kCb; ++c) t[c] = B.L[row + (size_t)(k + c) * n3];
 #pragma unroll
-  for (int c = 0; c < kCb; ++c) {
-    double s = x[c];
-#pragma unroll
-    for (int l = 0; l < c; ++l) s -= x[l] * T[c][l];
-    x[c] = s / T[c][c];
+    for (int c = 0; c < kCb; ++c) X[c * kCb + lane] = live ? t[c] : 0.0;
   }
-#pragma unroll
-  for (int c = 0; c < kCb; ++c) {
-    B.L[(size_t)r + (size_t)(k + c) * n3] = x[c];
-    P[(size_t)c * mpad + rl] = (float)x[c];
-  }
+  for (int c = 0; c < kCb; ++c)
+    X[c * kCb + lane] = chol_dot(X[c * kCb + lane], X + lane, A + c, c) / A[c * kCb + c];
+  if (rl < mpad)
+#pragma unroll 8
+    for (int c = 0; c < kCb; ++c) {
+      const double xv = X[c * kCb + lane];
+      if (live) B.L[(size_t)r + (size_t)(k + c) * n3] = xv;
+      P[(size_t)c * mpad + rl] = live ? (float)xv : 0.0f;
+    }
 }
 
 // trailing update of the lower triangle by 64x64 tiles; 4 waves, each a
@@ -238,10 +257,20 @@ __global__ __launch_bounds__(256) void k_chol_syrk(BdDev B, int k, const float *
   while ((ti + 1) * (ti + 2) / 2 <= bidx) ++ti;
   while (ti * (ti + 1) / 2 > bidx) --ti;
   const int tj = bidx - ti * (ti + 1) / 2;
-  for (int idx = tid; idx < kCb * kCb; idx += 256) {
-    const int r = idx % kCb, c = idx / kCb;
-    Pa[r][c] = P[(size_t)c * mpad + ti * kCb + r];
-    Pb[r][c] = P[(size_t)c * mpad + tj * kCb + r];
+  {
+    float ta[kCb * kCb / 256], tb[kCb * kCb / 256];
+#pragma unroll
+    for (int q = 0; q < kCb * kCb / 256; ++q) {
+      const int idx = tid + 256 * q, r = idx % kCb, c = idx / kCb;
+      ta[q] = P[(size_t)c * mpad + ti * kCb + r];
+      tb[q] = P[(size_t)c * mpad + tj * kCb + r];
+    }
+#pragma unroll
+    for (int q = 0; q < kCb * kCb / 256; ++q) {
+      const int idx = tid + 256 * q, r = idx % kCb, c = idx / kCb;
+      Pa[r][c] = ta[q];
+      Pb[r][c] = tb[q];
+    }
   }
   __syncthreads();
   const int w = tid / 64, lane = tid & 63;
@@ -260,10 +289,17 @@ __global__ __launch_bounds__(256) void k_chol_syrk(BdDev B, int k, const float *
   }
   __syncthreads();
   const int r0 = k + kCb + ti * kCb, c0 = k + kCb + tj * kCb;
-  for (int idx = tid; idx < kCb * kCb; idx += 256) {
-    const int r = idx % kCb, c = idx / kCb;
-    const int gr = r0 + r, gc = c0 + c;
-    if (gr < n3 && gc < n3 && gr >= gc) B.L[(size_t)gr + (size_t)gc * n3] -= (double)C[r][c];
+  double old[kCb * kCb / 256];
+#pragma unroll
+  for (int q = 0; q < kCb * kCb / 256; ++q) {
+    const int idx = tid + 256 * q;
+    const int gr = min(r0 + idx % kCb, n3 - 1), gc = min(c0 + idx / kCb, n3 - 1);
+    old[q] = B.L[(size_t)gr + (size_t)gc * n3];   // clamped: in bounds, unconditional
+  }
+#pragma unroll
+  for (int q = 0; q < kCb * kCb / 256; ++q) {
+    const int idx = tid + 256 * q, r = idx % kCb, c = idx / kCb, gr = r0 + r, gc = c0 + c;
+    if (gr < n3 && gc < n3 && gr >= gc) B.L[(size_t)gr + (size_t)gc * n3] = old[q] - (double)C[r][c];
   }
 }
 
@@ -430,11 +466,10 @@ void launch_factor(tis_ctx *c, BdState *b) {
   hipStream_t st = ctx_stream(c);
   const BdDev &d = b->d;
   for (int k = 0; k < d.n3; k += kCb) {
-    hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(256), 0, st, d, k);
     const int m = d.n3 - k - kCb;
-    if (m <= 0) break;
-    const int mt = (m + kCb - 1) / kCb;
-    hipLaunchKernelGGL(k_chol_trsm, dim3(mt), dim3(kCb), 0, st, d, k, b->P, mt * kCb);
+    const int mt = m > 0 ? (m + kCb - 1) / kCb : 0;
+    hipLaunchKernelGGL(k_chol_panel, dim3(1 + mt), dim3(kCb), 0, st, d, k, b->P, mt * kCb);
+    if (mt == 0) break;
     hipLaunchKernelGGL(k_chol_syrk, dim3(mt * (mt + 1) / 2), dim3(256), 0, st, d, k, (const float *)b->P, mt * kCb);
   }
 }
