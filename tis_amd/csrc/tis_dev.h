@@ -108,7 +108,7 @@ struct DevPtrs {
   const int32_t *inc_off;
   const uint32_t *inc;
   int32_t ninc;
-  double *fslot;             // [3][ninc] slot forces written by the term passes
+  double *fslot;             // [ninc][3] slot forces written by the term passes
   const int32_t *bd_pos, *ba_pos, *st_pos, *tst_pos, *hb_pos;  // slot k of term t -> incidence index
   // terms (global bead indices)
   const int32_t *bd_mp; const double *bd_nat, *bd_coef; int32_t nbd;

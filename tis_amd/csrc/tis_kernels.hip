@@ -131,9 +131,8 @@ __device__ inline void put_slots(const DevPtrs &P, const int32_t *pos, int nterm
                                  const double *fo) {
   for (int k = 0; k < nslot; ++k) {
     const int q = pos[(size_t)k * nterm + term];
-    P.fslot[q] = fo[3 * k];
-    P.fslot[(size_t)P.ninc + q] = fo[3 * k + 1];
-    P.fslot[2 * (size_t)P.ninc + q] = fo[3 * k + 2];
+    double *o = P.fslot + 3 * (size_t)q;   // one 24-byte slot
+    o[0] = fo[3 * k]; o[1] = fo[3 * k + 1]; o[2] = fo[3 * k + 2];
   }
 }
 
@@ -485,20 +484,26 @@ __global__ __launch_bounds__(kTile) void k_terms(DevPtrs P, const double *X, con
 // forms: Debye-Hueckel (force_ele_DH.F90:100-112, clamped at DE_MAX), plain
 // Coulomb (force_ele_coulomb.F90:75-79) and the Ewald real-space term
 // (force_ele_coulomb_ewld.F90:128-132).
+// Hardware reciprocal square root and base-2 exponential (about 1 ulp);
+// d2 is a normal positive number here and exp underflows to 0 gracefully.
+__device__ inline float hw_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ inline float hw_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
+
 template <int FORM>
 __device__ inline float ele_dv(const DSys &s, float qq, float d2, unsigned &nclamp, bool count) {
-  const float rinv = rsqrtf(d2);
+  const float rinv = hw_rsq(d2);
   if (FORM == 0) {
     const float rcd = (float)s.rcdist;
     const float d1 = d2 * rinv;
-    float dv = qq * rinv * rinv * (rinv + rcd) * __expf(-d1 * rcd);
-    if (dv > (float)kDeMax) { dv = (float)kDeMax; if (count) ++nclamp; }
-    return dv;
+    const float dv = qq * rinv * rinv * (rinv + rcd) * hw_exp(-d1 * rcd);
+    const bool cl = dv > (float)kDeMax;
+    nclamp += (cl && count) ? 1u : 0u;
+    return cl ? (float)kDeMax : dv;
   } else if (FORM == 1) {
     return qq * rinv * rinv * rinv;
   } else {
     const float d1 = d2 * rinv;
-    return qq * (erfcf((float)s.ew_alpha * d1) * rinv + (float)s.ew_beta * __expf(-(float)s.ew_a2 * d2)) * rinv * rinv;
+    return qq * (erfcf((float)s.ew_alpha * d1) * rinv + (float)s.ew_beta * hw_exp(-(float)s.ew_a2 * d2)) * rinv * rinv;
   }
 }
 
@@ -551,14 +556,18 @@ __device__ inline void ele_tile(const DevPtrs &P, const DSys &s, const double *X
       const double *SX = sx[w], *SY = sy[w], *SZ = sz[w];
       const float *SQ = sq[w];
       if (!s.periodic) {
+        // branch-free: pairs outside the list or the cutoff contribute 0
+        // (their distance is replaced by 1 so that no inf/NaN arises)
 #pragma unroll 4
         for (int k = 0; k < kend; ++k) {
           const uint32_t bit = k < 32 ? (w0 >> k) & 1u : (w1 >> (k - 32)) & 1u;
           const double v0 = SX[k] - xi0, v1 = SY[k] - xi1, v2 = SZ[k] - xi2;
           const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-          if (!bit || d2 > cut2) continue;
-          const float dv = ele_dv<FORM>(s, qiC * SQ[k], (float)d2, nclamp, qi < jt + k);
-          F0 -= (double)(dv * (float)v0); F1 -= (double)(dv * (float)v1); F2 -= (double)(dv * (float)v2);
+          const bool on = bit && d2 <= cut2;
+          float dv = ele_dv<FORM>(s, qiC * SQ[k], on ? (float)d2 : 1.0f, nclamp, on && qi < jt + k);
+          dv = on ? dv : 0.0f;
+          const double dd = (double)dv;
+          F0 -= dd * v0; F1 -= dd * v1; F2 -= dd * v2;
         }
       } else {
         for (int k = 0; k < kend; ++k) {
@@ -788,9 +797,11 @@ __global__ __launch_bounds__(kTile * 2) void k_main(DevPtrs P, const double *X, 
   if (valid) {
     if (w == 0) {
       // bonded slots in incidence order
+      // bonded slots in incidence order
       const int ib = P.inc_off[g], ie = P.inc_off[g + 1];
       for (int k = ib; k < ie; ++k) {
-        F[0] += P.fslot[k]; F[1] += P.fslot[(size_t)P.ninc + k]; F[2] += P.fslot[2 * (size_t)P.ninc + k];
+        const double *sl = P.fslot + 3 * (size_t)k;
+        F[0] += sl[0]; F[1] += sl[1]; F[2] += sl[2];
       }
     }
     // excluded volume (force_exv_dt15.F90:53-115), owner view of each pair;
