@@ -62,6 +62,10 @@ struct DBuf {
 struct tis_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // second stream (default priority): the term passes run beside the
+  // electrostatic pass; fork/join by events (captured into the graphs too)
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   std::string err;
   // batch
   int32_t nsys = 0, ntot = 0, nqtot = 0, nhbtot = 0, ntsttot = 0, nsttot = 0;
@@ -303,6 +307,9 @@ int32_t tis_ctx_create(int32_t device, tis_ctx **out) {
   c->device = device;
   if (hipSetDevice(device) != hipSuccess) { delete c; return TIS_ERR_HIP; }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return TIS_ERR_HIP; }
+  if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) { delete c; return TIS_ERR_HIP; }
+  (void)hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming);
+  (void)hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming);
   (void)hipEventCreate(&c->ev0);
   (void)hipEventCreate(&c->ev1);
   *out = c;
@@ -320,6 +327,9 @@ int32_t tis_ctx_destroy(tis_ctx *c) {
   for (hipEvent_t e : c->tev) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
+  if (c->join_ev) (void)hipEventDestroy(c->join_ev);
+  if (c->side) (void)hipStreamDestroy(c->side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return TIS_OK;
@@ -855,18 +865,28 @@ static void launch_force_pair(tis_ctx *c, const Range &r, const double *X, const
     c->tev_used += 3;
     (void)hipEventRecord(e3[0], c->stream);
   }
-  if (r.taskn > 0)
-    hipLaunchKernelGGL(k_terms, dim3(r.taskn), dim3(kTile), 0, c->stream, c->P, X, PX, c->d_systask + r.task0, pre_t);
   const int nterm = r.nb + r.na + r.ns;
+  const bool fork = (r.taskn > 0 || nterm > 0) && ((r.qtn > 0 && c->any_ele) || r.ekn > 0);
+  hipStream_t ts = fork ? c->side : c->stream;
+  if (fork) {
+    (void)hipEventRecord(c->fork_ev, c->stream);
+    (void)hipStreamWaitEvent(c->side, c->fork_ev, 0);
+  }
+  // tertiary stacks and hbonds (k_terms) then the bonded slots (k_bonded,
+  // which reads the stack status k_terms sets)
+  if (r.taskn > 0)
+    hipLaunchKernelGGL(k_terms, dim3(r.taskn), dim3(kTile), 0, ts, c->P, X, PX, c->d_systask + r.task0, pre_t);
   if (nterm > 0)
-    hipLaunchKernelGGL(k_bonded, dim3((nterm + 255) / 256), dim3(256), 0, c->stream, c->P, X, r.b0, r.nb, r.a0,
+    hipLaunchKernelGGL(k_bonded, dim3((nterm + 255) / 256), dim3(256), 0, ts, c->P, X, r.b0, r.nb, r.a0,
                        r.na, r.s0, r.ns);
+  if (fork) (void)hipEventRecord(c->join_ev, c->side);
   if (r.qtn > 0 && c->any_ele)
     hipLaunchKernelGGL(k_ele, dim3(r.qtn), dim3(kTile * kWaves), 0, c->stream, c->P, X, PX, c->d_qtiles + r.qt0);
   if (r.ekn > 0) {
     hipLaunchKernelGGL(k_ewald_sk, dim3(r.ekn), dim3(kTile), 0, c->stream, c->P, PX, c->d_ewsk + r.ek0, 0);
     hipLaunchKernelGGL(k_ewald_f, dim3(r.efn), dim3(kTile * kWaves), 0, c->stream, c->P, PX, c->d_ewf + r.ef0);
   }
+  if (fork) (void)hipStreamWaitEvent(c->stream, c->join_ev, 0);
   if (c->timing) (void)hipEventRecord(e3[1], c->stream);
   if (r.btn > 0)
     hipLaunchKernelGGL(k_main, dim3(r.btn), dim3(kTile * 2), 0, c->stream, c->P, X, PX, XN, PXN,
