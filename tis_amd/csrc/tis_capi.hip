@@ -288,10 +288,20 @@ int launch_lists(tis_ctx *c, const Range &r, const double *X, double *PXcur, dou
   if (r.btn + r.qtn > 0)
     hipLaunchKernelGGL(k_bbox, dim3(r.btn + r.qtn), dim3(kTile), 0, c->stream, c->P, X, PXcur, c->d_btiles,
                        r.bt0, r.btn, c->d_qtiles, r.qt0);
+  // the excluded-volume and electrostatic lists are independent: the
+  // electrostatic one is built on the second stream
+  const bool fork = r.btn > 0 && r.qtn > 0;
+  if (fork) {
+    (void)hipEventRecord(c->fork_ev, c->stream);
+    (void)hipStreamWaitEvent(c->side, c->fork_ev, 0);
+  }
+  if (r.qtn > 0)
+    hipLaunchKernelGGL(k_list_ele, dim3(r.qtn), dim3(kTile), 0, fork ? c->side : c->stream, c->P, X, PXcur,
+                       c->d_qtiles, r.qt0);
+  if (fork) (void)hipEventRecord(c->join_ev, c->side);
   if (r.btn > 0)
     hipLaunchKernelGGL(k_list_exv, dim3(r.btn), dim3(kTile), 0, c->stream, c->P, X, PXcur, c->d_btiles, r.bt0);
-  if (r.qtn > 0)
-    hipLaunchKernelGGL(k_list_ele, dim3(r.qtn), dim3(kTile), 0, c->stream, c->P, X, PXcur, c->d_qtiles, r.qt0);
+  if (fork) (void)hipStreamWaitEvent(c->stream, c->join_ev, 0);
   return 0;
 }
 
