@@ -134,6 +134,9 @@ int32_t tis_synchronize(tis_ctx *ctx);
  * events on the context's stream. */
 int32_t tis_last_timing(tis_ctx *ctx, double *total_ms, double *step_kernel_ms,
                         double *pre_kernel_ms, int64_t *nlaunch);
+/* Of the last eager tis_langevin_steps call: average time of the
+ * electrostatic branch (k_ele and the Ewald passes) per step, ms. */
+int32_t tis_last_ele_ms(tis_ctx *ctx, double *ele_ms);
 
 #ifdef __cplusplus
 }

@@ -55,3 +55,12 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(engine, "_LIB", None)
     with pytest.raises(engine.TisError):
         engine.load_library()
+
+
+def test_fortran_module_binds_every_entry_point():
+    """module tis_hip (tis_amd/fortran/tis_hip.F90) has a bind(C) interface
+    for every function the headers declare."""
+    src = open(os.path.join(ROOT, "tis_amd", "fortran", "tis_hip.F90")).read()
+    bound = set(re.findall(r'bind\(C,\s*name="(tis_[a-z0-9_]+)"\)', src))
+    missing = [n for n in declared_functions() if n not in bound]
+    assert not missing, missing

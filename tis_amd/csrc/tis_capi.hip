@@ -59,6 +59,8 @@ struct DBuf {
 
 }  // namespace
 
+constexpr int kTev = 4;  // timing events per eager step
+
 struct tis_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -97,7 +99,7 @@ struct tis_ctx {
   int graph_len = 0;
   // timing
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  double last_total_ms = 0.0, last_main_ms = 0.0, last_pre_ms = 0.0;
+  double last_total_ms = 0.0, last_main_ms = 0.0, last_pre_ms = 0.0, last_ele_ms = 0.0;
   int64_t last_nlaunch = 0;
   // per-kernel events of eager runs: (before k_terms, before k_main, after k_main)
   bool timing = false;
@@ -864,15 +866,17 @@ int32_t tis_pairs_get(tis_ctx *c, int32_t isys, int32_t which, int32_t *pairs, d
 static void launch_force_pair(tis_ctx *c, const Range &r, const double *X, const double *PX, double *XN,
                               double *PXN, int t_local, int mode) {
   const int32_t pre_t = mode == 1 ? (int32_t)t_local : -1;  // k_terms draws at the force step
-  hipEvent_t e3[3] = {nullptr, nullptr, nullptr};
+  // timing events on this stream: e[0] start, e[1] after the electrostatic
+  // branch, e[2] after the join of every branch, e[3] after k_main
+  hipEvent_t e3[kTev] = {nullptr, nullptr, nullptr, nullptr};
   if (c->timing) {
-    while (c->tev.size() < c->tev_used + 3) {
+    while (c->tev.size() < c->tev_used + kTev) {
       hipEvent_t e;
       (void)hipEventCreate(&e);
       c->tev.push_back(e);
     }
-    for (int k = 0; k < 3; ++k) e3[k] = c->tev[c->tev_used + k];
-    c->tev_used += 3;
+    for (int k = 0; k < kTev; ++k) e3[k] = c->tev[c->tev_used + k];
+    c->tev_used += kTev;
     (void)hipEventRecord(e3[0], c->stream);
   }
   const int nterm = r.nb + r.na + r.ns;
@@ -896,12 +900,13 @@ static void launch_force_pair(tis_ctx *c, const Range &r, const double *X, const
     hipLaunchKernelGGL(k_ewald_sk, dim3(r.ekn), dim3(kTile), 0, c->stream, c->P, PX, c->d_ewsk + r.ek0, 0);
     hipLaunchKernelGGL(k_ewald_f, dim3(r.efn), dim3(kTile * kWaves), 0, c->stream, c->P, PX, c->d_ewf + r.ef0);
   }
-  if (fork) (void)hipStreamWaitEvent(c->stream, c->join_ev, 0);
   if (c->timing) (void)hipEventRecord(e3[1], c->stream);
+  if (fork) (void)hipStreamWaitEvent(c->stream, c->join_ev, 0);
+  if (c->timing) (void)hipEventRecord(e3[2], c->stream);
   if (r.btn > 0)
     hipLaunchKernelGGL(k_main, dim3(r.btn), dim3(kTile * 2), 0, c->stream, c->P, X, PX, XN, PXN,
                        c->d_btiles + r.bt0, r.btn, (int32_t)t_local, mode, c->any_ele ? 1 : 0);
-  if (c->timing) (void)hipEventRecord(e3[2], c->stream);
+  if (c->timing) (void)hipEventRecord(e3[3], c->stream);
 }
 
 int32_t tis_force_sumup(tis_ctx *c, int32_t isys, double *force_out) {
@@ -1190,15 +1195,16 @@ int32_t tis_langevin_steps(tis_ctx *c, int64_t nsteps, int32_t mode) {
   c->last_nlaunch = nlaunch;
   c->last_main_ms = c->last_pre_ms = 0.0;
   if (c->timing) {
-    double sp = 0.0, sm = 0.0;
-    size_t nstep = c->tev_used / 3;
+    double sp = 0.0, sm = 0.0, se = 0.0;
+    size_t nstep = c->tev_used / kTev;
     for (size_t k = 0; k < nstep; ++k) {
-      float a = 0.0f, b = 0.0f;
-      (void)hipEventElapsedTime(&a, c->tev[3 * k], c->tev[3 * k + 1]);
-      (void)hipEventElapsedTime(&b, c->tev[3 * k + 1], c->tev[3 * k + 2]);
-      sp += a; sm += b;
+      float a = 0.0f, b = 0.0f, e = 0.0f;
+      (void)hipEventElapsedTime(&a, c->tev[kTev * k], c->tev[kTev * k + 2]);
+      (void)hipEventElapsedTime(&b, c->tev[kTev * k + 2], c->tev[kTev * k + 3]);
+      (void)hipEventElapsedTime(&e, c->tev[kTev * k], c->tev[kTev * k + 1]);
+      sp += a; sm += b; se += e;
     }
-    if (nstep) { c->last_pre_ms = sp / nstep; c->last_main_ms = sm / nstep; }
+    if (nstep) { c->last_pre_ms = sp / nstep; c->last_main_ms = sm / nstep; c->last_ele_ms = se / nstep; }
     c->timing = false;
     c->tev_used = 0;
   }
@@ -1223,6 +1229,12 @@ int32_t tis_synchronize(tis_ctx *c) {
   if (!c) return TIS_ERR_ARG;
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TIS_OK;
+}
+
+int32_t tis_last_ele_ms(tis_ctx *c, double *ele_ms) {
+  if (!c || !ele_ms) return TIS_ERR_ARG;
+  *ele_ms = c->last_ele_ms;
   return TIS_OK;
 }
 

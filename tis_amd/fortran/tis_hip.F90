@@ -50,10 +50,24 @@ module tis_hip
      type(tis_params_c) :: p
   end type tis_system_c
 
+  ! include/tis_bdhi.h
+  type, bind(C), public :: tis_bd_system_c
+     integer(c_int32_t) :: nmp
+     type(c_ptr) :: radius
+     integer(c_int32_t) :: nfene
+     type(c_ptr) :: fene_mp, fene_nat, fene_coef, fene_dist2
+     integer(c_int32_t) :: nwca
+     type(c_ptr) :: wca_mp, wca_nat2, wca_coef
+     real(c_double) :: cutoff_wca, tempk, viscosity
+     integer(c_int32_t) :: hydro_tensor
+  end type tis_bd_system_c
+
   public :: tis_ctx_create, tis_ctx_destroy, tis_last_error, tis_system_upload, tis_set_system_T, &
        tis_state_set, tis_state_get, tis_neighbor, tis_pairs_get, tis_force_sumup, tis_force_get, &
        tis_energy_sumup, tis_langevin_init, tis_langevin_steps, tis_system_origin, tis_device_bytes, &
-       tis_status_counters, tis_synchronize, tis_last_timing
+       tis_status_counters, tis_synchronize, tis_last_timing, tis_last_ele_ms, tis_label_define, &
+       tis_label_apply, tis_bdhi_setup, tis_bdhi_state_set, tis_bdhi_state_get, tis_bdhi_forces, &
+       tis_bdhi_tensors, tis_bdhi_tensor_get, tis_bdhi_factor_error, tis_bdhi_steps, tis_bdhi_timing
   public :: hip_check, hip_neighbor, hip_force_sumup, hip_energy_sumup
 
   interface
@@ -165,6 +179,74 @@ module tis_hip
        type(c_ptr), value :: ctx
        real(c_double), intent(out) :: total_ms, step_ms, pre_ms
        integer(c_int64_t), intent(out) :: nlaunch
+     end function
+     integer(c_int32_t) function tis_last_ele_ms(ctx, ele_ms) bind(C, name="tis_last_ele_ms")
+       import :: c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: ele_ms
+     end function
+     integer(c_int32_t) function tis_label_define(ctx, label, isys, sys) bind(C, name="tis_label_define")
+       import :: c_int32_t, c_ptr, tis_system_c
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: label, isys
+       type(tis_system_c), intent(in) :: sys
+     end function
+     integer(c_int32_t) function tis_label_apply(ctx, n, isys, label, vscale) bind(C, name="tis_label_apply")
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: ctx, vscale
+       integer(c_int32_t), value :: n
+       integer(c_int32_t), intent(in) :: isys(*), label(*)
+     end function
+     integer(c_int32_t) function tis_bdhi_setup(ctx, sys, key, seed) bind(C, name="tis_bdhi_setup")
+       import :: c_int32_t, c_ptr, c_int64_t, tis_bd_system_c
+       type(c_ptr), value :: ctx
+       type(tis_bd_system_c), intent(in) :: sys
+       integer(c_int32_t), value :: key
+       integer(c_int64_t), value :: seed
+     end function
+     integer(c_int32_t) function tis_bdhi_state_set(ctx, which, buf) bind(C, name="tis_bdhi_state_set")
+       import :: c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: which
+       real(c_double), intent(in) :: buf(*)
+     end function
+     integer(c_int32_t) function tis_bdhi_state_get(ctx, which, buf) bind(C, name="tis_bdhi_state_get")
+       import :: c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: which
+       real(c_double), intent(out) :: buf(*)
+     end function
+     integer(c_int32_t) function tis_bdhi_forces(ctx, force_out) bind(C, name="tis_bdhi_forces")
+       import :: c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: force_out(*)
+     end function
+     integer(c_int32_t) function tis_bdhi_tensors(ctx) bind(C, name="tis_bdhi_tensors")
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+     end function
+     integer(c_int32_t) function tis_bdhi_tensor_get(ctx, which, buf) bind(C, name="tis_bdhi_tensor_get")
+       import :: c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: which
+       real(c_double), intent(out) :: buf(*)
+     end function
+     integer(c_int32_t) function tis_bdhi_factor_error(ctx, rel_err) bind(C, name="tis_bdhi_factor_error")
+       import :: c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: rel_err
+     end function
+     integer(c_int32_t) function tis_bdhi_steps(ctx, tstep, nsteps, refactor_every, step0) bind(C, name="tis_bdhi_steps")
+       import :: c_int32_t, c_ptr, c_double, c_int64_t
+       type(c_ptr), value :: ctx
+       real(c_double), value :: tstep
+       integer(c_int64_t), value :: nsteps, step0
+       integer(c_int32_t), value :: refactor_every
+     end function
+     integer(c_int32_t) function tis_bdhi_timing(ctx, total_ms, fill_ms, factor_ms) bind(C, name="tis_bdhi_timing")
+       import :: c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: total_ms, fill_ms, factor_ms
      end function
      integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
        import :: c_size_t, c_ptr
