@@ -79,6 +79,24 @@ def algorithmic_bytes(systems, eng):
     return {"beads": nb, "exv_pairs": nexv, "ele_pairs": nele, "main_bytes": main, "pre_bytes": pre}
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "dt15_batched_step_hbm_bytes.md")
+
+
+def pmc_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (calibrated against a copy of known
+    size); None when the summary or the row is absent."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            for line in fh:
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cells) == 6 and cells[0] == kernel:
+                    return int(cells[5])
+    except OSError:
+        return None
+    return None
+
+
 def force_check(systems, coords_now, eng, sample):
     import oracle
     worst = 0.0
@@ -351,7 +369,8 @@ def main():
     dom_ms = main_ms if dom == "k_main" else ele_ms
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
     roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(dom),
+            "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT),
             "avg_launch_ms": round(dom_ms, 5), "bytes_per_launch": dom_bytes,
             "k_main_ms": round(main_ms, 5), "k_ele_ms": round(ele_ms, 5), "pre_branches_ms": round(pre_ms, 5)}
 

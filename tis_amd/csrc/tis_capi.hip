@@ -3,6 +3,7 @@
 // launches are stream-ordered on the context's own stream (default
 // priority); the batched Langevin loop is captured once into a hipGraph and
 // replayed.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -59,7 +60,7 @@ struct DBuf {
 
 }  // namespace
 
-constexpr int kTev = 4;  // timing events per eager step
+constexpr int kTev = 6;  // timing events per eager step
 
 struct tis_ctx {
   int device = 0;
@@ -866,9 +867,10 @@ int32_t tis_pairs_get(tis_ctx *c, int32_t isys, int32_t which, int32_t *pairs, d
 static void launch_force_pair(tis_ctx *c, const Range &r, const double *X, const double *PX, double *XN,
                               double *PXN, int t_local, int mode) {
   const int32_t pre_t = mode == 1 ? (int32_t)t_local : -1;  // k_terms draws at the force step
-  // timing events on this stream: e[0] start, e[1] after the electrostatic
-  // branch, e[2] after the join of every branch, e[3] after k_main
-  hipEvent_t e3[kTev] = {nullptr, nullptr, nullptr, nullptr};
+  // timing events on this stream: e[0] start, e[3] after the join of every
+  // branch; e[1]/e[2] and e[4]/e[5] are bound to the k_ele and k_main
+  // dispatches (start and end of the kernel itself, no marker packets)
+  hipEvent_t e3[kTev] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   if (c->timing) {
     while (c->tev.size() < c->tev_used + kTev) {
       hipEvent_t e;
@@ -894,19 +896,33 @@ static void launch_force_pair(tis_ctx *c, const Range &r, const double *X, const
     hipLaunchKernelGGL(k_bonded, dim3((nterm + 255) / 256), dim3(256), 0, ts, c->P, X, r.b0, r.nb, r.a0,
                        r.na, r.s0, r.ns);
   if (fork) (void)hipEventRecord(c->join_ev, c->side);
-  if (r.qtn > 0 && c->any_ele)
-    hipLaunchKernelGGL(k_ele, dim3(r.qtn), dim3(kTile * kWaves), 0, c->stream, c->P, X, PX, c->d_qtiles + r.qt0);
+  if (r.qtn > 0 && c->any_ele) {
+    if (c->timing)
+      hipExtLaunchKernelGGL(k_ele, dim3(r.qtn), dim3(kTile * kWaves), 0, c->stream, e3[1], e3[2], 0, c->P, X, PX,
+                            c->d_qtiles + r.qt0);
+    else
+      hipLaunchKernelGGL(k_ele, dim3(r.qtn), dim3(kTile * kWaves), 0, c->stream, c->P, X, PX, c->d_qtiles + r.qt0);
+  } else if (c->timing) {
+    (void)hipEventRecord(e3[1], c->stream);
+    (void)hipEventRecord(e3[2], c->stream);
+  }
   if (r.ekn > 0) {
     hipLaunchKernelGGL(k_ewald_sk, dim3(r.ekn), dim3(kTile), 0, c->stream, c->P, PX, c->d_ewsk + r.ek0, 0);
     hipLaunchKernelGGL(k_ewald_f, dim3(r.efn), dim3(kTile * kWaves), 0, c->stream, c->P, PX, c->d_ewf + r.ef0);
   }
-  if (c->timing) (void)hipEventRecord(e3[1], c->stream);
   if (fork) (void)hipStreamWaitEvent(c->stream, c->join_ev, 0);
-  if (c->timing) (void)hipEventRecord(e3[2], c->stream);
-  if (r.btn > 0)
-    hipLaunchKernelGGL(k_main, dim3(r.btn), dim3(kTile * 2), 0, c->stream, c->P, X, PX, XN, PXN,
-                       c->d_btiles + r.bt0, r.btn, (int32_t)t_local, mode, c->any_ele ? 1 : 0);
   if (c->timing) (void)hipEventRecord(e3[3], c->stream);
+  const int32_t ele_on = c->any_ele ? 1 : 0;
+  if (r.btn > 0 && c->timing)
+    hipExtLaunchKernelGGL(k_main, dim3(r.btn), dim3(kTile * 2), 0, c->stream, e3[4], e3[5], 0, c->P, X, PX, XN,
+                          PXN, c->d_btiles + r.bt0, r.btn, (int32_t)t_local, mode, ele_on);
+  else if (r.btn > 0)
+    hipLaunchKernelGGL(k_main, dim3(r.btn), dim3(kTile * 2), 0, c->stream, c->P, X, PX, XN, PXN,
+                       c->d_btiles + r.bt0, r.btn, (int32_t)t_local, mode, ele_on);
+  else if (c->timing) {
+    (void)hipEventRecord(e3[4], c->stream);
+    (void)hipEventRecord(e3[5], c->stream);
+  }
 }
 
 int32_t tis_force_sumup(tis_ctx *c, int32_t isys, double *force_out) {
@@ -1199,9 +1215,9 @@ int32_t tis_langevin_steps(tis_ctx *c, int64_t nsteps, int32_t mode) {
     size_t nstep = c->tev_used / kTev;
     for (size_t k = 0; k < nstep; ++k) {
       float a = 0.0f, b = 0.0f, e = 0.0f;
-      (void)hipEventElapsedTime(&a, c->tev[kTev * k], c->tev[kTev * k + 2]);
-      (void)hipEventElapsedTime(&b, c->tev[kTev * k + 2], c->tev[kTev * k + 3]);
-      (void)hipEventElapsedTime(&e, c->tev[kTev * k], c->tev[kTev * k + 1]);
+      if (hipEventElapsedTime(&a, c->tev[kTev * k], c->tev[kTev * k + 3]) != hipSuccess) a = 0.0f;
+      if (hipEventElapsedTime(&e, c->tev[kTev * k + 1], c->tev[kTev * k + 2]) != hipSuccess) e = 0.0f;
+      if (hipEventElapsedTime(&b, c->tev[kTev * k + 4], c->tev[kTev * k + 5]) != hipSuccess) b = 0.0f;
       sp += a; sm += b; se += e;
     }
     if (nstep) { c->last_pre_ms = sp / nstep; c->last_main_ms = sm / nstep; c->last_ele_ms = se / nstep; }
