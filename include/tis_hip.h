@@ -130,13 +130,15 @@ int32_t tis_status_counters(tis_ctx *ctx, int64_t *counters);
 int32_t tis_synchronize(tis_ctx *ctx);
 
 /* Device-side timing of the last tis_langevin_steps call: total milliseconds
- * and the per-launch average of the force/integrate kernel (ms), from HIP
- * events on the context's stream. */
+ * and, for eager runs, per-step averages of the gather/integrate kernel
+ * (k_main, step_kernel_ms) and of everything before it (pre_kernel_ms), from
+ * HIP events on the context's stream. */
 int32_t tis_last_timing(tis_ctx *ctx, double *total_ms, double *step_kernel_ms,
                         double *pre_kernel_ms, int64_t *nlaunch);
-/* Of the last eager tis_langevin_steps call: average time of the
- * electrostatic branch (k_ele and the Ewald passes) per step, ms. */
-int32_t tis_last_ele_ms(tis_ctx *ctx, double *ele_ms);
+/* Of the last eager tis_langevin_steps call: average duration of the force
+ * pass kernel (k_force: hbonds and tertiary stacks, electrostatic pairs,
+ * bonded terms) per step, ms, from events bound to its dispatch. */
+int32_t tis_last_force_ms(tis_ctx *ctx, double *force_ms);
 
 #ifdef __cplusplus
 }

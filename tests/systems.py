@@ -46,11 +46,26 @@ def small_ions_box(seed=5, n=20, box=60.0, alpha=0.2, rcut=25.0, hmax=6, form=EL
     return s, x
 
 
+def coulomb_chain(seed=21, n=160, rcut=40.0):
+    """A compact 160-nt chain (phosphate charge -1) with plain Coulomb
+    electrostatics cut at 40 A, no box: three charge chunks of 64 with a
+    partial last one."""
+    rng = np.random.default_rng(seed)
+    seq = T.random_sequence(rng, n)
+    s = T.build_rna_system([(seq, [])], ele_form=ELE_COULOMB, cutoff_ele=rcut, phos_charge=-1.0,
+                           name="coulomb_chain")
+    x = T.compact_coords(s, rng, box=60, min_dist=2.6)
+    return s, x
+
+
 def parity_cases(ewald=True):
     out = [("hairpin64_helix",) + hairpin64(0, "helix"),
            ("hairpin64_compact",) + hairpin64(3, "compact")]
     for k, kw in enumerate([{}, {"cutoff_ele": 2.0}, {"ions": True}, {"ions": True, "periodic": True}]):
         out.append((f"compact{k}",) + compact_allpairs(seed=10 + k, **kw))
+    # several 64-charge chunks: pair chunks I < J of the electrostatic sweep
+    out.append(("hairpin300_helix",) + hairpin_design(7, 300))
+    out.append(("coulomb_chain160",) + coulomb_chain())
     if ewald:
         out.append(("ions_box_ewald",) + small_ions_box())
         out.append(("ions_box_coulomb",) + small_ions_box(form=ELE_COULOMB))

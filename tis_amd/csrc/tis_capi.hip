@@ -19,11 +19,10 @@
 #include "tis_internal.h"
 
 namespace tis {
-__global__ void k_terms(DevPtrs, const double *, const double *, const int32_t *, int32_t);
-__global__ void k_ele(DevPtrs, const double *, const double *, const Tile *);
+__global__ void k_force(DevPtrs, const double *, const double *, const int32_t *, int, const Tile *, int, BondedRange,
+                        int32_t);
 __global__ void k_ewald_sk(DevPtrs, const double *, const EwTask *, int);
 __global__ void k_ewald_f(DevPtrs, const double *, const EwTask *);
-__global__ void k_bonded(DevPtrs, const double *, int, int, int, int, int, int);
 __global__ void k_bbox(DevPtrs, const double *, const double *, const Tile *, int, int, const Tile *, int);
 __global__ void k_main(DevPtrs, const double *, const double *, double *, double *, const Tile *, int, int32_t, int, int);
 __global__ void k_wrap(DevPtrs, const double *, double *, double *, const Tile *, int);
@@ -77,7 +76,7 @@ struct tis_ctx {
   size_t dev_bytes = 0;                  // bytes held in allocs
   DevPtrs P{};
   DSys *d_sys = nullptr;
-  Tile *d_btiles = nullptr, *d_qtiles = nullptr;
+  Tile *d_btiles = nullptr, *d_qtiles = nullptr, *d_qtiles_lpt = nullptr;
   int32_t nbtile = 0, nqtile = 0;
   int32_t *d_systask = nullptr, *d_sysidx = nullptr;
   int32_t nsystask = 0;
@@ -100,9 +99,9 @@ struct tis_ctx {
   int graph_len = 0;
   // timing
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  double last_total_ms = 0.0, last_main_ms = 0.0, last_pre_ms = 0.0, last_ele_ms = 0.0;
+  double last_total_ms = 0.0, last_main_ms = 0.0, last_pre_ms = 0.0, last_force_ms = 0.0;
   int64_t last_nlaunch = 0;
-  // per-kernel events of eager runs: (before k_terms, before k_main, after k_main)
+  // per-kernel events of eager runs (kTev per step, see launch_force_pair)
   bool timing = false;
   std::vector<hipEvent_t> tev;
   size_t tev_used = 0;
@@ -259,6 +258,7 @@ double *PXc(tis_ctx *c) { return c->P.px[c->cur]; }
 
 // --- launch helpers on a system subset: isys = -1 all, else one system
 struct Range {
+  bool all;                 // the whole batch
   int bt0, btn, qt0, qtn, task0, taskn;
   const int32_t *sysidx; int nsel;
   int b0, nb, a0, na, s0, ns;
@@ -268,6 +268,7 @@ struct Range {
 Range range_of(tis_ctx *c, int32_t isys) {
   Range r{};
   if (isys < 0) {
+    r.all = true;
     r.bt0 = 0; r.btn = c->nbtile; r.qt0 = 0; r.qtn = c->nqtile; r.task0 = 0; r.taskn = c->nsystask;
     r.sysidx = c->d_sysidx; r.nsel = c->nsys;
     r.b0 = 0; r.nb = c->P.nbd; r.a0 = 0; r.na = c->P.nba; r.s0 = 0; r.ns = c->P.nst;
@@ -383,7 +384,7 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
     return (int)specs.size() - 1;
   };
   c->systask_of.assign(nsys, -1);
-  int64_t moff = 0, mioff = 0;
+  int64_t moff = 0, mioff = 0, pj_off = 0;
   int32_t off = 0, qoff = 0, nbd = 0, nba = 0, nst = 0, ntst = 0, nhb = 0, nsite = 0, uoff = 0;
   c->any_ele = false;
   c->any_periodic = false;
@@ -517,6 +518,9 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
     for (int st0 = 0; st0 < nq; st0 += kTile) qtiles.push_back(Tile{is, st0});
     h.qt_count = (int32_t)qtiles.size() - h.qt_begin;
     d.bt0 = h.bt_begin; d.nbt = h.bt_count; d.qt0 = h.qt_begin; d.nqt = h.qt_count;
+    d.ele_once = (!s.p.periodic && (s.p.ele_form == TIS_ELE_DH || s.p.ele_form == TIS_ELE_COULOMB) && nq > 0);
+    d.pj_off = pj_off;
+    if (d.ele_once) pj_off += (int64_t)d.nqt * (d.nqt - 1) / 2 * 3 * kTile;
     h.has_task = (s.nhb > 0 || s.ntst > 0);
     if (h.has_task) { c->systask_of[is] = (int32_t)systask.size(); systask.push_back(is); }
     // Ewald reciprocal vectors and work items
@@ -645,9 +649,20 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
   UP(i32, hb_site); P.hb_site = i32; UP(i32, hb_tert); P.hb_tert = i32; P.nhb = nhb;
   UP(i32, hb_sys); P.hb_sys = i32; UP(i32, tst_sys); P.tst_sys = i32;
   UP(i32, site_val); P.site_val = i32;
-  std::vector<uint8_t> ones(std::max(nst, 1), 1);
-  uint8_t *u8;
-  UP(u8, ones); P.st_status = u8;
+  {
+    // per consecutive stack, the exclusive tertiary stacks that name it
+    std::vector<int32_t> cnt(nst + 1, 0), exoff(nst + 1, 0), ex;
+    for (int t = 0; t < ntst; ++t)
+      for (int k = 0; k < 2; ++k)
+        if (tst_excl[2 * t + k] && tst2st[2 * t + k] >= 0) ++cnt[tst2st[2 * t + k]];
+    for (int st = 0; st < nst; ++st) exoff[st + 1] = exoff[st] + cnt[st];
+    ex.assign(std::max(exoff[nst], 1), 0);
+    std::vector<int32_t> fill(exoff.begin(), exoff.end() - 1);
+    for (int t = 0; t < ntst; ++t)
+      for (int k = 0; k < 2; ++k)
+        if (tst_excl[2 * t + k] && tst2st[2 * t + k] >= 0) ex[fill[tst2st[2 * t + k]]++] = t;
+    UP(i32, exoff); P.st_exoff = i32; UP(i32, ex); P.st_ex = i32;
+  }
   rc = dalloc(c, &P.hb_inlist, (size_t)nhb); if (rc) return rc;
   rc = dalloc(c, &P.hb_status, (size_t)nhb); if (rc) return rc;
   rc = dalloc(c, &P.hb_ene, (size_t)nhb); if (rc) return rc;
@@ -657,6 +672,7 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
   P.nb_cap = 64;
   rc = dalloc(c, &P.nb, (size_t)P.nb_cap * off); if (rc) return rc;
   rc = dalloc(c, &P.fdh, 3 * (size_t)qoff); if (rc) return rc;
+  rc = dalloc(c, &P.pj, (size_t)std::max<int64_t>(pj_off, 1)); if (rc) return rc;
   rc = dalloc(c, &P.qmask, (size_t)moff); if (rc) return rc;
   rc = dalloc(c, &P.qmirror, (size_t)std::max<int64_t>(mioff, 1)); if (rc) return rc;
   rc = dalloc(c, &c->d_counters, TIS_CNT_MAX); if (rc) return rc;
@@ -666,6 +682,19 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
   P.seed = seed;
   UP(c->d_btiles, btiles);
   UP(c->d_qtiles, qtiles);
+  {
+    // whole-batch k_force launches take the charge tiles longest first (chunk
+    // pairs of the sweep, then the original order): the block order changes
+    // no result
+    std::vector<Tile> lpt(qtiles);
+    auto work = [&](const Tile &t) {
+      const DSys &d = c->hs[t.sys].d;
+      const int I = t.start / kTile;
+      return d.ele_once ? d.nqt - I : d.nqt;
+    };
+    std::stable_sort(lpt.begin(), lpt.end(), [&](const Tile &a, const Tile &b) { return work(a) > work(b); });
+    UP(c->d_qtiles_lpt, lpt);
+  }
   {
     std::vector<double> h3;
     h3.insert(h3.end(), ewhx.begin(), ewhx.end());
@@ -861,15 +890,15 @@ int32_t tis_pairs_get(tis_ctx *c, int32_t isys, int32_t which, int32_t *pairs, d
                  : TIS_OK;
 }
 
-// k_terms, k_bonded, k_ele, the Ewald passes, then k_main.  Steps draw noise/valence numbers for step
+// k_force (terms, charge tiles, bonded terms), the Ewald passes, then k_main.  Steps draw noise/valence numbers for step
 // stepbase+t+1; a stand-alone force evaluation or the initial acceleration
 // uses stepbase.
 static void launch_force_pair(tis_ctx *c, const Range &r, const double *X, const double *PX, double *XN,
                               double *PXN, int t_local, int mode) {
-  const int32_t pre_t = mode == 1 ? (int32_t)t_local : -1;  // k_terms draws at the force step
-  // timing events on this stream: e[0] start, e[3] after the join of every
-  // branch; e[1]/e[2] and e[4]/e[5] are bound to the k_ele and k_main
-  // dispatches (start and end of the kernel itself, no marker packets)
+  const int32_t pre_t = mode == 1 ? (int32_t)t_local : -1;  // valence draws at the force step
+  // timing events on this stream: e[0] start, e[3] before k_main; e[1]/e[2]
+  // and e[4]/e[5] are bound to the k_force and k_main dispatches (start and
+  // end of the kernel itself, no marker packets)
   hipEvent_t e3[kTev] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   if (c->timing) {
     while (c->tev.size() < c->tev_used + kTev) {
@@ -881,27 +910,21 @@ static void launch_force_pair(tis_ctx *c, const Range &r, const double *X, const
     c->tev_used += kTev;
     (void)hipEventRecord(e3[0], c->stream);
   }
+  // k_force: per-system tertiary stacks and hbonds, the charge tiles, then
+  // the bonded terms in blocks of 256 lanes
   const int nterm = r.nb + r.na + r.ns;
-  const bool fork = (r.taskn > 0 || nterm > 0) && ((r.qtn > 0 && c->any_ele) || r.ekn > 0);
-  hipStream_t ts = fork ? c->side : c->stream;
-  if (fork) {
-    (void)hipEventRecord(c->fork_ev, c->stream);
-    (void)hipStreamWaitEvent(c->side, c->fork_ev, 0);
-  }
-  // tertiary stacks and hbonds (k_terms) then the bonded slots (k_bonded,
-  // which reads the stack status k_terms sets)
-  if (r.taskn > 0)
-    hipLaunchKernelGGL(k_terms, dim3(r.taskn), dim3(kTile), 0, ts, c->P, X, PX, c->d_systask + r.task0, pre_t);
-  if (nterm > 0)
-    hipLaunchKernelGGL(k_bonded, dim3((nterm + 255) / 256), dim3(256), 0, ts, c->P, X, r.b0, r.nb, r.a0,
-                       r.na, r.s0, r.ns);
-  if (fork) (void)hipEventRecord(c->join_ev, c->side);
-  if (r.qtn > 0 && c->any_ele) {
+  const int nqt = (r.qtn > 0 && c->any_ele) ? r.qtn : 0;
+  const int nblk = r.taskn + nqt + (nterm + kTile * kWaves - 1) / (kTile * kWaves);
+  if (nblk > 0) {
+    const Tile *qt = r.all ? c->d_qtiles_lpt : c->d_qtiles + r.qt0;
+    const BondedRange br{r.b0, r.nb, r.a0, r.na, r.s0, r.ns};
+    const int32_t *task = c->d_systask + r.task0;
     if (c->timing)
-      hipExtLaunchKernelGGL(k_ele, dim3(r.qtn), dim3(kTile * kWaves), 0, c->stream, e3[1], e3[2], 0, c->P, X, PX,
-                            c->d_qtiles + r.qt0);
+      hipExtLaunchKernelGGL(k_force, dim3(nblk), dim3(kTile * kWaves), 0, c->stream, e3[1], e3[2], 0, c->P, X, PX,
+                            task, r.taskn, qt, nqt, br, pre_t);
     else
-      hipLaunchKernelGGL(k_ele, dim3(r.qtn), dim3(kTile * kWaves), 0, c->stream, c->P, X, PX, c->d_qtiles + r.qt0);
+      hipLaunchKernelGGL(k_force, dim3(nblk), dim3(kTile * kWaves), 0, c->stream, c->P, X, PX, task, r.taskn, qt,
+                         nqt, br, pre_t);
   } else if (c->timing) {
     (void)hipEventRecord(e3[1], c->stream);
     (void)hipEventRecord(e3[2], c->stream);
@@ -910,7 +933,6 @@ static void launch_force_pair(tis_ctx *c, const Range &r, const double *X, const
     hipLaunchKernelGGL(k_ewald_sk, dim3(r.ekn), dim3(kTile), 0, c->stream, c->P, PX, c->d_ewsk + r.ek0, 0);
     hipLaunchKernelGGL(k_ewald_f, dim3(r.efn), dim3(kTile * kWaves), 0, c->stream, c->P, PX, c->d_ewf + r.ef0);
   }
-  if (fork) (void)hipStreamWaitEvent(c->stream, c->join_ev, 0);
   if (c->timing) (void)hipEventRecord(e3[3], c->stream);
   const int32_t ele_on = c->any_ele ? 1 : 0;
   if (r.btn > 0 && c->timing)
@@ -1220,7 +1242,7 @@ int32_t tis_langevin_steps(tis_ctx *c, int64_t nsteps, int32_t mode) {
       if (hipEventElapsedTime(&b, c->tev[kTev * k + 4], c->tev[kTev * k + 5]) != hipSuccess) b = 0.0f;
       sp += a; sm += b; se += e;
     }
-    if (nstep) { c->last_pre_ms = sp / nstep; c->last_main_ms = sm / nstep; c->last_ele_ms = se / nstep; }
+    if (nstep) { c->last_pre_ms = sp / nstep; c->last_main_ms = sm / nstep; c->last_force_ms = se / nstep; }
     c->timing = false;
     c->tev_used = 0;
   }
@@ -1248,9 +1270,9 @@ int32_t tis_synchronize(tis_ctx *c) {
   return TIS_OK;
 }
 
-int32_t tis_last_ele_ms(tis_ctx *c, double *ele_ms) {
-  if (!c || !ele_ms) return TIS_ERR_ARG;
-  *ele_ms = c->last_ele_ms;
+int32_t tis_last_force_ms(tis_ctx *c, double *force_ms) {
+  if (!c || !force_ms) return TIS_ERR_ARG;
+  *force_ms = c->last_force_ms;
   return TIS_OK;
 }
 

@@ -70,6 +70,10 @@ struct DSys {
   int32_t ewk0, nk;          // reciprocal vectors [ewk0, ewk0 + nk)
   int32_t nqc, nkc;          // charge chunks of the S pass, k chunks of the force pass
   int64_t psk_off, pf_off;   // partial structure factors [nqc][nk][2], partial forces [nkc][3][nq]
+  // electrostatic pairs taken once (non-periodic, ele_form 0 or 1): partial
+  // forces on chunk J from chunk I < J, [nqt(nqt-1)/2][3][kTile] at pj_off
+  int32_t ele_once;
+  int64_t pj_off;
 };
 
 // Ewald work item: (system, tile a, chunk b)
@@ -77,6 +81,9 @@ struct EwTask { int32_t sys, a, b; };
 constexpr int kEwKChunk = 256;   // reciprocal vectors per block of the force pass
 
 struct Tile { int32_t sys, start; };
+
+// bonds [b0, b0+nb), angles [a0, a0+na), consecutive stacks [s0, s0+ns)
+struct BondedRange { int32_t b0, nb, a0, na, s0, ns; };
 
 // excluded-volume parameters of a species pair: D_ij (FP64 and its square)
 // and eps_i*eps_j (force_exv_dt15.F90:60-80,92)
@@ -108,7 +115,7 @@ struct DevPtrs {
   const int32_t *inc_off;
   const uint32_t *inc;
   int32_t ninc;
-  double *fslot;             // [ninc][3] slot forces written by the term passes
+  float *fslot;              // [ninc][3] slot forces written by the term passes (summed in FP64)
   const int32_t *bd_pos, *ba_pos, *st_pos, *tst_pos, *hb_pos;  // slot k of term t -> incidence index
   // terms (global bead indices)
   const int32_t *bd_mp; const double *bd_nat, *bd_coef; int32_t nbd;
@@ -118,8 +125,9 @@ struct DevPtrs {
   const int32_t *hb_mp; const double *hb_nat, *hb_coef; const int32_t *hb_site; const int32_t *hb_tert; int32_t nhb;
   const int32_t *hb_sys, *tst_sys;  // owning system of each candidate / tertiary stack
   const int32_t *site_val;
+  // tertiary stacks that switch consecutive stack st off: st_ex[st_exoff[st] .. st_exoff[st+1])
+  const int32_t *st_exoff, *st_ex;
   // per-step state of terms
-  uint8_t *st_status;        // [nst]
   uint8_t *hb_inlist, *hb_status; // [nhb]
   double *hb_ene;            // [nhb]
   double *for_hb;            // [18][nhb]
@@ -128,6 +136,7 @@ struct DevPtrs {
   int32_t *nb_cnt; uint32_t *nb; int32_t nb_cap;
   const int32_t *qbead;      // [nq] global bead of each charge
   double *fdh;               // [3][nq]
+  double *pj;                // partial forces of the pair-once sweep
   uint32_t *qmask;           // bit matrix
   uint8_t *qmirror;          // periodic ele mirror codes
   // Ewald reciprocal space

@@ -131,15 +131,36 @@ __device__ inline void put_slots(const DevPtrs &P, const int32_t *pos, int nterm
                                  const double *fo) {
   for (int k = 0; k < nslot; ++k) {
     const int q = pos[(size_t)k * nterm + term];
-    double *o = P.fslot + 3 * (size_t)q;   // one 24-byte slot
-    o[0] = fo[3 * k]; o[1] = fo[3 * k + 1]; o[2] = fo[3 * k + 2];
+    float *o = P.fslot + 3 * (size_t)q;   // one 12-byte slot
+    o[0] = (float)fo[3 * k]; o[1] = (float)fo[3 * k + 1]; o[2] = (float)fo[3 * k + 2];
   }
 }
 
-// One lane per bond, angle or consecutive stack of the launch range.
-__global__ __launch_bounds__(256) void k_bonded(DevPtrs P, const double *X, int b0, int nb, int a0, int na,
-                                                int s0, int ns) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+// st_status of force_dtrna_stack_nlocal.F90:83-92 evaluated where it is
+// read: consecutive stack st is off when one of the exclusive tertiary
+// stacks that name it has d - d0 < 10 (the same FP64 operations as
+// tstack_eval).
+__device__ inline bool stack_on(const DevPtrs &P, const double *X, const double *PX, int st) {
+  const int n = P.ntot;
+  for (int e = P.st_exoff[st]; e < P.st_exoff[st + 1]; ++e) {
+    const int t = P.st_ex[e];
+    const DSys *sp = P.sys + P.tst_sys[t];
+    const int32_t *m6 = P.tst_mp + 6 * t;
+    double v12[3];
+    if (sp->periodic) { vsub(PX, n, m6[0], m6[1], v12); pbneighbor(*sp, v12); }
+    else vsub(X, n, m6[0], m6[1], v12);
+    const double d1212 = dot3(v12, v12), a12 = sqrt(d1212);
+    if (a12 - P.tst_nat[6 * t] < 10.0) return false;
+  }
+  return true;
+}
+
+// Bonded terms of the launch range: one lane per bond, angle or consecutive
+// stack.
+
+__device__ inline void bonded_lane(const DevPtrs &P, const double *X, const double *PX, const BondedRange &R,
+                                   int t) {
+  const int nb = R.nb, na = R.na, ns = R.ns, b0 = R.b0, a0 = R.a0, s0 = R.s0;
   if (t < nb) {
     double fo[6];
     bond_eval(P, X, b0 + t, fo);
@@ -157,7 +178,7 @@ __global__ __launch_bounds__(256) void k_bonded(DevPtrs P, const double *X, int 
   if (t < ns) {
     const int st = s0 + t;
     double fo[21];
-    if (P.st_status[st]) stack_eval(P, X, st, fo);
+    if (stack_on(P, X, PX, st)) stack_eval(P, X, st, fo);
     else for (int k = 0; k < 21; ++k) fo[k] = 0.0;
     put_slots(P, P.st_pos, P.nst, st, 7, fo);
   }
@@ -165,7 +186,7 @@ __global__ __launch_bounds__(256) void k_bonded(DevPtrs P, const double *X, int 
 
 // ------------------------------------------------------------------ tertiary stack
 // force_dtrna_stack_nlocal.F90:48-263 for one tertiary stack: all six slots
-// into fo[18]; sets the exclusive secondary stacks off when d < 10.
+// into fo[18] (the exclusion of secondary stacks when d < 10 is stack_on).
 __device__ void tstack_eval(const DevPtrs &P, const DSys &s, const double *X, const double *PX,
                             int t, double fo[18]) {
   const int n = P.ntot;
@@ -177,10 +198,6 @@ __device__ void tstack_eval(const DevPtrs &P, const DSys &s, const double *X, co
   if (s.periodic) pbneighbor(s, v12);
   double d1212 = dot3(v12, v12), a12 = sqrt(d1212);
   double dd = a12 - nat[0];
-  if (dd < 10.0) {
-    if (P.tst_excl[2 * t] && P.tst2st[2 * t] >= 0) P.st_status[P.tst2st[2 * t]] = 0;
-    if (P.tst_excl[2 * t + 1] && P.tst2st[2 * t + 1] >= 0) P.st_status[P.tst2st[2 * t + 1]] = 0;
-  }
   vsub(Y, n, m6[0], m6[2], v13);
   vsub(Y, n, m6[4], m6[2], v53);
   vsub(Y, n, m6[3], m6[1], v42);
@@ -423,25 +440,35 @@ __device__ int hbond_valence_serial(const DevPtrs &P, const DSys &s, uint64_t st
 }
 
 // ------------------------------------------------------------------ terms pass
-// One wave per system that has tertiary stacks or hbond candidates: tertiary
-// stack forces and st_status, hbond geometry of listed candidates and the
-// valence pass.  Forces go to per-term slots gathered by k_main.
-__global__ __launch_bounds__(kTile) void k_terms(DevPtrs P, const double *X, const double *PX,
-                                                 const int32_t *systask, int32_t t_local) {
-  const int lane = threadIdx.x;
-  const DSys s = P.sys[systask[blockIdx.x]];
+// One block per system that has tertiary stacks or hbond candidates:
+// tertiary stack forces, hbond geometry of listed candidates and the valence
+// pass.  Forces go to per-term slots gathered by k_main.
+__device__ void terms_block(const DevPtrs &P, const double *X, const double *PX, int isys, int32_t t_local) {
+  const int lane = threadIdx.x, nt = blockDim.x;
+  const DSys s = P.sys[isys];
   const uint64_t step = *P.stepbase + (uint64_t)t_local + 1u - s.step_origin;
-  for (int k = lane; k < s.nst; k += kTile) P.st_status[s.soff + k] = 1;
-  __syncthreads();
-  for (int k = lane; k < s.ntst; k += kTile) {
+  for (int k = lane; k < s.ntst; k += nt) {
     double fo[18];
     tstack_eval(P, s, X, PX, s.toff + k, fo);
     put_slots(P, P.tst_pos, P.ntst, s.toff + k, 6, fo);
   }
-  if (s.hb_excess_possible)
-    for (int k = lane; k < s.nsite; k += kTile) P.site_cnt[s.sitoff + k] = 0;
+  if (!s.hb_excess_possible) {
+    // no site can exceed its valence: every hbond within the cutoff stays
+    for (int k = lane; k < s.nhb; k += nt) {
+      const int h = s.hoff + k;
+      int st = 0;
+      double e = 0.0, fo[18];
+      if (P.hb_inlist[h]) st = hbond_eval(P, s, X, PX, h, fo, &e);
+      if (!st) for (int q = 0; q < 18; ++q) fo[q] = 0.0;
+      P.hb_status[h] = (uint8_t)st;
+      P.hb_ene[h] = e;
+      put_slots(P, P.hb_pos, P.nhb, h, 6, fo);
+    }
+    return;
+  }
+  for (int k = lane; k < s.nsite; k += nt) P.site_cnt[s.sitoff + k] = 0;
   __syncthreads();
-  for (int k = lane; k < s.nhb; k += kTile) {
+  for (int k = lane; k < s.nhb; k += nt) {
     int h = s.hoff + k;
     int st = 0;
     double e = 0.0, fo[18];
@@ -450,27 +477,24 @@ __global__ __launch_bounds__(kTile) void k_terms(DevPtrs P, const double *X, con
     P.hb_ene[h] = e;
     if (st) {
       for (int q = 0; q < 18; ++q) P.for_hb[(size_t)q * P.nhb + h] = fo[q];
-      if (s.hb_excess_possible)
-        for (int q = 0; q < 6; ++q) {
-          int site = P.hb_site[6 * h + q];
-          if (site >= 0) atomicAdd(&P.site_cnt[site], 1);
-        }
+      for (int q = 0; q < 6; ++q) {
+        int site = P.hb_site[6 * h + q];
+        if (site >= 0) atomicAdd(&P.site_cnt[site], 1);
+      }
     }
   }
   __syncthreads();
-  if (s.hb_excess_possible) {
-    int any = 0;
-    for (int k = lane; k < s.nsite; k += kTile)
-      if (site_count(P, s.sitoff + k) - P.site_val[s.sitoff + k] > 0) any = 1;
-    any = __syncthreads_or(any);
-    if (any && lane == 0) {
-      int nd = hbond_valence_serial(P, s, step);
-      if (nd) atomicAdd(&P.counters[2], (unsigned long long)nd);
-    }
+  int any = 0;
+  for (int k = lane; k < s.nsite; k += nt)
+    if (site_count(P, s.sitoff + k) - P.site_val[s.sitoff + k] > 0) any = 1;
+  any = __syncthreads_or(any);
+  if (any && lane == 0) {
+    int nd = hbond_valence_serial(P, s, step);
+    if (nd) atomicAdd(&P.counters[2], (unsigned long long)nd);
   }
   __syncthreads();
   // forces of the hbonds that survive into their beads' slots
-  for (int k = lane; k < s.nhb; k += kTile) {
+  for (int k = lane; k < s.nhb; k += nt) {
     const int h = s.hoff + k;
     double fo[18];
     const bool on = P.hb_status[h] != 0;
@@ -596,13 +620,173 @@ __device__ inline void ele_tile(const DevPtrs &P, const DSys &s, const double *X
   if (nclamp) atomicAdd(&P.counters[1], (unsigned long long)nclamp);
 }
 
+// Whole-wave rotation by one lane (DPP wave_rol:1): each lane receives the
+// value of a neighbouring lane.  The direction is read off at run time from
+// the lane ids (wave_dir), so the pair schedule below does not depend on it.
+__device__ inline uint32_t wave_rot(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x134, 0xF, 0xF, false);
+}
+__device__ inline double wave_rot(double v) {
+  const uint64_t b = __double_as_longlong(v);
+  const uint64_t lo = wave_rot((uint32_t)b), hi = wave_rot((uint32_t)(b >> 32));
+  return __longlong_as_double((long long)(lo | (hi << 32)));
+}
+// +1 or 63 (= -1 mod 64) when wave_rot is a uniform rotation of the wave,
+// else 0
+__device__ inline int wave_dir() {
+  const int lane = threadIdx.x & (kTile - 1);
+  const int d = ((int)wave_rot((uint32_t)lane) - lane) & (kTile - 1);
+  const int d0 = __shfl(d, 0);
+  const bool ok = (d0 == 1 || d0 == kTile - 1) && __all(d == d0);
+  return ok ? d0 : 0;
+}
+
+// Index of the off-diagonal chunk pair (I, J), I < J < nch, in a system's
+// partial-force buffer.
+__host__ __device__ inline int64_t ele_pair_index(int I, int J, int nch) {
+  return (int64_t)I * (2 * nch - I - 1) / 2 + (J - I - 1);
+}
+
+// Each pair once (non-periodic systems).  Block = charge tile I; wave w takes
+// the chunks J = I + w, I + w + 4, ...  Inside a chunk pair, at iteration k
+// lane l pairs its charge i with j = (l + dir k) mod 64 of chunk J; the force
+// on i accumulates in the lane, the opposite force on j in a register that
+// rotates with the schedule, so after 64 iterations lane l holds the sum for
+// j = l.  The diagonal chunk (J = I) runs k = 1..32 (k = 32 for lanes < 32
+// only), which meets every unordered pair of the tile once.  Forces on j
+// from J > I go to the partial buffer pj[(I, J)], summed by k_main in I
+// order after fdh; everything is in a fixed order, so results are bitwise
+// reproducible.  Pair math as ele_tile; charges beyond nq are staged as zeros
+// with their bits clear.
+template <int FORM>
+__device__ inline void ele_tile_once(const DevPtrs &P, const DSys &s, const double *X, int qi0, int dir,
+                                     EleLds &L) {
+  auto &sx = L.sx; auto &sy = L.sy; auto &sz = L.sz; auto &sq = L.sq; auto &red = L.red;
+  const int lane = threadIdx.x & (kTile - 1), w = threadIdx.x / kTile;
+  const int I = qi0 / kTile;
+  const int qi = qi0 + lane;
+  const bool valid = qi < s.nq;
+  const int n = P.ntot;
+  double xi0 = 0.0, xi1 = 0.0, xi2 = 0.0, chi = 0.0;
+  if (valid) {
+    const int gi = P.qbead[s.qoff + qi];
+    xi0 = X[gi]; xi1 = X[n + gi]; xi2 = X[2 * n + gi];
+    chi = P.charge[gi];
+  }
+  const float qiC = (float)(chi * s.ele_coef);
+  const double cut2 = FORM == 0 ? s.dh_cut2 : s.cl_cut2;
+  double F0 = 0.0, F1 = 0.0, F2 = 0.0;   // on i
+  double D0 = 0.0, D1 = 0.0, D2 = 0.0;   // on j of the diagonal chunk
+  unsigned nclamp = 0;
+  const uint32_t *row = P.qmask + s.moff + (int64_t)(valid ? qi : 0) * s.mwords;
+  const int nch = s.nqt;
+  for (int J0 = I; J0 < nch; J0 += kWaves) {
+    const int J = J0 + w;
+    const int jt = J * kTile;
+    if (J < nch) {
+      const bool vj = jt + lane < s.nq;
+      const int gj = vj ? P.qbead[s.qoff + jt + lane] : 0;
+      sx[w][lane] = vj ? X[gj] : 0.0; sy[w][lane] = vj ? X[n + gj] : 0.0; sz[w][lane] = vj ? X[2 * n + gj] : 0.0;
+      sq[w][lane] = vj ? (float)P.charge[gj] : 0.0f;
+    }
+    __syncthreads();
+    if (J < nch) {
+      const int wb = jt >> 5;
+      uint64_t m64 = 0;
+      if (valid) {
+        m64 = row[wb];
+        if (wb + 1 < s.mwords) m64 |= (uint64_t)row[wb + 1] << 32;
+      }
+      const bool diag = J == I;
+      const double *SX = sx[w], *SY = sy[w], *SZ = sz[w];
+      const float *SQ = sq[w];
+      double G0 = 0.0, G1 = 0.0, G2 = 0.0;
+      // one pair of the schedule at iteration k
+      auto pair = [&](int k, bool last_half) {
+        const int jl = (lane + dir * k) & (kTile - 1);
+        bool bit = (m64 >> jl) & 1u;
+        if (last_half) bit = bit && lane < 32;
+        const double v0 = SX[jl] - xi0, v1 = SY[jl] - xi1, v2 = SZ[jl] - xi2;
+        const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+        const bool on = bit && d2 <= cut2;
+        float dv = ele_dv<FORM>(s, qiC * SQ[jl], on ? (float)d2 : 1.0f, nclamp, on);
+        dv = on ? dv : 0.0f;
+        const double dd = (double)dv;
+        F0 -= dd * v0; F1 -= dd * v1; F2 -= dd * v2;
+        G0 += dd * v0; G1 += dd * v1; G2 += dd * v2;
+        G0 = wave_rot(G0); G1 = wave_rot(G1); G2 = wave_rot(G2);
+      };
+      if (__any(m64 != 0)) {
+        if (diag) {
+#pragma unroll 1
+          for (int k = 1; k < 32; ++k) pair(k, false);
+          pair(32, true);
+        } else {
+#pragma unroll 1
+          for (int k = 0; k < kTile; ++k) pair(k, false);
+        }
+      }
+      if (diag) {
+        // lane l holds j = l + dir * 33; bring j = l home
+        const int src = (lane - dir * 33) & (kTile - 1);
+        D0 = __shfl(G0, src); D1 = __shfl(G1, src); D2 = __shfl(G2, src);
+      } else {
+        double *pj = P.pj + s.pj_off + ele_pair_index(I, J, nch) * (3 * kTile);
+        pj[lane] = G0; pj[kTile + lane] = G1; pj[2 * kTile + lane] = G2;
+      }
+    }
+    __syncthreads();
+  }
+  red[0][w][lane] = F0; red[1][w][lane] = F1; red[2][w][lane] = F2;
+  __syncthreads();
+  if (w == 0 && valid) {
+    const int q = s.qoff + qi;
+    P.fdh[q] = (((red[0][0][lane] + red[0][1][lane]) + red[0][2][lane]) + red[0][3][lane]) + D0;
+    P.fdh[(size_t)P.nqtot + q] = (((red[1][0][lane] + red[1][1][lane]) + red[1][2][lane]) + red[1][3][lane]) + D1;
+    P.fdh[2 * (size_t)P.nqtot + q] = (((red[2][0][lane] + red[2][1][lane]) + red[2][2][lane]) + red[2][3][lane]) + D2;
+  }
+  if (nclamp) atomicAdd(&P.counters[1], (unsigned long long)nclamp);
+}
+
 // force_ele_DH.F90:68-118, force_ele_coulomb.F90:63-84 and the real-space
 // loop of force_ele_coulomb_ewld.F90:118-137, one block per charge tile.
-__global__ __launch_bounds__(kTile * kWaves) void k_ele(DevPtrs P, const double *X, const double *PX,
-                                                        const Tile *qtiles) {
+// Non-periodic systems take each pair once (ele_tile_once); periodic ones
+// sweep the full row of each charge (ele_tile).
+// The force pass, one launch with three kinds of blocks: [0, ntask) one
+// system's tertiary stacks and hbonds each (terms_block), [ntask,
+// ntask + nqt) charge tiles, then the bonded terms, 256 lanes per block
+// (bonded_lane).  The memory-bound term blocks overlap the compute-bound
+// pair sweep; k_main gathers all of their outputs.
+__global__ __launch_bounds__(kTile * kWaves) __attribute__((amdgpu_waves_per_eu(4)))
+void k_force(DevPtrs P, const double *X, const double *PX, const int32_t *systask, int ntask, const Tile *qtiles,
+             int nqt, BondedRange br, int32_t t_local) {
   __shared__ EleLds lds;
-  const Tile tl = qtiles[blockIdx.x];
+  const int b = (int)blockIdx.x;
+  if (b < ntask) {
+    terms_block(P, X, PX, systask[b], t_local);
+    return;
+  }
+  if (b >= ntask + nqt) {
+    bonded_lane(P, X, PX, br, (b - ntask - nqt) * (kTile * kWaves) + (int)threadIdx.x);
+    return;
+  }
+  const Tile tl = qtiles[b - ntask];
   const DSys s = P.sys[tl.sys];
+  const int dir = s.ele_once ? wave_dir() : 0;
+  if (s.ele_once && dir != 0) {
+    if (s.ele_form == 0) ele_tile_once<0>(P, s, X, tl.start, dir, lds);
+    else ele_tile_once<1>(P, s, X, tl.start, dir, lds);
+    return;
+  }
+  if (s.ele_once) {
+    // wave_rot is not a rotation of the whole wave on this device: the full
+    // sweep below computes every force; this block's partials are zero
+    const int I = tl.start / kTile;
+    for (int J = I + 1; J < s.nqt; ++J) {
+      double *pj = P.pj + s.pj_off + ele_pair_index(I, J, s.nqt) * (3 * kTile);
+      for (int k = threadIdx.x; k < 3 * kTile; k += blockDim.x) pj[k] = 0.0;
+    }
+  }
   if (s.ele_form == 0) ele_tile<0>(P, s, X, PX, tl.start, lds);
   else if (s.ele_form == 1) ele_tile<1>(P, s, X, PX, tl.start, lds);
   else if (s.ele_form == 2) ele_tile<2>(P, s, X, PX, tl.start, lds);
@@ -800,8 +984,8 @@ __global__ __launch_bounds__(kTile * 2) void k_main(DevPtrs P, const double *X, 
       // bonded slots in incidence order
       const int ib = P.inc_off[g], ie = P.inc_off[g + 1];
       for (int k = ib; k < ie; ++k) {
-        const double *sl = P.fslot + 3 * (size_t)k;
-        F[0] += sl[0]; F[1] += sl[1]; F[2] += sl[2];
+        const float *sl = P.fslot + 3 * (size_t)k;
+        F[0] += (double)sl[0]; F[1] += (double)sl[1]; F[2] += (double)sl[2];
       }
     }
     // excluded volume (force_exv_dt15.F90:53-115), owner view of each pair;
@@ -824,6 +1008,14 @@ __global__ __launch_bounds__(kTile * 2) void k_main(DevPtrs P, const double *X, 
     const int q = P.qidx[g];
     if (q >= 0) {
       F[0] += P.fdh[q]; F[1] += P.fdh[(size_t)P.nqtot + q]; F[2] += P.fdh[2 * (size_t)P.nqtot + q];
+      if (s.ele_once) {
+        // forces from the chunks I < J of the pair-once sweep, in I order
+        const int lq = q - s.qoff, J = lq / kTile, lj = lq % kTile;
+        for (int I = 0; I < J; ++I) {
+          const double *pj = P.pj + s.pj_off + ele_pair_index(I, J, s.nqt) * (3 * kTile) + lj;
+          F[0] += pj[0]; F[1] += pj[kTile]; F[2] += pj[2 * kTile];
+        }
+      }
       if (s.ele_form == 2) {
         const double *pf = P.pf + s.pf_off + (q - s.qoff);
         for (int c = 0; c < s.nkc; ++c)
@@ -1121,7 +1313,7 @@ __global__ __launch_bounds__(kEBlock) void k_energy(DevPtrs P, const double *X, 
   }
   // consecutive stacks (energy_dtrna_stack.F90:85-160)
   for (int st = s.soff + t; st < s.soff + s.nst; st += kEBlock) {
-    if (!P.st_status[st]) continue;
+    if (!stack_on(P, X, PX, st)) continue;
     const int32_t *m7 = P.st_mp + 7 * st;
     const double *nat = P.st_nat + 3 * st, *cf = P.st_coef + 4 * st;
     double v21[3], v34[3], v54[3], v56[3], v76[3], m[3], nn[3];

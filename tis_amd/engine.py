@@ -67,12 +67,12 @@ def load_library():
     L.tis_status_counters.argtypes = [vp, C.POINTER(C.c_int64)]
     L.tis_synchronize.argtypes = [vp]
     L.tis_last_timing.argtypes = [vp, _f64p, _f64p, _f64p, C.POINTER(C.c_int64)]
-    L.tis_last_ele_ms.argtypes = [vp, _f64p]
+    L.tis_last_force_ms.argtypes = [vp, _f64p]
     for fn in ("tis_ctx_create", "tis_ctx_destroy", "tis_system_upload", "tis_set_system_T", "tis_state_set",
                "tis_state_get", "tis_neighbor", "tis_pairs_get", "tis_force_sumup", "tis_force_get",
                "tis_energy_sumup", "tis_langevin_init", "tis_langevin_steps", "tis_status_counters",
                "tis_system_origin", "tis_device_bytes", "tis_label_define", "tis_label_apply",
-               "tis_synchronize", "tis_last_timing", "tis_last_ele_ms"):
+               "tis_synchronize", "tis_last_timing", "tis_last_force_ms"):
         getattr(L, fn).restype = C.c_int32
     _LIB = L
     return L
@@ -232,8 +232,8 @@ class Engine:
         self._chk(self.L.tis_last_timing(self.ctx, C.byref(t), C.byref(m), C.byref(p), C.byref(n)),
                   "tis_last_timing")
         e = C.c_double()
-        self._chk(self.L.tis_last_ele_ms(self.ctx, C.byref(e)), "tis_last_ele_ms")
-        return {"total_ms": t.value, "main_ms": m.value, "pre_ms": p.value, "ele_ms": e.value, "nlaunch": n.value}
+        self._chk(self.L.tis_last_force_ms(self.ctx, C.byref(e)), "tis_last_force_ms")
+        return {"total_ms": t.value, "main_ms": m.value, "pre_ms": p.value, "force_ms": e.value, "nlaunch": n.value}
 
 
 class BDEngine:

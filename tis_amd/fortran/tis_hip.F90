@@ -65,7 +65,7 @@ module tis_hip
   public :: tis_ctx_create, tis_ctx_destroy, tis_last_error, tis_system_upload, tis_set_system_T, &
        tis_state_set, tis_state_get, tis_neighbor, tis_pairs_get, tis_force_sumup, tis_force_get, &
        tis_energy_sumup, tis_langevin_init, tis_langevin_steps, tis_system_origin, tis_device_bytes, &
-       tis_status_counters, tis_synchronize, tis_last_timing, tis_last_ele_ms, tis_label_define, &
+       tis_status_counters, tis_synchronize, tis_last_timing, tis_last_force_ms, tis_label_define, &
        tis_label_apply, tis_bdhi_setup, tis_bdhi_state_set, tis_bdhi_state_get, tis_bdhi_forces, &
        tis_bdhi_tensors, tis_bdhi_tensor_get, tis_bdhi_factor_error, tis_bdhi_steps, tis_bdhi_timing
   public :: hip_check, hip_neighbor, hip_force_sumup, hip_energy_sumup
@@ -180,10 +180,10 @@ module tis_hip
        real(c_double), intent(out) :: total_ms, step_ms, pre_ms
        integer(c_int64_t), intent(out) :: nlaunch
      end function
-     integer(c_int32_t) function tis_last_ele_ms(ctx, ele_ms) bind(C, name="tis_last_ele_ms")
+     integer(c_int32_t) function tis_last_force_ms(ctx, force_ms) bind(C, name="tis_last_force_ms")
        import :: c_int32_t, c_ptr, c_double
        type(c_ptr), value :: ctx
-       real(c_double), intent(out) :: ele_ms
+       real(c_double), intent(out) :: force_ms
      end function
      integer(c_int32_t) function tis_label_define(ctx, label, isys, sys) bind(C, name="tis_label_define")
        import :: c_int32_t, c_ptr, tis_system_c
