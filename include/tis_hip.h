@@ -37,7 +37,8 @@ enum { TIS_OK = 0, TIS_ERR_ARG = -1, TIS_ERR_HIP = -2, TIS_ERR_STATE = -3,
 /* which list for tis_pairs_get */
 enum { TIS_LIST_EXV = 0, TIS_LIST_ELE = 1, TIS_LIST_HB = 2 };
 
-/* counters of tis_status_counters */
+/* counters of tis_status_counters (TIS_CNT_REBUILDS: whole-batch rebuilds,
+ * plus one per system rebuild fired by the displacement trigger) */
 enum { TIS_CNT_CLAMP_EXV = 0, TIS_CNT_CLAMP_ELE = 1, TIS_CNT_HB_DELETED = 2,
        TIS_CNT_REBUILDS = 3, TIS_CNT_STEPS = 4, TIS_CNT_NB_OVERFLOW = 5,
        TIS_CNT_MAX = 8 };
@@ -95,10 +96,14 @@ int32_t tis_force_get(tis_ctx *ctx, int32_t isys, double *force_out);
 int32_t tis_energy_sumup(tis_ctx *ctx, int32_t isys, double *energy_out);
 
 /* Langevin dynamics for every system of the batch.
- * tis_langevin_init: time step tstep (tau), neighbour rebuild every
- * rebuild_every steps (> 0), first step index step0; if init_accel, the
- * accelerations are set to F/m + c1*xi(step0) (time_integral_pre.F90:176-179),
- * if 0 the stored accelerations are used, if -1 per system (tis_system_origin). */
+ * tis_langevin_init: time step tstep (tau); neighbour lists rebuilt every
+ * rebuild_every steps (> 0) for the whole batch, or (rebuild_every = 0) per
+ * system by the reference's displacement trigger: a system's lists are
+ * rebuilt at the start of a step when the two largest displacements since
+ * its last rebuild sum to more than neigh_margin (time_integral.F90:57-93);
+ * first step index step0; if init_accel, the accelerations are set to
+ * F/m + c1*xi(step0) (time_integral_pre.F90:176-179), if 0 the stored
+ * accelerations are used, if -1 per system (tis_system_origin). */
 int32_t tis_langevin_init(tis_ctx *ctx, double tstep, int32_t rebuild_every, uint64_t step0,
                           int32_t init_accel);
 int32_t tis_langevin_steps(tis_ctx *ctx, int64_t nsteps, int32_t mode);

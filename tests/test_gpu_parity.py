@@ -5,6 +5,8 @@ import pytest
 import oracle
 from tis_amd.engine import Engine, LIST_ELE, LIST_EXV, LIST_HB, MODE_EAGER, MODE_GRAPH, MODE_PLAIN
 
+from tis_amd import topology as T
+
 from systems import force_error, hairpin_design, hairpin64, parity_cases
 
 pytestmark = pytest.mark.gpu
@@ -119,3 +121,63 @@ def test_modes_bitwise_equal():
     for k in range(len(systems)):
         assert np.array_equal(res[0][k], res[1][k])
         assert np.array_equal(res[0][k], res[2][k])
+
+
+def _small_margin_systems(margin=1.5):
+    """Chains with a 1.5 A list margin, so that the displacement trigger
+    fires every few steps."""
+    out = []
+    for k, n in enumerate([64, 30, 150]):
+        rng = np.random.default_rng(300 + k)
+        seq, pairs = T.hairpin_design(rng, n, 6, 4)
+        s = T.build_rna_system([(seq, pairs)], neigh_margin=margin, name=f"m{k}")
+        out.append((s, T.helix_coords(s, rng)))
+    return out
+
+
+def test_displacement_trigger_matches_oracle():
+    """Displacement-triggered rebuilds (time_integral.F90:57-93): the GPU
+    trajectory and rebuild count against the oracle's trigger loop."""
+    s, x = _small_margin_systems()[0]
+    v = np.zeros_like(x)
+    eng = Engine([s], keys=[4], seed=17)
+    eng.set_coords(0, x)
+    eng.set_state(0, 1, v)
+    c0 = eng.counters()["rebuilds"]
+    eng.langevin_init(tstep=0.05, rebuild_every=0, step0=0, init_accel=True)
+    eng.langevin_steps(60, MODE_EAGER)
+    ng = eng.counters()["rebuilds"] - c0
+    xg = eng.coords(0)
+    o = oracle.Oracle(s)
+    xo, vo, ao, nr = o.langevin(x, v, np.zeros_like(x), 60, tstep=0.05, rebuild_every=0, seed=17, sys_id=4)
+    d = np.abs(xg - xo).max()
+    print(f"   rebuilds gpu {ng} oracle {nr}; 60-step max |dx| = {d:.3e} A", flush=True)
+    assert nr >= 4
+    assert ng == nr
+    assert d < 1e-4
+
+
+def test_displacement_trigger_modes_and_batch_bitwise():
+    """With per-system rebuild decisions, graph == eager == plain and each
+    system in the batch == the same system alone, bitwise."""
+    systems = _small_margin_systems()
+    res = []
+    for mode in (MODE_EAGER, MODE_GRAPH, MODE_PLAIN):
+        eng = Engine([s for s, _ in systems], keys=[0, 1, 2], seed=5)
+        for k, (s, x) in enumerate(systems):
+            eng.set_coords(k, x)
+        eng.langevin_init(tstep=0.05, rebuild_every=0)
+        eng.langevin_steps(40, mode)
+        res.append([eng.coords(k) for k in range(len(systems))])
+        eng.close()
+    for k in range(len(systems)):
+        assert np.array_equal(res[0][k], res[1][k])
+        assert np.array_equal(res[0][k], res[2][k])
+    for k, (s, x) in enumerate(systems):
+        alone = Engine([s], keys=[k], seed=5)
+        alone.set_coords(0, x)
+        alone.langevin_init(tstep=0.05, rebuild_every=0)
+        alone.langevin_steps(40, MODE_GRAPH)
+        assert np.array_equal(alone.coords(0), res[1][k])
+        alone.close()
+

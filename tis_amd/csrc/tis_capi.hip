@@ -23,12 +23,11 @@ __global__ void k_force(DevPtrs, const double *, const double *, const int32_t *
                         int32_t);
 __global__ void k_ewald_sk(DevPtrs, const double *, const EwTask *, int);
 __global__ void k_ewald_f(DevPtrs, const double *, const EwTask *);
-__global__ void k_bbox(DevPtrs, const double *, const double *, const Tile *, int, int, const Tile *, int);
 __global__ void k_main(DevPtrs, const double *, const double *, double *, double *, const Tile *, int, int32_t, int, int);
-__global__ void k_wrap(DevPtrs, const double *, double *, double *, const Tile *, int);
-__global__ void k_list_exv(DevPtrs, const double *, const double *, const Tile *, int);
-__global__ void k_list_ele(DevPtrs, const double *, const double *, const Tile *, int);
-__global__ void k_list_hb(DevPtrs, const double *, const double *, int, int);
+__global__ void k_prep(DevPtrs, const double *, double *, double *, const Tile *, int, int, const Tile *, int, int,
+                       int, int);
+__global__ void k_lists(DevPtrs, const double *, const double *, const Tile *, int, int, const Tile *, int, int, int,
+                        int, int);
 __global__ void k_advance(uint64_t *, uint64_t, unsigned long long *);
 __global__ void k_count_rebuild(unsigned long long *);
 __global__ void k_energy(DevPtrs, const double *, const double *, const int32_t *, double *);
@@ -259,6 +258,7 @@ double *PXc(tis_ctx *c) { return c->P.px[c->cur]; }
 // --- launch helpers on a system subset: isys = -1 all, else one system
 struct Range {
   bool all;                 // the whole batch
+  int isys;                 // else this system
   int bt0, btn, qt0, qtn, task0, taskn;
   const int32_t *sysidx; int nsel;
   int b0, nb, a0, na, s0, ns;
@@ -275,6 +275,7 @@ Range range_of(tis_ctx *c, int32_t isys) {
     r.ek0 = 0; r.ekn = c->n_ewsk; r.ef0 = 0; r.efn = c->n_ewf;
   } else {
     const HostSys &h = c->hs[isys];
+    r.isys = isys;
     r.bt0 = h.bt_begin; r.btn = h.bt_count; r.qt0 = h.qt_begin; r.qtn = h.qt_count;
     r.task0 = c->systask_of[isys] < 0 ? 0 : c->systask_of[isys];
     r.taskn = c->systask_of[isys] < 0 ? 0 : 1;
@@ -285,27 +286,19 @@ Range range_of(tis_ctx *c, int32_t isys) {
   return r;
 }
 
-int launch_lists(tis_ctx *c, const Range &r, const double *X, double *PXcur, double *PXpred, int with_pred) {
-  if (c->any_periodic && r.btn > 0)
-    hipLaunchKernelGGL(k_wrap, dim3(r.btn), dim3(kTile), 0, c->stream, c->P, X, PXcur, PXpred,
-                       c->d_btiles + r.bt0, with_pred);
+// The lists of the range from X (wrapped copies into PXcur, predicted
+// wrapped positions into PXpred when with_pred): k_prep then k_lists; with
+// gate only the systems whose displacement trigger fired.
+int launch_lists(tis_ctx *c, const Range &r, const double *X, double *PXcur, double *PXpred, int with_pred,
+                 int gate) {
+  const int h0 = r.all ? 0 : c->hs[r.isys].d.hoff, nh = r.all ? c->nhbtot : c->hs[r.isys].d.nhb;
   if (r.btn + r.qtn > 0)
-    hipLaunchKernelGGL(k_bbox, dim3(r.btn + r.qtn), dim3(kTile), 0, c->stream, c->P, X, PXcur, c->d_btiles,
-                       r.bt0, r.btn, c->d_qtiles, r.qt0);
-  // the excluded-volume and electrostatic lists are independent: the
-  // electrostatic one is built on the second stream
-  const bool fork = r.btn > 0 && r.qtn > 0;
-  if (fork) {
-    (void)hipEventRecord(c->fork_ev, c->stream);
-    (void)hipStreamWaitEvent(c->side, c->fork_ev, 0);
-  }
-  if (r.qtn > 0)
-    hipLaunchKernelGGL(k_list_ele, dim3(r.qtn), dim3(kTile), 0, fork ? c->side : c->stream, c->P, X, PXcur,
-                       c->d_qtiles, r.qt0);
-  if (fork) (void)hipEventRecord(c->join_ev, c->side);
-  if (r.btn > 0)
-    hipLaunchKernelGGL(k_list_exv, dim3(r.btn), dim3(kTile), 0, c->stream, c->P, X, PXcur, c->d_btiles, r.bt0);
-  if (fork) (void)hipStreamWaitEvent(c->stream, c->join_ev, 0);
+    hipLaunchKernelGGL(k_prep, dim3(r.btn + r.qtn), dim3(kTile), 0, c->stream, c->P, X, PXcur, PXpred, c->d_btiles,
+                       r.bt0, r.btn, c->d_qtiles, r.qt0, r.qtn, with_pred, gate);
+  const int nblk = r.btn + r.qtn + (nh + kTile - 1) / kTile;
+  if (nblk > 0)
+    hipLaunchKernelGGL(k_lists, dim3(nblk), dim3(kTile), 0, c->stream, c->P, X, PXcur, c->d_btiles, r.bt0, r.btn,
+                       c->d_qtiles, r.qt0, r.qtn, h0, nh, gate);
   return 0;
 }
 
@@ -636,6 +629,7 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
   P.species = d_species; P.sptab = d_sptab; P.nspecies = NS;
   rc = dalloc(c, &P.bbox, (size_t)c->nbtile); if (rc) return rc;
   rc = dalloc(c, &P.qbbox, (size_t)c->nqtile); if (rc) return rc;
+  rc = dalloc(c, &P.sbox, 4 * (size_t)c->nbtile); if (rc) return rc;
   P.rcm = d_rcm; P.mass = d_mass; P.exv_rad = d_rad; P.charge = d_charge; P.rl2 = d_rl2;
   P.mptype = d_mpt; P.iclass = d_icl; P.unit = d_unit; P.qidx = d_qidx; P.qbead = d_qbead;
   P.inc_off = d_incoff; P.inc = d_inc; P.exv_eps = d_eps;
@@ -672,6 +666,10 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
   P.nb_cap = 64;
   rc = dalloc(c, &P.nb, (size_t)P.nb_cap * off); if (rc) return rc;
   rc = dalloc(c, &P.fdh, 3 * (size_t)qoff); if (rc) return rc;
+  rc = dalloc(c, &P.dxyz, 3 * (size_t)off); if (rc) return rc;
+  rc = dalloc(c, &P.dlast, 3 * (size_t)off); if (rc) return rc;
+  rc = dalloc(c, &P.ttop, 2 * btiles.size()); if (rc) return rc;
+  rc = dalloc(c, &P.sys_flag, (size_t)nsys); if (rc) return rc;
   rc = dalloc(c, &P.pj, (size_t)std::max<int64_t>(pj_off, 1)); if (rc) return rc;
   rc = dalloc(c, &P.qmask, (size_t)moff); if (rc) return rc;
   rc = dalloc(c, &P.qmirror, (size_t)std::max<int64_t>(mioff, 1)); if (rc) return rc;
@@ -798,12 +796,7 @@ static int rebuild_lists(tis_ctx *c, int32_t isys) {
   Range r = range_of(c, isys);
   for (int attempt = 0; attempt < 4; ++attempt) {
     HIPCHK(c, hipMemsetAsync(c->d_counters + TIS_CNT_NB_OVERFLOW, 0, sizeof(unsigned long long), c->stream));
-    launch_lists(c, r, Xc(c), PXc(c), nullptr, 0);
-    if (c->nhbtot > 0) {
-      int h0 = isys < 0 ? 0 : c->hs[isys].d.hoff, nh = isys < 0 ? c->nhbtot : c->hs[isys].d.nhb;
-      if (nh > 0)
-        hipLaunchKernelGGL(k_list_hb, dim3((nh + 255) / 256), dim3(256), 0, c->stream, c->P, Xc(c), PXc(c), h0, nh);
-    }
+    launch_lists(c, r, Xc(c), PXc(c), nullptr, 0, 0);
     hipLaunchKernelGGL(k_count_rebuild, dim3(1), dim3(1), 0, c->stream, c->d_counters);
     HIPCHK(c, hipGetLastError());
     unsigned long long ovf = 0;
@@ -1094,12 +1087,13 @@ int32_t tis_device_bytes(tis_ctx *c, int64_t *bytes) {
 }
 
 int32_t tis_langevin_init(tis_ctx *c, double tstep, int32_t rebuild_every, uint64_t step0, int32_t init_accel) {
-  if (!c || tstep <= 0.0 || rebuild_every <= 0) return fail(c, TIS_ERR_ARG, "bad Langevin parameters");
+  if (!c || tstep <= 0.0 || rebuild_every < 0) return fail(c, TIS_ERR_ARG, "bad Langevin parameters");
   int rc = check_sys(c, -1);
   if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->device));
   c->tstep = tstep;
   c->rebuild_every = rebuild_every;
+  c->P.dyn = rebuild_every == 0 ? 1 : 0;
   c->P.tstep2 = 0.5 * tstep * tstep;
   c->P.tstep = tstep;
   for (int k = 0; k < 2; ++k)
@@ -1119,6 +1113,10 @@ int32_t tis_langevin_init(tis_ctx *c, double tstep, int32_t rebuild_every, uint6
   HIPCHK(c, hipMemcpyAsync(c->d_stepbase, &step0, sizeof(step0), hipMemcpyHostToDevice, c->stream));
   rc = rebuild_lists(c, -1);
   if (rc) return rc;
+  // displacements since this rebuild start at zero, no rebuild pending
+  HIPCHK(c, hipMemsetAsync(c->P.dxyz, 0, 3 * sizeof(double) * (size_t)std::max(c->ntot, 1), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->P.sys_flag, 0, (size_t)std::max(c->nsys, 1), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->P.ttop, 0, 2 * sizeof(double) * (size_t)std::max(c->nbtile, 1), c->stream));
   Range r = range_of(c, -1);
   const int cu = c->cur;
   if (init_accel < 0) {
@@ -1133,8 +1131,8 @@ int32_t tis_langevin_init(tis_ctx *c, double tstep, int32_t rebuild_every, uint6
   }
   HIPCHK(c, hipGetLastError());
   if (c->any_periodic)
-    hipLaunchKernelGGL(k_wrap, dim3(c->nbtile), dim3(kTile), 0, c->stream, c->P, c->P.x[cu], c->P.px[cu],
-                       c->P.px[cu ^ 1], c->d_btiles, 1);
+    hipLaunchKernelGGL(k_prep, dim3(c->nbtile), dim3(kTile), 0, c->stream, c->P, c->P.x[cu], c->P.px[cu],
+                       c->P.px[cu ^ 1], c->d_btiles, 0, c->nbtile, c->d_qtiles, 0, 0, 1, 0);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->lv_init = true;
   c->t_since_init = 0;
@@ -1145,32 +1143,34 @@ int32_t tis_langevin_init(tis_ctx *c, double tstep, int32_t rebuild_every, uint6
 static int eager_step(tis_ctx *c, int32_t isys) {
   const int cu = c->cur;
   Range r = range_of(c, isys);
-  if (c->t_since_init % c->rebuild_every == 0) {
-    launch_lists(c, r, c->P.x[cu], c->P.px[cu], c->P.px[cu ^ 1], 1);
-    if (c->nhbtot > 0) {
-      int h0 = isys < 0 ? 0 : c->hs[isys].d.hoff, nh = isys < 0 ? c->nhbtot : c->hs[isys].d.nhb;
-      if (nh > 0)
-        hipLaunchKernelGGL(k_list_hb, dim3((nh + 255) / 256), dim3(256), 0, c->stream, c->P, c->P.x[cu],
-                           c->P.px[cu], h0, nh);
-    }
+  if (c->P.dyn) {
+    launch_lists(c, r, c->P.x[cu], c->P.px[cu], c->P.px[cu ^ 1], 1, 1);
+  } else if (c->t_since_init % c->rebuild_every == 0) {
+    launch_lists(c, r, c->P.x[cu], c->P.px[cu], c->P.px[cu ^ 1], 1, 0);
     if (isys <= 0) hipLaunchKernelGGL(k_count_rebuild, dim3(1), dim3(1), 0, c->stream, c->d_counters);
   }
   launch_force_pair(c, r, c->P.x[cu ^ 1], c->P.px[cu ^ 1], c->P.x[cu], c->P.px[cu], 0, 1);
   return 0;
 }
 
+// steps per captured graph: even (the ping-pong parity returns) and a
+// multiple of the fixed rebuild interval
+static int steps_per_graph(const tis_ctx *c) {
+  if (c->P.dyn) return 10;
+  return (c->rebuild_every % 2 == 0) ? c->rebuild_every : 2 * c->rebuild_every;
+}
+
 static int capture_graph(tis_ctx *c, int parity) {
-  const int L = (c->rebuild_every % 2 == 0) ? c->rebuild_every : 2 * c->rebuild_every;
+  const int L = steps_per_graph(c);
   hipGraph_t g = nullptr;
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   int cu = parity;
   Range r = range_of(c, -1);
   for (int t = 0; t < L; ++t) {
-    if (t % c->rebuild_every == 0) {
-      launch_lists(c, r, c->P.x[cu], c->P.px[cu], c->P.px[cu ^ 1], 1);
-      if (c->nhbtot > 0)
-        hipLaunchKernelGGL(k_list_hb, dim3((c->nhbtot + 255) / 256), dim3(256), 0, c->stream, c->P, c->P.x[cu],
-                           c->P.px[cu], 0, c->nhbtot);
+    if (c->P.dyn) {
+      launch_lists(c, r, c->P.x[cu], c->P.px[cu], c->P.px[cu ^ 1], 1, 1);
+    } else if (t % c->rebuild_every == 0) {
+      launch_lists(c, r, c->P.x[cu], c->P.px[cu], c->P.px[cu ^ 1], 1, 0);
       hipLaunchKernelGGL(k_count_rebuild, dim3(1), dim3(1), 0, c->stream, c->d_counters);
     }
     launch_force_pair(c, r, c->P.x[cu ^ 1], c->P.px[cu ^ 1], c->P.x[cu], c->P.px[cu], t, 1);
@@ -1193,7 +1193,7 @@ int32_t tis_langevin_steps(tis_ctx *c, int64_t nsteps, int32_t mode) {
   HIPCHK(c, hipMemsetAsync(c->d_counters + TIS_CNT_NB_OVERFLOW, 0, sizeof(unsigned long long), c->stream));
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
   int64_t done = 0, nlaunch = 0;
-  const int L = (c->rebuild_every % 2 == 0) ? c->rebuild_every : 2 * c->rebuild_every;
+  const int L = steps_per_graph(c);
   if (mode == TIS_MODE_GRAPH) {
     while (done < nsteps) {
       if (c->t_since_init % L == 0 && nsteps - done >= L) {

@@ -107,6 +107,7 @@ struct DevPtrs {
   const uint8_t *species;       // excluded-volume species of each bead
   const SpPair *sptab; int32_t nspecies;
   BBox *bbox, *qbbox;          // per bead tile / per charge tile
+  BBox *sbox;                  // per 16-bead group of each bead tile [4 * nbtile]
   const double *exv_rad;
   const float *exv_eps;
   const double *charge;
@@ -145,6 +146,14 @@ struct DevPtrs {
   int32_t nktot;
   double *psk;               // partial structure factors
   double *pf;                // partial reciprocal forces
+  // displacement-triggered rebuilds (time_integral.F90:57-93): displacement
+  // since the last rebuild and of the last step per bead, the two largest
+  // squared displacements of each bead tile, and the rebuild decision of
+  // the current step per system
+  int32_t dyn;
+  double *dxyz, *dlast;      // [3][ntot]
+  double *ttop;              // [nbtile][2]
+  uint8_t *sys_flag;         // [nsys]
   // counters / timing
   unsigned long long *counters;  // TIS_CNT_*
   uint64_t *stepbase;

@@ -902,7 +902,7 @@ __global__ __launch_bounds__(kTile * kWaves) void k_ewald_f(DevPtrs P, const dou
 // x_next = x + c4 v + (dt^2/2) a.
 __device__ inline void integrate(const DevPtrs &P, const DSys &s, int g, int li, const double F[3],
                                  const double *X, const double *PX, double *XN, double *PXN,
-                                 uint64_t step, int mode) {
+                                 uint64_t step, int mode, double dxo[3]) {
 #pragma clang fp contract(off)
   const int n = P.ntot;
   double gs[3];
@@ -918,6 +918,7 @@ __device__ inline void integrate(const DevPtrs &P, const DSys &s, int g, int li,
     P.v[d * n + g] = vv;
     P.a[d * n + g] = af;
     double dx = r3 * vv + P.tstep2 * af;
+    dxo[d] = dx;
     XN[d * n + g] = X[d * n + g] + dx;
     if (s.periodic) PXN[d * n + g] = PX[d * n + g] + dx;
   }
@@ -966,6 +967,72 @@ __device__ inline void exv_group(const DevPtrs &P, const DSys &s, const double *
   }
 }
 
+// The two largest squared displacements of a bead tile since the last
+// rebuild (wave merge of top-2 pairs; -1 marks lanes without a bead), for
+// the rebuild decision of the next step (system_trigger).
+__device__ inline void tile_top2(const DevPtrs &P, int tile, double d2) {
+  double a = d2, b = -1.0;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double a2 = __shfl_xor(a, off), b2 = __shfl_xor(b, off);
+    const double hi = fmax(a, a2), lo = fmax(fmin(a, a2), fmax(b, b2));
+    a = hi; b = lo;
+  }
+  if ((threadIdx.x & (kTile - 1)) == 0) { P.ttop[2 * tile] = a; P.ttop[2 * tile + 1] = b; }
+}
+
+// dxyz_mp of time_integral.F90:160: |dxyz|^2 before the update (as
+// :64 computes it), then dxyz += dx; the update is kept for a rebuild.
+__device__ inline double accumulate_displacement(const DevPtrs &P, int g, const double dx[3]) {
+#pragma clang fp contract(off)
+  const int n = P.ntot;
+  const double o0 = P.dxyz[g], o1 = P.dxyz[n + g], o2 = P.dxyz[2 * n + g];
+  const double d2 = o0 * o0 + o1 * o1 + o2 * o2;
+  P.dxyz[g] = o0 + dx[0]; P.dxyz[n + g] = o1 + dx[1]; P.dxyz[2 * n + g] = o2 + dx[2];
+  P.dlast[g] = dx[0]; P.dlast[n + g] = dx[1]; P.dlast[2 * n + g] = dx[2];
+  return d2;
+}
+
+// Forces of one bead after the exv waves, then the integrator; returns the
+// squared displacement since the last rebuild before this step's update
+// (and accumulates the update) when rebuilds are displacement-triggered.
+__device__ inline double main_tail(const DevPtrs &P, const DSys &s, int g, int li, double F[3],
+                                   double (*red)[2][kTile], int lane, const double *X, const double *PX, double *XN,
+                                   double *PXN, int32_t t_local, int mode, int do_dh) {
+  const int n = P.ntot;
+  for (int d = 0; d < 3; ++d) F[d] = red[d][0][lane] + red[d][1][lane];
+  // electrostatic pair sums of the ele pass, then the Ewald reciprocal
+  // partial forces in chunk order
+  if (do_dh && s.ele_form >= 0) {
+    const int q = P.qidx[g];
+    if (q >= 0) {
+      F[0] += P.fdh[q]; F[1] += P.fdh[(size_t)P.nqtot + q]; F[2] += P.fdh[2 * (size_t)P.nqtot + q];
+      if (s.ele_once) {
+        // forces from the chunks I < J of the pair-once sweep, in I order
+        const int lq = q - s.qoff, J = lq / kTile, lj = lq % kTile;
+        for (int I = 0; I < J; ++I) {
+          const double *pj = P.pj + s.pj_off + ele_pair_index(I, J, s.nqt) * (3 * kTile) + lj;
+          F[0] += pj[0]; F[1] += pj[kTile]; F[2] += pj[2 * kTile];
+        }
+      }
+      if (s.ele_form == 2) {
+        const double *pf = P.pf + s.pf_off + (q - s.qoff);
+        for (int c = 0; c < s.nkc; ++c)
+          for (int d = 0; d < 3; ++d) F[d] += pf[((int64_t)c * 3 + d) * s.nq];
+      }
+    }
+  }
+  P.f[g] = F[0]; P.f[n + g] = F[1]; P.f[2 * n + g] = F[2];
+  if (mode == 0) return -1.0;
+  if (mode == 4) mode = s.init_accel ? 2 : 3;
+  const uint64_t step = *P.stepbase + (uint64_t)t_local + (mode == 2 ? 0u : 1u) - s.step_origin;
+  double dx[3];
+  integrate(P, s, g, li, F, X, PX, XN, PXN, step, mode, dx);
+  if (!P.dyn) return -1.0;
+  return accumulate_displacement(P, g, dx);
+}
+
+
 __global__ __launch_bounds__(kTile * 2) void k_main(DevPtrs P, const double *X, const double *PX,
                                                     double *XN, double *PXN, const Tile *tiles,
                                                     int ntile, int32_t t_local, int mode, int do_dh) {
@@ -1000,47 +1067,19 @@ __global__ __launch_bounds__(kTile * 2) void k_main(DevPtrs P, const double *X, 
   }
   for (int d = 0; d < 3; ++d) red[d][w][lane] = F[d];
   __syncthreads();
-  if (w != 0 || !valid) return;
-  for (int d = 0; d < 3; ++d) F[d] = red[d][0][lane] + red[d][1][lane];
-  // electrostatic pair sums of the ele pass, then the Ewald reciprocal
-  // partial forces in chunk order
-  if (do_dh && s.ele_form >= 0) {
-    const int q = P.qidx[g];
-    if (q >= 0) {
-      F[0] += P.fdh[q]; F[1] += P.fdh[(size_t)P.nqtot + q]; F[2] += P.fdh[2 * (size_t)P.nqtot + q];
-      if (s.ele_once) {
-        // forces from the chunks I < J of the pair-once sweep, in I order
-        const int lq = q - s.qoff, J = lq / kTile, lj = lq % kTile;
-        for (int I = 0; I < J; ++I) {
-          const double *pj = P.pj + s.pj_off + ele_pair_index(I, J, s.nqt) * (3 * kTile) + lj;
-          F[0] += pj[0]; F[1] += pj[kTile]; F[2] += pj[2 * kTile];
-        }
-      }
-      if (s.ele_form == 2) {
-        const double *pf = P.pf + s.pf_off + (q - s.qoff);
-        for (int c = 0; c < s.nkc; ++c)
-          for (int d = 0; d < 3; ++d) F[d] += pf[((int64_t)c * 3 + d) * s.nq];
-      }
-    }
-  }
-  P.f[g] = F[0]; P.f[n + g] = F[1]; P.f[2 * n + g] = F[2];
-  if (mode == 0) return;
-  if (mode == 4) mode = s.init_accel ? 2 : 3;
-  const uint64_t step = *P.stepbase + (uint64_t)t_local + (mode == 2 ? 0u : 1u) - s.step_origin;
-  integrate(P, s, g, li, F, X, PX, XN, PXN, step, mode);
+  if (w != 0) return;
+  double d2 = -1.0;   // squared displacement since the last rebuild, before this update
+  if (valid) d2 = main_tail(P, s, g, li, F, red, lane, X, PX, XN, PXN, t_local, mode, do_dh);
+  if (P.dyn && mode != 0) tile_top2(P, s.bt0 + tl.start / kTile, d2);
 }
 
 // ------------------------------------------------------------------ lists
 // util_periodic.F90:26-36 on the current coordinates; with with_pred the
 // predicted wrapped positions px + (c4 v + dt^2/2 a) are refreshed too.
-__global__ __launch_bounds__(kTile) void k_wrap(DevPtrs P, const double *X, double *PXc, double *PXp,
-                                                const Tile *tiles, int with_pred) {
+__device__ inline void wrap_bead(const DevPtrs &P, const DSys &s, const double *X, double *PXc, double *PXp,
+                                 int g, int with_pred) {
 #pragma clang fp contract(off)
-  const Tile tl = tiles[blockIdx.x];
-  const DSys s = P.sys[tl.sys];
-  const int li = tl.start + threadIdx.x;
-  if (!s.periodic || li >= s.nmp) return;
-  const int g = s.off + li, n = P.ntot;
+  const int n = P.ntot;
   for (int d = 0; d < 3; ++d) {
     double pw = s.box[d], pmax = 0.5 * pw, pmin = -0.5 * pw;
     double tx = X[d * n + g];
@@ -1063,17 +1102,13 @@ __device__ inline bool exv_keep(const DSys &s, int ulo, int uhi, int clo, int tl
   return hi >= lo + sep;
 }
 
-// Bounding boxes of bead tiles (blocks [0, nbt)) and charge tiles (blocks
-// [nbt, nbt + nqt)), from the coordinates the lists are built from.
-__global__ __launch_bounds__(kTile) void k_bbox(DevPtrs P, const double *X, const double *PX,
-                                                const Tile *btiles, int bt0, int nbt, const Tile *qtiles,
-                                                int qt0) {
+// Bounding box of bead tile or charge tile `it` (tile tl) from the
+// coordinates the lists are built from (non-periodic systems only cull by
+// boxes).
+__device__ inline void bbox_tile(const DevPtrs &P, const DSys &s, const double *X, const Tile tl, int it,
+                                 bool charge) {
   const int lane = threadIdx.x;
-  const bool charge = (int)blockIdx.x >= nbt;
-  const int it = charge ? qt0 + (int)blockIdx.x - nbt : bt0 + (int)blockIdx.x;
-  const Tile tl = charge ? qtiles[it] : btiles[it];
-  const DSys s = P.sys[tl.sys];
-  const double *Y = s.periodic ? PX : X;
+  const double *Y = X;
   const int n = P.ntot;
   const int li = tl.start + lane;
   const int lim = charge ? s.nq : s.nmp;
@@ -1082,11 +1117,18 @@ __global__ __launch_bounds__(kTile) void k_bbox(DevPtrs P, const double *X, cons
   if (li < lim) g = charge ? P.qbead[s.qoff + li] : s.off + li;
   double lo[3] = {Y[g], Y[n + g], Y[2 * n + g]}, hi[3] = {lo[0], lo[1], lo[2]};
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1)
+  for (int off = 1; off < kTile; off <<= 1) {
     for (int d = 0; d < 3; ++d) {
       lo[d] = fmin(lo[d], __shfl_xor(lo[d], off));
       hi[d] = fmax(hi[d], __shfl_xor(hi[d], off));
     }
+    // after the offsets 1..8 each group of 16 lanes holds its group's box
+    if (off == 8 && !charge && (lane & 15) == 0) {
+      BBox b;
+      for (int d = 0; d < 3; ++d) { b.lo[d] = lo[d]; b.hi[d] = hi[d]; }
+      P.sbox[4 * it + lane / 16] = b;
+    }
+  }
   if (lane == 0) {
     BBox b;
     for (int d = 0; d < 3; ++d) { b.lo[d] = lo[d]; b.hi[d] = hi[d]; }
@@ -1117,27 +1159,49 @@ __device__ inline double box_dmax2(const BBox &a, const BBox &b) {
 // the tile scans the system's bead tiles that its tile's box can reach,
 // staged through LDS, with the FP64 test d^2 < (cutoff + margin)^2 rounded as
 // the reference rounds it.
-__global__ __launch_bounds__(kTile) void k_list_exv(DevPtrs P, const double *X, const double *PX, const Tile *tiles,
-                                                    int tile0) {
-  __shared__ double sx[kTile], sy[kTile], sz[kTile];
-  __shared__ int su[kTile], stc[kTile];
+constexpr int kRl2Lds = 16;       // unit-pair radius tables up to 16 x 16 units are staged in LDS
+
+struct ListLds {
+  double sx[kTile], sy[kTile], sz[kTile];
+  int su[kTile], stc[kTile];
+  double rl2[kRl2Lds * kRl2Lds];
+};
+
+// squared distance from a point to a box (FP64)
+__device__ inline double point_box_d2(const double p[3], const BBox &b) {
+  double s2 = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    double g = fmax(0.0, fmax(b.lo[d] - p[d], p[d] - b.hi[d]));
+    s2 += g * g;
+  }
+  return s2;
+}
+
+// One wave per bead tile sweeps the system's bead tiles in order; tiles and
+// 16-bead groups whose boxes no bead of the tile can reach within the
+// largest list radius are skipped.  The unit-pair radii sit in LDS, so the
+// per-pair tests issue no dependent global loads.
+__device__ inline void list_exv_tile(const DevPtrs &P, const DSys &s, const double *X, const double *PX,
+                                     const Tile tl, int it, ListLds &L) {
+  auto &sx = L.sx; auto &sy = L.sy; auto &sz = L.sz; auto &su = L.su; auto &stc = L.stc;
   const int lane = threadIdx.x;
-  const int it = tile0 + blockIdx.x;
-  const Tile tl = tiles[it];
-  const DSys s = P.sys[tl.sys];
   const int li = tl.start + lane;
   const bool valid = li < s.nmp;
   const int n = P.ntot;
   const double *Y = s.periodic ? PX : X;
   const int g = s.off + (valid ? li : 0);
+  const bool rl2_lds = s.nunit <= kRl2Lds;
+  if (rl2_lds)
+    for (int k = lane; k < s.nunit * s.nunit; k += kTile) L.rl2[k] = P.rl2[s.uoff + k];
   double xi[3] = {0, 0, 0};
   int ui = 0, ti = 0, ci = 0;
   if (valid) { ld3(Y, n, g, xi); ui = P.unit[g]; ti = P.mptype[g]; ci = P.iclass[g]; }
   const BBox bi = P.bbox[it];
+  const double cut2 = s.rl2max * (1.0 + 1e-9);
   int cnt = 0;
   for (int jtile = 0; jtile < s.nbt; ++jtile) {
     const int jt = jtile * kTile;
-    if (!s.periodic && box_dmin2(bi, P.bbox[s.bt0 + jtile]) > s.rl2max * (1.0 + 1e-9)) continue;
+    if (!s.periodic && box_dmin2(bi, P.bbox[s.bt0 + jtile]) > cut2) continue;
     int lj = jt + lane;
     if (lj < s.nmp) {
       int gj = s.off + lj;
@@ -1147,27 +1211,35 @@ __global__ __launch_bounds__(kTile) void k_list_exv(DevPtrs P, const double *X, 
     }
     __syncthreads();
     const int kend = min(kTile, s.nmp - jt);
-    if (valid)
-      for (int k = 0; k < kend; ++k) {
-        int lj2 = jt + k;
-        if (lj2 == li) continue;
-        const bool own_lo = li < lj2;
-        double v[3];
-        if (own_lo) { v[0] = __dadd_rn(sx[k], -xi[0]); v[1] = __dadd_rn(sy[k], -xi[1]); v[2] = __dadd_rn(sz[k], -xi[2]); }
-        else { v[0] = __dadd_rn(xi[0], -sx[k]); v[1] = __dadd_rn(xi[1], -sy[k]); v[2] = __dadd_rn(xi[2], -sz[k]); }
-        int code = s.periodic ? pbneighbor(s, v) : 0;
-        double d2 = sq3_exact(v[0], v[1], v[2]);
-        int uj = su[k];
-        int ulo = own_lo ? ui : uj, uhi = own_lo ? uj : ui;
-        double r2 = P.rl2[s.uoff + ulo * s.nunit + uhi];
-        if (!(d2 < r2)) continue;
-        int tj = stc[k] & 0xFF, cj = stc[k] >> 8;
-        bool keep = own_lo ? exv_keep(s, ui, uj, ci, ti, li, lj2) : exv_keep(s, uj, ui, cj, tj, lj2, li);
-        if (!keep) continue;
-        int oc = own_lo ? code : mirror_flip(code);
-        if (cnt < P.nb_cap) P.nb[(size_t)cnt * n + g] = ((uint32_t)oc << kMirrorShift) | (uint32_t)(s.off + lj2);
-        ++cnt;
+    for (int q0 = 0; q0 < kend; q0 += 16) {
+      if (!s.periodic) {
+        const bool need = valid && point_box_d2(xi, P.sbox[4 * (s.bt0 + jtile) + q0 / 16]) <= cut2;
+        if (!__any(need)) continue;
       }
+      const int q1 = min(q0 + 16, kend);
+      if (valid)
+        for (int k = q0; k < q1; ++k) {
+          const int lj2 = jt + k;
+          if (lj2 == li) continue;
+          const bool own_lo = li < lj2;
+          double v[3];
+          if (own_lo) { v[0] = __dadd_rn(sx[k], -xi[0]); v[1] = __dadd_rn(sy[k], -xi[1]); v[2] = __dadd_rn(sz[k], -xi[2]); }
+          else { v[0] = __dadd_rn(xi[0], -sx[k]); v[1] = __dadd_rn(xi[1], -sy[k]); v[2] = __dadd_rn(xi[2], -sz[k]); }
+          const int code = s.periodic ? pbneighbor(s, v) : 0;
+          const double d2 = sq3_exact(v[0], v[1], v[2]);
+          if (!(d2 < s.rl2max)) continue;   // rl2max bounds every unit-pair radius
+          const int uj = su[k];
+          const int ulo = own_lo ? ui : uj, uhi = own_lo ? uj : ui;
+          const double r2 = rl2_lds ? L.rl2[ulo * s.nunit + uhi] : P.rl2[s.uoff + ulo * s.nunit + uhi];
+          if (!(d2 < r2)) continue;
+          const int tj = stc[k] & 0xFF, cj = stc[k] >> 8;
+          const bool keep = own_lo ? exv_keep(s, ui, uj, ci, ti, li, lj2) : exv_keep(s, uj, ui, cj, tj, lj2, li);
+          if (!keep) continue;
+          const int oc = own_lo ? code : mirror_flip(code);
+          if (cnt < P.nb_cap) P.nb[(size_t)cnt * n + g] = ((uint32_t)oc << kMirrorShift) | (uint32_t)(s.off + lj2);
+          ++cnt;
+        }
+    }
     __syncthreads();
   }
   if (valid) {
@@ -1180,13 +1252,10 @@ __global__ __launch_bounds__(kTile) void k_list_exv(DevPtrs P, const double *X, 
 // pair is in the list; radius (margin + cutoff)^2 per i_DH_cutoff_type.  Tile
 // pairs whose boxes lie entirely inside (outside) the radius are filled with
 // ones (zeros) without per-pair tests; the rest are tested pair by pair.
-__global__ __launch_bounds__(kTile) void k_list_ele(DevPtrs P, const double *X, const double *PX, const Tile *qtiles,
-                                                    int tile0) {
-  __shared__ double sx[kTile], sy[kTile], sz[kTile];
+__device__ inline void list_ele_tile(const DevPtrs &P, const DSys &s, const double *X, const double *PX,
+                                     const Tile tl, int it, ListLds &L) {
+  auto &sx = L.sx; auto &sy = L.sy; auto &sz = L.sz;
   const int lane = threadIdx.x;
-  const int it = tile0 + blockIdx.x;
-  const Tile tl = qtiles[it];
-  const DSys s = P.sys[tl.sys];
   const int qi = tl.start + lane;
   const bool valid = qi < s.nq;
   const int n = P.ntot;
@@ -1240,12 +1309,8 @@ __global__ __launch_bounds__(kTile) void k_list_ele(DevPtrs P, const double *X, 
 }
 
 // hb candidate list (neighbor_list_hb.F90:59-78; radius mloop_dtrna.F90:149)
-__global__ void k_list_hb(DevPtrs P, const double *X, const double *PX, int h0, int nh) {
+__device__ inline void list_hb_entry(const DevPtrs &P, const DSys &s, const double *X, const double *PX, int h) {
 #pragma clang fp contract(off)
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nh) return;
-  int h = h0 + k;
-  const DSys s = P.sys[P.hb_sys[h]];
   const int n = P.ntot;
   int a = P.hb_mp[6 * h], b = P.hb_mp[6 * h + 1];
   double v[3];
@@ -1258,6 +1323,91 @@ __global__ void k_list_hb(DevPtrs P, const double *X, const double *PX, int h0, 
   double d2 = sq3_exact(v[0], v[1], v[2]);
   double r = __dadd_rn(__dadd_rn(P.hb_nat[6 * h], s.hb_cut), s.margin);
   P.hb_inlist[h] = (uint8_t)(d2 < __dmul_rn(r, r));
+}
+
+
+// The list passes of one rebuild in two launches.  gate = 0 builds for every
+// system of the launch range; gate = 1 only for the systems whose
+// displacement trigger fires (decided in k_prep from the tile pairs the
+// previous k_main wrote, recorded in sys_flag for k_lists); all other blocks
+// return at once.
+//
+// k_prep: blocks [0, nbt) bead tiles (periodic: wrap into the box and refresh
+// the predicted wrapped positions; else the tile's bounding box; with gate,
+// dxyz restarts from the last step's displacement), blocks [nbt, nbt + nqt)
+// charge-tile boxes.
+// time_integral.F90:57-75: the two largest squared displacements of the
+// system (merged over its tiles' pairs; the merge is exact in any order) and
+// sqrt(m1) + sqrt(m2) > margin.
+__device__ inline bool system_trigger(const DevPtrs &P, const DSys &s) {
+  double a = 0.0, b = 0.0;
+  for (int t = (int)threadIdx.x; t < s.nbt; t += kTile) {
+    const double t1 = P.ttop[2 * (s.bt0 + t)], t2 = P.ttop[2 * (s.bt0 + t) + 1];
+    const double hi = fmax(a, t1), lo = fmax(fmin(a, t1), fmax(b, t2));
+    a = hi; b = lo;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double a2 = __shfl_xor(a, off), b2 = __shfl_xor(b, off);
+    const double hi = fmax(a, a2), lo = fmax(fmin(a, a2), fmax(b, b2));
+    a = hi; b = lo;
+  }
+  return sqrt(a) + sqrt(b) > s.margin;
+}
+
+__global__ __launch_bounds__(kTile) void k_prep(DevPtrs P, const double *X, double *PXc, double *PXp,
+                                                const Tile *btiles, int bt0, int nbt, const Tile *qtiles, int qt0,
+                                                int nqt, int with_pred, int gate) {
+  const int b = (int)blockIdx.x;
+  const bool charge = b >= nbt;
+  const int it = charge ? qt0 + b - nbt : bt0 + b;
+  const Tile tl = charge ? qtiles[it] : btiles[it];
+  const DSys s = P.sys[tl.sys];
+  if (gate) {
+    // every block of the system reaches the same decision; the first bead
+    // tile records it for k_lists and counts the rebuild
+    const bool fire = system_trigger(P, s);
+    if (!charge && tl.start == 0 && threadIdx.x == 0) {
+      P.sys_flag[tl.sys] = fire ? 1 : 0;
+      if (fire) atomicAdd(&P.counters[3], 1ull);
+    }
+    if (!fire) return;
+  }
+  if (!charge) {
+    const int li = tl.start + (int)threadIdx.x;
+    if (li < s.nmp) {
+      const int g = s.off + li, n = P.ntot;
+      if (s.periodic) wrap_bead(P, s, X, PXc, PXp, g, with_pred);
+      if (gate)
+        for (int d = 0; d < 3; ++d) P.dxyz[d * n + g] = P.dlast[d * n + g];
+    }
+  }
+  if (!s.periodic) bbox_tile(P, s, X, tl, it, charge);
+}
+
+// k_lists: one wave per block; [0, nbt) exv lists of bead tiles, [nbt,
+// nbt + nqt) ele bit rows of charge tiles, then 64 hbond candidates per
+// block.
+__global__ __launch_bounds__(kTile) void k_lists(DevPtrs P, const double *X, const double *PX, const Tile *btiles,
+                                                 int bt0, int nbt, const Tile *qtiles, int qt0, int nqt, int h0,
+                                                 int nh, int gate) {
+  __shared__ ListLds lds;
+  const int b = (int)blockIdx.x;
+  if (b < nbt + nqt) {
+    const bool charge = b >= nbt;
+    const int it = charge ? qt0 + b - nbt : bt0 + b;
+    const Tile tl = charge ? qtiles[it] : btiles[it];
+    if (gate && !P.sys_flag[tl.sys]) return;
+    const DSys s = P.sys[tl.sys];
+    if (charge) list_ele_tile(P, s, X, PX, tl, it, lds);
+    else list_exv_tile(P, s, X, PX, tl, it, lds);
+    return;
+  }
+  const int k = (b - nbt - nqt) * kTile + (int)threadIdx.x;
+  if (k >= nh) return;
+  const int h = h0 + k;
+  if (gate && !P.sys_flag[P.hb_sys[h]]) return;
+  list_hb_entry(P, P.sys[P.hb_sys[h]], X, PX, h);
 }
 
 __global__ void k_advance(uint64_t *stepbase, uint64_t by, unsigned long long *counters) {
