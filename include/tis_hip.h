@@ -94,6 +94,14 @@ int32_t tis_force_get(tis_ctx *ctx, int32_t isys, double *force_out);
  * launch, energy_out[nsys * TIS_E_MAX] (the replica loop of
  * energy_allrep.F90:123-125). */
 int32_t tis_energy_sumup(tis_ctx *ctx, int32_t isys, double *energy_out);
+/* energy_unit of energy_sumup (energy_sumup.F90:17-36, 76-508) for one
+ * system: energy_unit[(iunit, junit, type)] column-major (nunit, nunit,
+ * TIS_E_MAX), each term added at the unit pair the reference's term routine
+ * uses, then the post-processing of i_output_energy_style = style (0: half
+ * of each upper off-diagonal entry to both diagonal entries; 1: symmetric
+ * off-diagonal sums) and TOTAL over the types other than TOTAL and VELO.
+ * The Ewald terms contribute nothing here, as in energy_ele_coulomb_ewld. */
+int32_t tis_energy_unit(tis_ctx *ctx, int32_t isys, int32_t style, double *energy_unit);
 
 /* Langevin dynamics for every system of the batch.
  * tis_langevin_init: time step tstep (tau); neighbour lists rebuilt every

@@ -43,6 +43,7 @@ def _lib():
     lib.oracle_force_sumup.restype = C.c_int32
     lib.oracle_force_sumup.argtypes = [vp, _f64p, _f64p, C.c_uint64, C.c_uint32, C.c_uint64]
     lib.oracle_energy_sumup.argtypes = [vp, _f64p, _f64p, _f64p]
+    lib.oracle_energy_unit.argtypes = [vp, _f64p, _f64p, C.c_int32, _f64p]
     lib.oracle_langevin_run.restype = C.c_int32
     lib.oracle_langevin_run.argtypes = [vp, C.c_double, C.c_int64, C.c_int32, _f64p, _f64p, _f64p,
                                         C.c_uint64, C.c_uint32, C.c_uint64, C.c_int32]
@@ -143,6 +144,17 @@ class Oracle:
         v = None if velo is None else np.ascontiguousarray(velo, dtype=np.float64)
         self.L.oracle_energy_sumup(self.ws, fp(x), fp(v) if v is not None else None, fp(e))
         return e
+
+    def energy_unit(self, xyz, velo=None, style=0):
+        """energy_unit (nunit, nunit, 25) after the post-processing of
+        i_output_energy_style = style (lists and hbond state of the last
+        neighbor()/force() calls, like energy())."""
+        x = np.ascontiguousarray(xyz, dtype=np.float64)
+        nu = self.sys.nunit
+        eu = np.zeros(25 * nu * nu)
+        v = None if velo is None else np.ascontiguousarray(velo, dtype=np.float64)
+        self.L.oracle_energy_unit(self.ws, fp(x), fp(v) if v is not None else None, style, fp(eu))
+        return eu.reshape(25, nu, nu).transpose(2, 1, 0)
 
     def langevin(self, xyz, velo, accel, nsteps, tstep=0.05, rebuild_every=10, seed=1,
                  sys_id=0, step0=0, init_accel=True):
@@ -339,6 +351,7 @@ class RefLib:
         L.tisref_force_term.argtypes = [C.c_int32, _f64p]
         L.tisref_hb_state.argtypes = [_i32p, _f64p]
         L.tisref_energy.argtypes = [_f64p, C.c_int32, _f64p]
+        L.tisref_energy_unit.argtypes = [_f64p, C.c_int32, C.c_int32, _f64p]
         L.tisref_set_mass.argtypes = [_f64p]
         L.tisref_temperature.argtypes = [C.c_double, C.c_double, _f64p, _f64p, _f64p, _f64p,
                                          _f64p, _f64p]
@@ -394,6 +407,16 @@ class RefLib:
             v = np.ascontiguousarray(velo, dtype=np.float64)
             _on_big_stack(self.L.tisref_energy, fp(v), 1, fp(e))
         return e
+
+    def energy_unit(self, velo=None, style=0):
+        nu = self.sys.nunit
+        eu = np.zeros(25 * nu * nu)
+        if velo is None:
+            _on_big_stack(self.L.tisref_energy_unit, fp(np.zeros(3)), 0, style, fp(eu))
+        else:
+            v = np.ascontiguousarray(velo, dtype=np.float64)
+            _on_big_stack(self.L.tisref_energy_unit, fp(v), 1, style, fp(eu))
+        return eu.reshape(25, nu, nu).transpose(2, 1, 0)
 
     def hydro(self, bd, xyz):
         """(D, L) of the reference's simu_hydro_tensors."""

@@ -510,6 +510,69 @@ contains
     end do
   end subroutine tisref_energy
 
+  ! energy_unit of the reference's own term routines at the current
+  ! coordinates (the kinetic term as energy_kinetic.F90:49-56 adds it), then
+  ! the per-unit post-processing of energy_sumup.F90:480-508 for
+  ! i_output_energy_style = style.
+  subroutine tisref_energy_unit(velo, nvelo, style, eu_out) bind(C, name="tisref_energy_unit")
+    real(c_double), intent(in) :: velo(3, *)
+    integer(c_int32_t), value :: nvelo, style
+    real(c_double), intent(out) :: eu_out(nunit_all, nunit_all, E_TYPE%MAX)
+    real(PREC) :: e(E_TYPE%MAX), eu(nunit_all, nunit_all, E_TYPE%MAX), sume, ev
+    real(PREC) :: ene_st(max(ndtrna_st,1)), ene_tst(max(ndtrna_tst,1))
+    logical :: st_l(max(ndtrna_st,1))
+    integer :: i, iunit, junit
+    e(:) = 0.0e0_PREC
+    eu(:,:,:) = 0.0e0_PREC
+    call energy_bond(1, eu, e)
+    call energy_bangle(1, eu, e)
+    call energy_dtrna_stack_nlocal(1, eu, e, ene_tst, st_l)
+    st_status(1:ndtrna_st, 1) = st_l(1:ndtrna_st)
+    call energy_dtrna_stack(1, eu, e, ene_st)
+    call energy_dtrna_hbond15(1, eu, e)
+    call energy_exv_dt15(1, eu, e)
+    if (inmisc%force_flag(INTERACT%ELE) .and. inele%i_function_form == 0) then
+       call energy_ele_DH(1, e, eu)
+    else if (inmisc%force_flag(INTERACT%ELE) .and. inele%i_function_form == 1) then
+       call energy_ele_coulomb(1, e, eu)
+    else if (inmisc%force_flag(INTERACT%ELE) .and. inele%i_function_form == 2) then
+       call energy_ele_coulomb_ewld(1, e, eu)
+    end if
+    if (nvelo > 0) then
+       do i = 1, nmp_all
+          ev = 0.5e0_PREC * g_mass(i) * dot_product(velo(:,i), velo(:,i))
+          iunit = imp2unit(i)
+          eu(iunit, iunit, E_TYPE%VELO) = eu(iunit, iunit, E_TYPE%VELO) + ev
+       end do
+    end if
+    if (style == 0) then
+       do iunit = 1, nunit_all
+          do junit = 1, iunit - 1
+             eu(iunit, iunit, 1:E_TYPE%MAX) = eu(iunit, iunit, 1:E_TYPE%MAX) + 0.5e0_PREC * eu(junit, iunit, 1:E_TYPE%MAX)
+          end do
+          do junit = iunit + 1, nunit_all
+             eu(iunit, iunit, 1:E_TYPE%MAX) = eu(iunit, iunit, 1:E_TYPE%MAX) + 0.5e0_PREC * eu(iunit, junit, 1:E_TYPE%MAX)
+          end do
+       end do
+    else
+       do iunit = 1, nunit_all
+          do junit = iunit + 1, nunit_all
+             do i = 1, E_TYPE%MAX
+                sume = eu(iunit, junit, i) + eu(junit, iunit, i)
+                eu(iunit, junit, i) = sume
+                eu(junit, iunit, i) = sume
+             end do
+          end do
+       end do
+    end if
+    do i = 1, E_TYPE%MAX
+       if (i == E_TYPE%TOTAL .or. i == E_TYPE%VELO) cycle
+       eu(1:nunit_all, 1:nunit_all, E_TYPE%TOTAL) = eu(1:nunit_all, 1:nunit_all, E_TYPE%TOTAL) &
+            + eu(1:nunit_all, 1:nunit_all, i)
+    end do
+    eu_out(:,:,:) = eu(:,:,:)
+  end subroutine tisref_energy_unit
+
   ! mass for the kinetic term (the glue keeps its own copy)
   subroutine tisref_set_mass(m) bind(C, name="tisref_set_mass")
     real(c_double), intent(in) :: m(nmp_all)

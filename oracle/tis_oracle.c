@@ -367,6 +367,18 @@ void oracle_force_coulomb(const tis_system *s, const double *pxyz, const int32_t
   }
 }
 
+/* Per-unit-pair energies (energy_unit of energy_sumup.F90:17-36): while
+ * eu_acc is set, every energy term also adds its energy to
+ * eu_acc[(unit(i), unit(j), type)] (column-major nunit x nunit x TIS_E_MAX)
+ * with the beads each term of the reference uses for (iunit, junit). */
+static __thread double *eu_acc = NULL;
+static __thread const tis_system *eu_sys = NULL;
+static void eu_add(int type, int i, int j, double e) {
+  if (!eu_acc) return;
+  const int nu = eu_sys->nunit;
+  eu_acc[(size_t)eu_sys->unit[i] + (size_t)nu * eu_sys->unit[j] + (size_t)nu * nu * type] += e;
+}
+
 /* energy_ele_coulomb.F90:47-80: coef_ij (1/r - 1/r_c) */
 double oracle_energy_coulomb(const tis_system *s, const double *pxyz, const int32_t *pairs,
                              const double *coef, int64_t n) {
@@ -380,6 +392,7 @@ double oracle_energy_coulomb(const tis_system *s, const double *pxyz, const int3
     double d2 = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
     if (d2 > cutoff2) continue;
     e += coef[k] * (1.0 / sqrt(d2) - cut_inv);
+    eu_add(TIS_E_ELE, i, j, coef[k] * (1.0 / sqrt(d2) - cut_inv));
   }
   return e;
 }
@@ -898,6 +911,7 @@ double oracle_energy_bond(const tis_system *s, const double *xyz) {
     sub3(xyz, s->bd_mp[2 * b + 1], s->bd_mp[2 * b], v);
     double dd = sqrt(sq3(v)) - s->bd_nat[b];
     e += s->bd_coef[b] * (dd * dd);
+    eu_add(TIS_E_BOND, s->bd_mp[2 * b], s->bd_mp[2 * b + 1], s->bd_coef[b] * (dd * dd));
   }
   return e;
 }
@@ -917,6 +931,7 @@ double oracle_energy_bangle(const tis_system *s, const double *xyz) {
     if (ct > 1.0) ct = 1.0; else if (ct < -1.0) ct = -1.0;
     double dba = acos(ct) - s->ba_nat[a];
     e += s->ba_coef[a] * (dba * dba);
+    eu_add(TIS_E_BANGLE, i1, i2, s->ba_coef[a] * (dba * dba));
   }
   return e;
 }
@@ -942,6 +957,7 @@ double oracle_energy_exv(const tis_system *s, const double *pxyz, const int32_t 
       ene = HIGH_ENERGY;
     }
     e += ene;
+    eu_add(TIS_E_EXV, i, j, ene);
   }
   return e;
 }
@@ -962,6 +978,7 @@ double oracle_energy_dh(const tis_system *s, const double *pxyz, const int32_t *
     if (d2 > cutoff2) continue;
     double d1 = sqrt(d2);
     e += coef[k] / d1 * exp(-d1 * rcdist);
+    eu_add(TIS_E_ELE, i, j, coef[k] / d1 * exp(-d1 * rcdist));
   }
   return e;
 }
@@ -993,6 +1010,7 @@ double oracle_energy_stack(const tis_system *s, const double *xyz, const uint8_t
     dph = wrap_pi(dih - nat[2]);
     ediv += cf[3] * dph * dph;
     e += cf[0] / ediv;
+    eu_add(TIS_E_STACK, m7[0], m7[0], cf[0] / ediv);
   }
   return e;
 }
@@ -1034,6 +1052,7 @@ double oracle_energy_tstack(const tis_system *s, const double *xyz, const double
     dph = wrap_pi(atan2(dot3(v12, n) * a42, dot3(m, n)) - nat[5]);
     ediv += cf[6] * dph * dph;
     e += cf[0] / ediv;
+    eu_add(TIS_E_TSTACK, m6[0], m6[1], cf[0] / ediv);
   }
   return e;
 }
@@ -1044,6 +1063,7 @@ double oracle_energy_kinetic(const tis_system *s, const double *velo) {
   for (int i = 0; i < s->nmp; ++i) {
     const double *v = velo + 3 * i;
     e += 0.5 * s->mass[i] * dot3(v, v);
+    eu_add(TIS_E_VELO, i, i, 0.5 * s->mass[i] * dot3(v, v));
   }
   return e;
 }
@@ -1143,6 +1163,7 @@ void oracle_energy_sumup(oracle_ws *w, const double *xyz, const double *velo, do
     int h = w->hb[q];
     if (!w->hb_status[h]) continue;
     if (s->hb_tertiary[h]) ethb += w->ene_hb[h]; else ehb += w->ene_hb[h];
+    eu_add(s->hb_tertiary[h] ? TIS_E_THBOND : TIS_E_HBOND, s->hb_mp[6 * h], s->hb_mp[6 * h + 1], w->ene_hb[h]);
   }
   energy[TIS_E_HBOND] = ehb;
   energy[TIS_E_THBOND] = ethb;
@@ -1158,6 +1179,45 @@ void oracle_energy_sumup(oracle_ws *w, const double *xyz, const double *velo, do
   for (int k = 0; k < TIS_E_MAX; ++k)
     if (k != TIS_E_TOTAL && k != TIS_E_VELO) tot += energy[k];
   energy[TIS_E_TOTAL] = tot;
+}
+
+/* energy_unit of energy_sumup.F90:76-507: the per-unit-pair sums of every
+ * term (the Ewald terms add none: energy_ele_coulomb_ewld.F90:168-170 leaves
+ * them out), then for style 0 each diagonal entry takes half of the upper
+ * off-diagonal entries of its row and column (:480-492), for style 1 the
+ * off-diagonal pairs are symmetrised (:494-503), and TOTAL is the sum over
+ * the types other than TOTAL and VELO (:504-508). */
+void oracle_energy_unit(oracle_ws *w, const double *xyz, const double *velo, int32_t style, double *eu) {
+  const tis_system *s = w->s;
+  const int nu = s->nunit;
+  const size_t nn = (size_t)nu * nu;
+  for (size_t k = 0; k < nn * TIS_E_MAX; ++k) eu[k] = 0.0;
+  double energy[TIS_E_MAX];
+  eu_acc = eu; eu_sys = s;
+  oracle_energy_sumup(w, xyz, velo, energy);
+  eu_acc = NULL; eu_sys = NULL;
+#define EU(a, b, t) eu[(size_t)(a) + (size_t)nu * (b) + nn * (t)]
+  if (style == 0) {
+    for (int iu = 0; iu < nu; ++iu) {
+      for (int ju = 0; ju < iu; ++ju)
+        for (int t = 0; t < TIS_E_MAX; ++t) EU(iu, iu, t) = EU(iu, iu, t) + 0.5 * EU(ju, iu, t);
+      for (int ju = iu + 1; ju < nu; ++ju)
+        for (int t = 0; t < TIS_E_MAX; ++t) EU(iu, iu, t) = EU(iu, iu, t) + 0.5 * EU(iu, ju, t);
+    }
+  } else {
+    for (int iu = 0; iu < nu; ++iu)
+      for (int ju = iu + 1; ju < nu; ++ju)
+        for (int t = 0; t < TIS_E_MAX; ++t) {
+          double sume = EU(iu, ju, t) + EU(ju, iu, t);
+          EU(iu, ju, t) = sume;
+          EU(ju, iu, t) = sume;
+        }
+  }
+  for (int t = 0; t < TIS_E_MAX; ++t) {
+    if (t == TIS_E_TOTAL || t == TIS_E_VELO) continue;
+    for (size_t k = 0; k < nn; ++k) eu[k + nn * TIS_E_TOTAL] += eu[k + nn * t];
+  }
+#undef EU
 }
 
 /* time_integral_pre.F90:160-170 */

@@ -102,6 +102,7 @@ void oracle_neighbor(oracle_ws *w, const double *xyz);
 int32_t oracle_force_sumup(oracle_ws *w, const double *xyz, double *f,
                            uint64_t seed, uint32_t sys, uint64_t step);
 void oracle_energy_sumup(oracle_ws *w, const double *xyz, const double *velo, double *energy);
+void oracle_energy_unit(oracle_ws *w, const double *xyz, const double *velo, int32_t style, double *eu);
 int64_t oracle_ws_lists(oracle_ws *w, int32_t which, const int32_t **pairs, const double **coef);
 
 /* ---- Brownian dynamics with hydrodynamic interaction ----

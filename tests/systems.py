@@ -78,3 +78,22 @@ def force_error(f, fo):
     nf = np.linalg.norm(fo, axis=1)
     floor = 1e-3 * max(nf.max(), 1e-12)
     return np.linalg.norm(f - fo, axis=1) / np.maximum(nf, floor)
+
+
+def two_chain_system(seed=31):
+    """Two hairpins (units 0 and 1) and a unit of ions: every off-diagonal
+    unit pair has exv and DH energies."""
+    rng = np.random.default_rng(seed)
+    chains = [T.hairpin_design(rng, 24, 5, 4), T.hairpin_design(rng, 18, 4, 4)]
+    s = T.build_rna_system(chains, ions=["Mg"] * 2 + ["K"] * 3 + ["Cl"] * 3, name="two_chains")
+    x = T.compact_coords(s, rng, box=34, min_dist=2.9)
+    return s, x
+
+
+def unit_cases(cases=None):
+    """Systems of several units for energy_unit: two chains + ions, the
+    ion-rich compact chains (one periodic) and an Ewald box."""
+    cases = parity_cases(ewald=False) if cases is None else cases
+    return ([("two_chains",) + two_chain_system()] + [c for c in cases if c[0] in ("compact2", "compact3")]
+            + [("ions_box_ewald",) + small_ions_box()])
+

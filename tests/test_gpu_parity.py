@@ -7,7 +7,7 @@ from tis_amd.engine import Engine, LIST_ELE, LIST_EXV, LIST_HB, MODE_EAGER, MODE
 
 from tis_amd import topology as T
 
-from systems import force_error, hairpin_design, hairpin64, parity_cases
+from systems import force_error, hairpin_design, hairpin64, parity_cases, unit_cases
 
 pytestmark = pytest.mark.gpu
 CASES = parity_cases()
@@ -180,4 +180,29 @@ def test_displacement_trigger_modes_and_batch_bitwise():
         alone.langevin_steps(40, MODE_GRAPH)
         assert np.array_equal(alone.coords(0), res[1][k])
         alone.close()
+
+
+@pytest.mark.parametrize("style", [0, 1])
+def test_energy_unit_matches_oracle(style):
+    """energy_unit (per unit pair and type, post-processed) through the
+    C-ABI against the FP64 oracle, on two chains + ions, ion-rich compact
+    chains (one periodic) and an Ewald box."""
+    for name, s, x in unit_cases():
+        print(f"[energy_unit] {name} nunit={s.nunit} style={style}", flush=True)
+        v = np.random.default_rng(5).normal(size=x.shape)
+        eng = Engine([s], keys=[0], seed=0)
+        eng.set_coords(0, x)
+        eng.set_state(0, 1, v)
+        eng.neighbor(0)
+        eng.force_sumup(0)
+        eu = eng.energy_unit(0, style)
+        o = oracle.Oracle(s)
+        o.neighbor(x)
+        o.force(x, seed=0, sys_id=0, step=0)
+        euo = o.energy_unit(x, v, style)
+        scale = np.abs(euo).max()
+        err = np.abs(eu - euo) / np.maximum(np.abs(euo), 1e-6 * scale)
+        print(f"   max rel err {err.max():.3e}", flush=True)
+        assert err.max() <= 1e-7
+        eng.close()
 

@@ -19,7 +19,7 @@ from tis_amd import topology as T
 from tis_amd.sysdesc import (E_BANGLE, E_BOND, E_ELE, E_EXV, E_HBOND, E_STACK, E_TOTAL,
                              E_TSTACK)
 
-from systems import compact_allpairs, hairpin64, parity_cases
+from systems import compact_allpairs, hairpin64, parity_cases, unit_cases
 
 CASES = parity_cases(ewald=False)
 needs_ref = pytest.mark.skipif(not RefLib.available(), reason="oracle/_ref not built (reference absent)")
@@ -140,6 +140,62 @@ def test_energies_match_reference(case):
     er = R.energy(v)
     for k in (E_BOND, E_BANGLE, E_ELE, E_STACK, E_HBOND, E_EXV, E_TSTACK, E_TOTAL, 1):
         assert abs(e[k] - er[k]) <= 1e-12 * max(abs(er[k]), 1e-6), k
+
+
+UNIT_CASES = unit_cases(CASES)
+
+
+@needs_ref
+@pytest.mark.parametrize("case", UNIT_CASES, ids=[c[0] for c in UNIT_CASES])
+@pytest.mark.parametrize("style", [0, 1])
+def test_energy_unit_matches_reference(case, style):
+    """energy_unit of energy_sumup (per unit pair and type, after the
+    post-processing of i_output_energy_style) against the reference's term
+    routines and post-processing."""
+    name, s, x = case
+    o = Oracle(s)
+    o.neighbor(x)
+    o.force(x)
+    R = RefLib()
+    R.load(s)
+    R.neighbor(x)
+    R.force_term("stack")
+    if not _valence_free(o, s, x):
+        pytest.skip("reference draws Mersenne-Twister numbers for this configuration")
+    R.force_term("hbond")
+    v = np.random.default_rng(3).normal(size=x.shape)
+    eu = o.energy_unit(x, v, style)
+    eur = R.energy_unit(v, style)
+    assert s.nunit >= 2 and eu.shape == (s.nunit, s.nunit, 25)
+    assert np.abs(eu - eur).max() <= 1e-12 * max(np.abs(eur).max(), 1e-6)
+
+
+@pytest.mark.parametrize("case", UNIT_CASES, ids=[c[0] for c in UNIT_CASES])
+def test_energy_unit_diagonal_sums_to_energy(case):
+    """With style 0 each off-diagonal pair energy is split between the two
+    diagonal entries: the diagonal summed over units gives every term's
+    total (pairs are stored with the lower bead's unit first)."""
+    name, s, x = case
+    o = Oracle(s)
+    o.neighbor(x)
+    o.force(x)
+    v = np.random.default_rng(4).normal(size=x.shape)
+    e = o.energy(x, v)
+    eu = o.energy_unit(x, v, 0)
+    diag = np.einsum("iik->k", eu)
+    assert np.abs(np.tril(eu[:, :, E_EXV], -1)).max() == 0.0
+    ewald = s.params.ele_form == 2
+    if ewald:
+        # energy_ele_coulomb_ewld.F90:168-170 adds nothing per unit
+        assert np.abs(eu[:, :, E_ELE]).max() == 0.0
+        e = e.copy()
+        e[E_TOTAL] -= e[E_ELE]
+        e[E_ELE] = 0.0
+    for k in range(25):
+        assert abs(diag[k] - e[k]) <= 1e-12 * max(abs(e[k]), 1.0), k
+    # style 1 keeps the raw diagonal and symmetric off-diagonal sums
+    eu1 = o.energy_unit(x, v, 1)
+    assert np.array_equal(eu1[:, :, E_BOND], eu1[:, :, E_BOND].T)
 
 
 @needs_ref

@@ -30,7 +30,7 @@ __global__ void k_lists(DevPtrs, const double *, const double *, const Tile *, i
                         int, int);
 __global__ void k_advance(uint64_t *, uint64_t, unsigned long long *);
 __global__ void k_count_rebuild(unsigned long long *);
-__global__ void k_energy(DevPtrs, const double *, const double *, const int32_t *, double *);
+__global__ void k_energy(DevPtrs, const double *, const double *, const int32_t *, double *, int);
 __global__ void k_label_apply(DevPtrs, LabelDev, const int32_t *, const int32_t *, const double *);
 }  // namespace tis
 
@@ -81,6 +81,8 @@ struct tis_ctx {
   int32_t nsystask = 0;
   std::vector<int32_t> systask_of;       // index into d_systask per system or -1
   double *d_energy = nullptr;
+  double *d_eunit = nullptr;   // raw per-unit-pair energies [nunit^2][25]
+  size_t eunit_cap = 0;
   unsigned long long *d_counters = nullptr;
   uint64_t *d_stepbase = nullptr;
   bool any_ele = false, any_periodic = false;
@@ -972,11 +974,72 @@ int32_t tis_energy_sumup(tis_ctx *c, int32_t isys, double *energy_out) {
     hipLaunchKernelGGL(k_ewald_sk, dim3(ekn), dim3(kTile), 0, c->stream, c->P, PXc(c), c->d_ewsk + ek0, 1);
   const int nsel = isys < 0 ? c->nsys : 1;
   hipLaunchKernelGGL(k_energy, dim3(nsel), dim3(256), 0, c->stream, c->P, Xc(c), PXc(c),
-                     c->d_sysidx + (isys < 0 ? 0 : isys), c->d_energy);
+                     c->d_sysidx + (isys < 0 ? 0 : isys), c->d_energy, 0);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(energy_out, c->d_energy, 25 * sizeof(double) * (size_t)nsel, hipMemcpyDeviceToHost,
                            c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TIS_OK;
+}
+
+int32_t tis_energy_unit(tis_ctx *c, int32_t isys, int32_t style, double *energy_unit) {
+  if (!c || !energy_unit || (style != 0 && style != 1)) return TIS_ERR_ARG;
+  int rc = check_sys(c, isys);
+  if (rc) return rc;
+  if (isys < 0) return fail(c, TIS_ERR_ARG, "tis_energy_unit needs one system");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (!c->lists_valid) { rc = rebuild_lists(c, -1); if (rc) return rc; }
+  const int nu = c->hs[isys].d.nunit;
+  const size_t nn = (size_t)nu * nu;
+  if (nu < 1 || nn > 65535) return fail(c, TIS_ERR_ARG, "system %d has %d units", isys, nu);
+  if (c->eunit_cap < nn * 25) {
+    if (c->d_eunit) {
+      (void)hipFree(c->d_eunit);
+      c->allocs.erase(std::remove(c->allocs.begin(), c->allocs.end(), (void *)c->d_eunit), c->allocs.end());
+      c->dev_bytes -= c->eunit_cap * sizeof(double);
+      c->d_eunit = nullptr;
+    }
+    rc = dalloc(c, &c->d_eunit, nn * 25);
+    if (rc) return rc;
+    c->eunit_cap = nn * 25;
+  }
+  hipLaunchKernelGGL(k_energy, dim3(1, (unsigned)nn), dim3(256), 0, c->stream, c->P, Xc(c), PXc(c),
+                     c->d_sysidx + isys, c->d_eunit, 1);
+  HIPCHK(c, hipGetLastError());
+  std::vector<double> raw(nn * 25);
+  HIPCHK(c, hipMemcpyAsync(raw.data(), c->d_eunit, raw.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  // energy_unit(iunit, junit, type), column-major, from the raw block sums
+  // raw[(a + nu*b)*25 + k]
+  std::vector<double> m(nn * 25);
+  for (int b = 0; b < nu; ++b)
+    for (int a = 0; a < nu; ++a)
+      for (int k = 0; k < 25; ++k) m[(size_t)a + (size_t)nu * b + nn * k] = raw[((size_t)a + (size_t)nu * b) * 25 + k];
+  auto E = [&](int a, int b, int k) -> double & { return m[(size_t)a + (size_t)nu * b + nn * k]; };
+  for (int a = 0; a < nu; ++a)
+    for (int b = 0; b < nu; ++b) E(a, b, TIS_E_TOTAL) = 0.0;
+  // energy_sumup.F90:480-508
+  if (style == 0) {
+    for (int iu = 0; iu < nu; ++iu) {
+      for (int ju = 0; ju < iu; ++ju)
+        for (int k = 0; k < 25; ++k) E(iu, iu, k) = E(iu, iu, k) + 0.5 * E(ju, iu, k);
+      for (int ju = iu + 1; ju < nu; ++ju)
+        for (int k = 0; k < 25; ++k) E(iu, iu, k) = E(iu, iu, k) + 0.5 * E(iu, ju, k);
+    }
+  } else {
+    for (int iu = 0; iu < nu; ++iu)
+      for (int ju = iu + 1; ju < nu; ++ju)
+        for (int k = 0; k < 25; ++k) {
+          const double sume = E(iu, ju, k) + E(ju, iu, k);
+          E(iu, ju, k) = sume;
+          E(ju, iu, k) = sume;
+        }
+  }
+  for (int k = 0; k < 25; ++k) {
+    if (k == TIS_E_TOTAL || k == TIS_E_VELO) continue;
+    for (size_t q = 0; q < nn; ++q) m[q + nn * TIS_E_TOTAL] += m[q + nn * k];
+  }
+  std::memcpy(energy_unit, m.data(), m.size() * sizeof(double));
   return TIS_OK;
 }
 

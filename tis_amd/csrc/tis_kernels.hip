@@ -1435,17 +1435,24 @@ __device__ inline double block_sum(double v, double *red) {
   return r;
 }
 
+// unit_mode 1: block (x, y) sums only the terms whose unit pair, taken as
+// energy_sumup.F90's terms take it, is (y mod nunit, y / nunit): raw
+// energy_unit entries (the Ewald terms add none, as in
+// energy_ele_coulomb_ewld.F90:168-170).
 __global__ __launch_bounds__(kEBlock) void k_energy(DevPtrs P, const double *X, const double *PX,
-                                                    const int32_t *sysidx, double *out) {
+                                                    const int32_t *sysidx, double *out, int unit_mode) {
   __shared__ double red[kEBlock];
   const int t = threadIdx.x;
   const int is = sysidx[blockIdx.x];
   const DSys s = P.sys[is];
   const int n = P.ntot;
+  const int ua = unit_mode ? (int)blockIdx.y % s.nunit : 0, ub = unit_mode ? (int)blockIdx.y / s.nunit : 0;
+  auto take = [&](int gi, int gj) { return !unit_mode || (P.unit[gi] == ua && P.unit[gj] == ub); };
   double e[kNE];
   for (int k = 0; k < kNE; ++k) e[k] = 0.0;
   // bonds (energy_bond.F90:39-57)
   for (int b = s.boff + t; b < s.boff + s.nbd; b += kEBlock) {
+    if (!take(P.bd_mp[2 * b], P.bd_mp[2 * b + 1])) continue;
     double v[3];
     vsub(X, n, P.bd_mp[2 * b + 1], P.bd_mp[2 * b], v);
     double dd = sqrt(dot3(v, v)) - P.bd_nat[b];
@@ -1453,6 +1460,7 @@ __global__ __launch_bounds__(kEBlock) void k_energy(DevPtrs P, const double *X, 
   }
   // angles (energy_bangle.F90:36-52)
   for (int a = s.aoff + t; a < s.aoff + s.nba; a += kEBlock) {
+    if (!take(P.ba_mp[3 * a], P.ba_mp[3 * a + 1])) continue;
     double v21[3], v32[3];
     vsub(X, n, P.ba_mp[3 * a + 1], P.ba_mp[3 * a], v21);
     vsub(X, n, P.ba_mp[3 * a + 2], P.ba_mp[3 * a + 1], v32);
@@ -1463,6 +1471,7 @@ __global__ __launch_bounds__(kEBlock) void k_energy(DevPtrs P, const double *X, 
   }
   // consecutive stacks (energy_dtrna_stack.F90:85-160)
   for (int st = s.soff + t; st < s.soff + s.nst; st += kEBlock) {
+    if (!take(P.st_mp[7 * st], P.st_mp[7 * st])) continue;
     if (!stack_on(P, X, PX, st)) continue;
     const int32_t *m7 = P.st_mp + 7 * st;
     const double *nat = P.st_nat + 3 * st, *cf = P.st_coef + 4 * st;
@@ -1482,6 +1491,7 @@ __global__ __launch_bounds__(kEBlock) void k_energy(DevPtrs P, const double *X, 
   }
   // tertiary stacks (energy_dtrna_stack_nlocal.F90:55-230)
   for (int k = s.toff + t; k < s.toff + s.ntst; k += kEBlock) {
+    if (!take(P.tst_mp[6 * k], P.tst_mp[6 * k + 1])) continue;
     const int32_t *m6 = P.tst_mp + 6 * k;
     const double *nat = P.tst_nat + 6 * k, *cf = P.tst_coef + 7 * k;
     const double *Y = s.periodic ? PX : X;
@@ -1512,7 +1522,8 @@ __global__ __launch_bounds__(kEBlock) void k_energy(DevPtrs P, const double *X, 
   }
   // hbonds: energies of the last force evaluation (energy_dtrna_hbond15.F90:119-148)
   for (int h = s.hoff + t; h < s.hoff + s.nhb; h += kEBlock)
-    if (P.hb_inlist[h] && P.hb_status[h]) e[P.hb_tert[h] ? 5 : 4] += P.hb_ene[h];
+    if (P.hb_inlist[h] && P.hb_status[h] && take(P.hb_mp[6 * h], P.hb_mp[6 * h + 1]))
+      e[P.hb_tert[h] ? 5 : 4] += P.hb_ene[h];
   // excluded volume, each pair once by its lower bead (energy_exv_dt15.F90:60-116)
   {
     const double *Y = s.periodic ? PX : X;
@@ -1525,7 +1536,7 @@ __global__ __launch_bounds__(kEBlock) void k_energy(DevPtrs P, const double *X, 
       for (int m = 0; m < cnt; ++m) {
         uint32_t ent = P.nb[(size_t)m * n + g];
         int j = (int)(ent & kIdxMask);
-        if (j < g) continue;
+        if (j < g || !take(g, j)) continue;
         double v[3] = {Y[j] - xi[0], Y[n + j] - xi[1], Y[2 * n + j] - xi[2]};
         if (s.periodic) { int c = (int)(ent >> kMirrorShift); for (int d = 0; d < 3; ++d) v[d] += mirror_shift(s, c, d); }
         int ti = P.mptype[g], tj = P.mptype[j];
@@ -1549,6 +1560,7 @@ __global__ __launch_bounds__(kEBlock) void k_energy(DevPtrs P, const double *X, 
   if (s.ele_form == 0) {
     const double *Y = s.periodic ? PX : X;
     for (int qi = t; qi < s.nq; qi += kEBlock) {
+      if (unit_mode && P.unit[P.qbead[s.qoff + qi]] != ua) continue;
       int gi = P.qbead[s.qoff + qi];
       double xi[3];
       ld3(Y, n, gi, xi);
@@ -1556,6 +1568,7 @@ __global__ __launch_bounds__(kEBlock) void k_energy(DevPtrs P, const double *X, 
       for (int qj = qi + 1; qj < s.nq; ++qj) {
         if (!((row[qj >> 5] >> (qj & 31)) & 1u)) continue;
         int gj = P.qbead[s.qoff + qj];
+        if (unit_mode && P.unit[gj] != ub) continue;
         double v[3] = {Y[gj] - xi[0], Y[n + gj] - xi[1], Y[2 * n + gj] - xi[2]};
         if (s.periodic) { int c = P.qmirror[s.mioff + (int64_t)qi * s.nq + qj]; for (int d = 0; d < 3; ++d) v[d] += mirror_shift(s, c, d); }
         double d2 = dot3(v, v);
@@ -1567,9 +1580,10 @@ __global__ __launch_bounds__(kEBlock) void k_energy(DevPtrs P, const double *X, 
   }
   // plain Coulomb (energy_ele_coulomb.F90:55-66) and the Ewald real-space
   // term (energy_ele_coulomb_ewld.F90:126-138), each pair once
-  if (s.ele_form == 1 || s.ele_form == 2) {
+  if (s.ele_form == 1 || (s.ele_form == 2 && !unit_mode)) {
     const double *Y = s.periodic ? PX : X;
     for (int qi = t; qi < s.nq; qi += kEBlock) {
+      if (unit_mode && P.unit[P.qbead[s.qoff + qi]] != ua) continue;
       int gi = P.qbead[s.qoff + qi];
       double xi[3];
       ld3(Y, n, gi, xi);
@@ -1577,6 +1591,7 @@ __global__ __launch_bounds__(kEBlock) void k_energy(DevPtrs P, const double *X, 
       for (int qj = qi + 1; qj < s.nq; ++qj) {
         if (!((row[qj >> 5] >> (qj & 31)) & 1u)) continue;
         int gj = P.qbead[s.qoff + qj];
+        if (unit_mode && P.unit[gj] != ub) continue;
         double v[3] = {Y[gj] - xi[0], Y[n + gj] - xi[1], Y[2 * n + gj] - xi[2]};
         if (s.periodic) { int c = P.qmirror[s.mioff + (int64_t)qi * s.nq + qj]; for (int d = 0; d < 3; ++d) v[d] += mirror_shift(s, c, d); }
         double d2 = dot3(v, v);
@@ -1589,7 +1604,7 @@ __global__ __launch_bounds__(kEBlock) void k_energy(DevPtrs P, const double *X, 
   }
   // Ewald Fourier sum (energy_ele_coulomb_ewld.F90:70-90) from the partial
   // structure factors of the last k_ewald_sk pass, plus the self term
-  if (s.ele_form == 2) {
+  if (s.ele_form == 2 && !unit_mode) {
     for (int k = t; k < s.nk; k += kEBlock) {
       double sc = 0.0, ss = 0.0;
       for (int c = 0; c < s.nqc; ++c) {
@@ -1603,13 +1618,14 @@ __global__ __launch_bounds__(kEBlock) void k_energy(DevPtrs P, const double *X, 
   // kinetic (energy_kinetic.F90:33-44)
   for (int li = t; li < s.nmp; li += kEBlock) {
     int g = s.off + li;
+    if (!take(g, g)) continue;
     double v[3] = {P.v[g], P.v[n + g], P.v[2 * n + g]};
     e[8] += 0.5 * P.mass[g] * dot3(v, v);
   }
   double tot[kNE];
   for (int k = 0; k < kNE; ++k) tot[k] = block_sum(e[k], red);
   if (t == 0) {
-    double *o = out + (size_t)blockIdx.x * 25;
+    double *o = out + ((size_t)blockIdx.x * gridDim.y + blockIdx.y) * 25;
     for (int k = 0; k < 25; ++k) o[k] = 0.0;
     o[2] = tot[0]; o[3] = tot[1]; o[15] = tot[2]; o[18] = tot[3]; o[16] = tot[4]; o[19] = tot[5];
     o[17] = tot[6]; o[7] = tot[7]; o[1] = tot[8];

@@ -58,6 +58,7 @@ def load_library():
     L.tis_force_sumup.argtypes = [vp, C.c_int32, _f64p]
     L.tis_force_get.argtypes = [vp, C.c_int32, _f64p]
     L.tis_energy_sumup.argtypes = [vp, C.c_int32, _f64p]
+    L.tis_energy_unit.argtypes = [vp, C.c_int32, C.c_int32, _f64p]
     L.tis_langevin_init.argtypes = [vp, C.c_double, C.c_int32, C.c_uint64, C.c_int32]
     L.tis_langevin_steps.argtypes = [vp, C.c_int64, C.c_int32]
     L.tis_system_origin.argtypes = [vp, C.c_int32, C.c_uint64, C.c_int32]
@@ -70,7 +71,7 @@ def load_library():
     L.tis_last_force_ms.argtypes = [vp, _f64p]
     for fn in ("tis_ctx_create", "tis_ctx_destroy", "tis_system_upload", "tis_set_system_T", "tis_state_set",
                "tis_state_get", "tis_neighbor", "tis_pairs_get", "tis_force_sumup", "tis_force_get",
-               "tis_energy_sumup", "tis_langevin_init", "tis_langevin_steps", "tis_status_counters",
+               "tis_energy_sumup", "tis_energy_unit", "tis_langevin_init", "tis_langevin_steps", "tis_status_counters",
                "tis_system_origin", "tis_device_bytes", "tis_label_define", "tis_label_apply",
                "tis_synchronize", "tis_last_timing", "tis_last_force_ms"):
         getattr(L, fn).restype = C.c_int32
@@ -175,6 +176,13 @@ class Engine:
         e = np.zeros(E_MAX)
         self._chk(self.L.tis_energy_sumup(self.ctx, isys, _fp(e)), "tis_energy_sumup")
         return e
+
+    def energy_unit(self, isys, style=0):
+        """energy_unit (nunit, nunit, E_MAX) of system isys, indexed [iunit, junit, type]."""
+        nu = self.systems[isys].nunit
+        eu = np.zeros(E_MAX * nu * nu)
+        self._chk(self.L.tis_energy_unit(self.ctx, isys, style, _fp(eu)), "tis_energy_unit")
+        return eu.reshape(E_MAX, nu, nu).transpose(2, 1, 0)
 
     def energy_all(self):
         """(nsys, E_MAX) energies of every system in one launch."""

@@ -64,7 +64,7 @@ module tis_hip
 
   public :: tis_ctx_create, tis_ctx_destroy, tis_last_error, tis_system_upload, tis_set_system_T, &
        tis_state_set, tis_state_get, tis_neighbor, tis_pairs_get, tis_force_sumup, tis_force_get, &
-       tis_energy_sumup, tis_langevin_init, tis_langevin_steps, tis_system_origin, tis_device_bytes, &
+       tis_energy_sumup, tis_energy_unit, tis_langevin_init, tis_langevin_steps, tis_system_origin, tis_device_bytes, &
        tis_status_counters, tis_synchronize, tis_last_timing, tis_last_force_ms, tis_label_define, &
        tis_label_apply, tis_bdhi_setup, tis_bdhi_state_set, tis_bdhi_state_get, tis_bdhi_forces, &
        tis_bdhi_tensors, tis_bdhi_tensor_get, tis_bdhi_factor_error, tis_bdhi_steps, tis_bdhi_timing
@@ -139,6 +139,12 @@ module tis_hip
        type(c_ptr), value :: ctx
        integer(c_int32_t), value :: isys
        real(c_double), intent(out) :: energy_out(*)
+     end function
+     integer(c_int32_t) function tis_energy_unit(ctx, isys, style, energy_unit) bind(C, name="tis_energy_unit")
+       import :: c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: isys, style
+       real(c_double), intent(out) :: energy_unit(*)
      end function
      integer(c_int32_t) function tis_langevin_init(ctx, tstep, rebuild_every, step0, init_accel) &
           bind(C, name="tis_langevin_init")
@@ -288,17 +294,22 @@ contains
     call hip_check(ctx, tis_force_sumup(ctx, int(irep - 1, c_int32_t), force_mp), 'tis_force_sumup')
   end subroutine hip_force_sumup
 
-  ! energy_sumup(irep, velo_mp, energy, energy_unit), energy_sumup.F90:17-36
-  ! (the per-unit-pair matrix is not produced: energy_unit is zeroed)
-  subroutine hip_energy_sumup(ctx, irep, velo_mp, energy, energy_unit)
+  ! energy_sumup(irep, velo_mp, energy, energy_unit), energy_sumup.F90:17-36;
+  ! energy_unit(nunit, nunit, E_TYPE%MAX) with the post-processing of
+  ! i_output_energy_style (style, default 0)
+  subroutine hip_energy_sumup(ctx, irep, velo_mp, energy, energy_unit, style)
     type(c_ptr), intent(in) :: ctx
     integer, intent(in) :: irep
     real(c_double), intent(in) :: velo_mp(:,:)
     real(c_double), intent(out) :: energy(:)
     real(c_double), intent(out) :: energy_unit(:,:,:)
+    integer, intent(in), optional :: style
+    integer(c_int32_t) :: st
+    st = 0_c_int32_t
+    if (present(style)) st = int(style, c_int32_t)
     call hip_check(ctx, tis_state_set(ctx, int(irep - 1, c_int32_t), 1_c_int32_t, velo_mp), 'tis_state_set')
     call hip_check(ctx, tis_energy_sumup(ctx, int(irep - 1, c_int32_t), energy), 'tis_energy_sumup')
-    energy_unit(:,:,:) = 0.0_c_double
+    call hip_check(ctx, tis_energy_unit(ctx, int(irep - 1, c_int32_t), st, energy_unit), 'tis_energy_unit')
   end subroutine hip_energy_sumup
 
 end module tis_hip
