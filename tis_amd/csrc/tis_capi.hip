@@ -19,8 +19,8 @@
 #include "tis_internal.h"
 
 namespace tis {
-__global__ void k_force(DevPtrs, const double *, const double *, const int32_t *, int, const Tile *, int, BondedRange,
-                        int32_t);
+__global__ void k_force(DevPtrs, const double *, const double *, const int32_t *, int, const Tile *, int,
+                        const EwTask *, int, BondedRange, int32_t);
 __global__ void k_ewald_sk(DevPtrs, const double *, const EwTask *, int);
 __global__ void k_ewald_f(DevPtrs, const double *, const EwTask *);
 __global__ void k_main(DevPtrs, const double *, const double *, double *, double *, const Tile *, int, int32_t, int, int);
@@ -46,6 +46,7 @@ struct HostSys {
   bool has_task = false;                // hbonds or tertiary stacks
   int32_t esk_begin = 0, esk_count = 0;  // Ewald structure-factor tasks
   int32_t ef_begin = 0, ef_count = 0;    // Ewald force tasks
+  int32_t pt_begin = 0, pt_count = 0;    // periodic electrostatic chunk-pair tasks
   std::vector<double> mass, fric;
   std::vector<int32_t> stack_ids;       // global stack indices
 };
@@ -86,8 +87,8 @@ struct tis_ctx {
   unsigned long long *d_counters = nullptr;
   uint64_t *d_stepbase = nullptr;
   bool any_ele = false, any_periodic = false;
-  EwTask *d_ewsk = nullptr, *d_ewf = nullptr;
-  int32_t n_ewsk = 0, n_ewf = 0;
+  EwTask *d_ewsk = nullptr, *d_ewf = nullptr, *d_ptask = nullptr;
+  int32_t n_ewsk = 0, n_ewf = 0, n_ptask = 0;
   // state
   int cur = 0;
   bool lists_valid = false;
@@ -265,6 +266,7 @@ struct Range {
   const int32_t *sysidx; int nsel;
   int b0, nb, a0, na, s0, ns;
   int ek0, ekn, ef0, efn;   // Ewald tasks
+  int pt0, ptn;             // periodic electrostatic chunk-pair tasks
 };
 
 Range range_of(tis_ctx *c, int32_t isys) {
@@ -275,6 +277,7 @@ Range range_of(tis_ctx *c, int32_t isys) {
     r.sysidx = c->d_sysidx; r.nsel = c->nsys;
     r.b0 = 0; r.nb = c->P.nbd; r.a0 = 0; r.na = c->P.nba; r.s0 = 0; r.ns = c->P.nst;
     r.ek0 = 0; r.ekn = c->n_ewsk; r.ef0 = 0; r.efn = c->n_ewf;
+    r.pt0 = 0; r.ptn = c->n_ptask;
   } else {
     const HostSys &h = c->hs[isys];
     r.isys = isys;
@@ -284,6 +287,7 @@ Range range_of(tis_ctx *c, int32_t isys) {
     r.sysidx = c->d_sysidx + isys; r.nsel = 1;
     r.b0 = h.d.boff; r.nb = h.d.nbd; r.a0 = h.d.aoff; r.na = h.d.nba; r.s0 = h.d.soff; r.ns = h.d.nst;
     r.ek0 = h.esk_begin; r.ekn = h.esk_count; r.ef0 = h.ef_begin; r.efn = h.ef_count;
+    r.pt0 = h.pt_begin; r.ptn = h.pt_count;
   }
   return r;
 }
@@ -366,7 +370,7 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
   std::vector<Tile> btiles, qtiles;
   std::vector<int32_t> systask;
   std::vector<double> ewhx, ewhy, ewhz, ewc;
-  std::vector<EwTask> ewsk, ewf;
+  std::vector<EwTask> ewsk, ewf, ptask;
   int64_t psk_off = 0, pf_off = 0;
   // excluded-volume species: distinct (mptype, class, radius, epsilon)
   struct Spec { int8_t t, c; double r, e; };
@@ -379,7 +383,7 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
     return (int)specs.size() - 1;
   };
   c->systask_of.assign(nsys, -1);
-  int64_t moff = 0, mioff = 0, pj_off = 0;
+  int64_t moff = 0, mioff = 0, pj_off = 0, pp_off = 0;
   int32_t off = 0, qoff = 0, nbd = 0, nba = 0, nst = 0, ntst = 0, nhb = 0, nsite = 0, uoff = 0;
   c->any_ele = false;
   c->any_periodic = false;
@@ -516,6 +520,15 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
     d.ele_once = (!s.p.periodic && (s.p.ele_form == TIS_ELE_DH || s.p.ele_form == TIS_ELE_COULOMB) && nq > 0);
     d.pj_off = pj_off;
     if (d.ele_once) pj_off += (int64_t)d.nqt * (d.nqt - 1) / 2 * 3 * kTile;
+    d.ele_pairs = (s.p.periodic && s.p.ele_form != TIS_ELE_NONE && nq > 0);
+    d.pp_off = pp_off;
+    h.pt_begin = (int32_t)ptask.size();
+    if (d.ele_pairs) {
+      pp_off += (int64_t)d.nqt * (d.nqt + 1) / 2 * 6 * kTile;
+      for (int I = 0; I < d.nqt; ++I)
+        for (int J = I; J < d.nqt; ++J) ptask.push_back(EwTask{is, I, J});
+    }
+    h.pt_count = (int32_t)ptask.size() - h.pt_begin;
     h.has_task = (s.nhb > 0 || s.ntst > 0);
     if (h.has_task) { c->systask_of[is] = (int32_t)systask.size(); systask.push_back(is); }
     // Ewald reciprocal vectors and work items
@@ -673,6 +686,7 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
   rc = dalloc(c, &P.ttop, 2 * btiles.size()); if (rc) return rc;
   rc = dalloc(c, &P.sys_flag, (size_t)nsys); if (rc) return rc;
   rc = dalloc(c, &P.pj, (size_t)std::max<int64_t>(pj_off, 1)); if (rc) return rc;
+  rc = dalloc(c, &P.pp, (size_t)std::max<int64_t>(pp_off, 1)); if (rc) return rc;
   rc = dalloc(c, &P.qmask, (size_t)moff); if (rc) return rc;
   rc = dalloc(c, &P.qmirror, (size_t)std::max<int64_t>(mioff, 1)); if (rc) return rc;
   rc = dalloc(c, &c->d_counters, TIS_CNT_MAX); if (rc) return rc;
@@ -690,7 +704,7 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
     auto work = [&](const Tile &t) {
       const DSys &d = c->hs[t.sys].d;
       const int I = t.start / kTile;
-      return d.ele_once ? d.nqt - I : d.nqt;
+      return d.ele_pairs ? 0 : (d.ele_once ? d.nqt - I : d.nqt);
     };
     std::stable_sort(lpt.begin(), lpt.end(), [&](const Tile &a, const Tile &b) { return work(a) > work(b); });
     UP(c->d_qtiles_lpt, lpt);
@@ -707,6 +721,8 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
     rc = dalloc(c, &P.pf, (size_t)std::max<int64_t>(pf_off, 1)); if (rc) return rc;
     UP(c->d_ewsk, ewsk);
     UP(c->d_ewf, ewf);
+    UP(c->d_ptask, ptask);
+    c->n_ptask = (int32_t)ptask.size();
     c->n_ewsk = (int32_t)ewsk.size();
     c->n_ewf = (int32_t)ewf.size();
   }
@@ -909,17 +925,19 @@ static void launch_force_pair(tis_ctx *c, const Range &r, const double *X, const
   // the bonded terms in blocks of 256 lanes
   const int nterm = r.nb + r.na + r.ns;
   const int nqt = (r.qtn > 0 && c->any_ele) ? r.qtn : 0;
-  const int nblk = r.taskn + nqt + (nterm + kTile * kWaves - 1) / (kTile * kWaves);
+  const int npt = c->any_ele ? r.ptn : 0;
+  const int nblk = r.taskn + nqt + (npt + kWaves - 1) / kWaves + (nterm + kTile * kWaves - 1) / (kTile * kWaves);
   if (nblk > 0) {
     const Tile *qt = r.all ? c->d_qtiles_lpt : c->d_qtiles + r.qt0;
     const BondedRange br{r.b0, r.nb, r.a0, r.na, r.s0, r.ns};
     const int32_t *task = c->d_systask + r.task0;
+    const EwTask *pt = c->d_ptask + r.pt0;
     if (c->timing)
       hipExtLaunchKernelGGL(k_force, dim3(nblk), dim3(kTile * kWaves), 0, c->stream, e3[1], e3[2], 0, c->P, X, PX,
-                            task, r.taskn, qt, nqt, br, pre_t);
+                            task, r.taskn, qt, nqt, pt, npt, br, pre_t);
     else
       hipLaunchKernelGGL(k_force, dim3(nblk), dim3(kTile * kWaves), 0, c->stream, c->P, X, PX, task, r.taskn, qt,
-                         nqt, br, pre_t);
+                         nqt, pt, npt, br, pre_t);
   } else if (c->timing) {
     (void)hipEventRecord(e3[1], c->stream);
     (void)hipEventRecord(e3[2], c->stream);

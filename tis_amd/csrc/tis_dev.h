@@ -74,6 +74,10 @@ struct DSys {
   // forces on chunk J from chunk I < J, [nqt(nqt-1)/2][3][kTile] at pj_off
   int32_t ele_once;
   int64_t pj_off;
+  // periodic electrostatics by chunk-pair tasks: partial forces
+  // [nqt(nqt+1)/2][2][3][kTile] at pp_off
+  int32_t ele_pairs;
+  int64_t pp_off;
 };
 
 // Ewald work item: (system, tile a, chunk b)
@@ -138,6 +142,7 @@ struct DevPtrs {
   const int32_t *qbead;      // [nq] global bead of each charge
   double *fdh;               // [3][nq]
   double *pj;                // partial forces of the pair-once sweep
+  double *pp;                // partial forces of the periodic chunk-pair tasks
   uint32_t *qmask;           // bit matrix
   uint8_t *qmirror;          // periodic ele mirror codes
   // Ewald reciprocal space

@@ -748,6 +748,98 @@ __device__ inline void ele_tile_once(const DevPtrs &P, const DSys &s, const doub
   if (nclamp) atomicAdd(&P.counters[1], (unsigned long long)nclamp);
 }
 
+// Periodic systems: one wave per chunk pair (I, J), I <= J (more waves in
+// flight than one block per tile gives a single box).  The same rotating
+// schedule as ele_tile_once takes each pair once with the mirror code the
+// list stored for it (v = x_j - x_i + shift, util_pbneighbor.F90:22-35);
+// the forces on chunk I go to slot 0 of the task's partial buffer, those on
+// chunk J (I < J) to slot 1, and k_main sums a charge's slots in task order.
+// dir = 0 (no whole-wave DPP rotation) rotates with __shfl instead.
+__host__ __device__ inline int64_t ele_task_index(int I, int J, int nch) {
+  return (int64_t)I * (2 * nch - I + 1) / 2 + (J - I);
+}
+
+__device__ inline double rot_next(double v, int dir) {
+  if (dir != 0) return wave_rot(v);
+  const int lane = threadIdx.x & (kTile - 1);
+  return __shfl(v, (lane + 1) & (kTile - 1));
+}
+
+template <int FORM>
+__device__ inline void ele_pair_task(const DevPtrs &P, const DSys &s, const double *PX, int I, int J, int dir,
+                                     double *SX, double *SY, double *SZ, float *SQ) {
+  const int lane = threadIdx.x & (kTile - 1);
+  const int rd = dir != 0 ? dir : 1;
+  const int n = P.ntot;
+  const int qi = I * kTile + lane;
+  const bool valid = qi < s.nq;
+  double xi0 = 0.0, xi1 = 0.0, xi2 = 0.0, chi = 0.0;
+  if (valid) {
+    const int gi = P.qbead[s.qoff + qi];
+    xi0 = PX[gi]; xi1 = PX[n + gi]; xi2 = PX[2 * n + gi];
+    chi = P.charge[gi];
+  }
+  const int jt = J * kTile;
+  {
+    const bool vj = jt + lane < s.nq;
+    const int gj = vj ? P.qbead[s.qoff + jt + lane] : 0;
+    SX[lane] = vj ? PX[gj] : 0.0; SY[lane] = vj ? PX[n + gj] : 0.0; SZ[lane] = vj ? PX[2 * n + gj] : 0.0;
+    SQ[lane] = vj ? (float)P.charge[gj] : 0.0f;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const float qiC = (float)(chi * s.ele_coef);
+  const double cut2 = FORM == 0 ? s.dh_cut2 : s.cl_cut2;
+  const uint32_t *row = P.qmask + s.moff + (int64_t)(valid ? qi : 0) * s.mwords;
+  const uint8_t *mrow = P.qmirror + s.mioff + (int64_t)(valid ? qi : 0) * s.nq + jt;
+  uint64_t m64 = 0;
+  if (valid) {
+    const int wb = jt >> 5;
+    m64 = row[wb];
+    if (wb + 1 < s.mwords) m64 |= (uint64_t)row[wb + 1] << 32;
+  }
+  const bool diag = I == J;
+  double F0 = 0.0, F1 = 0.0, F2 = 0.0, G0 = 0.0, G1 = 0.0, G2 = 0.0;
+  unsigned nclamp = 0;
+  auto pair = [&](int k, bool last_half) {
+    const int jl = (lane + rd * k) & (kTile - 1);
+    bool bit = (m64 >> jl) & 1u;
+    if (last_half) bit = bit && lane < 32;
+    const int code = bit ? (int)mrow[jl] : 0;
+    const double v0 = SX[jl] - xi0 + mirror_shift(s, code, 0);
+    const double v1 = SY[jl] - xi1 + mirror_shift(s, code, 1);
+    const double v2 = SZ[jl] - xi2 + mirror_shift(s, code, 2);
+    const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+    const bool on = bit && d2 <= cut2;
+    float dv = ele_dv<FORM>(s, qiC * SQ[jl], on ? (float)d2 : 1.0f, nclamp, on);
+    dv = on ? dv : 0.0f;
+    const double dd = (double)dv;
+    F0 -= dd * v0; F1 -= dd * v1; F2 -= dd * v2;
+    G0 += dd * v0; G1 += dd * v1; G2 += dd * v2;
+    G0 = rot_next(G0, dir); G1 = rot_next(G1, dir); G2 = rot_next(G2, dir);
+  };
+  if (__any(m64 != 0)) {
+    if (diag) {
+#pragma unroll 1
+      for (int k = 1; k < 32; ++k) pair(k, false);
+      pair(32, true);
+    } else {
+#pragma unroll 1
+      for (int k = 0; k < kTile; ++k) pair(k, false);
+    }
+  }
+  double *pp = P.pp + s.pp_off + ele_task_index(I, J, s.nqt) * (6 * kTile);
+  if (diag) {
+    const int src = (lane - rd * 33) & (kTile - 1);
+    F0 += __shfl(G0, src); F1 += __shfl(G1, src); F2 += __shfl(G2, src);
+  } else {
+    pp[3 * kTile + lane] = G0; pp[4 * kTile + lane] = G1; pp[5 * kTile + lane] = G2;
+  }
+  pp[lane] = F0; pp[kTile + lane] = F1; pp[2 * kTile + lane] = F2;
+  if (nclamp) atomicAdd(&P.counters[1], (unsigned long long)nclamp);
+}
+
 // force_ele_DH.F90:68-118, force_ele_coulomb.F90:63-84 and the real-space
 // loop of force_ele_coulomb_ewld.F90:118-137, one block per charge tile.
 // Non-periodic systems take each pair once (ele_tile_once); periodic ones
@@ -759,19 +851,35 @@ __device__ inline void ele_tile_once(const DevPtrs &P, const DSys &s, const doub
 // pair sweep; k_main gathers all of their outputs.
 __global__ __launch_bounds__(kTile * kWaves) __attribute__((amdgpu_waves_per_eu(4)))
 void k_force(DevPtrs P, const double *X, const double *PX, const int32_t *systask, int ntask, const Tile *qtiles,
-             int nqt, BondedRange br, int32_t t_local) {
+             int nqt, const EwTask *ptasks, int npt, BondedRange br, int32_t t_local) {
   __shared__ EleLds lds;
   const int b = (int)blockIdx.x;
+  const int npb = (npt + kWaves - 1) / kWaves;   // blocks of four pair tasks
   if (b < ntask) {
     terms_block(P, X, PX, systask[b], t_local);
     return;
   }
+  if (b >= ntask + nqt + npb) {
+    bonded_lane(P, X, PX, br, (b - ntask - nqt - npb) * (kTile * kWaves) + (int)threadIdx.x);
+    return;
+  }
   if (b >= ntask + nqt) {
-    bonded_lane(P, X, PX, br, (b - ntask - nqt) * (kTile * kWaves) + (int)threadIdx.x);
+    const int w = (int)threadIdx.x / kTile;
+    const int task = (b - ntask - nqt) * kWaves + w;
+    const int dir = wave_dir();
+    if (task >= npt) return;
+    const EwTask tk = ptasks[task];
+    const DSys s = P.sys[tk.sys];
+    double *SX = lds.sx[w], *SY = lds.sy[w], *SZ = lds.sz[w];
+    float *SQ = lds.sq[w];
+    if (s.ele_form == 0) ele_pair_task<0>(P, s, PX, tk.a, tk.b, dir, SX, SY, SZ, SQ);
+    else if (s.ele_form == 1) ele_pair_task<1>(P, s, PX, tk.a, tk.b, dir, SX, SY, SZ, SQ);
+    else ele_pair_task<2>(P, s, PX, tk.a, tk.b, dir, SX, SY, SZ, SQ);
     return;
   }
   const Tile tl = qtiles[b - ntask];
   const DSys s = P.sys[tl.sys];
+  if (s.ele_pairs) return;   // taken by the pair tasks
   const int dir = s.ele_once ? wave_dir() : 0;
   if (s.ele_once && dir != 0) {
     if (s.ele_form == 0) ele_tile_once<0>(P, s, X, tl.start, dir, lds);
@@ -1006,7 +1114,18 @@ __device__ inline double main_tail(const DevPtrs &P, const DSys &s, int g, int l
   if (do_dh && s.ele_form >= 0) {
     const int q = P.qidx[g];
     if (q >= 0) {
-      F[0] += P.fdh[q]; F[1] += P.fdh[(size_t)P.nqtot + q]; F[2] += P.fdh[2 * (size_t)P.nqtot + q];
+      if (s.ele_pairs) {
+        // the slots of the chunk-pair tasks that hold this charge, in task order
+        const int lq = q - s.qoff, C = lq / kTile, lc = lq % kTile;
+        for (int I = 0; I < s.nqt; ++I) {
+          const int a = I <= C ? I : C, bb = I <= C ? C : I;
+          const int slot = I < C ? 1 : 0;
+          const double *pp = P.pp + s.pp_off + ele_task_index(a, bb, s.nqt) * (6 * kTile) + slot * 3 * kTile + lc;
+          F[0] += pp[0]; F[1] += pp[kTile]; F[2] += pp[2 * kTile];
+        }
+      } else {
+        F[0] += P.fdh[q]; F[1] += P.fdh[(size_t)P.nqtot + q]; F[2] += P.fdh[2 * (size_t)P.nqtot + q];
+      }
       if (s.ele_once) {
         // forces from the chunks I < J of the pair-once sweep, in I order
         const int lq = q - s.qoff, J = lq / kTile, lj = lq % kTile;
