@@ -926,7 +926,8 @@ static void launch_force_pair(tis_ctx *c, const Range &r, const double *X, const
   const int nterm = r.nb + r.na + r.ns;
   const int nqt = (r.qtn > 0 && c->any_ele) ? r.qtn : 0;
   const int npt = c->any_ele ? r.ptn : 0;
-  const int nblk = r.taskn + nqt + (npt + kWaves - 1) / kWaves + (nterm + kTile * kWaves - 1) / (kTile * kWaves);
+  const int nblk = (r.taskn + kWaves - 1) / kWaves + nqt + (npt + kWaves - 1) / kWaves +
+                   (nterm + kTile * kWaves - 1) / (kTile * kWaves);
   if (nblk > 0) {
     const Tile *qt = r.all ? c->d_qtiles_lpt : c->d_qtiles + r.qt0;
     const BondedRange br{r.b0, r.nb, r.a0, r.na, r.s0, r.ns};

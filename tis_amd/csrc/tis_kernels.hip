@@ -440,11 +440,19 @@ __device__ int hbond_valence_serial(const DevPtrs &P, const DSys &s, uint64_t st
 }
 
 // ------------------------------------------------------------------ terms pass
-// One block per system that has tertiary stacks or hbond candidates:
+// One wave per system that has tertiary stacks or hbond candidates:
 // tertiary stack forces, hbond geometry of listed candidates and the valence
-// pass.  Forces go to per-term slots gathered by k_main.
-__device__ void terms_block(const DevPtrs &P, const double *X, const double *PX, int isys, int32_t t_local) {
-  const int lane = threadIdx.x, nt = blockDim.x;
+// pass.  Forces go to per-term slots gathered by k_main.  The phases of a
+// system are ordered by wave-level fences: the four waves of a block serve
+// four systems.
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
+__device__ void terms_wave(const DevPtrs &P, const double *X, const double *PX, int isys, int32_t t_local) {
+  const int lane = threadIdx.x & (kTile - 1), nt = kTile;
   const DSys s = P.sys[isys];
   const uint64_t step = *P.stepbase + (uint64_t)t_local + 1u - s.step_origin;
   for (int k = lane; k < s.ntst; k += nt) {
@@ -467,7 +475,7 @@ __device__ void terms_block(const DevPtrs &P, const double *X, const double *PX,
     return;
   }
   for (int k = lane; k < s.nsite; k += nt) P.site_cnt[s.sitoff + k] = 0;
-  __syncthreads();
+  wave_sync();
   for (int k = lane; k < s.nhb; k += nt) {
     int h = s.hoff + k;
     int st = 0;
@@ -483,16 +491,15 @@ __device__ void terms_block(const DevPtrs &P, const double *X, const double *PX,
       }
     }
   }
-  __syncthreads();
+  wave_sync();
   int any = 0;
   for (int k = lane; k < s.nsite; k += nt)
     if (site_count(P, s.sitoff + k) - P.site_val[s.sitoff + k] > 0) any = 1;
-  any = __syncthreads_or(any);
-  if (any && lane == 0) {
+  if (__any(any) && lane == 0) {
     int nd = hbond_valence_serial(P, s, step);
     if (nd) atomicAdd(&P.counters[2], (unsigned long long)nd);
   }
-  __syncthreads();
+  wave_sync();
   // forces of the hbonds that survive into their beads' slots
   for (int k = lane; k < s.nhb; k += nt) {
     const int h = s.hoff + k;
@@ -844,28 +851,31 @@ __device__ inline void ele_pair_task(const DevPtrs &P, const DSys &s, const doub
 // loop of force_ele_coulomb_ewld.F90:118-137, one block per charge tile.
 // Non-periodic systems take each pair once (ele_tile_once); periodic ones
 // sweep the full row of each charge (ele_tile).
-// The force pass, one launch with three kinds of blocks: [0, ntask) one
-// system's tertiary stacks and hbonds each (terms_block), [ntask,
-// ntask + nqt) charge tiles, then the bonded terms, 256 lanes per block
-// (bonded_lane).  The memory-bound term blocks overlap the compute-bound
+// The force pass, one launch with four kinds of blocks: four systems'
+// tertiary stacks and hbonds per block (terms_wave, one wave each), charge
+// tiles, periodic chunk-pair tasks (four per block), then the bonded terms,
+// 256 lanes per block (bonded_lane).  The memory-bound term blocks overlap the compute-bound
 // pair sweep; k_main gathers all of their outputs.
 __global__ __launch_bounds__(kTile * kWaves) __attribute__((amdgpu_waves_per_eu(4)))
 void k_force(DevPtrs P, const double *X, const double *PX, const int32_t *systask, int ntask, const Tile *qtiles,
              int nqt, const EwTask *ptasks, int npt, BondedRange br, int32_t t_local) {
   __shared__ EleLds lds;
-  const int b = (int)blockIdx.x;
+  int b = (int)blockIdx.x;
   const int npb = (npt + kWaves - 1) / kWaves;   // blocks of four pair tasks
-  if (b < ntask) {
-    terms_block(P, X, PX, systask[b], t_local);
+  const int ntb = (ntask + kWaves - 1) / kWaves;   // blocks of four system terms passes
+  if (b < ntb) {
+    const int task = b * kWaves + (int)threadIdx.x / kTile;
+    if (task < ntask) terms_wave(P, X, PX, systask[task], t_local);
     return;
   }
-  if (b >= ntask + nqt + npb) {
-    bonded_lane(P, X, PX, br, (b - ntask - nqt - npb) * (kTile * kWaves) + (int)threadIdx.x);
+  b -= ntb;
+  if (b >= nqt + npb) {
+    bonded_lane(P, X, PX, br, (b - nqt - npb) * (kTile * kWaves) + (int)threadIdx.x);
     return;
   }
-  if (b >= ntask + nqt) {
+  if (b >= nqt) {
     const int w = (int)threadIdx.x / kTile;
-    const int task = (b - ntask - nqt) * kWaves + w;
+    const int task = (b - nqt) * kWaves + w;
     const int dir = wave_dir();
     if (task >= npt) return;
     const EwTask tk = ptasks[task];
@@ -877,7 +887,7 @@ void k_force(DevPtrs P, const double *X, const double *PX, const int32_t *systas
     else ele_pair_task<2>(P, s, PX, tk.a, tk.b, dir, SX, SY, SZ, SQ);
     return;
   }
-  const Tile tl = qtiles[b - ntask];
+  const Tile tl = qtiles[b];
   const DSys s = P.sys[tl.sys];
   if (s.ele_pairs) return;   // taken by the pair tasks
   const int dir = s.ele_once ? wave_dir() : 0;
