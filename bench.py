@@ -272,6 +272,8 @@ def main():
     ap.add_argument("--no-ewald", action="store_true")
     ap.add_argument("--no-bdhi", action="store_true")
     ap.add_argument("--swap-every", type=int, default=1000)
+    ap.add_argument("--trigger-steps", type=int, default=1000,
+                    help="steps of the displacement-trigger leg (0 = skip)")
     ap.add_argument("--rebuild-every", type=int, default=10,
                     help="neighbour-list rebuild interval in steps; 0 = displacement trigger (margin 8 A)")
     ap.add_argument("--only", choices=["ewald", "bdhi", "copy"], default=None,
@@ -356,6 +358,26 @@ def main():
     print(f"[bench] plain: {(tp1 - tp0) * 1e3 / args.plain_steps:.3f} ms/step, {plain:.4e} bead-steps/s",
           file=sys.stderr, flush=True)
 
+    # the reference's own rebuild rule on the same batch: lists rebuilt per
+    # system by the displacement trigger (margin 8 A) instead of every N steps
+    trig = None
+    if args.rebuild_every > 0 and args.trigger_steps > 0:
+        eng.langevin_init(tstep=0.05, rebuild_every=0,
+                          step0=args.warmup + args.steps + args.eager_steps + args.plain_steps, init_accel=False)
+        rb_t = eng.counters()["rebuilds"]
+        eng.langevin_steps(100, MODE_GRAPH)
+        eng.synchronize()
+        tt0 = time.perf_counter()
+        eng.langevin_steps(args.trigger_steps, MODE_GRAPH)
+        eng.synchronize()
+        tt1 = time.perf_counter()
+        nreb = eng.counters()["rebuilds"] - rb_t
+        trig = {"ms_per_step": (tt1 - tt0) * 1e3 / args.trigger_steps,
+                "bead_steps_per_s": nbeads * args.trigger_steps / (tt1 - tt0),
+                "rebuilds_per_system_step": nreb / len(systems) / (100 + args.trigger_steps)}
+        print(f"[bench] displacement trigger: {trig['ms_per_step']:.4f} ms/step, "
+              f"{trig['rebuilds_per_system_step']:.4f} rebuilds per system-step", file=sys.stderr, flush=True)
+
     # lists and forces at the final state; error against the FP64 oracle
     eng.neighbor(-1)
     eng.force_all()
@@ -425,6 +447,7 @@ def main():
         "serving": serve,
         "remd": {"replicas": len(rep_slots), "swap_every": args.swap_every, "attempts": ladder.attempts,
                  "acceptance": ladder.acceptance},
+        "displacement_trigger": trig,
         "ewald": ewald,
         "bdhi": bdhi,
         "roofline": roof,
