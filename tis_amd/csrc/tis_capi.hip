@@ -28,6 +28,8 @@ __global__ void k_prep(DevPtrs, const double *, double *, double *, const Tile *
                        int, int);
 __global__ void k_lists(DevPtrs, const double *, const double *, const Tile *, int, int, const Tile *, int, int, int,
                         int, int);
+__global__ void k_lpair(DevPtrs, const double *, const double *, const EwTask *, int, const EwTask *, int, int, int,
+                        int, int);
 __global__ void k_advance(uint64_t *, uint64_t, unsigned long long *);
 __global__ void k_count_rebuild(unsigned long long *);
 __global__ void k_energy(DevPtrs, const double *, const double *, const int32_t *, double *, int);
@@ -47,6 +49,8 @@ struct HostSys {
   int32_t esk_begin = 0, esk_count = 0;  // Ewald structure-factor tasks
   int32_t ef_begin = 0, ef_count = 0;    // Ewald force tasks
   int32_t pt_begin = 0, pt_count = 0;    // periodic electrostatic chunk-pair tasks
+  int32_t bp_begin = 0, bp_count = 0;    // bead tile pairs of the tile-pair list build
+  int32_t qp_begin = 0, qp_count = 0;    // charge tile pairs
   std::vector<double> mass, fric;
   std::vector<int32_t> stack_ids;       // global stack indices
 };
@@ -87,8 +91,8 @@ struct tis_ctx {
   unsigned long long *d_counters = nullptr;
   uint64_t *d_stepbase = nullptr;
   bool any_ele = false, any_periodic = false;
-  EwTask *d_ewsk = nullptr, *d_ewf = nullptr, *d_ptask = nullptr;
-  int32_t n_ewsk = 0, n_ewf = 0, n_ptask = 0;
+  EwTask *d_ewsk = nullptr, *d_ewf = nullptr, *d_ptask = nullptr, *d_bpair = nullptr, *d_qpair = nullptr;
+  int32_t n_ewsk = 0, n_ewf = 0, n_ptask = 0, n_bpair = 0, n_qpair = 0;
   // state
   int cur = 0;
   bool lists_valid = false;
@@ -292,12 +296,28 @@ Range range_of(tis_ctx *c, int32_t isys) {
   return r;
 }
 
+static void launch_lpair(tis_ctx *c, const Range &r, const double *X, const double *PX, int mode, int h0, int nh) {
+  const int bp0 = r.all ? 0 : c->hs[r.isys].bp_begin, nbp = r.all ? c->n_bpair : c->hs[r.isys].bp_count;
+  const int qp0 = r.all ? 0 : c->hs[r.isys].qp_begin, nqp = r.all ? c->n_qpair : c->hs[r.isys].qp_count;
+  const int nbb = (nbp + kWaves - 1) / kWaves, nqb = (nqp + kWaves - 1) / kWaves;
+  const int nhb = (nh + kTile * kWaves - 1) / (kTile * kWaves);
+  if (nbb + nqb + nhb > 0)
+    hipLaunchKernelGGL(k_lpair, dim3(nbb + nqb + nhb), dim3(kTile * kWaves), 0, c->stream, c->P, X, PX,
+                       c->d_bpair + bp0, nbp, c->d_qpair + qp0, nqp, h0, nh, mode, 0);
+  if (nbb > 0)
+    hipLaunchKernelGGL(k_lpair, dim3(nbb), dim3(kTile * kWaves), 0, c->stream, c->P, X, PX, c->d_bpair + bp0, nbp,
+                       c->d_qpair + qp0, nqp, h0, 0, mode, 1);
+}
+
 // The lists of the range from X (wrapped copies into PXcur, predicted
-// wrapped positions into PXpred when with_pred): k_prep then k_lists; with
-// gate only the systems whose displacement trigger fired.
+// wrapped positions into PXpred when with_pred): k_prep, then k_lists by
+// bead / charge tile for non-periodic systems and the hbond candidates, and
+// k_lpair by tile pair for periodic systems; with gate, only the systems
+// whose displacement trigger fired.
 int launch_lists(tis_ctx *c, const Range &r, const double *X, double *PXcur, double *PXpred, int with_pred,
                  int gate) {
   const int h0 = r.all ? 0 : c->hs[r.isys].d.hoff, nh = r.all ? c->nhbtot : c->hs[r.isys].d.nhb;
+  const bool periodic = r.all ? c->any_periodic : c->hs[r.isys].d.periodic != 0;
   if (r.btn + r.qtn > 0)
     hipLaunchKernelGGL(k_prep, dim3(r.btn + r.qtn), dim3(kTile), 0, c->stream, c->P, X, PXcur, PXpred, c->d_btiles,
                        r.bt0, r.btn, c->d_qtiles, r.qt0, r.qtn, with_pred, gate);
@@ -305,6 +325,7 @@ int launch_lists(tis_ctx *c, const Range &r, const double *X, double *PXcur, dou
   if (nblk > 0)
     hipLaunchKernelGGL(k_lists, dim3(nblk), dim3(kTile), 0, c->stream, c->P, X, PXcur, c->d_btiles, r.bt0, r.btn,
                        c->d_qtiles, r.qt0, r.qtn, h0, nh, gate);
+  if (periodic) launch_lpair(c, r, X, PXcur, gate ? 3 : 2, h0, 0);
   return 0;
 }
 
@@ -370,7 +391,7 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
   std::vector<Tile> btiles, qtiles;
   std::vector<int32_t> systask;
   std::vector<double> ewhx, ewhy, ewhz, ewc;
-  std::vector<EwTask> ewsk, ewf, ptask;
+  std::vector<EwTask> ewsk, ewf, ptask, bpair, qpair;
   int64_t psk_off = 0, pf_off = 0;
   // excluded-volume species: distinct (mptype, class, radius, epsilon)
   struct Spec { int8_t t, c; double r, e; };
@@ -383,7 +404,7 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
     return (int)specs.size() - 1;
   };
   c->systask_of.assign(nsys, -1);
-  int64_t moff = 0, mioff = 0, pj_off = 0, pp_off = 0;
+  int64_t moff = 0, mioff = 0, pj_off = 0, pp_off = 0, lp_off = 0;
   int32_t off = 0, qoff = 0, nbd = 0, nba = 0, nst = 0, ntst = 0, nhb = 0, nsite = 0, uoff = 0;
   c->any_ele = false;
   c->any_periodic = false;
@@ -529,6 +550,17 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
         for (int J = I; J < d.nqt; ++J) ptask.push_back(EwTask{is, I, J});
     }
     h.pt_count = (int32_t)ptask.size() - h.pt_begin;
+    // tile pairs for the one-step (triggered) and periodic list builds
+    d.lp_off = lp_off;
+    lp_off += (int64_t)d.nbt * d.nbt;
+    h.bp_begin = (int32_t)bpair.size();
+    for (int a = 0; a < d.nbt; ++a)
+      for (int b = 0; b < d.nbt; ++b) bpair.push_back(EwTask{is, a, b});
+    h.bp_count = (int32_t)bpair.size() - h.bp_begin;
+    h.qp_begin = (int32_t)qpair.size();
+    for (int a = 0; a < d.nqt; ++a)
+      for (int b = 0; b < d.nqt; ++b) qpair.push_back(EwTask{is, a, b});
+    h.qp_count = (int32_t)qpair.size() - h.qp_begin;
     h.has_task = (s.nhb > 0 || s.ntst > 0);
     if (h.has_task) { c->systask_of[is] = (int32_t)systask.size(); systask.push_back(is); }
     // Ewald reciprocal vectors and work items
@@ -645,6 +677,7 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
   rc = dalloc(c, &P.bbox, (size_t)c->nbtile); if (rc) return rc;
   rc = dalloc(c, &P.qbbox, (size_t)c->nqtile); if (rc) return rc;
   rc = dalloc(c, &P.sbox, 4 * (size_t)c->nbtile); if (rc) return rc;
+  rc = dalloc(c, &P.lcnt, (size_t)std::max<int64_t>(lp_off, 1) * kTile); if (rc) return rc;
   P.rcm = d_rcm; P.mass = d_mass; P.exv_rad = d_rad; P.charge = d_charge; P.rl2 = d_rl2;
   P.mptype = d_mpt; P.iclass = d_icl; P.unit = d_unit; P.qidx = d_qidx; P.qbead = d_qbead;
   P.inc_off = d_incoff; P.inc = d_inc; P.exv_eps = d_eps;
@@ -723,6 +756,10 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
     UP(c->d_ewf, ewf);
     UP(c->d_ptask, ptask);
     c->n_ptask = (int32_t)ptask.size();
+    UP(c->d_bpair, bpair);
+    c->n_bpair = (int32_t)bpair.size();
+    UP(c->d_qpair, qpair);
+    c->n_qpair = (int32_t)qpair.size();
     c->n_ewsk = (int32_t)ewsk.size();
     c->n_ewf = (int32_t)ewf.size();
   }

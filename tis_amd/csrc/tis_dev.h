@@ -78,6 +78,8 @@ struct DSys {
   // [nqt(nqt+1)/2][2][3][kTile] at pp_off
   int32_t ele_pairs;
   int64_t pp_off;
+  // tile-pair list build: counts [nbt * nbt][kTile] at lp_off * kTile
+  int64_t lp_off;
 };
 
 // Ewald work item: (system, tile a, chunk b)
@@ -112,6 +114,7 @@ struct DevPtrs {
   const SpPair *sptab; int32_t nspecies;
   BBox *bbox, *qbbox;          // per bead tile / per charge tile
   BBox *sbox;                  // per 16-bead group of each bead tile [4 * nbtile]
+  int *lcnt;                   // exv entry counts per (bead tile pair, lane) of the tile-pair build
   const double *exv_rad;
   const float *exv_eps;
   const double *charge;

@@ -1455,6 +1455,192 @@ __device__ inline void list_hb_entry(const DevPtrs &P, const DSys &s, const doub
 }
 
 
+// ------------------------------------------------------------------ lists by tile pairs
+// Periodic systems (no box culling, so one wave per tile would sweep every
+// tile of the box in turn): every (bead tile, bead tile) and (charge tile,
+// charge tile) pair is its own wave.  The excluded-volume lists take two passes: pass 0 counts each
+// bead's entries per tile pair, pass 1 writes them after the counts of the
+// earlier tile pairs of its row, so each list keeps the tile order of
+// list_exv_tile.  mode bit 1 takes only the systems whose trigger fired,
+// bit 2 only the periodic systems.
+
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// exv entries of bead tile ia (local index) against bead tile jb: counts
+// them (write = false) or writes them from slot `off` on; returns the count
+__device__ inline int exv_pair_scan(const DevPtrs &P, const DSys &s, const double *X, const double *PX, int ia,
+                                    int jb, bool write, int off, ListLds &L) {
+  const int lane = threadIdx.x & (kTile - 1);
+  const int li = ia * kTile + lane;
+  const bool valid = li < s.nmp;
+  const int n = P.ntot;
+  const double *Y = s.periodic ? PX : X;
+  const int g = s.off + (valid ? li : 0);
+  const double cut2 = s.rl2max * (1.0 + 1e-9);
+  if (!s.periodic && box_dmin2(P.bbox[s.bt0 + ia], P.bbox[s.bt0 + jb]) > cut2) return 0;
+  const bool rl2_lds = s.nunit <= kRl2Lds;
+  if (rl2_lds)
+    for (int k = lane; k < s.nunit * s.nunit; k += kTile) L.rl2[k] = P.rl2[s.uoff + k];
+  double xi[3] = {0, 0, 0};
+  int ui = 0, ti = 0, ci = 0;
+  if (valid) { ld3(Y, n, g, xi); ui = P.unit[g]; ti = P.mptype[g]; ci = P.iclass[g]; }
+  const int jt = jb * kTile;
+  const int lj = jt + lane;
+  if (lj < s.nmp) {
+    const int gj = s.off + lj;
+    L.sx[lane] = Y[gj]; L.sy[lane] = Y[n + gj]; L.sz[lane] = Y[2 * n + gj];
+    L.su[lane] = P.unit[gj];
+    L.stc[lane] = (int)P.mptype[gj] | ((int)P.iclass[gj] << 8);
+  }
+  wave_lds_sync();
+  int cnt = 0;
+  const int kend = min(kTile, s.nmp - jt);
+  for (int q0 = 0; q0 < kend; q0 += 16) {
+    if (!s.periodic) {
+      const bool need = valid && point_box_d2(xi, P.sbox[4 * (s.bt0 + jb) + q0 / 16]) <= cut2;
+      if (!__any(need)) continue;
+    }
+    const int q1 = min(q0 + 16, kend);
+    if (valid)
+      for (int k = q0; k < q1; ++k) {
+        const int lj2 = jt + k;
+        if (lj2 == li) continue;
+        const bool own_lo = li < lj2;
+        double v[3];
+        if (own_lo) { v[0] = __dadd_rn(L.sx[k], -xi[0]); v[1] = __dadd_rn(L.sy[k], -xi[1]); v[2] = __dadd_rn(L.sz[k], -xi[2]); }
+        else { v[0] = __dadd_rn(xi[0], -L.sx[k]); v[1] = __dadd_rn(xi[1], -L.sy[k]); v[2] = __dadd_rn(xi[2], -L.sz[k]); }
+        const int code = s.periodic ? pbneighbor(s, v) : 0;
+        const double d2 = sq3_exact(v[0], v[1], v[2]);
+        if (!(d2 < s.rl2max)) continue;
+        const int uj = L.su[k];
+        const int ulo = own_lo ? ui : uj, uhi = own_lo ? uj : ui;
+        const double r2 = rl2_lds ? L.rl2[ulo * s.nunit + uhi] : P.rl2[s.uoff + ulo * s.nunit + uhi];
+        if (!(d2 < r2)) continue;
+        const int tj = L.stc[k] & 0xFF, cj = L.stc[k] >> 8;
+        const bool keep = own_lo ? exv_keep(s, ui, uj, ci, ti, li, lj2) : exv_keep(s, uj, ui, cj, tj, lj2, li);
+        if (!keep) continue;
+        if (write) {
+          const int slot = off + cnt;
+          const int oc = own_lo ? code : mirror_flip(code);
+          if (slot < P.nb_cap) P.nb[(size_t)slot * n + g] = ((uint32_t)oc << kMirrorShift) | (uint32_t)(s.off + lj2);
+        }
+        ++cnt;
+      }
+  }
+  wave_lds_sync();
+  return cnt;
+}
+
+// ele bit words of charge tile I's rows for charge tile J (and, periodic,
+// the mirror codes), as list_ele_tile does for one J
+__device__ inline void ele_pair_rows(const DevPtrs &P, const DSys &s, const double *X, const double *PX, int I, int J,
+                                     ListLds &L) {
+  const int lane = threadIdx.x & (kTile - 1);
+  const int qi = I * kTile + lane;
+  const bool valid = qi < s.nq;
+  const int n = P.ntot;
+  const double *Y = s.periodic ? PX : X;
+  double xi[3] = {0, 0, 0};
+  if (valid) ld3(Y, n, P.qbead[s.qoff + qi], xi);
+  uint32_t *row = P.qmask + s.moff + (int64_t)(valid ? qi : 0) * s.mwords;
+  const int jt = J * kTile;
+  const int kend = min(kTile, s.nq - jt);
+  int mode = 2;  // 0 none, 1 all, 2 test
+  if (!s.periodic) {
+    const BBox bi = P.qbbox[s.qt0 + I], bj = P.qbbox[s.qt0 + J];
+    if (box_dmin2(bi, bj) > s.ele_rl2 * (1.0 + 1e-9)) mode = 0;
+    else if (box_dmax2(bi, bj) < s.ele_rl2 * (1.0 - 1e-9)) mode = 1;
+  }
+  uint32_t w[2] = {0u, 0u};
+  if (mode == 1) {
+    w[0] = kend >= 32 ? 0xFFFFFFFFu : ((1u << kend) - 1u);
+    w[1] = kend >= 64 ? 0xFFFFFFFFu : (kend > 32 ? ((1u << (kend - 32)) - 1u) : 0u);
+    if (qi >= jt && qi < jt + kend) w[(qi - jt) >> 5] &= ~(1u << ((qi - jt) & 31));
+  } else if (mode == 2) {
+    const int qj = jt + lane;
+    if (qj < s.nq) {
+      const int gj = P.qbead[s.qoff + qj];
+      L.sx[lane] = Y[gj]; L.sy[lane] = Y[n + gj]; L.sz[lane] = Y[2 * n + gj];
+    }
+    wave_lds_sync();
+    if (valid)
+      for (int k = 0; k < kend; ++k) {
+        const int qj2 = jt + k;
+        if (qj2 == qi) continue;
+        const bool own_lo = qi < qj2;
+        double v[3];
+        if (own_lo) { v[0] = __dadd_rn(L.sx[k], -xi[0]); v[1] = __dadd_rn(L.sy[k], -xi[1]); v[2] = __dadd_rn(L.sz[k], -xi[2]); }
+        else { v[0] = __dadd_rn(xi[0], -L.sx[k]); v[1] = __dadd_rn(xi[1], -L.sy[k]); v[2] = __dadd_rn(xi[2], -L.sz[k]); }
+        const int code = s.periodic ? pbneighbor(s, v) : 0;
+        const double d2 = sq3_exact(v[0], v[1], v[2]);
+        if (!(d2 < s.ele_rl2)) continue;
+        w[k >> 5] |= 1u << (k & 31);
+        if (s.periodic) P.qmirror[s.mioff + (int64_t)qi * s.nq + qj2] = (uint8_t)(own_lo ? code : mirror_flip(code));
+      }
+    wave_lds_sync();
+  }
+  if (valid) {
+    row[jt >> 5] = w[0];
+    if ((jt >> 5) + 1 < s.mwords) row[(jt >> 5) + 1] = w[1];
+  }
+}
+
+__device__ inline bool lpair_take(const DevPtrs &P, int sys, int mode) {
+  return (!(mode & 1) || P.sys_flag[sys] != 0) && (!(mode & 2) || P.sys[sys].periodic != 0);
+}
+
+// pass 0: exv counts of bead tile pairs, ele rows of charge tile pairs and
+// hbond candidates (nh > 0); pass 1: exv entries.  One pair per wave.
+__global__ __launch_bounds__(kTile * kWaves) void k_lpair(DevPtrs P, const double *X, const double *PX,
+                                                          const EwTask *bpairs, int nbp, const EwTask *qpairs,
+                                                          int nqp, int h0, int nh, int mode, int pass) {
+  __shared__ ListLds lds[kWaves];
+  const int w = (int)threadIdx.x / kTile, lane = (int)threadIdx.x & (kTile - 1);
+  const int nbb = (nbp + kWaves - 1) / kWaves, nqb = (nqp + kWaves - 1) / kWaves;
+  const int b = (int)blockIdx.x;
+  if (b < nbb) {
+    const int p = b * kWaves + w;
+    if (p >= nbp) return;
+    const EwTask bp = bpairs[p];
+    if (!lpair_take(P, bp.sys, mode)) return;
+    const DSys s = P.sys[bp.sys];
+    int *cnt = P.lcnt + (size_t)(s.lp_off + bp.a * s.nbt) * kTile;   // counts of this row of tile pairs
+    if (pass == 0) {
+      cnt[bp.b * kTile + lane] = exv_pair_scan(P, s, X, PX, bp.a, bp.b, false, 0, lds[w]);
+      return;
+    }
+    int off = 0;
+    for (int j = 0; j < bp.b; ++j) off += cnt[j * kTile + lane];
+    const int c = exv_pair_scan(P, s, X, PX, bp.a, bp.b, true, off, lds[w]);
+    const int li = bp.a * kTile + lane;
+    if (bp.b == s.nbt - 1 && li < s.nmp) {
+      const int tot = off + c, g = s.off + li;
+      P.nb_cnt[g] = tot;
+      if (tot > P.nb_cap) atomicMax(&P.counters[5], (unsigned long long)tot);
+    }
+    return;
+  }
+  if (pass == 1) return;
+  if (b < nbb + nqb) {
+    const int p = (b - nbb) * kWaves + w;
+    if (p >= nqp) return;
+    const EwTask qp = qpairs[p];
+    if (!lpair_take(P, qp.sys, mode)) return;
+    const DSys s = P.sys[qp.sys];
+    ele_pair_rows(P, s, X, PX, qp.a, qp.b, lds[w]);
+    return;
+  }
+  const int k = (b - nbb - nqb) * (kTile * kWaves) + (int)threadIdx.x;
+  if (k >= nh) return;
+  const int h = h0 + k;
+  if (!lpair_take(P, P.hb_sys[h], mode)) return;
+  list_hb_entry(P, P.sys[P.hb_sys[h]], X, PX, h);
+}
+
 // The list passes of one rebuild in two launches.  gate = 0 builds for every
 // system of the launch range; gate = 1 only for the systems whose
 // displacement trigger fires (decided in k_prep from the tile pairs the
@@ -1528,6 +1714,7 @@ __global__ __launch_bounds__(kTile) void k_lists(DevPtrs P, const double *X, con
     const Tile tl = charge ? qtiles[it] : btiles[it];
     if (gate && !P.sys_flag[tl.sys]) return;
     const DSys s = P.sys[tl.sys];
+    if (s.periodic) return;   // built by tile pairs (k_lpair)
     if (charge) list_ele_tile(P, s, X, PX, tl, it, lds);
     else list_exv_tile(P, s, X, PX, tl, it, lds);
     return;
