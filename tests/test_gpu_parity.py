@@ -135,10 +135,13 @@ def _small_margin_systems(margin=1.5):
     return out
 
 
-def test_displacement_trigger_matches_oracle():
+@pytest.mark.parametrize("which", [0, 2])
+def test_displacement_trigger_matches_oracle(which):
     """Displacement-triggered rebuilds (time_integral.F90:57-93): the GPU
-    trajectory and rebuild count against the oracle's trigger loop."""
-    s, x = _small_margin_systems()[0]
+    trajectory and rebuild count against the oracle's trigger loop.  The
+    150-nt chain (8 bead tiles) rebuilds its lists by tile pairs."""
+    s, x = _small_margin_systems()[which]
+    print(f"[trigger] system {which} nmp={s.nmp}", flush=True)
     v = np.zeros_like(x)
     eng = Engine([s], keys=[4], seed=17)
     eng.set_coords(0, x)

@@ -1042,25 +1042,27 @@ __device__ inline void integrate(const DevPtrs &P, const DSys &s, int g, int li,
   }
 }
 
-// Four waves share a tile of 64 beads: waves 0-1 take alternate bonded
-// entries of each bead's incidence list, waves 2-3 alternate groups of four
-// excluded-volume neighbours; the partial sums are added in wave order (a
-// fixed order: results are bitwise reproducible), then wave 0 adds the DH
-// sum and integrates.
+// Two waves share a tile of 64 beads: wave 0 sums each bead's bonded slots,
+// and the two waves take alternate groups of four excluded-volume
+// neighbours; the partial sums are added in wave order (a fixed order:
+// results are bitwise reproducible), then wave 0 adds the electrostatic
+// sums and integrates.
 __device__ inline void exv_group(const DevPtrs &P, const DSys &s, const double *Y, int g, int m0, int cnt,
                                  double xi0, double xi1, double xi2, const SpPair *tab, double F[3],
                                  unsigned &nclamp) {
   const int n = P.ntot;
+  // every load of the group is issued before the arithmetic: entries past
+  // cnt re-read entry cnt - 1 (m0 < cnt) and are skipped below
   uint32_t e[4];
   double y0[4], y1[4], y2[4];
-  int sp[4];
+  SpPair tp[4];
 #pragma unroll
-  for (int u = 0; u < 4; ++u) e[u] = (m0 + u < cnt) ? P.nb[(size_t)(m0 + u) * n + g] : (uint32_t)g;
+  for (int u = 0; u < 4; ++u) e[u] = P.nb[(size_t)min(m0 + u, cnt - 1) * n + g];
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     const int j = (int)(e[u] & kIdxMask);
     y0[u] = Y[j]; y1[u] = Y[n + j]; y2[u] = Y[2 * n + j];
-    sp[u] = P.species[j];
+    tp[u] = tab[P.species[j]];
   }
   const double a = s.exv_a;
 #pragma unroll
@@ -1071,7 +1073,7 @@ __device__ inline void exv_group(const DevPtrs &P, const DSys &s, const double *
       const int code = (int)(e[u] >> kMirrorShift);
       v0 += mirror_shift(s, code, 0); v1 += mirror_shift(s, code, 1); v2 += mirror_shift(s, code, 2);
     }
-    const SpPair t = tab[sp[u]];
+    const SpPair t = tp[u];
     const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
     if (d2 > t.dij2) continue;
     // contact pair: the mWCA factor (a/dr)^14 amplifies rounding 14-fold, so
@@ -1125,13 +1127,22 @@ __device__ inline double main_tail(const DevPtrs &P, const DSys &s, int g, int l
     const int q = P.qidx[g];
     if (q >= 0) {
       if (s.ele_pairs) {
-        // the slots of the chunk-pair tasks that hold this charge, in task order
+        // the slots of the chunk-pair tasks that hold this charge, in task
+        // order, four tasks' loads in flight at a time
         const int lq = q - s.qoff, C = lq / kTile, lc = lq % kTile;
-        for (int I = 0; I < s.nqt; ++I) {
-          const int a = I <= C ? I : C, bb = I <= C ? C : I;
-          const int slot = I < C ? 1 : 0;
-          const double *pp = P.pp + s.pp_off + ele_task_index(a, bb, s.nqt) * (6 * kTile) + slot * 3 * kTile + lc;
-          F[0] += pp[0]; F[1] += pp[kTile]; F[2] += pp[2 * kTile];
+        for (int I0 = 0; I0 < s.nqt; I0 += 4) {
+          double f[4][3];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int I = min(I0 + u, s.nqt - 1);
+            const int a = I <= C ? I : C, bb = I <= C ? C : I;
+            const int slot = I < C ? 1 : 0;
+            const double *pp = P.pp + s.pp_off + ele_task_index(a, bb, s.nqt) * (6 * kTile) + slot * 3 * kTile + lc;
+            f[u][0] = pp[0]; f[u][1] = pp[kTile]; f[u][2] = pp[2 * kTile];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (I0 + u < s.nqt) { F[0] += f[u][0]; F[1] += f[u][1]; F[2] += f[u][2]; }
         }
       } else {
         F[0] += P.fdh[q]; F[1] += P.fdh[(size_t)P.nqtot + q]; F[2] += P.fdh[2 * (size_t)P.nqtot + q];
@@ -1139,9 +1150,17 @@ __device__ inline double main_tail(const DevPtrs &P, const DSys &s, int g, int l
       if (s.ele_once) {
         // forces from the chunks I < J of the pair-once sweep, in I order
         const int lq = q - s.qoff, J = lq / kTile, lj = lq % kTile;
-        for (int I = 0; I < J; ++I) {
-          const double *pj = P.pj + s.pj_off + ele_pair_index(I, J, s.nqt) * (3 * kTile) + lj;
-          F[0] += pj[0]; F[1] += pj[kTile]; F[2] += pj[2 * kTile];
+        for (int I0 = 0; I0 < J; I0 += 4) {
+          double f[4][3];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int I = min(I0 + u, J - 1);
+            const double *pj = P.pj + s.pj_off + ele_pair_index(I, J, s.nqt) * (3 * kTile) + lj;
+            f[u][0] = pj[0]; f[u][1] = pj[kTile]; f[u][2] = pj[2 * kTile];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (I0 + u < J) { F[0] += f[u][0]; F[1] += f[u][1]; F[2] += f[u][2]; }
         }
       }
       if (s.ele_form == 2) {
@@ -1176,12 +1195,19 @@ __global__ __launch_bounds__(kTile * 2) void k_main(DevPtrs P, const double *X, 
   double F[3] = {0.0, 0.0, 0.0};
   if (valid) {
     if (w == 0) {
-      // bonded slots in incidence order
-      // bonded slots in incidence order
+      // bonded slots in incidence order, eight loads in flight at a time
+      // (slots past ie re-read slot ie - 1 and are not added)
       const int ib = P.inc_off[g], ie = P.inc_off[g + 1];
-      for (int k = ib; k < ie; ++k) {
-        const float *sl = P.fslot + 3 * (size_t)k;
-        F[0] += (double)sl[0]; F[1] += (double)sl[1]; F[2] += (double)sl[2];
+      for (int k0 = ib; k0 < ie; k0 += 8) {
+        float sl[8][3];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const float *q = P.fslot + 3 * (size_t)min(k0 + u, ie - 1);
+          sl[u][0] = q[0]; sl[u][1] = q[1]; sl[u][2] = q[2];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (k0 + u < ie) { F[0] += (double)sl[u][0]; F[1] += (double)sl[u][1]; F[2] += (double)sl[u][2]; }
       }
     }
     // excluded volume (force_exv_dt15.F90:53-115), owner view of each pair;
