@@ -155,12 +155,15 @@ __device__ inline bool stack_on(const DevPtrs &P, const double *X, const double 
   return true;
 }
 
-// Bonded terms of the launch range: one lane per bond, angle or consecutive
-// stack.
+// Bonded terms of the launch range: one lane per consecutive stack, angle or
+// bond, in that order (the heaviest terms start first; the force pass runs
+// these blocks last).
 
 __device__ inline void bonded_lane(const DevPtrs &P, const double *X, const double *PX, const BondedRange &R,
                                    int t) {
   const int nb = R.nb, na = R.na, ns = R.ns, b0 = R.b0, a0 = R.a0, s0 = R.s0;
+  if (t >= nb + na + ns) return;
+  t = t < ns ? t + nb + na : (t < ns + na ? t - ns + nb : t - ns - na);   // stacks, angles, bonds
   if (t < nb) {
     double fo[6];
     bond_eval(P, X, b0 + t, fo);
@@ -1018,13 +1021,11 @@ __global__ __launch_bounds__(kTile * kWaves) void k_ewald_f(DevPtrs P, const dou
 // Langevin (time_integral.F90:169-183, then the next step's position update
 // :152-160): a' = F/m + c1 xi; v = c2 v + c3 (a + a'); a = a';
 // x_next = x + c4 v + (dt^2/2) a.
-__device__ inline void integrate(const DevPtrs &P, const DSys &s, int g, int li, const double F[3],
-                                 const double *X, const double *PX, double *XN, double *PXN,
-                                 uint64_t step, int mode, double dxo[3]) {
+__device__ inline void integrate(const DevPtrs &P, const DSys &s, int g, const double gs[3], const double F[3],
+                                 const double *X, const double *PX, double *XN, double *PXN, int mode,
+                                 double dxo[3]) {
 #pragma clang fp contract(off)
   const int n = P.ntot;
-  double gs[3];
-  gauss3(P.seed, s.key, step, (uint32_t)li, gs);
   const double r0 = P.rlan[g], r1 = P.rlan[n + g], r2 = P.rlan[2 * n + g], r3 = P.rlan[3 * n + g];
   const double rcm = P.rcm[g];
   for (int d = 0; d < 3; ++d) {
@@ -1043,10 +1044,10 @@ __device__ inline void integrate(const DevPtrs &P, const DSys &s, int g, int li,
 }
 
 // Two waves share a tile of 64 beads: wave 0 sums each bead's bonded slots,
-// and the two waves take alternate groups of four excluded-volume
-// neighbours; the partial sums are added in wave order (a fixed order:
-// results are bitwise reproducible), then wave 0 adds the electrostatic
-// sums and integrates.
+// wave 1 draws its Langevin noise and adds its electrostatic sums, and the
+// two waves take alternate groups of four excluded-volume neighbours; the
+// partial sums are added in wave order (a fixed order: results are bitwise
+// reproducible), then wave 0 integrates.
 __device__ inline void exv_group(const DevPtrs &P, const DSys &s, const double *Y, int g, int m0, int cnt,
                                  double xi0, double xi1, double xi2, const SpPair *tab, double F[3],
                                  unsigned &nclamp) {
@@ -1113,69 +1114,77 @@ __device__ inline double accumulate_displacement(const DevPtrs &P, int g, const 
   return d2;
 }
 
-// Forces of one bead after the exv waves, then the integrator; returns the
-// squared displacement since the last rebuild before this step's update
-// (and accumulates the update) when rebuilds are displacement-triggered.
-__device__ inline double main_tail(const DevPtrs &P, const DSys &s, int g, int li, double F[3],
-                                   double (*red)[2][kTile], int lane, const double *X, const double *PX, double *XN,
-                                   double *PXN, int32_t t_local, int mode, int do_dh) {
-  const int n = P.ntot;
-  for (int d = 0; d < 3; ++d) F[d] = red[d][0][lane] + red[d][1][lane];
-  // electrostatic pair sums of the ele pass, then the Ewald reciprocal
-  // partial forces in chunk order
-  if (do_dh && s.ele_form >= 0) {
-    const int q = P.qidx[g];
-    if (q >= 0) {
-      if (s.ele_pairs) {
-        // the slots of the chunk-pair tasks that hold this charge, in task
-        // order, four tasks' loads in flight at a time
-        const int lq = q - s.qoff, C = lq / kTile, lc = lq % kTile;
-        for (int I0 = 0; I0 < s.nqt; I0 += 4) {
-          double f[4][3];
+// Electrostatic pair sums of the ele pass (chunk-pair task slots in task
+// order, or fdh then the pair-once partials in I order), then the Ewald
+// reciprocal partial forces in chunk order, added to F.
+__device__ inline void ele_sums(const DevPtrs &P, const DSys &s, int g, double F[3]) {
+  const int q = P.qidx[g];
+  if (q >= 0) {
+    if (s.ele_pairs) {
+      // the slots of the chunk-pair tasks that hold this charge, in task
+      // order, four tasks' loads in flight at a time
+      const int lq = q - s.qoff, C = lq / kTile, lc = lq % kTile;
+      for (int I0 = 0; I0 < s.nqt; I0 += 4) {
+        double f[4][3];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const int I = min(I0 + u, s.nqt - 1);
-            const int a = I <= C ? I : C, bb = I <= C ? C : I;
-            const int slot = I < C ? 1 : 0;
-            const double *pp = P.pp + s.pp_off + ele_task_index(a, bb, s.nqt) * (6 * kTile) + slot * 3 * kTile + lc;
-            f[u][0] = pp[0]; f[u][1] = pp[kTile]; f[u][2] = pp[2 * kTile];
-          }
-#pragma unroll
-          for (int u = 0; u < 4; ++u)
-            if (I0 + u < s.nqt) { F[0] += f[u][0]; F[1] += f[u][1]; F[2] += f[u][2]; }
+        for (int u = 0; u < 4; ++u) {
+          const int I = min(I0 + u, s.nqt - 1);
+          const int a = I <= C ? I : C, bb = I <= C ? C : I;
+          const int slot = I < C ? 1 : 0;
+          const double *pp = P.pp + s.pp_off + ele_task_index(a, bb, s.nqt) * (6 * kTile) + slot * 3 * kTile + lc;
+          f[u][0] = pp[0]; f[u][1] = pp[kTile]; f[u][2] = pp[2 * kTile];
         }
-      } else {
-        F[0] += P.fdh[q]; F[1] += P.fdh[(size_t)P.nqtot + q]; F[2] += P.fdh[2 * (size_t)P.nqtot + q];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (I0 + u < s.nqt) { F[0] += f[u][0]; F[1] += f[u][1]; F[2] += f[u][2]; }
       }
-      if (s.ele_once) {
-        // forces from the chunks I < J of the pair-once sweep, in I order
-        const int lq = q - s.qoff, J = lq / kTile, lj = lq % kTile;
-        for (int I0 = 0; I0 < J; I0 += 4) {
-          double f[4][3];
+    } else {
+      F[0] += P.fdh[q]; F[1] += P.fdh[(size_t)P.nqtot + q]; F[2] += P.fdh[2 * (size_t)P.nqtot + q];
+    }
+    if (s.ele_once) {
+      // forces from the chunks I < J of the pair-once sweep, in I order
+      const int lq = q - s.qoff, J = lq / kTile, lj = lq % kTile;
+      for (int I0 = 0; I0 < J; I0 += 4) {
+        double f[4][3];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const int I = min(I0 + u, J - 1);
-            const double *pj = P.pj + s.pj_off + ele_pair_index(I, J, s.nqt) * (3 * kTile) + lj;
-            f[u][0] = pj[0]; f[u][1] = pj[kTile]; f[u][2] = pj[2 * kTile];
-          }
-#pragma unroll
-          for (int u = 0; u < 4; ++u)
-            if (I0 + u < J) { F[0] += f[u][0]; F[1] += f[u][1]; F[2] += f[u][2]; }
+        for (int u = 0; u < 4; ++u) {
+          const int I = min(I0 + u, J - 1);
+          const double *pj = P.pj + s.pj_off + ele_pair_index(I, J, s.nqt) * (3 * kTile) + lj;
+          f[u][0] = pj[0]; f[u][1] = pj[kTile]; f[u][2] = pj[2 * kTile];
         }
-      }
-      if (s.ele_form == 2) {
-        const double *pf = P.pf + s.pf_off + (q - s.qoff);
-        for (int c = 0; c < s.nkc; ++c)
-          for (int d = 0; d < 3; ++d) F[d] += pf[((int64_t)c * 3 + d) * s.nq];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (I0 + u < J) { F[0] += f[u][0]; F[1] += f[u][1]; F[2] += f[u][2]; }
       }
     }
+    if (s.ele_form == 2) {
+      const double *pf = P.pf + s.pf_off + (q - s.qoff);
+      for (int c = 0; c < s.nkc; ++c)
+        for (int d = 0; d < 3; ++d) F[d] += pf[((int64_t)c * 3 + d) * s.nq];
+    }
   }
+}
+
+// The step index of a bead's noise and valence draws in mode (after mode 4
+// resolves to 2 or 3 per system).
+__device__ inline int resolve_mode(const DSys &s, int mode) { return mode == 4 ? (s.init_accel ? 2 : 3) : mode; }
+__device__ inline uint64_t noise_step(const DevPtrs &P, const DSys &s, int32_t t_local, int mode) {
+  return *P.stepbase + (uint64_t)t_local + (mode == 2 ? 0u : 1u) - s.step_origin;
+}
+
+// Forces of one bead from the two waves' partial sums, then the integrator
+// with the noise gs; returns the squared displacement since the last
+// rebuild before this step's update (and accumulates the update) when
+// rebuilds are displacement-triggered.
+__device__ inline double main_tail(const DevPtrs &P, const DSys &s, int g, double F[3], double (*red)[2][kTile],
+                                   const double gs[3], int lane, const double *X, const double *PX, double *XN,
+                                   double *PXN, int mode) {
+  const int n = P.ntot;
+  for (int d = 0; d < 3; ++d) F[d] = red[d][0][lane] + red[d][1][lane];
   P.f[g] = F[0]; P.f[n + g] = F[1]; P.f[2 * n + g] = F[2];
   if (mode == 0) return -1.0;
-  if (mode == 4) mode = s.init_accel ? 2 : 3;
-  const uint64_t step = *P.stepbase + (uint64_t)t_local + (mode == 2 ? 0u : 1u) - s.step_origin;
   double dx[3];
-  integrate(P, s, g, li, F, X, PX, XN, PXN, step, mode, dx);
+  integrate(P, s, g, gs, F, X, PX, XN, PXN, resolve_mode(s, mode), dx);
   if (!P.dyn) return -1.0;
   return accumulate_displacement(P, g, dx);
 }
@@ -1184,7 +1193,7 @@ __device__ inline double main_tail(const DevPtrs &P, const DSys &s, int g, int l
 __global__ __launch_bounds__(kTile * 2) void k_main(DevPtrs P, const double *X, const double *PX,
                                                     double *XN, double *PXN, const Tile *tiles,
                                                     int ntile, int32_t t_local, int mode, int do_dh) {
-  __shared__ double red[3][2][kTile];
+  __shared__ double red[3][2][kTile], gsh[3][kTile];
   const int lane = threadIdx.x & (kTile - 1), w = threadIdx.x / kTile;
   const Tile tl = tiles[blockIdx.x];
   const DSys s = P.sys[tl.sys];
@@ -1194,7 +1203,17 @@ __global__ __launch_bounds__(kTile * 2) void k_main(DevPtrs P, const double *X, 
   const int n = P.ntot;
   double F[3] = {0.0, 0.0, 0.0};
   if (valid) {
-    if (w == 0) {
+    if (w == 1) {
+      // the Langevin noise of the bead (wave 0 integrates after the
+      // barrier) and its electrostatic sums
+      if (mode != 0) {
+        const int m = resolve_mode(s, mode);
+        double gs[3];
+        gauss3(P.seed, s.key, noise_step(P, s, t_local, m), (uint32_t)li, gs);
+        gsh[0][lane] = gs[0]; gsh[1][lane] = gs[1]; gsh[2][lane] = gs[2];
+      }
+      if (do_dh && s.ele_form >= 0) ele_sums(P, s, g, F);
+    } else {
       // bonded slots in incidence order, eight loads in flight at a time
       // (slots past ie re-read slot ie - 1 and are not added)
       const int ib = P.inc_off[g], ie = P.inc_off[g + 1];
@@ -1224,7 +1243,10 @@ __global__ __launch_bounds__(kTile * 2) void k_main(DevPtrs P, const double *X, 
   __syncthreads();
   if (w != 0) return;
   double d2 = -1.0;   // squared displacement since the last rebuild, before this update
-  if (valid) d2 = main_tail(P, s, g, li, F, red, lane, X, PX, XN, PXN, t_local, mode, do_dh);
+  if (valid) {
+    const double gs[3] = {gsh[0][lane], gsh[1][lane], gsh[2][lane]};
+    d2 = main_tail(P, s, g, F, red, gs, lane, X, PX, XN, PXN, mode);
+  }
   if (P.dyn && mode != 0) tile_top2(P, s.bt0 + tl.start / kTile, d2);
 }
 
