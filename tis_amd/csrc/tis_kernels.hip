@@ -716,14 +716,17 @@ __device__ inline void ele_tile_once(const DevPtrs &P, const DSys &s, const doub
         const int jl = (lane + dir * k) & (kTile - 1);
         bool bit = (m64 >> jl) & 1u;
         if (last_half) bit = bit && lane < 32;
-        const double v0 = SX[jl] - xi0, v1 = SY[jl] - xi1, v2 = SZ[jl] - xi2;
-        const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-        const bool on = bit && d2 <= cut2;
-        float dv = ele_dv<FORM>(s, qiC * SQ[jl], on ? (float)d2 : 1.0f, nclamp, on);
-        dv = on ? dv : 0.0f;
-        const double dd = (double)dv;
-        F0 -= dd * v0; F1 -= dd * v1; F2 -= dd * v2;
-        G0 += dd * v0; G1 += dd * v1; G2 += dd * v2;
+        // a step with no listed pair in the wave adds zeros: only the sums move
+        if (__any(bit)) {
+          const double v0 = SX[jl] - xi0, v1 = SY[jl] - xi1, v2 = SZ[jl] - xi2;
+          const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+          const bool on = bit && d2 <= cut2;
+          float dv = ele_dv<FORM>(s, qiC * SQ[jl], on ? (float)d2 : 1.0f, nclamp, on);
+          dv = on ? dv : 0.0f;
+          const double dd = (double)dv;
+          F0 -= dd * v0; F1 -= dd * v1; F2 -= dd * v2;
+          G0 += dd * v0; G1 += dd * v1; G2 += dd * v2;
+        }
         G0 = wave_rot(G0); G1 = wave_rot(G1); G2 = wave_rot(G2);
       };
       if (__any(m64 != 0)) {
@@ -816,17 +819,20 @@ __device__ inline void ele_pair_task(const DevPtrs &P, const DSys &s, const doub
     const int jl = (lane + rd * k) & (kTile - 1);
     bool bit = (m64 >> jl) & 1u;
     if (last_half) bit = bit && lane < 32;
-    const int code = bit ? (int)mrow[jl] : 0;
-    const double v0 = SX[jl] - xi0 + mirror_shift(s, code, 0);
-    const double v1 = SY[jl] - xi1 + mirror_shift(s, code, 1);
-    const double v2 = SZ[jl] - xi2 + mirror_shift(s, code, 2);
-    const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-    const bool on = bit && d2 <= cut2;
-    float dv = ele_dv<FORM>(s, qiC * SQ[jl], on ? (float)d2 : 1.0f, nclamp, on);
-    dv = on ? dv : 0.0f;
-    const double dd = (double)dv;
-    F0 -= dd * v0; F1 -= dd * v1; F2 -= dd * v2;
-    G0 += dd * v0; G1 += dd * v1; G2 += dd * v2;
+    // a step with no listed pair in the wave adds zeros: only the sums move
+    if (__any(bit)) {
+      const int code = bit ? (int)mrow[jl] : 0;
+      const double v0 = SX[jl] - xi0 + mirror_shift(s, code, 0);
+      const double v1 = SY[jl] - xi1 + mirror_shift(s, code, 1);
+      const double v2 = SZ[jl] - xi2 + mirror_shift(s, code, 2);
+      const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+      const bool on = bit && d2 <= cut2;
+      float dv = ele_dv<FORM>(s, qiC * SQ[jl], on ? (float)d2 : 1.0f, nclamp, on);
+      dv = on ? dv : 0.0f;
+      const double dd = (double)dv;
+      F0 -= dd * v0; F1 -= dd * v1; F2 -= dd * v2;
+      G0 += dd * v0; G1 += dd * v1; G2 += dd * v2;
+    }
     G0 = rot_next(G0, dir); G1 = rot_next(G1, dir); G2 = rot_next(G2, dir);
   };
   if (__any(m64 != 0)) {
@@ -1021,25 +1027,41 @@ __global__ __launch_bounds__(kTile * kWaves) void k_ewald_f(DevPtrs P, const dou
 // Langevin (time_integral.F90:169-183, then the next step's position update
 // :152-160): a' = F/m + c1 xi; v = c2 v + c3 (a + a'); a = a';
 // x_next = x + c4 v + (dt^2/2) a.
-__device__ inline void integrate(const DevPtrs &P, const DSys &s, int g, const double gs[3], const double F[3],
-                                 const double *X, const double *PX, double *XN, double *PXN, int mode,
-                                 double dxo[3]) {
+// A bead's integrator inputs (Langevin constants, 1/m, v, a, x, wrapped x,
+// displacement since the last rebuild), loaded before the barrier of k_main
+// so that their latency overlaps the other wave's work.
+struct BeadState { double r[4], rcm, v[3], a[3], x[3], px[3], o[3]; };
+
+__device__ inline void load_state(const DevPtrs &P, const DSys &s, int g, const double *X, const double *PX,
+                                  BeadState &b) {
+  const int n = P.ntot;
+  for (int k = 0; k < 4; ++k) b.r[k] = P.rlan[k * n + g];
+  b.rcm = P.rcm[g];
+  for (int d = 0; d < 3; ++d) {
+    b.v[d] = P.v[d * n + g];
+    b.a[d] = P.a[d * n + g];
+    b.x[d] = X[d * n + g];
+    b.px[d] = s.periodic ? PX[d * n + g] : 0.0;
+    b.o[d] = P.dyn ? P.dxyz[d * n + g] : 0.0;
+  }
+}
+
+__device__ inline void integrate(const DevPtrs &P, const DSys &s, int g, const BeadState &b, const double gs[3],
+                                 const double F[3], double *XN, double *PXN, int mode, double dxo[3]) {
 #pragma clang fp contract(off)
   const int n = P.ntot;
-  const double r0 = P.rlan[g], r1 = P.rlan[n + g], r2 = P.rlan[2 * n + g], r3 = P.rlan[3 * n + g];
-  const double rcm = P.rcm[g];
   for (int d = 0; d < 3; ++d) {
-    double af = F[d] * rcm + r0 * gs[d];
-    double vv = P.v[d * n + g];
-    double a0 = P.a[d * n + g];
+    double af = F[d] * b.rcm + b.r[0] * gs[d];
+    double vv = b.v[d];
+    const double a0 = b.a[d];
     if (mode == 3) af = a0;
-    if (mode == 1) vv = r1 * vv + r2 * (a0 + af);
+    if (mode == 1) vv = b.r[1] * vv + b.r[2] * (a0 + af);
     P.v[d * n + g] = vv;
     P.a[d * n + g] = af;
-    double dx = r3 * vv + P.tstep2 * af;
+    double dx = b.r[3] * vv + P.tstep2 * af;
     dxo[d] = dx;
-    XN[d * n + g] = X[d * n + g] + dx;
-    if (s.periodic) PXN[d * n + g] = PX[d * n + g] + dx;
+    XN[d * n + g] = b.x[d] + dx;
+    if (s.periodic) PXN[d * n + g] = b.px[d] + dx;
   }
 }
 
@@ -1104,10 +1126,10 @@ __device__ inline void tile_top2(const DevPtrs &P, int tile, double d2) {
 
 // dxyz_mp of time_integral.F90:160: |dxyz|^2 before the update (as
 // :64 computes it), then dxyz += dx; the update is kept for a rebuild.
-__device__ inline double accumulate_displacement(const DevPtrs &P, int g, const double dx[3]) {
+__device__ inline double accumulate_displacement(const DevPtrs &P, int g, const double o[3], const double dx[3]) {
 #pragma clang fp contract(off)
   const int n = P.ntot;
-  const double o0 = P.dxyz[g], o1 = P.dxyz[n + g], o2 = P.dxyz[2 * n + g];
+  const double o0 = o[0], o1 = o[1], o2 = o[2];
   const double d2 = o0 * o0 + o1 * o1 + o2 * o2;
   P.dxyz[g] = o0 + dx[0]; P.dxyz[n + g] = o1 + dx[1]; P.dxyz[2 * n + g] = o2 + dx[2];
   P.dlast[g] = dx[0]; P.dlast[n + g] = dx[1]; P.dlast[2 * n + g] = dx[2];
@@ -1177,16 +1199,16 @@ __device__ inline uint64_t noise_step(const DevPtrs &P, const DSys &s, int32_t t
 // rebuild before this step's update (and accumulates the update) when
 // rebuilds are displacement-triggered.
 __device__ inline double main_tail(const DevPtrs &P, const DSys &s, int g, double F[3], double (*red)[2][kTile],
-                                   const double gs[3], int lane, const double *X, const double *PX, double *XN,
-                                   double *PXN, int mode) {
+                                   const BeadState &b, const double gs[3], int lane, double *XN, double *PXN,
+                                   int mode) {
   const int n = P.ntot;
   for (int d = 0; d < 3; ++d) F[d] = red[d][0][lane] + red[d][1][lane];
   P.f[g] = F[0]; P.f[n + g] = F[1]; P.f[2 * n + g] = F[2];
   if (mode == 0) return -1.0;
   double dx[3];
-  integrate(P, s, g, gs, F, X, PX, XN, PXN, resolve_mode(s, mode), dx);
+  integrate(P, s, g, b, gs, F, XN, PXN, resolve_mode(s, mode), dx);
   if (!P.dyn) return -1.0;
-  return accumulate_displacement(P, g, dx);
+  return accumulate_displacement(P, g, b.o, dx);
 }
 
 
@@ -1240,12 +1262,14 @@ __global__ __launch_bounds__(kTile * 2) void k_main(DevPtrs P, const double *X, 
     if (nclamp) atomicAdd(&P.counters[0], (unsigned long long)nclamp);
   }
   for (int d = 0; d < 3; ++d) red[d][w][lane] = F[d];
+  BeadState bs;
+  if (w == 0 && valid && mode != 0) load_state(P, s, g, X, PX, bs);
   __syncthreads();
   if (w != 0) return;
   double d2 = -1.0;   // squared displacement since the last rebuild, before this update
   if (valid) {
     const double gs[3] = {gsh[0][lane], gsh[1][lane], gsh[2][lane]};
-    d2 = main_tail(P, s, g, F, red, gs, lane, X, PX, XN, PXN, mode);
+    d2 = main_tail(P, s, g, F, red, bs, gs, lane, XN, PXN, mode);
   }
   if (P.dyn && mode != 0) tile_top2(P, s.bt0 + tl.start / kTile, d2);
 }
