@@ -46,15 +46,16 @@ def small_ions_box(seed=5, n=20, box=60.0, alpha=0.2, rcut=25.0, hmax=6, form=EL
     return s, x
 
 
-def coulomb_chain(seed=21, n=160, rcut=40.0):
-    """A compact 160-nt chain (phosphate charge -1) with plain Coulomb
-    electrostatics cut at 40 A, no box: three charge chunks of 64 with a
-    partial last one."""
+def coulomb_chain(seed=21, n=160, rcut=40.0, box=60.0):
+    """A compact n-nt chain (phosphate charge -1) with plain Coulomb
+    electrostatics cut at 40 A, no box: 160 nt give three charge chunks of 64
+    with a partial last one (each pair once per charge tile), 320 nt five
+    (one wave per chunk pair)."""
     rng = np.random.default_rng(seed)
     seq = T.random_sequence(rng, n)
     s = T.build_rna_system([(seq, [])], ele_form=ELE_COULOMB, cutoff_ele=rcut, phos_charge=-1.0,
                            name="coulomb_chain")
-    x = T.compact_coords(s, rng, box=60, min_dist=2.6)
+    x = T.compact_coords(s, rng, box=box, min_dist=2.6)
     return s, x
 
 
@@ -66,6 +67,7 @@ def parity_cases(ewald=True):
     # several 64-charge chunks: pair chunks I < J of the electrostatic sweep
     out.append(("hairpin300_helix",) + hairpin_design(7, 300))
     out.append(("coulomb_chain160",) + coulomb_chain())
+    out.append(("coulomb_chain320",) + coulomb_chain(seed=22, n=320, box=76.0))
     if ewald:
         out.append(("ions_box_ewald",) + small_ions_box())
         out.append(("ions_box_coulomb",) + small_ions_box(form=ELE_COULOMB))

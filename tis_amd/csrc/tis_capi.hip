@@ -19,8 +19,8 @@
 #include "tis_internal.h"
 
 namespace tis {
-__global__ void k_force(DevPtrs, const double *, const double *, const int32_t *, int, const Tile *, int,
-                        const EwTask *, int, BondedRange, int32_t);
+__global__ void k_force(DevPtrs, const double *, const double *, const int32_t *, int, const EwTask *, int,
+                        BondedRange, int32_t);
 __global__ void k_ewald_sk(DevPtrs, const double *, const EwTask *, int);
 __global__ void k_ewald_f(DevPtrs, const double *, const EwTask *);
 __global__ void k_main(DevPtrs, const double *, const double *, double *, double *, const Tile *, int, int32_t, int, int);
@@ -80,7 +80,7 @@ struct tis_ctx {
   size_t dev_bytes = 0;                  // bytes held in allocs
   DevPtrs P{};
   DSys *d_sys = nullptr;
-  Tile *d_btiles = nullptr, *d_qtiles = nullptr, *d_qtiles_lpt = nullptr;
+  Tile *d_btiles = nullptr, *d_qtiles = nullptr;
   int32_t nbtile = 0, nqtile = 0;
   int32_t *d_systask = nullptr, *d_sysidx = nullptr;
   int32_t nsystask = 0;
@@ -91,7 +91,8 @@ struct tis_ctx {
   unsigned long long *d_counters = nullptr;
   uint64_t *d_stepbase = nullptr;
   bool any_ele = false, any_periodic = false;
-  EwTask *d_ewsk = nullptr, *d_ewf = nullptr, *d_ptask = nullptr, *d_bpair = nullptr, *d_qpair = nullptr;
+  EwTask *d_ewsk = nullptr, *d_ewf = nullptr, *d_ptask = nullptr, *d_ptask_lpt = nullptr, *d_bpair = nullptr,
+         *d_qpair = nullptr;
   int32_t n_ewsk = 0, n_ewf = 0, n_ptask = 0, n_bpair = 0, n_qpair = 0;
   // state
   int cur = 0;
@@ -404,7 +405,7 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
     return (int)specs.size() - 1;
   };
   c->systask_of.assign(nsys, -1);
-  int64_t moff = 0, mioff = 0, pj_off = 0, pp_off = 0, lp_off = 0;
+  int64_t moff = 0, mioff = 0, pp_off = 0, lp_off = 0;
   int32_t off = 0, qoff = 0, nbd = 0, nba = 0, nst = 0, ntst = 0, nhb = 0, nsite = 0, uoff = 0;
   c->any_ele = false;
   c->any_periodic = false;
@@ -538,13 +539,10 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
     for (int st0 = 0; st0 < nq; st0 += kTile) qtiles.push_back(Tile{is, st0});
     h.qt_count = (int32_t)qtiles.size() - h.qt_begin;
     d.bt0 = h.bt_begin; d.nbt = h.bt_count; d.qt0 = h.qt_begin; d.nqt = h.qt_count;
-    d.ele_once = (!s.p.periodic && (s.p.ele_form == TIS_ELE_DH || s.p.ele_form == TIS_ELE_COULOMB) && nq > 0);
-    d.pj_off = pj_off;
-    if (d.ele_once) pj_off += (int64_t)d.nqt * (d.nqt - 1) / 2 * 3 * kTile;
-    d.ele_pairs = (s.p.periodic && s.p.ele_form != TIS_ELE_NONE && nq > 0);
+    // electrostatic pairs: one wave per chunk pair (I, J), I <= J
     d.pp_off = pp_off;
     h.pt_begin = (int32_t)ptask.size();
-    if (d.ele_pairs) {
+    if (s.p.ele_form != TIS_ELE_NONE && nq > 0) {
       pp_off += (int64_t)d.nqt * (d.nqt + 1) / 2 * 6 * kTile;
       for (int I = 0; I < d.nqt; ++I)
         for (int J = I; J < d.nqt; ++J) ptask.push_back(EwTask{is, I, J});
@@ -713,12 +711,10 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
   rc = dalloc(c, &P.nb_cnt, (size_t)off); if (rc) return rc;
   P.nb_cap = 64;
   rc = dalloc(c, &P.nb, (size_t)P.nb_cap * off); if (rc) return rc;
-  rc = dalloc(c, &P.fdh, 3 * (size_t)qoff); if (rc) return rc;
   rc = dalloc(c, &P.dxyz, 3 * (size_t)off); if (rc) return rc;
   rc = dalloc(c, &P.dlast, 3 * (size_t)off); if (rc) return rc;
   rc = dalloc(c, &P.ttop, 2 * btiles.size()); if (rc) return rc;
   rc = dalloc(c, &P.sys_flag, (size_t)nsys); if (rc) return rc;
-  rc = dalloc(c, &P.pj, (size_t)std::max<int64_t>(pj_off, 1)); if (rc) return rc;
   rc = dalloc(c, &P.pp, (size_t)std::max<int64_t>(pp_off, 1)); if (rc) return rc;
   rc = dalloc(c, &P.qmask, (size_t)moff); if (rc) return rc;
   rc = dalloc(c, &P.qmirror, (size_t)std::max<int64_t>(mioff, 1)); if (rc) return rc;
@@ -729,19 +725,6 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
   P.seed = seed;
   UP(c->d_btiles, btiles);
   UP(c->d_qtiles, qtiles);
-  {
-    // whole-batch k_force launches take the charge tiles longest first (chunk
-    // pairs of the sweep, then the original order): the block order changes
-    // no result
-    std::vector<Tile> lpt(qtiles);
-    auto work = [&](const Tile &t) {
-      const DSys &d = c->hs[t.sys].d;
-      const int I = t.start / kTile;
-      return d.ele_pairs ? 0 : (d.ele_once ? d.nqt - I : d.nqt);
-    };
-    std::stable_sort(lpt.begin(), lpt.end(), [&](const Tile &a, const Tile &b) { return work(a) > work(b); });
-    UP(c->d_qtiles_lpt, lpt);
-  }
   {
     std::vector<double> h3;
     h3.insert(h3.end(), ewhx.begin(), ewhx.end());
@@ -756,6 +739,11 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
     UP(c->d_ewf, ewf);
     UP(c->d_ptask, ptask);
     c->n_ptask = (int32_t)ptask.size();
+    // whole-batch k_force launches take the off-diagonal chunk pairs (64
+    // steps) before the diagonal ones (32): the order changes no result
+    std::vector<EwTask> lpt(ptask);
+    std::stable_partition(lpt.begin(), lpt.end(), [](const EwTask &t) { return t.a != t.b; });
+    UP(c->d_ptask_lpt, lpt);
     UP(c->d_bpair, bpair);
     c->n_bpair = (int32_t)bpair.size();
     UP(c->d_qpair, qpair);
@@ -938,7 +926,8 @@ int32_t tis_pairs_get(tis_ctx *c, int32_t isys, int32_t which, int32_t *pairs, d
                  : TIS_OK;
 }
 
-// k_force (terms, charge tiles, bonded terms), the Ewald passes, then k_main.  Steps draw noise/valence numbers for step
+// k_force (terms, electrostatic chunk pairs, bonded terms), the Ewald
+// passes, then k_main.  Steps draw noise/valence numbers for step
 // stepbase+t+1; a stand-alone force evaluation or the initial acceleration
 // uses stepbase.
 static void launch_force_pair(tis_ctx *c, const Range &r, const double *X, const double *PX, double *XN,
@@ -958,24 +947,22 @@ static void launch_force_pair(tis_ctx *c, const Range &r, const double *X, const
     c->tev_used += kTev;
     (void)hipEventRecord(e3[0], c->stream);
   }
-  // k_force: per-system tertiary stacks and hbonds, the charge tiles, then
-  // the bonded terms in blocks of 256 lanes
+  // k_force: per-system tertiary stacks and hbonds, the electrostatic chunk
+  // pairs, then the bonded terms in blocks of 256 lanes
   const int nterm = r.nb + r.na + r.ns;
-  const int nqt = (r.qtn > 0 && c->any_ele) ? r.qtn : 0;
   const int npt = c->any_ele ? r.ptn : 0;
-  const int nblk = (r.taskn + kWaves - 1) / kWaves + nqt + (npt + kWaves - 1) / kWaves +
+  const int nblk = (r.taskn + kWaves - 1) / kWaves + (npt + kWaves - 1) / kWaves +
                    (nterm + kTile * kWaves - 1) / (kTile * kWaves);
   if (nblk > 0) {
-    const Tile *qt = r.all ? c->d_qtiles_lpt : c->d_qtiles + r.qt0;
     const BondedRange br{r.b0, r.nb, r.a0, r.na, r.s0, r.ns};
     const int32_t *task = c->d_systask + r.task0;
-    const EwTask *pt = c->d_ptask + r.pt0;
+    const EwTask *pt = r.all ? c->d_ptask_lpt : c->d_ptask + r.pt0;
     if (c->timing)
       hipExtLaunchKernelGGL(k_force, dim3(nblk), dim3(kTile * kWaves), 0, c->stream, e3[1], e3[2], 0, c->P, X, PX,
-                            task, r.taskn, qt, nqt, pt, npt, br, pre_t);
+                            task, r.taskn, pt, npt, br, pre_t);
     else
-      hipLaunchKernelGGL(k_force, dim3(nblk), dim3(kTile * kWaves), 0, c->stream, c->P, X, PX, task, r.taskn, qt,
-                         nqt, pt, npt, br, pre_t);
+      hipLaunchKernelGGL(k_force, dim3(nblk), dim3(kTile * kWaves), 0, c->stream, c->P, X, PX, task, r.taskn, pt,
+                         npt, br, pre_t);
   } else if (c->timing) {
     (void)hipEventRecord(e3[1], c->stream);
     (void)hipEventRecord(e3[2], c->stream);

@@ -70,13 +70,8 @@ struct DSys {
   int32_t ewk0, nk;          // reciprocal vectors [ewk0, ewk0 + nk)
   int32_t nqc, nkc;          // charge chunks of the S pass, k chunks of the force pass
   int64_t psk_off, pf_off;   // partial structure factors [nqc][nk][2], partial forces [nkc][3][nq]
-  // electrostatic pairs taken once (non-periodic, ele_form 0 or 1): partial
-  // forces on chunk J from chunk I < J, [nqt(nqt-1)/2][3][kTile] at pj_off
-  int32_t ele_once;
-  int64_t pj_off;
-  // periodic electrostatics by chunk-pair tasks: partial forces
+  // electrostatics by chunk-pair tasks: partial forces
   // [nqt(nqt+1)/2][2][3][kTile] at pp_off
-  int32_t ele_pairs;
   int64_t pp_off;
   // tile-pair list build: counts [nbt * nbt][kTile] at lp_off * kTile
   int64_t lp_off;
@@ -143,9 +138,7 @@ struct DevPtrs {
   // lists
   int32_t *nb_cnt; uint32_t *nb; int32_t nb_cap;
   const int32_t *qbead;      // [nq] global bead of each charge
-  double *fdh;               // [3][nq]
-  double *pj;                // partial forces of the pair-once sweep
-  double *pp;                // partial forces of the periodic chunk-pair tasks
+  double *pp;                // partial forces of the electrostatic chunk-pair tasks
   uint32_t *qmask;           // bit matrix
   uint8_t *qmirror;          // periodic ele mirror codes
   // Ewald reciprocal space

@@ -541,94 +541,11 @@ __device__ inline float ele_dv(const DSys &s, float qq, float d2, unsigned &ncla
   }
 }
 
-// Owner-computes over one charge tile: each lane of wave w owns a charge of
-// the tile and sweeps the system's charge chunks w, w+4, ... staged through
-// LDS; list membership from the bit matrix, cutoff test in FP64, pair
-// arithmetic in FP32, accumulation in FP64; the four partial sums are added
-// in wave order.
+// Staging of the J chunk of each wave's chunk pair.
 struct EleLds {
   double sx[kWaves][kTile], sy[kWaves][kTile], sz[kWaves][kTile];
   float sq[kWaves][kTile];
-  double red[3][kWaves][kTile];
 };
-
-template <int FORM>
-__device__ inline void ele_tile(const DevPtrs &P, const DSys &s, const double *X, const double *PX, int qi0,
-                                EleLds &L) {
-  auto &sx = L.sx; auto &sy = L.sy; auto &sz = L.sz; auto &sq = L.sq; auto &red = L.red;
-  const int lane = threadIdx.x & (kTile - 1), w = threadIdx.x / kTile;
-  const int qi = qi0 + lane;
-  const bool valid = qi < s.nq;
-  const double *Y = s.periodic ? PX : X;
-  const int n = P.ntot;
-  double xi0 = 0.0, xi1 = 0.0, xi2 = 0.0, chi = 0.0;
-  if (valid) {
-    int gi = P.qbead[s.qoff + qi];
-    xi0 = Y[gi]; xi1 = Y[n + gi]; xi2 = Y[2 * n + gi];
-    chi = P.charge[gi];
-  }
-  const float qiC = (float)(chi * s.ele_coef);
-  const double cut2 = FORM == 0 ? s.dh_cut2 : s.cl_cut2;
-  double F0 = 0.0, F1 = 0.0, F2 = 0.0;
-  unsigned nclamp = 0;
-  const uint32_t *row = P.qmask + s.moff + (int64_t)(valid ? qi : 0) * s.mwords;
-  const int nch = (s.nq + kTile - 1) / kTile;
-  for (int it = 0; it < nch; it += kWaves) {
-    const int jc = it + w;
-    const int jt = jc * kTile;
-    if (jc < nch && jt + lane < s.nq) {
-      int gj = P.qbead[s.qoff + jt + lane];
-      sx[w][lane] = Y[gj]; sy[w][lane] = Y[n + gj]; sz[w][lane] = Y[2 * n + gj];
-      sq[w][lane] = (float)P.charge[gj];
-    }
-    __syncthreads();
-    if (valid && jc < nch) {
-      const int wbase = jt >> 5;
-      const uint32_t w0 = row[wbase];
-      const uint32_t w1 = (wbase + 1 < s.mwords) ? row[wbase + 1] : 0u;
-      const int kend = min(kTile, s.nq - jt);
-      const double *SX = sx[w], *SY = sy[w], *SZ = sz[w];
-      const float *SQ = sq[w];
-      if (!s.periodic) {
-        // branch-free: pairs outside the list or the cutoff contribute 0
-        // (their distance is replaced by 1 so that no inf/NaN arises)
-#pragma unroll 4
-        for (int k = 0; k < kend; ++k) {
-          const uint32_t bit = k < 32 ? (w0 >> k) & 1u : (w1 >> (k - 32)) & 1u;
-          const double v0 = SX[k] - xi0, v1 = SY[k] - xi1, v2 = SZ[k] - xi2;
-          const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-          const bool on = bit && d2 <= cut2;
-          float dv = ele_dv<FORM>(s, qiC * SQ[k], on ? (float)d2 : 1.0f, nclamp, on && qi < jt + k);
-          dv = on ? dv : 0.0f;
-          const double dd = (double)dv;
-          F0 -= dd * v0; F1 -= dd * v1; F2 -= dd * v2;
-        }
-      } else {
-        for (int k = 0; k < kend; ++k) {
-          const uint32_t bit = k < 32 ? (w0 >> k) & 1u : (w1 >> (k - 32)) & 1u;
-          if (!bit) continue;
-          const int code = P.qmirror[s.mioff + (int64_t)qi * s.nq + jt + k];
-          const double v0 = SX[k] - xi0 + mirror_shift(s, code, 0);
-          const double v1 = SY[k] - xi1 + mirror_shift(s, code, 1);
-          const double v2 = SZ[k] - xi2 + mirror_shift(s, code, 2);
-          const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-          if (d2 > cut2) continue;
-          const float dv = ele_dv<FORM>(s, qiC * SQ[k], (float)d2, nclamp, qi < jt + k);
-          F0 -= (double)(dv * (float)v0); F1 -= (double)(dv * (float)v1); F2 -= (double)(dv * (float)v2);
-        }
-      }
-    }
-    __syncthreads();
-  }
-  red[0][w][lane] = F0; red[1][w][lane] = F1; red[2][w][lane] = F2;
-  __syncthreads();
-  if (w == 0 && valid) {
-    const int q = s.qoff + qi;
-    for (int d = 0; d < 3; ++d)
-      P.fdh[(size_t)d * P.nqtot + q] = ((red[d][0][lane] + red[d][1][lane]) + red[d][2][lane]) + red[d][3][lane];
-  }
-  if (nclamp) atomicAdd(&P.counters[1], (unsigned long long)nclamp);
-}
 
 // Whole-wave rotation by one lane (DPP wave_rol:1): each lane receives the
 // value of a neighbouring lane.  The direction is read off at run time from
@@ -651,123 +568,23 @@ __device__ inline int wave_dir() {
   return ok ? d0 : 0;
 }
 
-// Index of the off-diagonal chunk pair (I, J), I < J < nch, in a system's
-// partial-force buffer.
-__host__ __device__ inline int64_t ele_pair_index(int I, int J, int nch) {
-  return (int64_t)I * (2 * nch - I - 1) / 2 + (J - I - 1);
-}
-
-// Each pair once (non-periodic systems).  Block = charge tile I; wave w takes
-// the chunks J = I + w, I + w + 4, ...  Inside a chunk pair, at iteration k
-// lane l pairs its charge i with j = (l + dir k) mod 64 of chunk J; the force
-// on i accumulates in the lane, the opposite force on j in a register that
-// rotates with the schedule, so after 64 iterations lane l holds the sum for
-// j = l.  The diagonal chunk (J = I) runs k = 1..32 (k = 32 for lanes < 32
-// only), which meets every unordered pair of the tile once.  Forces on j
-// from J > I go to the partial buffer pj[(I, J)], summed by k_main in I
-// order after fdh; everything is in a fixed order, so results are bitwise
-// reproducible.  Pair math as ele_tile; charges beyond nq are staged as zeros
-// with their bits clear.
-template <int FORM>
-__device__ inline void ele_tile_once(const DevPtrs &P, const DSys &s, const double *X, int qi0, int dir,
-                                     EleLds &L) {
-  auto &sx = L.sx; auto &sy = L.sy; auto &sz = L.sz; auto &sq = L.sq; auto &red = L.red;
-  const int lane = threadIdx.x & (kTile - 1), w = threadIdx.x / kTile;
-  const int I = qi0 / kTile;
-  const int qi = qi0 + lane;
-  const bool valid = qi < s.nq;
-  const int n = P.ntot;
-  double xi0 = 0.0, xi1 = 0.0, xi2 = 0.0, chi = 0.0;
-  if (valid) {
-    const int gi = P.qbead[s.qoff + qi];
-    xi0 = X[gi]; xi1 = X[n + gi]; xi2 = X[2 * n + gi];
-    chi = P.charge[gi];
-  }
-  const float qiC = (float)(chi * s.ele_coef);
-  const double cut2 = FORM == 0 ? s.dh_cut2 : s.cl_cut2;
-  double F0 = 0.0, F1 = 0.0, F2 = 0.0;   // on i
-  double D0 = 0.0, D1 = 0.0, D2 = 0.0;   // on j of the diagonal chunk
-  unsigned nclamp = 0;
-  const uint32_t *row = P.qmask + s.moff + (int64_t)(valid ? qi : 0) * s.mwords;
-  const int nch = s.nqt;
-  for (int J0 = I; J0 < nch; J0 += kWaves) {
-    const int J = J0 + w;
-    const int jt = J * kTile;
-    if (J < nch) {
-      const bool vj = jt + lane < s.nq;
-      const int gj = vj ? P.qbead[s.qoff + jt + lane] : 0;
-      sx[w][lane] = vj ? X[gj] : 0.0; sy[w][lane] = vj ? X[n + gj] : 0.0; sz[w][lane] = vj ? X[2 * n + gj] : 0.0;
-      sq[w][lane] = vj ? (float)P.charge[gj] : 0.0f;
-    }
-    __syncthreads();
-    if (J < nch) {
-      const int wb = jt >> 5;
-      uint64_t m64 = 0;
-      if (valid) {
-        m64 = row[wb];
-        if (wb + 1 < s.mwords) m64 |= (uint64_t)row[wb + 1] << 32;
-      }
-      const bool diag = J == I;
-      const double *SX = sx[w], *SY = sy[w], *SZ = sz[w];
-      const float *SQ = sq[w];
-      double G0 = 0.0, G1 = 0.0, G2 = 0.0;
-      // one pair of the schedule at iteration k
-      auto pair = [&](int k, bool last_half) {
-        const int jl = (lane + dir * k) & (kTile - 1);
-        bool bit = (m64 >> jl) & 1u;
-        if (last_half) bit = bit && lane < 32;
-        // a step with no listed pair in the wave adds zeros: only the sums move
-        if (__any(bit)) {
-          const double v0 = SX[jl] - xi0, v1 = SY[jl] - xi1, v2 = SZ[jl] - xi2;
-          const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
-          const bool on = bit && d2 <= cut2;
-          float dv = ele_dv<FORM>(s, qiC * SQ[jl], on ? (float)d2 : 1.0f, nclamp, on);
-          dv = on ? dv : 0.0f;
-          const double dd = (double)dv;
-          F0 -= dd * v0; F1 -= dd * v1; F2 -= dd * v2;
-          G0 += dd * v0; G1 += dd * v1; G2 += dd * v2;
-        }
-        G0 = wave_rot(G0); G1 = wave_rot(G1); G2 = wave_rot(G2);
-      };
-      if (__any(m64 != 0)) {
-        if (diag) {
-#pragma unroll 1
-          for (int k = 1; k < 32; ++k) pair(k, false);
-          pair(32, true);
-        } else {
-#pragma unroll 1
-          for (int k = 0; k < kTile; ++k) pair(k, false);
-        }
-      }
-      if (diag) {
-        // lane l holds j = l + dir * 33; bring j = l home
-        const int src = (lane - dir * 33) & (kTile - 1);
-        D0 = __shfl(G0, src); D1 = __shfl(G1, src); D2 = __shfl(G2, src);
-      } else {
-        double *pj = P.pj + s.pj_off + ele_pair_index(I, J, nch) * (3 * kTile);
-        pj[lane] = G0; pj[kTile + lane] = G1; pj[2 * kTile + lane] = G2;
-      }
-    }
-    __syncthreads();
-  }
-  red[0][w][lane] = F0; red[1][w][lane] = F1; red[2][w][lane] = F2;
-  __syncthreads();
-  if (w == 0 && valid) {
-    const int q = s.qoff + qi;
-    P.fdh[q] = (((red[0][0][lane] + red[0][1][lane]) + red[0][2][lane]) + red[0][3][lane]) + D0;
-    P.fdh[(size_t)P.nqtot + q] = (((red[1][0][lane] + red[1][1][lane]) + red[1][2][lane]) + red[1][3][lane]) + D1;
-    P.fdh[2 * (size_t)P.nqtot + q] = (((red[2][0][lane] + red[2][1][lane]) + red[2][2][lane]) + red[2][3][lane]) + D2;
-  }
-  if (nclamp) atomicAdd(&P.counters[1], (unsigned long long)nclamp);
-}
-
-// Periodic systems: one wave per chunk pair (I, J), I <= J (more waves in
-// flight than one block per tile gives a single box).  The same rotating
-// schedule as ele_tile_once takes each pair once with the mirror code the
-// list stored for it (v = x_j - x_i + shift, util_pbneighbor.F90:22-35);
-// the forces on chunk I go to slot 0 of the task's partial buffer, those on
-// chunk J (I < J) to slot 1, and k_main sums a charge's slots in task order.
-// dir = 0 (no whole-wave DPP rotation) rotates with __shfl instead.
+// force_ele_DH.F90:68-118, force_ele_coulomb.F90:63-84 and the real-space
+// loop of force_ele_coulomb_ewld.F90:118-137.  One wave per chunk pair
+// (I, J), I <= J, of a system's charges (chunks of 64), four pairs per
+// block.  At iteration k lane l pairs its charge i of chunk I with
+// j = (l + dir k) mod 64 of chunk J (staged in the wave's LDS slice); the
+// force on i accumulates in the lane, the opposite force on j in a register
+// that rotates with the schedule, so after 64 iterations lane l holds the
+// sum for j = l.  The diagonal chunk (J = I) runs k = 1..32 (k = 32 for
+// lanes < 32 only), which meets every unordered pair of the chunk once.
+// List membership from the bit matrix, cutoff test in FP64, pair arithmetic
+// in FP32, accumulation in FP64; periodic pairs take the mirror code the list
+// stored for them (v = x_j - x_i + shift, util_pbneighbor.F90:22-35).  The
+// forces on chunk I go to slot 0 of the task's partial buffer, those on
+// chunk J (I < J) to slot 1, and k_main sums a charge's slots in task order:
+// a fixed order, so results are bitwise reproducible.  Charges beyond nq are
+// staged as zeros with their bits clear.  dir = 0 (no whole-wave DPP
+// rotation) rotates with __shfl instead.
 __host__ __device__ inline int64_t ele_task_index(int I, int J, int nch) {
   return (int64_t)I * (2 * nch - I + 1) / 2 + (J - I);
 }
@@ -779,7 +596,7 @@ __device__ inline double rot_next(double v, int dir) {
 }
 
 template <int FORM>
-__device__ inline void ele_pair_task(const DevPtrs &P, const DSys &s, const double *PX, int I, int J, int dir,
+__device__ inline void ele_pair_task(const DevPtrs &P, const DSys &s, const double *Y, int I, int J, int dir,
                                      double *SX, double *SY, double *SZ, float *SQ) {
   const int lane = threadIdx.x & (kTile - 1);
   const int rd = dir != 0 ? dir : 1;
@@ -789,14 +606,14 @@ __device__ inline void ele_pair_task(const DevPtrs &P, const DSys &s, const doub
   double xi0 = 0.0, xi1 = 0.0, xi2 = 0.0, chi = 0.0;
   if (valid) {
     const int gi = P.qbead[s.qoff + qi];
-    xi0 = PX[gi]; xi1 = PX[n + gi]; xi2 = PX[2 * n + gi];
+    xi0 = Y[gi]; xi1 = Y[n + gi]; xi2 = Y[2 * n + gi];
     chi = P.charge[gi];
   }
   const int jt = J * kTile;
   {
     const bool vj = jt + lane < s.nq;
     const int gj = vj ? P.qbead[s.qoff + jt + lane] : 0;
-    SX[lane] = vj ? PX[gj] : 0.0; SY[lane] = vj ? PX[n + gj] : 0.0; SZ[lane] = vj ? PX[2 * n + gj] : 0.0;
+    SX[lane] = vj ? Y[gj] : 0.0; SY[lane] = vj ? Y[n + gj] : 0.0; SZ[lane] = vj ? Y[2 * n + gj] : 0.0;
     SQ[lane] = vj ? (float)P.charge[gj] : 0.0f;
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -805,7 +622,7 @@ __device__ inline void ele_pair_task(const DevPtrs &P, const DSys &s, const doub
   const float qiC = (float)(chi * s.ele_coef);
   const double cut2 = FORM == 0 ? s.dh_cut2 : s.cl_cut2;
   const uint32_t *row = P.qmask + s.moff + (int64_t)(valid ? qi : 0) * s.mwords;
-  const uint8_t *mrow = P.qmirror + s.mioff + (int64_t)(valid ? qi : 0) * s.nq + jt;
+  const uint8_t *mrow = s.periodic ? P.qmirror + s.mioff + (int64_t)(valid ? qi : 0) * s.nq + jt : nullptr;
   uint64_t m64 = 0;
   if (valid) {
     const int wb = jt >> 5;
@@ -821,10 +638,11 @@ __device__ inline void ele_pair_task(const DevPtrs &P, const DSys &s, const doub
     if (last_half) bit = bit && lane < 32;
     // a step with no listed pair in the wave adds zeros: only the sums move
     if (__any(bit)) {
-      const int code = bit ? (int)mrow[jl] : 0;
-      const double v0 = SX[jl] - xi0 + mirror_shift(s, code, 0);
-      const double v1 = SY[jl] - xi1 + mirror_shift(s, code, 1);
-      const double v2 = SZ[jl] - xi2 + mirror_shift(s, code, 2);
+      double v0 = SX[jl] - xi0, v1 = SY[jl] - xi1, v2 = SZ[jl] - xi2;
+      if (s.periodic) {
+        const int code = bit ? (int)mrow[jl] : 0;
+        v0 += mirror_shift(s, code, 0); v1 += mirror_shift(s, code, 1); v2 += mirror_shift(s, code, 2);
+      }
       const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
       const bool on = bit && d2 <= cut2;
       float dv = ele_dv<FORM>(s, qiC * SQ[jl], on ? (float)d2 : 1.0f, nclamp, on);
@@ -856,21 +674,17 @@ __device__ inline void ele_pair_task(const DevPtrs &P, const DSys &s, const doub
   if (nclamp) atomicAdd(&P.counters[1], (unsigned long long)nclamp);
 }
 
-// force_ele_DH.F90:68-118, force_ele_coulomb.F90:63-84 and the real-space
-// loop of force_ele_coulomb_ewld.F90:118-137, one block per charge tile.
-// Non-periodic systems take each pair once (ele_tile_once); periodic ones
-// sweep the full row of each charge (ele_tile).
-// The force pass, one launch with four kinds of blocks: four systems'
-// tertiary stacks and hbonds per block (terms_wave, one wave each), charge
-// tiles, periodic chunk-pair tasks (four per block), then the bonded terms,
-// 256 lanes per block (bonded_lane).  The memory-bound term blocks overlap the compute-bound
-// pair sweep; k_main gathers all of their outputs.
+// The force pass, one launch with three kinds of blocks: four systems'
+// tertiary stacks and hbonds per block (terms_wave, one wave each), four
+// electrostatic chunk pairs per block (ele_pair_task, one wave each), then
+// the bonded terms, 256 lanes per block (bonded_lane).  k_main gathers all of
+// their outputs.
 __global__ __launch_bounds__(kTile * kWaves) __attribute__((amdgpu_waves_per_eu(4)))
-void k_force(DevPtrs P, const double *X, const double *PX, const int32_t *systask, int ntask, const Tile *qtiles,
-             int nqt, const EwTask *ptasks, int npt, BondedRange br, int32_t t_local) {
+void k_force(DevPtrs P, const double *X, const double *PX, const int32_t *systask, int ntask,
+             const EwTask *ptasks, int npt, BondedRange br, int32_t t_local) {
   __shared__ EleLds lds;
   int b = (int)blockIdx.x;
-  const int npb = (npt + kWaves - 1) / kWaves;   // blocks of four pair tasks
+  const int npb = (npt + kWaves - 1) / kWaves;   // blocks of four chunk pairs
   const int ntb = (ntask + kWaves - 1) / kWaves;   // blocks of four system terms passes
   if (b < ntb) {
     const int task = b * kWaves + (int)threadIdx.x / kTile;
@@ -878,45 +692,22 @@ void k_force(DevPtrs P, const double *X, const double *PX, const int32_t *systas
     return;
   }
   b -= ntb;
-  if (b >= nqt + npb) {
-    bonded_lane(P, X, PX, br, (b - nqt - npb) * (kTile * kWaves) + (int)threadIdx.x);
+  if (b >= npb) {
+    bonded_lane(P, X, PX, br, (b - npb) * (kTile * kWaves) + (int)threadIdx.x);
     return;
   }
-  if (b >= nqt) {
-    const int w = (int)threadIdx.x / kTile;
-    const int task = (b - nqt) * kWaves + w;
-    const int dir = wave_dir();
-    if (task >= npt) return;
-    const EwTask tk = ptasks[task];
-    const DSys s = P.sys[tk.sys];
-    double *SX = lds.sx[w], *SY = lds.sy[w], *SZ = lds.sz[w];
-    float *SQ = lds.sq[w];
-    if (s.ele_form == 0) ele_pair_task<0>(P, s, PX, tk.a, tk.b, dir, SX, SY, SZ, SQ);
-    else if (s.ele_form == 1) ele_pair_task<1>(P, s, PX, tk.a, tk.b, dir, SX, SY, SZ, SQ);
-    else ele_pair_task<2>(P, s, PX, tk.a, tk.b, dir, SX, SY, SZ, SQ);
-    return;
-  }
-  const Tile tl = qtiles[b];
-  const DSys s = P.sys[tl.sys];
-  if (s.ele_pairs) return;   // taken by the pair tasks
-  const int dir = s.ele_once ? wave_dir() : 0;
-  if (s.ele_once && dir != 0) {
-    if (s.ele_form == 0) ele_tile_once<0>(P, s, X, tl.start, dir, lds);
-    else ele_tile_once<1>(P, s, X, tl.start, dir, lds);
-    return;
-  }
-  if (s.ele_once) {
-    // wave_rot is not a rotation of the whole wave on this device: the full
-    // sweep below computes every force; this block's partials are zero
-    const int I = tl.start / kTile;
-    for (int J = I + 1; J < s.nqt; ++J) {
-      double *pj = P.pj + s.pj_off + ele_pair_index(I, J, s.nqt) * (3 * kTile);
-      for (int k = threadIdx.x; k < 3 * kTile; k += blockDim.x) pj[k] = 0.0;
-    }
-  }
-  if (s.ele_form == 0) ele_tile<0>(P, s, X, PX, tl.start, lds);
-  else if (s.ele_form == 1) ele_tile<1>(P, s, X, PX, tl.start, lds);
-  else if (s.ele_form == 2) ele_tile<2>(P, s, X, PX, tl.start, lds);
+  const int w = (int)threadIdx.x / kTile;
+  const int task = b * kWaves + w;
+  const int dir = wave_dir();
+  if (task >= npt) return;
+  const EwTask tk = ptasks[task];
+  const DSys s = P.sys[tk.sys];
+  double *SX = lds.sx[w], *SY = lds.sy[w], *SZ = lds.sz[w];
+  float *SQ = lds.sq[w];
+  const double *Y = s.periodic ? PX : X;
+  if (s.ele_form == 0) ele_pair_task<0>(P, s, Y, tk.a, tk.b, dir, SX, SY, SZ, SQ);
+  else if (s.ele_form == 1) ele_pair_task<1>(P, s, Y, tk.a, tk.b, dir, SX, SY, SZ, SQ);
+  else ele_pair_task<2>(P, s, Y, tk.a, tk.b, dir, SX, SY, SZ, SQ);
 }
 
 // ------------------------------------------------------------------ Ewald reciprocal space
@@ -1136,54 +927,31 @@ __device__ inline double accumulate_displacement(const DevPtrs &P, int g, const 
   return d2;
 }
 
-// Electrostatic pair sums of the ele pass (chunk-pair task slots in task
-// order, or fdh then the pair-once partials in I order), then the Ewald
-// reciprocal partial forces in chunk order, added to F.
+// Electrostatic pair sums of the force pass (the chunk-pair task slots that
+// hold the charge, in task order, four tasks' loads in flight at a time),
+// then the Ewald reciprocal partial forces in chunk order, added to F.
 __device__ inline void ele_sums(const DevPtrs &P, const DSys &s, int g, double F[3]) {
   const int q = P.qidx[g];
-  if (q >= 0) {
-    if (s.ele_pairs) {
-      // the slots of the chunk-pair tasks that hold this charge, in task
-      // order, four tasks' loads in flight at a time
-      const int lq = q - s.qoff, C = lq / kTile, lc = lq % kTile;
-      for (int I0 = 0; I0 < s.nqt; I0 += 4) {
-        double f[4][3];
+  if (q < 0) return;
+  const int lq = q - s.qoff, C = lq / kTile, lc = lq % kTile;
+  for (int I0 = 0; I0 < s.nqt; I0 += 4) {
+    double f[4][3];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int I = min(I0 + u, s.nqt - 1);
-          const int a = I <= C ? I : C, bb = I <= C ? C : I;
-          const int slot = I < C ? 1 : 0;
-          const double *pp = P.pp + s.pp_off + ele_task_index(a, bb, s.nqt) * (6 * kTile) + slot * 3 * kTile + lc;
-          f[u][0] = pp[0]; f[u][1] = pp[kTile]; f[u][2] = pp[2 * kTile];
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-          if (I0 + u < s.nqt) { F[0] += f[u][0]; F[1] += f[u][1]; F[2] += f[u][2]; }
-      }
-    } else {
-      F[0] += P.fdh[q]; F[1] += P.fdh[(size_t)P.nqtot + q]; F[2] += P.fdh[2 * (size_t)P.nqtot + q];
+    for (int u = 0; u < 4; ++u) {
+      const int I = min(I0 + u, s.nqt - 1);
+      const int a = I <= C ? I : C, bb = I <= C ? C : I;
+      const int slot = I < C ? 1 : 0;
+      const double *pp = P.pp + s.pp_off + ele_task_index(a, bb, s.nqt) * (6 * kTile) + slot * 3 * kTile + lc;
+      f[u][0] = pp[0]; f[u][1] = pp[kTile]; f[u][2] = pp[2 * kTile];
     }
-    if (s.ele_once) {
-      // forces from the chunks I < J of the pair-once sweep, in I order
-      const int lq = q - s.qoff, J = lq / kTile, lj = lq % kTile;
-      for (int I0 = 0; I0 < J; I0 += 4) {
-        double f[4][3];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int I = min(I0 + u, J - 1);
-          const double *pj = P.pj + s.pj_off + ele_pair_index(I, J, s.nqt) * (3 * kTile) + lj;
-          f[u][0] = pj[0]; f[u][1] = pj[kTile]; f[u][2] = pj[2 * kTile];
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-          if (I0 + u < J) { F[0] += f[u][0]; F[1] += f[u][1]; F[2] += f[u][2]; }
-      }
-    }
-    if (s.ele_form == 2) {
-      const double *pf = P.pf + s.pf_off + (q - s.qoff);
-      for (int c = 0; c < s.nkc; ++c)
-        for (int d = 0; d < 3; ++d) F[d] += pf[((int64_t)c * 3 + d) * s.nq];
-    }
+    for (int u = 0; u < 4; ++u)
+      if (I0 + u < s.nqt) { F[0] += f[u][0]; F[1] += f[u][1]; F[2] += f[u][2]; }
+  }
+  if (s.ele_form == 2) {
+    const double *pf = P.pf + s.pf_off + lq;
+    for (int c = 0; c < s.nkc; ++c)
+      for (int d = 0; d < 3; ++d) F[d] += pf[((int64_t)c * 3 + d) * s.nq];
   }
 }
 
