@@ -49,8 +49,7 @@ def small_ions_box(seed=5, n=20, box=60.0, alpha=0.2, rcut=25.0, hmax=6, form=EL
 def coulomb_chain(seed=21, n=160, rcut=40.0, box=60.0):
     """A compact n-nt chain (phosphate charge -1) with plain Coulomb
     electrostatics cut at 40 A, no box: 160 nt give three charge chunks of 64
-    with a partial last one (each pair once per charge tile), 320 nt five
-    (one wave per chunk pair)."""
+    with a partial last one, 320 nt five."""
     rng = np.random.default_rng(seed)
     seq = T.random_sequence(rng, n)
     s = T.build_rna_system([(seq, [])], ele_form=ELE_COULOMB, cutoff_ele=rcut, phos_charge=-1.0,
