@@ -247,7 +247,8 @@ __device__ inline int pbneighbor(const DSys &s, double v[3]) {
 // shift of mirror code (setp_periodic.F90:80-115)
 __device__ inline double mirror_shift(const DSys &s, int code, int d) {
   int i = d == 0 ? code / 9 : (d == 1 ? (code / 3) % 3 : code % 3);
-  return i == 0 ? -s.box[d] : (i == 1 ? s.box[d] : 0.0);
+  // -L, +L or +0 for i = 0, 1, 2, without a branch
+  return (double)((i == 1) - (i == 0)) * s.box[d];
 }
 // the same pair seen from the other bead: swap -L and +L per component
 __device__ inline int mirror_flip(int code) {

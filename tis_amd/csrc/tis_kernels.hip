@@ -589,13 +589,17 @@ __host__ __device__ inline int64_t ele_task_index(int I, int J, int nch) {
   return (int64_t)I * (2 * nch - I + 1) / 2 + (J - I);
 }
 
-__device__ inline double rot_next(double v, int dir) {
-  if (dir != 0) return wave_rot(v);
+// the next value of the rotating sums: DPP when the whole-wave rotation is
+// available (decided once per wave, so the step loop has no branch), else
+// __shfl
+template <bool DPP>
+__device__ inline double rot_next(double v) {
+  if (DPP) return wave_rot(v);
   const int lane = threadIdx.x & (kTile - 1);
   return __shfl(v, (lane + 1) & (kTile - 1));
 }
 
-template <int FORM>
+template <int FORM, bool DPP>
 __device__ inline void ele_pair_task(const DevPtrs &P, const DSys &s, const double *Y, int I, int J, int dir,
                                      double *SX, double *SY, double *SZ, float *SQ) {
   const int lane = threadIdx.x & (kTile - 1);
@@ -651,7 +655,7 @@ __device__ inline void ele_pair_task(const DevPtrs &P, const DSys &s, const doub
       F0 -= dd * v0; F1 -= dd * v1; F2 -= dd * v2;
       G0 += dd * v0; G1 += dd * v1; G2 += dd * v2;
     }
-    G0 = rot_next(G0, dir); G1 = rot_next(G1, dir); G2 = rot_next(G2, dir);
+    G0 = rot_next<DPP>(G0); G1 = rot_next<DPP>(G1); G2 = rot_next<DPP>(G2);
   };
   if (__any(m64 != 0)) {
     if (diag) {
@@ -705,9 +709,15 @@ void k_force(DevPtrs P, const double *X, const double *PX, const int32_t *systas
   double *SX = lds.sx[w], *SY = lds.sy[w], *SZ = lds.sz[w];
   float *SQ = lds.sq[w];
   const double *Y = s.periodic ? PX : X;
-  if (s.ele_form == 0) ele_pair_task<0>(P, s, Y, tk.a, tk.b, dir, SX, SY, SZ, SQ);
-  else if (s.ele_form == 1) ele_pair_task<1>(P, s, Y, tk.a, tk.b, dir, SX, SY, SZ, SQ);
-  else ele_pair_task<2>(P, s, Y, tk.a, tk.b, dir, SX, SY, SZ, SQ);
+  if (dir != 0) {
+    if (s.ele_form == 0) ele_pair_task<0, true>(P, s, Y, tk.a, tk.b, dir, SX, SY, SZ, SQ);
+    else if (s.ele_form == 1) ele_pair_task<1, true>(P, s, Y, tk.a, tk.b, dir, SX, SY, SZ, SQ);
+    else ele_pair_task<2, true>(P, s, Y, tk.a, tk.b, dir, SX, SY, SZ, SQ);
+  } else {
+    if (s.ele_form == 0) ele_pair_task<0, false>(P, s, Y, tk.a, tk.b, dir, SX, SY, SZ, SQ);
+    else if (s.ele_form == 1) ele_pair_task<1, false>(P, s, Y, tk.a, tk.b, dir, SX, SY, SZ, SQ);
+    else ele_pair_task<2, false>(P, s, Y, tk.a, tk.b, dir, SX, SY, SZ, SQ);
+  }
 }
 
 // ------------------------------------------------------------------ Ewald reciprocal space
@@ -1133,6 +1143,7 @@ constexpr int kRl2Lds = 16;       // unit-pair radius tables up to 16 x 16 units
 struct ListLds {
   double sx[kTile], sy[kTile], sz[kTile];
   int su[kTile], stc[kTile];
+  BBox sb[kTile / 16];                 // boxes of the staged tile's 16-bead groups
   double rl2[kRl2Lds * kRl2Lds];
 };
 
@@ -1148,8 +1159,10 @@ __device__ inline double point_box_d2(const double p[3], const BBox &b) {
 
 // One wave per bead tile sweeps the system's bead tiles in order; tiles and
 // 16-bead groups whose boxes no bead of the tile can reach within the
-// largest list radius are skipped.  The unit-pair radii sit in LDS, so the
-// per-pair tests issue no dependent global loads.
+// largest list radius are skipped.  The tile boxes are tested 64 at a time
+// (one lane per tile), the group boxes are staged with the tile, and the
+// unit-pair radii sit in LDS, so neither the culling nor the per-pair tests
+// wait on dependent global loads.
 __device__ inline void list_exv_tile(const DevPtrs &P, const DSys &s, const double *X, const double *PX,
                                      const Tile tl, int it, ListLds &L) {
   auto &sx = L.sx; auto &sy = L.sy; auto &sz = L.sz; auto &su = L.su; auto &stc = L.stc;
@@ -1168,48 +1181,56 @@ __device__ inline void list_exv_tile(const DevPtrs &P, const DSys &s, const doub
   const BBox bi = P.bbox[it];
   const double cut2 = s.rl2max * (1.0 + 1e-9);
   int cnt = 0;
-  for (int jtile = 0; jtile < s.nbt; ++jtile) {
-    const int jt = jtile * kTile;
-    if (!s.periodic && box_dmin2(bi, P.bbox[s.bt0 + jtile]) > cut2) continue;
-    int lj = jt + lane;
-    if (lj < s.nmp) {
-      int gj = s.off + lj;
-      sx[lane] = Y[gj]; sy[lane] = Y[n + gj]; sz[lane] = Y[2 * n + gj];
-      su[lane] = P.unit[gj];
-      stc[lane] = (int)P.mptype[gj] | ((int)P.iclass[gj] << 8);
-    }
-    __syncthreads();
-    const int kend = min(kTile, s.nmp - jt);
-    for (int q0 = 0; q0 < kend; q0 += 16) {
-      if (!s.periodic) {
-        const bool need = valid && point_box_d2(xi, P.sbox[4 * (s.bt0 + jtile) + q0 / 16]) <= cut2;
-        if (!__any(need)) continue;
+  for (int j0 = 0; j0 < s.nbt; j0 += kTile) {
+    // tiles j0 .. j0 + 63 that a bead of this tile may reach, tested at once
+    const int jl = j0 + lane;
+    const bool reach = jl < s.nbt && (s.periodic || !(box_dmin2(bi, P.bbox[s.bt0 + jl]) > cut2));
+    uint64_t todo = __ballot(reach);
+    while (todo) {
+      const int jtile = j0 + __ffsll((unsigned long long)todo) - 1;
+      todo &= todo - 1;
+      const int jt = jtile * kTile;
+      int lj = jt + lane;
+      if (lj < s.nmp) {
+        int gj = s.off + lj;
+        sx[lane] = Y[gj]; sy[lane] = Y[n + gj]; sz[lane] = Y[2 * n + gj];
+        su[lane] = P.unit[gj];
+        stc[lane] = (int)P.mptype[gj] | ((int)P.iclass[gj] << 8);
       }
-      const int q1 = min(q0 + 16, kend);
-      if (valid)
-        for (int k = q0; k < q1; ++k) {
-          const int lj2 = jt + k;
-          if (lj2 == li) continue;
-          const bool own_lo = li < lj2;
-          double v[3];
-          if (own_lo) { v[0] = __dadd_rn(sx[k], -xi[0]); v[1] = __dadd_rn(sy[k], -xi[1]); v[2] = __dadd_rn(sz[k], -xi[2]); }
-          else { v[0] = __dadd_rn(xi[0], -sx[k]); v[1] = __dadd_rn(xi[1], -sy[k]); v[2] = __dadd_rn(xi[2], -sz[k]); }
-          const int code = s.periodic ? pbneighbor(s, v) : 0;
-          const double d2 = sq3_exact(v[0], v[1], v[2]);
-          if (!(d2 < s.rl2max)) continue;   // rl2max bounds every unit-pair radius
-          const int uj = su[k];
-          const int ulo = own_lo ? ui : uj, uhi = own_lo ? uj : ui;
-          const double r2 = rl2_lds ? L.rl2[ulo * s.nunit + uhi] : P.rl2[s.uoff + ulo * s.nunit + uhi];
-          if (!(d2 < r2)) continue;
-          const int tj = stc[k] & 0xFF, cj = stc[k] >> 8;
-          const bool keep = own_lo ? exv_keep(s, ui, uj, ci, ti, li, lj2) : exv_keep(s, uj, ui, cj, tj, lj2, li);
-          if (!keep) continue;
-          const int oc = own_lo ? code : mirror_flip(code);
-          if (cnt < P.nb_cap) P.nb[(size_t)cnt * n + g] = ((uint32_t)oc << kMirrorShift) | (uint32_t)(s.off + lj2);
-          ++cnt;
+      if (!s.periodic && lane < kTile / 16) L.sb[lane] = P.sbox[4 * (s.bt0 + jtile) + lane];
+      __syncthreads();
+      const int kend = min(kTile, s.nmp - jt);
+      for (int q0 = 0; q0 < kend; q0 += 16) {
+        if (!s.periodic) {
+          const bool need = valid && point_box_d2(xi, L.sb[q0 / 16]) <= cut2;
+          if (!__any(need)) continue;
         }
+        const int q1 = min(q0 + 16, kend);
+        if (valid)
+          for (int k = q0; k < q1; ++k) {
+            const int lj2 = jt + k;
+            if (lj2 == li) continue;
+            const bool own_lo = li < lj2;
+            double v[3];
+            if (own_lo) { v[0] = __dadd_rn(sx[k], -xi[0]); v[1] = __dadd_rn(sy[k], -xi[1]); v[2] = __dadd_rn(sz[k], -xi[2]); }
+            else { v[0] = __dadd_rn(xi[0], -sx[k]); v[1] = __dadd_rn(xi[1], -sy[k]); v[2] = __dadd_rn(xi[2], -sz[k]); }
+            const int code = s.periodic ? pbneighbor(s, v) : 0;
+            const double d2 = sq3_exact(v[0], v[1], v[2]);
+            if (!(d2 < s.rl2max)) continue;   // rl2max bounds every unit-pair radius
+            const int uj = su[k];
+            const int ulo = own_lo ? ui : uj, uhi = own_lo ? uj : ui;
+            const double r2 = rl2_lds ? L.rl2[ulo * s.nunit + uhi] : P.rl2[s.uoff + ulo * s.nunit + uhi];
+            if (!(d2 < r2)) continue;
+            const int tj = stc[k] & 0xFF, cj = stc[k] >> 8;
+            const bool keep = own_lo ? exv_keep(s, ui, uj, ci, ti, li, lj2) : exv_keep(s, uj, ui, cj, tj, lj2, li);
+            if (!keep) continue;
+            const int oc = own_lo ? code : mirror_flip(code);
+            if (cnt < P.nb_cap) P.nb[(size_t)cnt * n + g] = ((uint32_t)oc << kMirrorShift) | (uint32_t)(s.off + lj2);
+            ++cnt;
+          }
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
   if (valid) {
     P.nb_cnt[g] = cnt;
@@ -1233,46 +1254,51 @@ __device__ inline void list_ele_tile(const DevPtrs &P, const DSys &s, const doub
   if (valid) ld3(Y, n, P.qbead[s.qoff + qi], xi);
   uint32_t *row = P.qmask + s.moff + (int64_t)(valid ? qi : 0) * s.mwords;
   const BBox bi = P.qbbox[it];
-  for (int jtile = 0; jtile < s.nqt; ++jtile) {
-    const int jt = jtile * kTile;
-    const int kend = min(kTile, s.nq - jt);
-    int mode = 2;  // 0 none, 1 all, 2 test
-    if (!s.periodic) {
-      const BBox bj = P.qbbox[s.qt0 + jtile];
-      if (box_dmin2(bi, bj) > s.ele_rl2 * (1.0 + 1e-9)) mode = 0;
-      else if (box_dmax2(bi, bj) < s.ele_rl2 * (1.0 - 1e-9)) mode = 1;
+  for (int j0 = 0; j0 < s.nqt; j0 += kTile) {
+    // mode of the tile pairs (it, j0 + lane), decided at once: 0 none, 1 all,
+    // 2 test
+    int lmode = 2;
+    if (!s.periodic && j0 + lane < s.nqt) {
+      const BBox bj = P.qbbox[s.qt0 + j0 + lane];
+      if (box_dmin2(bi, bj) > s.ele_rl2 * (1.0 + 1e-9)) lmode = 0;
+      else if (box_dmax2(bi, bj) < s.ele_rl2 * (1.0 - 1e-9)) lmode = 1;
     }
-    uint32_t w[2] = {0u, 0u};
-    if (mode == 1) {
-      w[0] = kend >= 32 ? 0xFFFFFFFFu : ((1u << kend) - 1u);
-      w[1] = kend >= 64 ? 0xFFFFFFFFu : (kend > 32 ? ((1u << (kend - 32)) - 1u) : 0u);
-      if (qi >= jt && qi < jt + kend) w[(qi - jt) >> 5] &= ~(1u << ((qi - jt) & 31));
-    } else if (mode == 2) {
-      int qj = jt + lane;
-      if (qj < s.nq) {
-        int gj = P.qbead[s.qoff + qj];
-        sx[lane] = Y[gj]; sy[lane] = Y[n + gj]; sz[lane] = Y[2 * n + gj];
-      }
-      __syncthreads();
-      if (valid)
-        for (int k = 0; k < kend; ++k) {
-          int qj2 = jt + k;
-          if (qj2 == qi) continue;
-          const bool own_lo = qi < qj2;
-          double v[3];
-          if (own_lo) { v[0] = __dadd_rn(sx[k], -xi[0]); v[1] = __dadd_rn(sy[k], -xi[1]); v[2] = __dadd_rn(sz[k], -xi[2]); }
-          else { v[0] = __dadd_rn(xi[0], -sx[k]); v[1] = __dadd_rn(xi[1], -sy[k]); v[2] = __dadd_rn(xi[2], -sz[k]); }
-          int code = s.periodic ? pbneighbor(s, v) : 0;
-          double d2 = sq3_exact(v[0], v[1], v[2]);
-          if (!(d2 < s.ele_rl2)) continue;
-          w[k >> 5] |= 1u << (k & 31);
-          if (s.periodic) P.qmirror[s.mioff + (int64_t)qi * s.nq + qj2] = (uint8_t)(own_lo ? code : mirror_flip(code));
+    for (int jtile = j0; jtile < min(s.nqt, j0 + kTile); ++jtile) {
+      const int jt = jtile * kTile;
+      const int kend = min(kTile, s.nq - jt);
+      const int mode = __shfl(lmode, jtile - j0);
+      uint32_t w[2] = {0u, 0u};
+      if (mode == 1) {
+        w[0] = kend >= 32 ? 0xFFFFFFFFu : ((1u << kend) - 1u);
+        w[1] = kend >= 64 ? 0xFFFFFFFFu : (kend > 32 ? ((1u << (kend - 32)) - 1u) : 0u);
+        if (qi >= jt && qi < jt + kend) w[(qi - jt) >> 5] &= ~(1u << ((qi - jt) & 31));
+      } else if (mode == 2) {
+        int qj = jt + lane;
+        if (qj < s.nq) {
+          int gj = P.qbead[s.qoff + qj];
+          sx[lane] = Y[gj]; sy[lane] = Y[n + gj]; sz[lane] = Y[2 * n + gj];
         }
-      __syncthreads();
-    }
-    if (valid) {
-      row[jt >> 5] = w[0];
-      if ((jt >> 5) + 1 < s.mwords) row[(jt >> 5) + 1] = w[1];
+        __syncthreads();
+        if (valid)
+          for (int k = 0; k < kend; ++k) {
+            int qj2 = jt + k;
+            if (qj2 == qi) continue;
+            const bool own_lo = qi < qj2;
+            double v[3];
+            if (own_lo) { v[0] = __dadd_rn(sx[k], -xi[0]); v[1] = __dadd_rn(sy[k], -xi[1]); v[2] = __dadd_rn(sz[k], -xi[2]); }
+            else { v[0] = __dadd_rn(xi[0], -sx[k]); v[1] = __dadd_rn(xi[1], -sy[k]); v[2] = __dadd_rn(xi[2], -sz[k]); }
+            int code = s.periodic ? pbneighbor(s, v) : 0;
+            double d2 = sq3_exact(v[0], v[1], v[2]);
+            if (!(d2 < s.ele_rl2)) continue;
+            w[k >> 5] |= 1u << (k & 31);
+            if (s.periodic) P.qmirror[s.mioff + (int64_t)qi * s.nq + qj2] = (uint8_t)(own_lo ? code : mirror_flip(code));
+          }
+        __syncthreads();
+      }
+      if (valid) {
+        row[jt >> 5] = w[0];
+        if ((jt >> 5) + 1 < s.mwords) row[(jt >> 5) + 1] = w[1];
+      }
     }
   }
 }
