@@ -92,6 +92,27 @@ def test_batch_equals_alone_bitwise():
         assert np.array_equal(fa, fb[k])
 
 
+def test_mixed_forms_batch_equals_alone_bitwise():
+    """Every parity system (Debye-Hueckel and Coulomb chains, a periodic DH
+    system, Ewald boxes) in one batch: the electrostatic chunk-pair tasks of
+    all forms share one force pass, in the batch's order, and each system's
+    forces equal its forces alone, bitwise."""
+    batch = Engine([s for _, s, _ in CASES], keys=list(range(len(CASES))), seed=3)
+    for k, (_, s, x) in enumerate(CASES):
+        batch.set_coords(k, x)
+    batch.neighbor(-1)
+    fb = batch.force_all()
+    for k, (name, s, x) in enumerate(CASES):
+        print(f"[mixed batch] {name}", flush=True)
+        alone = Engine([s], keys=[k], seed=3)
+        alone.set_coords(0, x)
+        alone.neighbor(0)
+        fa = alone.force_sumup(0)
+        assert np.array_equal(fa, fb[k]), name
+        alone.close()
+    batch.close()
+
+
 def test_langevin_matches_oracle_short():
     s, x = hairpin64(0, "helix")
     v = np.zeros_like(x)
