@@ -548,16 +548,19 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
         for (int J = I; J < d.nqt; ++J) ptask.push_back(EwTask{is, I, J});
     }
     h.pt_count = (int32_t)ptask.size() - h.pt_begin;
-    // tile pairs for the one-step (triggered) and periodic list builds
+    // tile pairs of the periodic list builds (k_lpair; periodic systems
+    // only: their number and the count buffer grow as nbt^2)
     d.lp_off = lp_off;
-    lp_off += (int64_t)d.nbt * d.nbt;
     h.bp_begin = (int32_t)bpair.size();
-    for (int a = 0; a < d.nbt; ++a)
-      for (int b = 0; b < d.nbt; ++b) bpair.push_back(EwTask{is, a, b});
-    h.bp_count = (int32_t)bpair.size() - h.bp_begin;
     h.qp_begin = (int32_t)qpair.size();
-    for (int a = 0; a < d.nqt; ++a)
-      for (int b = 0; b < d.nqt; ++b) qpair.push_back(EwTask{is, a, b});
+    if (s.p.periodic) {
+      lp_off += (int64_t)d.nbt * d.nbt;
+      for (int a = 0; a < d.nbt; ++a)
+        for (int b = 0; b < d.nbt; ++b) bpair.push_back(EwTask{is, a, b});
+      for (int a = 0; a < d.nqt; ++a)
+        for (int b = 0; b < d.nqt; ++b) qpair.push_back(EwTask{is, a, b});
+    }
+    h.bp_count = (int32_t)bpair.size() - h.bp_begin;
     h.qp_count = (int32_t)qpair.size() - h.qp_begin;
     h.has_task = (s.nhb > 0 || s.ntst > 0);
     if (h.has_task) { c->systask_of[is] = (int32_t)systask.size(); systask.push_back(is); }
