@@ -233,6 +233,63 @@ def bdhi_leg(sizes=(300, 1000, 3000), steps=50, refactor=10):
     return out
 
 
+def cpu_baseline_reference(systems, coords, budget_s=15.0):
+    """The reference's own Fortran routines (oracle/_ref, compiled from its
+    sources) on this host, one thread, over chains of the same workload until
+    budget_s: per step the terms force_sumup evaluates for these chains
+    (force_bond, force_bangle, force_dtrna_stack_nlocal + force_dtrna_stack,
+    force_dtrna_hbond15, force_exv_dt15, force_ele_DH) and neighbor() every
+    10 steps, at the sample's coordinates (the O(N) Langevin update is not
+    counted).  None when the library is absent."""
+    import oracle
+    if not oracle.RefLib.available():
+        return None
+    R = oracle.RefLib()
+    steps = 300
+    terms = [R.TERMS[t] for t in ("bond", "angle", "stack", "hbond", "exv", "dh")]
+    order = list(range(0, len(systems), max(1, len(systems) // 64)))
+    res = {}
+
+    def run():
+        # one thread with a large stack for the whole loop (the reference's
+        # list builders keep pair buffers on the stack)
+        t0 = time.perf_counter()
+        bead_steps = nch = 0
+        for k in order:
+            s = systems[k]
+            x = np.ascontiguousarray(coords[k], dtype=np.float64)
+            f = np.zeros((s.nmp, 3))
+            R.load(s)
+            for t in range(steps):
+                if t % 10 == 0:
+                    R.L.tisref_neighbor(oracle.fp(x))
+                for term in terms:
+                    R.L.tisref_force_term(term, oracle.fp(f))
+            bead_steps += s.nmp * steps
+            nch += 1
+            if time.perf_counter() - t0 > budget_s:
+                break
+        res.update(dt=time.perf_counter() - t0, bead_steps=bead_steps, nch=nch)
+
+    # the reference writes a warning line to standard output for each force
+    # clamp: send its output to /dev/null while it runs (this script's
+    # standard output carries the one JSON line)
+    sys.stdout.flush()
+    saved = os.dup(1)
+    devnull = os.open(os.devnull, os.O_WRONLY)
+    os.dup2(devnull, 1)
+    try:
+        oracle._on_big_stack(run)
+        R.L.tisref_flush()
+    finally:
+        os.dup2(saved, 1)
+        os.close(saved)
+        os.close(devnull)
+    return {"value": res["bead_steps"] / res["dt"], "unit": "bead-steps/s", "cores": 1, "kind": "reference",
+            "sample": f"{res['nch']} systems of the workload x {steps} steps of the reference's force routines "
+                      f"and neighbor() every 10 steps (oracle/_ref, 1 thread, fixed coordinates), {res['dt']:.1f} s"}
+
+
 def cpu_baseline(systems, coords, budget_s=15.0):
     """Oracle Langevin (FP64, O(N^2) list build, rebuild every 10 steps) on
     this host, one core, over chains of the same workload until budget_s."""
@@ -419,7 +476,10 @@ def main():
         print(f"[bench] ewald: {ewald['ms_per_step']:.4f} ms/step, force err {ewald['force_max_rel_err_vs_fp64']:.2e}",
               file=sys.stderr, flush=True)
     bdhi = None if args.no_bdhi else bdhi_leg()
-    cpu = None if args.no_cpu else cpu_baseline(systems, coords, args.cpu_budget)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_reference(systems, coords, args.cpu_budget) or cpu_baseline(systems, coords,
+                                                                                       args.cpu_budget)
     out = {
         "metric": "batched bead-steps/s vs plain GPU @ force err <=1e-5 vs FP64; p50/p99 job latency",
         "value": value,

@@ -358,6 +358,7 @@ class RefLib:
         L.tisref_hydro.argtypes = [C.c_int32, _f64p, _f64p, C.c_double, C.c_double, C.c_int32, _f64p, _f64p]
         L.tisref_bd_forces.argtypes = [C.c_int32, _f64p, C.c_int32, _i32p, _f64p, _f64p, _f64p, C.c_int32,
                                        _i32p, _f64p, _f64p, C.c_double, _f64p]
+        L.tisref_flush.argtypes = []
         self.sys = None
 
     def load(self, system):

@@ -617,6 +617,12 @@ contains
     beta = beta_hbond15
   end subroutine tisref_temperature
 
+  ! Flush the standard output unit (the reference's warnings, e.g. the force
+  ! clamps of force_exv_dt15, are written there).
+  subroutine tisref_flush() bind(C, name="tisref_flush")
+    flush(6)
+  end subroutine tisref_flush
+
   integer function ifunc_seq2id_local()
     integer :: ifunc_seq2id
     ifunc_seq2id_local = ifunc_seq2id('P  ')
