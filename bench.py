@@ -63,9 +63,10 @@ def build_workload(nchains=512, nrep=64, seed=SEED, small=False):
 
 def algorithmic_bytes(systems, eng):
     """Algorithmic HBM bytes of one step, split by kernel.  k_main: per bead
-    36 (position read + force write) + 212 (integrator state), per exv list
-    entry 8.  k_force: per DH list entry 12, per term bond 16, angle 20,
-    stack 56, hbond and tertiary stack 76 (counts at the current lists)."""
+    169 (read x, v, a and the Langevin class byte; write F, v, a and the next
+    x), per exv list entry 8.  k_force: per DH list entry 12, per term bond
+    16, angle 20, stack 56, hbond and tertiary stack 76 (counts at the
+    current lists)."""
     nb = sum(s.nmp for s in systems)
     nexv = nele = 0
     for k, s in enumerate(systems):
@@ -76,7 +77,7 @@ def algorithmic_bytes(systems, eng):
     nst = sum(len(s.st_mp) for s in systems)
     nhb = sum(len(s.hb_mp) for s in systems)
     ntst = sum(len(s.tst_mp) for s in systems)
-    main = nb * (36 + 212) + 8 * nexv
+    main = nb * 169 + 8 * nexv
     force = 12 * nele + 16 * nbd + 20 * nba + 56 * nst + 76 * (nhb + ntst)
     return {"beads": nb, "exv_pairs": nexv, "ele_pairs": nele, "main_bytes": main, "force_bytes": force}
 

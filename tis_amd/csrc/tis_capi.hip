@@ -52,6 +52,7 @@ struct HostSys {
   int32_t bp_begin = 0, bp_count = 0;    // bead tile pairs of the tile-pair list build
   int32_t qp_begin = 0, qp_count = 0;    // charge tile pairs
   std::vector<double> mass, fric;
+  std::vector<double> lc_fric, lc_mass;   // its Langevin classes
   std::vector<int32_t> stack_ids;       // global stack indices
 };
 
@@ -384,7 +385,9 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
   std::vector<int8_t> mptype, iclass;
   std::vector<int32_t> unit, qidx, qbead, inc_off;
   std::vector<uint32_t> inc;
-  std::vector<double> exv_rad, charge, rcm, mass;
+  std::vector<double> exv_rad, charge, mass;
+  std::vector<uint8_t> lcl;
+  std::vector<LangClass> lcls;
   std::vector<float> exv_eps;
   std::vector<int32_t> bd_mp, ba_mp, st_mp, tst_mp, tst2st, tst_excl, hb_mp, hb_site, hb_tert, site_val,
       hb_sys, tst_sys;
@@ -448,9 +451,18 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
       species.push_back((uint8_t)spec_of((int8_t)s.mptype[i], (int8_t)s.iclass[i], s.exv_rad[i], s.exv_eps[i]));
       charge.push_back(s.charge[i]);
       mass.push_back(s.mass[i]);
-      rcm.push_back(1.0 / s.mass[i]);
       h.mass.push_back(s.mass[i]);
       h.fric.push_back(s.fric[i]);
+      // Langevin class: the system's (friction, mass) pairs in order of
+      // first appearance
+      int cl = 0;
+      while (cl < (int)h.lc_fric.size() && !(h.lc_fric[cl] == s.fric[i] && h.lc_mass[cl] == s.mass[i])) ++cl;
+      if (cl == (int)h.lc_fric.size()) {
+        if (cl == 256) return fail(c, TIS_ERR_ARG, "system %d: more than 256 (friction, mass) classes", is);
+        h.lc_fric.push_back(s.fric[i]);
+        h.lc_mass.push_back(s.mass[i]);
+      }
+      lcl.push_back((uint8_t)cl);
       // charges of the ele list: |q| >= ZERO_JUDGE (neighbor_list_ele.F90:134,147)
       if (s.p.ele_form != TIS_ELE_NONE && std::fabs(s.charge[i]) >= 1.0e-6) {
         qidx.push_back(qoff + nq);
@@ -460,6 +472,11 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
         qidx.push_back(-1);
       }
     }
+    // its Langevin classes (constants r[] set by tis_langevin_init)
+    d.lc0 = (int32_t)lcls.size();
+    d.nlc = (int32_t)h.lc_fric.size();
+    for (int k = 0; k < d.nlc; ++k)
+      lcls.push_back(LangClass{{0.0, 0.0, 0.0, 0.0}, 1.0 / h.lc_mass[k], h.lc_fric[k], h.lc_mass[k], 0.0});
     d.nq = nq;
     d.mwords = (nq + 31) / 32;
     d.moff = moff;
@@ -648,22 +665,15 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
   rc = dalloc(c, &P.v, 3 * (size_t)off); if (rc) return rc;
   rc = dalloc(c, &P.a, 3 * (size_t)off); if (rc) return rc;
   rc = dalloc(c, &P.f, 3 * (size_t)off); if (rc) return rc;
-  double *rlan = nullptr;
-  rc = dalloc(c, &rlan, 4 * (size_t)off); if (rc) return rc;
-  P.rlan = rlan;
-  double *d_rcm, *d_mass, *d_rad, *d_charge, *d_rl2;
+  uint8_t *d_lcl;
+  UP(d_lcl, lcl); P.lcl = d_lcl;
+  UP(P.lcls, lcls);
+  double *d_mass, *d_rad, *d_charge, *d_rl2;
   int8_t *d_mpt, *d_icl;
   int32_t *d_unit, *d_qidx, *d_qbead, *d_incoff;
   uint32_t *d_inc;
   float *d_eps;
-  UP(d_rcm, rcm); UP(d_mass, mass); UP(d_rad, exv_rad); UP(d_charge, charge); UP(d_rl2, rl2);
-  {
-    std::vector<double> fr;
-    for (int is = 0; is < nsys; ++is) fr.insert(fr.end(), c->hs[is].fric.begin(), c->hs[is].fric.end());
-    double *d_fric;
-    UP(d_fric, fr);
-    P.fric = d_fric;
-  }
+  UP(d_mass, mass); UP(d_rad, exv_rad); UP(d_charge, charge); UP(d_rl2, rl2);
   UP(d_mpt, mptype); UP(d_icl, iclass); UP(d_unit, unit); UP(d_qidx, qidx); UP(d_qbead, qbead);
   UP(d_incoff, inc_off); UP(d_inc, inc); UP(d_eps, exv_eps);
   int32_t *d_pos;
@@ -679,7 +689,7 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
   rc = dalloc(c, &P.qbbox, (size_t)c->nqtile); if (rc) return rc;
   rc = dalloc(c, &P.sbox, 4 * (size_t)c->nbtile); if (rc) return rc;
   rc = dalloc(c, &P.lcnt, (size_t)std::max<int64_t>(lp_off, 1) * kTile); if (rc) return rc;
-  P.rcm = d_rcm; P.mass = d_mass; P.exv_rad = d_rad; P.charge = d_charge; P.rl2 = d_rl2;
+  P.mass = d_mass; P.exv_rad = d_rad; P.charge = d_charge; P.rl2 = d_rl2;
   P.mptype = d_mpt; P.iclass = d_icl; P.unit = d_unit; P.qidx = d_qidx; P.qbead = d_qbead;
   P.inc_off = d_incoff; P.inc = d_inc; P.exv_eps = d_eps;
   int32_t *i32; double *f64;
@@ -783,17 +793,14 @@ int32_t tis_set_system_T(tis_ctx *c, int32_t isys, const tis_system *s) {
     HIPCHK(c, hipMemcpyAsync((void *)(c->P.st_coef + 4 * (size_t)h.d.soff), stc.data(), stc.size() * sizeof(double),
                              hipMemcpyHostToDevice, c->stream));
   if (c->lv_init) {
-    const int n = c->ntot;
-    for (int k = 0; k < 4; ++k) {
-      std::vector<double> r(h.d.nmp);
-      for (int i = 0; i < h.d.nmp; ++i) {
-        double rr[4];
-        langevin_consts(c->tstep, s->tempk, h.fric[i], h.mass[i], rr);
-        r[i] = rr[k];
-      }
-      HIPCHK(c, hipMemcpyAsync((void *)(c->P.rlan + (size_t)k * n + h.d.off), r.data(), r.size() * sizeof(double),
-                               hipMemcpyHostToDevice, c->stream));
+    std::vector<LangClass> lc(h.d.nlc);
+    for (int k = 0; k < h.d.nlc; ++k) {
+      lc[k] = LangClass{{0.0, 0.0, 0.0, 0.0}, 1.0 / h.lc_mass[k], h.lc_fric[k], h.lc_mass[k], 0.0};
+      langevin_consts(c->tstep, s->tempk, h.lc_fric[k], h.lc_mass[k], lc[k].r);
     }
+    if (!lc.empty())
+      HIPCHK(c, hipMemcpyAsync((void *)(c->P.lcls + h.d.lc0), lc.data(), lc.size() * sizeof(LangClass),
+                               hipMemcpyHostToDevice, c->stream));
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return TIS_OK;
@@ -1207,18 +1214,18 @@ int32_t tis_langevin_init(tis_ctx *c, double tstep, int32_t rebuild_every, uint6
   c->P.tstep = tstep;
   for (int k = 0; k < 2; ++k)
     if (c->graph[k]) { (void)hipGraphExecDestroy(c->graph[k]); c->graph[k] = nullptr; }
-  // per-bead constants
-  const int n = c->ntot;
-  std::vector<double> rl(4 * (size_t)n);
+  // Langevin constants of every system's (friction, mass) classes
+  std::vector<LangClass> lc;
   for (int is = 0; is < c->nsys; ++is) {
     const HostSys &h = c->hs[is];
-    for (int i = 0; i < h.d.nmp; ++i) {
-      double r[4];
-      langevin_consts(tstep, h.d.tempk, h.fric[i], h.mass[i], r);
-      for (int k = 0; k < 4; ++k) rl[(size_t)k * n + h.d.off + i] = r[k];
+    for (int k = 0; k < h.d.nlc; ++k) {
+      lc.push_back(LangClass{{0.0, 0.0, 0.0, 0.0}, 1.0 / h.lc_mass[k], h.lc_fric[k], h.lc_mass[k], 0.0});
+      langevin_consts(tstep, h.d.tempk, h.lc_fric[k], h.lc_mass[k], lc.back().r);
     }
   }
-  HIPCHK(c, hipMemcpyAsync((void *)c->P.rlan, rl.data(), rl.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (!lc.empty())
+    HIPCHK(c, hipMemcpyAsync((void *)c->P.lcls, lc.data(), lc.size() * sizeof(LangClass), hipMemcpyHostToDevice,
+                             c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_stepbase, &step0, sizeof(step0), hipMemcpyHostToDevice, c->stream));
   rc = rebuild_lists(c, -1);
   if (rc) return rc;

@@ -3,7 +3,7 @@
 // Layout in HBM (one context = a batch of systems, beads numbered globally):
 //   bead SoA:   x[2][3][N] FP64 (ping-pong: current / predicted positions),
 //               px[2][3][N] (periodic systems only: wrapped positions),
-//               v[3][N], a[3][N], f[3][N] FP64, rlan[4][N], rcm[N] FP64,
+//               v[3][N], a[3][N], f[3][N] FP64, a Langevin class byte,
 //               per-bead type/unit/class bytes, exv radius/epsilon, charge.
 //   incidence:  inc_off[N+1], inc[] -- the bonded terms each bead takes part
 //               in, so that a bead's force is gathered by one lane in a fixed
@@ -55,6 +55,7 @@ struct DSys {
   int32_t periodic;
   uint32_t key;              // identity in the random streams
   int32_t init_accel;        // mode-4 initialisation: 1 = new accelerations, 0 = keep
+  int32_t lc0, nlc;          // its Langevin classes [lc0, lc0 + nlc)
   uint64_t step_origin;      // global step at which this system's step count is zero
   int32_t nsep[3];           // P, S, B
   int32_t ele_form, dh_type;
@@ -79,6 +80,11 @@ struct DSys {
 
 // Ewald work item: (system, tile a, chunk b)
 struct EwTask { int32_t sys, a, b; };
+
+// Beads of one system with the same friction and mass share their Langevin
+// constants (time_integral_pre.F90:160-170): r[0] the noise amplitude c1,
+// r[1..3] c2, c3, c4, and 1/m.
+struct LangClass { double r[4], rcm, fric, mass, pad; };
 constexpr int kEwKChunk = 256;   // reciprocal vectors per block of the force pass
 
 struct Tile { int32_t sys, start; };
@@ -100,9 +106,9 @@ struct DevPtrs {
   // beads
   double *x[2], *px[2];      // [3][ntot] each
   double *v, *a, *f;         // [3][ntot]
-  const double *rlan;        // [4][ntot]
-  const double *fric;        // [ntot] friction / mass (Langevin)
-  const double *rcm, *mass;
+  LangClass *lcls;           // Langevin classes of every system (DSys::lc0 ..)
+  const uint8_t *lcl;        // [ntot] class of each bead within its system
+  const double *mass;
   const int8_t *mptype, *iclass;
   const int32_t *unit, *qidx;   // qidx: charge index (global) or -1
   const uint8_t *species;       // excluded-volume species of each bead

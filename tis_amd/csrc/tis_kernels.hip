@@ -836,8 +836,9 @@ struct BeadState { double r[4], rcm, v[3], a[3], x[3], px[3], o[3]; };
 __device__ inline void load_state(const DevPtrs &P, const DSys &s, int g, const double *X, const double *PX,
                                   BeadState &b) {
   const int n = P.ntot;
-  for (int k = 0; k < 4; ++k) b.r[k] = P.rlan[k * n + g];
-  b.rcm = P.rcm[g];
+  const LangClass &lc = P.lcls[s.lc0 + P.lcl[g]];
+  for (int k = 0; k < 4; ++k) b.r[k] = lc.r[k];
+  b.rcm = lc.rcm;
   for (int d = 0; d < 3; ++d) {
     b.v[d] = P.v[d * n + g];
     b.a[d] = P.a[d * n + g];
@@ -1066,7 +1067,7 @@ __device__ inline void wrap_bead(const DevPtrs &P, const DSys &s, const double *
     else if (tx < pmin) tx = tx + pw * (double)((int)((pmin - tx) / pw) + 1);
     PXc[d * n + g] = tx;
     if (with_pred) {
-      double dx = P.rlan[3 * n + g] * P.v[d * n + g] + P.tstep2 * P.a[d * n + g];
+      double dx = P.lcls[s.lc0 + P.lcl[g]].r[3] * P.v[d * n + g] + P.tstep2 * P.a[d * n + g];
       PXp[d * n + g] = tx + dx;
     }
   }
@@ -1833,13 +1834,15 @@ __global__ __launch_bounds__(256) void k_label_apply(DevPtrs P, LabelDev L, cons
   const DSys src = L.sys[lb];
   const int n = P.ntot;
   double *charge = const_cast<double *>(P.charge);
-  double *rlan = const_cast<double *>(P.rlan);
   double *stc = const_cast<double *>(P.st_coef);
   const double f = vscale ? vscale[k] : 1.0;
+  for (int q = threadIdx.x; q < t.nlc; q += blockDim.x) {
+    LangClass &c = P.lcls[t.lc0 + q];
+    if (P.tstep > 0.0) c.r[0] = sqrt(2.0 * c.fric * kBoltzKcalMol * src.tempk / (P.tstep * c.mass));
+  }
   for (int i = threadIdx.x; i < t.nmp; i += blockDim.x) {
     const int g = t.off + i;
     charge[g] = L.q[L.qoff[lb] + i];
-    if (P.tstep > 0.0) rlan[g] = sqrt(2.0 * P.fric[g] * kBoltzKcalMol * src.tempk / (P.tstep * P.mass[g]));
     if (vscale)
       for (int d = 0; d < 3; ++d) P.v[(size_t)d * n + g] *= f;
   }
