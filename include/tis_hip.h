@@ -57,7 +57,9 @@ const char *tis_last_error(const tis_ctx *ctx);
 /* Upload a batch of nsys systems (replaces any previous batch).  keys[s] is the
  * stable identity of system s in the random streams (noise of a system depends
  * only on (seed, key, step, bead), not on its slot in the batch); NULL means
- * key = s.  Coordinates start at zero; set them with tis_coords_set. */
+ * key = s.  Coordinates start at zero; set them with tis_state_set (which 0).
+ * TIS_ERR_ARG if a system has more than 256 distinct (friction, mass) pairs
+ * (its beads share Langevin constants by that pair). */
 int32_t tis_system_upload(tis_ctx *ctx, int32_t nsys, const tis_system *systems,
                           const uint32_t *keys, uint64_t seed);
 
