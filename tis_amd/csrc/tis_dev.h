@@ -20,7 +20,7 @@
 namespace tis {
 
 constexpr int kTile = 64;            // beads (or charges) per work tile = one wave
-constexpr int kWaves = 4;            // waves sharing one tile in k_main / k_dh
+constexpr int kWaves = 4;            // waves per block of the force pass, k_lpair and the Ewald force pass
 constexpr int kMirrorShift = 27;     // nb entry: (mirror << 27) | j
 constexpr uint32_t kIdxMask = (1u << 27) - 1u;
 constexpr double kDeMax = 50.0;      // const_physical.F90:35
