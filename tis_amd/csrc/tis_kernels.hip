@@ -1148,6 +1148,16 @@ struct ListLds {
   double rl2[kRl2Lds * kRl2Lds];
 };
 
+// The pair vector x_hi - x_lo of the reference's lower-index-first loops
+// from x_j - x_i (i: this lane's bead or charge): IEEE subtraction is
+// antisymmetric, so a negation gives the bits of the reversed difference;
+// non-periodic pairs need only |v|^2, which the sign does not change.
+__device__ inline void pair_vec(const DSys &s, double xj0, double xj1, double xj2, const double xi[3], bool own_lo,
+                                double v[3]) {
+  v[0] = __dadd_rn(xj0, -xi[0]); v[1] = __dadd_rn(xj1, -xi[1]); v[2] = __dadd_rn(xj2, -xi[2]);
+  if (s.periodic && !own_lo) { v[0] = -v[0]; v[1] = -v[1]; v[2] = -v[2]; }
+}
+
 // squared distance from a point to a box (FP64)
 __device__ inline double point_box_d2(const double p[3], const BBox &b) {
   double s2 = 0.0;
@@ -1164,8 +1174,9 @@ __device__ inline double point_box_d2(const double p[3], const BBox &b) {
 // (one lane per tile), the group boxes are staged with the tile, and the
 // unit-pair radii sit in LDS, so neither the culling nor the per-pair tests
 // wait on dependent global loads.
-__device__ inline void list_exv_tile(const DevPtrs &P, const DSys &s, const double *X, const double *PX,
-                                     const Tile tl, int it, ListLds &L) {
+template <bool RL2_LDS>
+__device__ inline void list_exv_tile_t(const DevPtrs &P, const DSys &s, const double *X, const double *PX,
+                                       const Tile tl, int it, ListLds &L) {
   auto &sx = L.sx; auto &sy = L.sy; auto &sz = L.sz; auto &su = L.su; auto &stc = L.stc;
   const int lane = threadIdx.x;
   const int li = tl.start + lane;
@@ -1173,8 +1184,7 @@ __device__ inline void list_exv_tile(const DevPtrs &P, const DSys &s, const doub
   const int n = P.ntot;
   const double *Y = s.periodic ? PX : X;
   const int g = s.off + (valid ? li : 0);
-  const bool rl2_lds = s.nunit <= kRl2Lds;
-  if (rl2_lds)
+  if (RL2_LDS)
     for (int k = lane; k < s.nunit * s.nunit; k += kTile) L.rl2[k] = P.rl2[s.uoff + k];
   double xi[3] = {0, 0, 0};
   int ui = 0, ti = 0, ci = 0;
@@ -1213,14 +1223,13 @@ __device__ inline void list_exv_tile(const DevPtrs &P, const DSys &s, const doub
             if (lj2 == li) continue;
             const bool own_lo = li < lj2;
             double v[3];
-            if (own_lo) { v[0] = __dadd_rn(sx[k], -xi[0]); v[1] = __dadd_rn(sy[k], -xi[1]); v[2] = __dadd_rn(sz[k], -xi[2]); }
-            else { v[0] = __dadd_rn(xi[0], -sx[k]); v[1] = __dadd_rn(xi[1], -sy[k]); v[2] = __dadd_rn(xi[2], -sz[k]); }
+            pair_vec(s, sx[k], sy[k], sz[k], xi, own_lo, v);
             const int code = s.periodic ? pbneighbor(s, v) : 0;
             const double d2 = sq3_exact(v[0], v[1], v[2]);
             if (!(d2 < s.rl2max)) continue;   // rl2max bounds every unit-pair radius
             const int uj = su[k];
             const int ulo = own_lo ? ui : uj, uhi = own_lo ? uj : ui;
-            const double r2 = rl2_lds ? L.rl2[ulo * s.nunit + uhi] : P.rl2[s.uoff + ulo * s.nunit + uhi];
+            const double r2 = RL2_LDS ? L.rl2[ulo * s.nunit + uhi] : P.rl2[s.uoff + ulo * s.nunit + uhi];
             if (!(d2 < r2)) continue;
             const int tj = stc[k] & 0xFF, cj = stc[k] >> 8;
             const bool keep = own_lo ? exv_keep(s, ui, uj, ci, ti, li, lj2) : exv_keep(s, uj, ui, cj, tj, lj2, li);
@@ -1237,6 +1246,14 @@ __device__ inline void list_exv_tile(const DevPtrs &P, const DSys &s, const doub
     P.nb_cnt[g] = cnt;
     if (cnt > P.nb_cap) atomicMax(&P.counters[5], (unsigned long long)cnt);
   }
+}
+
+// the unit-pair radii from LDS when the table fits there (a compile-time
+// choice: a run-time one makes every radius read a generic-pointer load)
+__device__ inline void list_exv_tile(const DevPtrs &P, const DSys &s, const double *X, const double *PX,
+                                     const Tile tl, int it, ListLds &L) {
+  if (s.nunit <= kRl2Lds) list_exv_tile_t<true>(P, s, X, PX, tl, it, L);
+  else list_exv_tile_t<false>(P, s, X, PX, tl, it, L);
 }
 
 // ele bit matrix (neighbor_list_ele.F90:123-210): bit (qi, qj) set when the
@@ -1286,8 +1303,7 @@ __device__ inline void list_ele_tile(const DevPtrs &P, const DSys &s, const doub
             if (qj2 == qi) continue;
             const bool own_lo = qi < qj2;
             double v[3];
-            if (own_lo) { v[0] = __dadd_rn(sx[k], -xi[0]); v[1] = __dadd_rn(sy[k], -xi[1]); v[2] = __dadd_rn(sz[k], -xi[2]); }
-            else { v[0] = __dadd_rn(xi[0], -sx[k]); v[1] = __dadd_rn(xi[1], -sy[k]); v[2] = __dadd_rn(xi[2], -sz[k]); }
+            pair_vec(s, sx[k], sy[k], sz[k], xi, own_lo, v);
             int code = s.periodic ? pbneighbor(s, v) : 0;
             double d2 = sq3_exact(v[0], v[1], v[2]);
             if (!(d2 < s.ele_rl2)) continue;
@@ -1339,8 +1355,9 @@ __device__ inline void wave_lds_sync() {
 
 // exv entries of bead tile ia (local index) against bead tile jb: counts
 // them (write = false) or writes them from slot `off` on; returns the count
-__device__ inline int exv_pair_scan(const DevPtrs &P, const DSys &s, const double *X, const double *PX, int ia,
-                                    int jb, bool write, int off, ListLds &L) {
+template <bool RL2_LDS>
+__device__ inline int exv_pair_scan_t(const DevPtrs &P, const DSys &s, const double *X, const double *PX, int ia,
+                                      int jb, bool write, int off, ListLds &L) {
   const int lane = threadIdx.x & (kTile - 1);
   const int li = ia * kTile + lane;
   const bool valid = li < s.nmp;
@@ -1349,8 +1366,7 @@ __device__ inline int exv_pair_scan(const DevPtrs &P, const DSys &s, const doubl
   const int g = s.off + (valid ? li : 0);
   const double cut2 = s.rl2max * (1.0 + 1e-9);
   if (!s.periodic && box_dmin2(P.bbox[s.bt0 + ia], P.bbox[s.bt0 + jb]) > cut2) return 0;
-  const bool rl2_lds = s.nunit <= kRl2Lds;
-  if (rl2_lds)
+  if (RL2_LDS)
     for (int k = lane; k < s.nunit * s.nunit; k += kTile) L.rl2[k] = P.rl2[s.uoff + k];
   double xi[3] = {0, 0, 0};
   int ui = 0, ti = 0, ci = 0;
@@ -1378,14 +1394,13 @@ __device__ inline int exv_pair_scan(const DevPtrs &P, const DSys &s, const doubl
         if (lj2 == li) continue;
         const bool own_lo = li < lj2;
         double v[3];
-        if (own_lo) { v[0] = __dadd_rn(L.sx[k], -xi[0]); v[1] = __dadd_rn(L.sy[k], -xi[1]); v[2] = __dadd_rn(L.sz[k], -xi[2]); }
-        else { v[0] = __dadd_rn(xi[0], -L.sx[k]); v[1] = __dadd_rn(xi[1], -L.sy[k]); v[2] = __dadd_rn(xi[2], -L.sz[k]); }
+        pair_vec(s, L.sx[k], L.sy[k], L.sz[k], xi, own_lo, v);
         const int code = s.periodic ? pbneighbor(s, v) : 0;
         const double d2 = sq3_exact(v[0], v[1], v[2]);
         if (!(d2 < s.rl2max)) continue;
         const int uj = L.su[k];
         const int ulo = own_lo ? ui : uj, uhi = own_lo ? uj : ui;
-        const double r2 = rl2_lds ? L.rl2[ulo * s.nunit + uhi] : P.rl2[s.uoff + ulo * s.nunit + uhi];
+        const double r2 = RL2_LDS ? L.rl2[ulo * s.nunit + uhi] : P.rl2[s.uoff + ulo * s.nunit + uhi];
         if (!(d2 < r2)) continue;
         const int tj = L.stc[k] & 0xFF, cj = L.stc[k] >> 8;
         const bool keep = own_lo ? exv_keep(s, ui, uj, ci, ti, li, lj2) : exv_keep(s, uj, ui, cj, tj, lj2, li);
@@ -1400,6 +1415,12 @@ __device__ inline int exv_pair_scan(const DevPtrs &P, const DSys &s, const doubl
   }
   wave_lds_sync();
   return cnt;
+}
+
+__device__ inline int exv_pair_scan(const DevPtrs &P, const DSys &s, const double *X, const double *PX, int ia,
+                                    int jb, bool write, int off, ListLds &L) {
+  return s.nunit <= kRl2Lds ? exv_pair_scan_t<true>(P, s, X, PX, ia, jb, write, off, L)
+                            : exv_pair_scan_t<false>(P, s, X, PX, ia, jb, write, off, L);
 }
 
 // ele bit words of charge tile I's rows for charge tile J (and, periodic,
@@ -1440,8 +1461,7 @@ __device__ inline void ele_pair_rows(const DevPtrs &P, const DSys &s, const doub
         if (qj2 == qi) continue;
         const bool own_lo = qi < qj2;
         double v[3];
-        if (own_lo) { v[0] = __dadd_rn(L.sx[k], -xi[0]); v[1] = __dadd_rn(L.sy[k], -xi[1]); v[2] = __dadd_rn(L.sz[k], -xi[2]); }
-        else { v[0] = __dadd_rn(xi[0], -L.sx[k]); v[1] = __dadd_rn(xi[1], -L.sy[k]); v[2] = __dadd_rn(xi[2], -L.sz[k]); }
+        pair_vec(s, L.sx[k], L.sy[k], L.sz[k], xi, own_lo, v);
         const int code = s.periodic ? pbneighbor(s, v) : 0;
         const double d2 = sq3_exact(v[0], v[1], v[2]);
         if (!(d2 < s.ele_rl2)) continue;
