@@ -340,7 +340,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the systems of this workload are independent and would be "
-              "sharded by bead budget across GPUs; multi-GPU runs are not part of this build.", file=sys.stderr)
+              "sharded by bead budget across GPUs, one process per GPU, with no per-step exchange.",
+              file=sys.stderr)
         sys.exit(2)
     if args.only == "ewald":
         print(json.dumps({"ewald": ewald_leg()}))
