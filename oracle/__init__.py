@@ -1,11 +1,12 @@
-"""Python handles on the CPU checkers (test infrastructure only).
+"""Python handles on the CPU oracle (test infrastructure only).
 
 ``Oracle``  wraps liboracle.so, the FP64 C restatement (tis_oracle.c).
 ``RefLib``  wraps _ref/libtisref.so, the reference's own Fortran routines
             compiled from its sources (built only where they are present).
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
-import this package; it is the checker, never the thing measured or shipped.
+import this package; it is what the HIP library is compared against, never
+the thing measured or shipped.
 """
 from __future__ import annotations
 

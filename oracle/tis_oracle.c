@@ -1,6 +1,6 @@
 /*
  * tis_oracle.c -- FP64 CPU restatement of the reference's DT15 per-step
- * force-and-integrate path.  TEST INFRASTRUCTURE ONLY: the checker for the
+ * force-and-integrate path.  TEST INFRASTRUCTURE ONLY: the oracle for the
  * HIP library and the CPU baseline timed beside it.  It is built with
  * -ffp-contract=off so that every a*b+c rounds twice, as the reference's
  * Fortran does on baseline x86-64.

@@ -1,6 +1,6 @@
 /*
  * tis_oracle.h -- FP64 CPU restatement of the reference's DT15 force-and-
- * integrate path.  TEST INFRASTRUCTURE ONLY: it is the checker that the HIP
+ * integrate path.  TEST INFRASTRUCTURE ONLY: the oracle that the HIP
  * library (tis_amd/csrc) is compared against, and the CPU baseline timed
  * beside it.  No product code path may call it.
  *
