@@ -100,9 +100,12 @@ def pmc_traffic(kernel):
     return None
 
 
-def force_check(systems, coords_now, eng, sample):
+def force_check(systems, coords_now, eng, sample, energies=None):
+    """Largest per-bead force error against the FP64 oracle over the sample
+    (relative, floor 1e-3 max|F|) and, with the GPU energies of the same
+    state, the largest per-term energy error (relative, floor 1e-6)."""
     import oracle
-    worst = 0.0
+    worst = worst_e = 0.0
     for k in sample:
         s = systems[k]
         x = coords_now[k]
@@ -114,7 +117,10 @@ def force_check(systems, coords_now, eng, sample):
         floor = 1e-3 * max(nf.max(), 1e-12)
         err = np.linalg.norm(fg - fo, axis=1) / np.maximum(nf, floor)
         worst = max(worst, float(err.max()))
-    return worst
+        if energies is not None:
+            eo = o.energy(x, eng.get_state(k, 1))
+            worst_e = max(worst_e, float((np.abs(energies[k] - eo) / np.maximum(np.abs(eo), 1e-6)).max()))
+    return worst, worst_e
 
 
 def serving_stream(njobs=100, scale=0.01, inflight=5.0, round_steps=500, seed=SEED + 4):
@@ -437,19 +443,39 @@ def main():
         print(f"[bench] displacement trigger: {trig['ms_per_step']:.4f} ms/step, "
               f"{trig['rebuilds_per_system_step']:.4f} rebuilds per system-step", file=sys.stderr, flush=True)
 
-    # lists and forces at the final state; error against the FP64 oracle
+    # lists and forces at the final state; energy_sumup of every system in
+    # one launch (the replica loop of energy_allrep.F90:123-125) and
+    # energy_unit of the largest system; errors against the FP64 oracle
     eng.neighbor(-1)
     eng.force_all()
     ab = algorithmic_bytes(systems, eng)
+    energies = eng.energy_all()
+    ncall = 20
+    t0 = time.perf_counter()
+    for _ in range(ncall):
+        eng.energy_all()
+    t_all = (time.perf_counter() - t0) / ncall
+    big = int(np.argmax([s.nmp for s in systems]))
+    eng.energy_unit(big)
+    t0 = time.perf_counter()
+    for _ in range(ncall):
+        eng.energy_unit(big)
+    t_unit = (time.perf_counter() - t0) / ncall
+    energy = {"systems": len(systems), "ms_per_batch_call": round(t_all * 1e3, 4),
+              "energy_unit_ms": round(t_unit * 1e3, 4), "energy_unit_beads": systems[big].nmp,
+              "note": "host-timed calls including the copy of the results to the host"}
+    print(f"[bench] energy_sumup of {len(systems)} systems: {t_all * 1e3:.3f} ms per call; energy_unit of "
+          f"{systems[big].nmp} beads: {t_unit * 1e3:.3f} ms", file=sys.stderr, flush=True)
     ferr = None
     if not args.no_check:
         rng = np.random.default_rng(1)
-        sample = sorted(set(rng.choice(len(systems), size=6, replace=False).tolist()) | {int(np.argmax([s.nmp for s in systems]))})
+        sample = sorted(set(rng.choice(len(systems), size=6, replace=False).tolist()) | {big})
         now = [eng.coords(k) for k in range(len(systems))]
         # replica slots carry the parameters of their current ladder label
-        ferr = force_check(eng.systems, now, eng, sample)
-        print(f"[bench] force max rel err vs FP64 oracle on {len(sample)} systems: {ferr:.3e}",
-              file=sys.stderr, flush=True)
+        ferr, eerr = force_check(eng.systems, now, eng, sample, energies)
+        energy["max_rel_err_vs_fp64"] = eerr
+        print(f"[bench] force max rel err vs FP64 oracle on {len(sample)} systems: {ferr:.3e}; "
+              f"energy terms {eerr:.3e}", file=sys.stderr, flush=True)
 
     # dominant kernel of the step by HIP-event time on the context's stream:
     # the force pass (k_force: hbond/tertiary-stack systems, electrostatic
@@ -504,6 +530,7 @@ def main():
         "plain_gpu_bead_steps_per_s": plain,
         "batched_over_plain": value / plain,
         "force_max_rel_err_vs_fp64": ferr,
+        "energy": energy,
         "p50_job_latency_s": serve.get("p50_latency") if serve else None,
         "p99_job_latency_s": serve.get("p99_latency") if serve else None,
         "serving": serve,
