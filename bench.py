@@ -169,6 +169,38 @@ def serving_stream(njobs=100, scale=0.01, inflight=5.0, round_steps=500, seed=SE
     return out
 
 
+def valence_leg(nsys=64, n=40, steps=200, seed=SEED + 7):
+    """Chains whose hbond candidates can exceed their sites' valence (every
+    base pair a candidate, compact seeded coordinates in a 28 A box): the
+    batched Langevin step with the valence-limit pass of
+    force_dtrna_hbond15.F90:373-510 at work, and its deletions per step."""
+    rng = np.random.default_rng(seed)
+    systems, coords = [], []
+    for k in range(nsys):
+        seq = T.random_sequence(rng, n)
+        s = T.build_rna_system([(seq, [])], all_pair_hbonds=True, name=f"valence{k}")
+        systems.append(s)
+        coords.append(T.compact_coords(s, rng, box=28, min_dist=2.4))
+    eng = Engine(systems, keys=list(range(nsys)), seed=seed)
+    for k, x in enumerate(coords):
+        eng.set_coords(k, x)
+    eng.langevin_init(tstep=0.05, rebuild_every=10)
+    eng.langevin_steps(20, MODE_GRAPH)
+    eng.synchronize()
+    c0 = eng.counters()
+    t0 = time.perf_counter()
+    eng.langevin_steps(steps, MODE_GRAPH)
+    eng.synchronize()
+    dt = time.perf_counter() - t0
+    c1 = eng.counters()
+    nb = sum(s.nmp for s in systems)
+    finite = all(bool(np.isfinite(eng.coords(k)).all()) for k in range(nsys))
+    eng.close()
+    return {"workload": f"{nsys} compact {n}-nt chains, every base pair an hbond candidate (valence-limited)",
+            "beads": nb, "ms_per_step": dt * 1e3 / steps, "bead_steps_per_s": nb * steps / dt,
+            "hb_deleted_per_step": (c1["hb_deleted"] - c0["hb_deleted"]) / steps, "finite": finite}
+
+
 def ewald_leg(steps=400, warmup=40):
     """Explicit ions with Ewald: the 76-nt chain + 565 ions in a 150 A cube
     (alpha 0.1, r_c 30 A, |h| <= 10), Langevin in hipGraph mode."""
@@ -334,6 +366,7 @@ def main():
     ap.add_argument("--serve-scale", type=float, default=0.01)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-ewald", action="store_true")
+    ap.add_argument("--no-valence", action="store_true")
     ap.add_argument("--no-bdhi", action="store_true")
     ap.add_argument("--swap-every", type=int, default=1000)
     ap.add_argument("--trigger-steps", type=int, default=1000,
@@ -499,6 +532,10 @@ def main():
         print(f"[bench] serving: {serve['jobs']} jobs, p50 {serve.get('p50_latency', 0):.3f} s, "
               f"p99 {serve.get('p99_latency', 0):.3f} s, {serve['bead_steps_per_s_busy']:.4e} bead-steps/s, "
               f"mean in flight {serve.get('mean_inflight', 0):.2f}", file=sys.stderr, flush=True)
+    valence = None if args.no_valence else valence_leg()
+    if valence:
+        print(f"[bench] valence pass: {valence['ms_per_step']:.4f} ms/step, "
+              f"{valence['hb_deleted_per_step']:.2f} hbonds deleted per step", file=sys.stderr, flush=True)
     ewald = None if args.no_ewald else ewald_leg()
     if ewald:
         print(f"[bench] ewald: {ewald['ms_per_step']:.4f} ms/step, force err {ewald['force_max_rel_err_vs_fp64']:.2e}",
@@ -537,6 +574,7 @@ def main():
         "remd": {"replicas": len(rep_slots), "swap_every": args.swap_every, "attempts": ladder.attempts,
                  "acceptance": ladder.acceptance},
         "displacement_trigger": trig,
+        "valence": valence,
         "ewald": ewald,
         "bdhi": bdhi,
         "roofline": roof,

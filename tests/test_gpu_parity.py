@@ -7,7 +7,7 @@ from tis_amd.engine import Engine, LIST_ELE, LIST_EXV, LIST_HB, MODE_EAGER, MODE
 
 from tis_amd import topology as T
 
-from systems import force_error, hairpin_design, hairpin64, parity_cases, unit_cases
+from systems import compact_allpairs, force_error, hairpin_design, hairpin64, parity_cases, unit_cases
 
 pytestmark = pytest.mark.gpu
 CASES = parity_cases()
@@ -126,6 +126,33 @@ def test_langevin_matches_oracle_short():
     xo, vo, ao, nr = o.langevin(x, v, np.zeros_like(x), 30, tstep=0.05, rebuild_every=10, seed=11, sys_id=9)
     d = np.abs(xg - xo).max()
     print(f"   30-step max |dx| = {d:.3e} A", flush=True)
+    assert d < 1e-4
+
+
+@pytest.mark.parametrize("seed", [10, 12])
+def test_langevin_valence_limited_matches_oracle(seed):
+    """A compact chain whose every base pair is an hbond candidate, so that
+    the valence-limit deletion of force_dtrna_hbond15.F90:373-510 runs at
+    each force step: 60 Langevin steps (list rebuilds at 10-step intervals)
+    against the oracle's, and the deletions are counted."""
+    s, x = compact_allpairs(seed=seed)
+    print(f"[valence trajectory] compact{seed} nmp={s.nmp}", flush=True)
+    v = np.zeros_like(x)
+    eng = Engine([s], keys=[seed], seed=23)
+    eng.set_coords(0, x)
+    eng.set_state(0, 1, v)
+    c0 = eng.counters()["hb_deleted"]
+    eng.langevin_init(tstep=0.05, rebuild_every=10, step0=0, init_accel=True)
+    eng.langevin_steps(60, MODE_EAGER)
+    ndel = eng.counters()["hb_deleted"] - c0
+    xg = eng.coords(0)
+    eng.close()
+    o = oracle.Oracle(s)
+    xo, vo, ao, nr = o.langevin(x, v, np.zeros_like(x), 60, tstep=0.05, rebuild_every=10, seed=23, sys_id=seed)
+    d = np.abs(xg - xo).max()
+    print(f"   {ndel} hbonds deleted; 60-step max |dx| = {d:.3e} A", flush=True)
+    assert ndel > 0
+    assert np.isfinite(xg).all()
     assert d < 1e-4
 
 

@@ -390,54 +390,98 @@ __device__ inline int site_count(const DevPtrs &P, int site) {
   return __hip_atomic_load(&P.site_cnt[site], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Orders the memory operations of a wave's lanes around it.
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
+__device__ inline bool site_in_excess(const DevPtrs &P, int site) {
+  return site_count(P, site) - P.site_val[site] > 0;
+}
+
 // Valence-limit deletion of force_dtrna_hbond15.F90:373-510 for one system,
-// run by one lane; the random draws come from the counter-based generator so
-// that the result equals the oracle's bit for bit.
-__device__ int hbond_valence_serial(const DevPtrs &P, const DSys &s, uint64_t step) {
+// run by the whole wave (every lane calls it).  The scans are wave-wide:
+// sites in excess are counted and the pick-th one found by ballots over
+// chunks of 64 sites, and the listed, formed hbonds of the picked site are
+// gathered in list order by ballots over chunks of 64 candidates into a
+// lane-distributed sequence (lane i holds element i; at most 64 kept).  The
+// shuffle and the Metropolis choice then run on every lane alike, with the
+// draws of the counter-based generator taken in the same order as a serial
+// pass, so that the result equals the oracle's bit for bit.  Lane 0 applies
+// each deletion; the fence after it orders the next scan's reads.
+__device__ int hbond_valence_wave(const DevPtrs &P, const DSys &s, uint64_t step, int lane) {
+  const uint64_t below = (1ull << lane) - 1ull;
   int ndel = 0;
   uint32_t draw = 0;
   for (int iter = 0; iter < (1 << 20); ++iter) {
     int nex = 0;
-    for (int k = 0; k < s.nsite; ++k)
-      if (site_count(P, s.sitoff + k) - P.site_val[s.sitoff + k] > 0) ++nex;
+    for (int k0 = 0; k0 < s.nsite; k0 += kTile) {
+      const int k = k0 + lane;
+      nex += __popcll(__ballot(k < s.nsite && site_in_excess(P, s.sitoff + k)));
+    }
     if (nex == 0) break;
     double rnd = uniform01(P.seed, s.key, step, 0, draw++);
-    int pick = (int)ceil(rnd * nex) - 1, site = -1;
-    for (int k = 0, c = 0; k < s.nsite; ++k)
-      if (site_count(P, s.sitoff + k) - P.site_val[s.sitoff + k] > 0) {
-        if (c == pick) { site = s.sitoff + k; break; }
-        ++c;
+    const int pick = (int)ceil(rnd * nex) - 1;  // rnd in (0,1): 0 <= pick < nex
+    int site = -1;
+    for (int k0 = 0, c = 0; k0 < s.nsite; k0 += kTile) {
+      const int k = k0 + lane;
+      const bool ex = k < s.nsite && site_in_excess(P, s.sitoff + k);
+      const uint64_t b = __ballot(ex);
+      const int n = __popcll(b);
+      if (pick < c + n) {
+        const uint64_t w = __ballot(ex && c + __popcll(b & below) == pick);
+        site = s.sitoff + k0 + __builtin_ctzll(w);
+        break;
       }
-    // hbonds of the site in list order, shuffled in place (at most 64 kept)
-    int seq[64];
-    int nseq = 0;
-    for (int q = 0; q < s.nhb && nseq < 64; ++q) {
-      int h = s.hoff + q;
-      if (!P.hb_inlist[h] || !P.hb_status[h]) continue;
-      for (int k = 0; k < 6; ++k)
-        if (P.hb_site[6 * h + k] == site) { seq[nseq++] = h; break; }
+      c += n;
+    }
+    // hbonds of the site in list order (at most 64), element i on lane i
+    int myh = -1, nseq = 0;
+    for (int q0 = 0; q0 < s.nhb && nseq < kTile; q0 += kTile) {
+      const int q = q0 + lane;
+      const int h = s.hoff + q;
+      bool m = false;
+      if (q < s.nhb && P.hb_inlist[h] && P.hb_status[h])
+        for (int k = 0; k < 6; ++k) m = m || P.hb_site[6 * h + k] == site;
+      uint64_t b = __ballot(m);
+      while (b && nseq < kTile) {
+        const int hv = __shfl(h, __builtin_ctzll(b));
+        b &= b - 1ull;
+        if (lane == nseq) myh = hv;
+        ++nseq;
+      }
     }
     if (nseq == 0) break;
     for (int i = 0; i < nseq; ++i) {
       rnd = uniform01(P.seed, s.key, step, 0, draw++);
-      int isw = (int)ceil(rnd * nseq) - 1;
-      int t = seq[i]; seq[i] = seq[isw]; seq[isw] = t;
+      const int isw = (int)ceil(rnd * nseq) - 1;  // 0 <= isw < nseq
+      const int a = __shfl(myh, i), c = __shfl(myh, isw);
+      if (lane == i) myh = c;
+      if (lane == isw) myh = a;
     }
-    int del = seq[0];
+    const double myene = lane < nseq ? P.hb_ene[myh] : 0.0;
+    int del = __shfl(myh, 0);
+    double edel = __shfl(myene, 0);
     for (int i = 1; i < nseq; ++i) {
-      int j = seq[i];
-      double ratio = exp((P.hb_ene[j] - P.hb_ene[del]) * s.beta_hb);
+      const int j = __shfl(myh, i);
+      const double ej = __shfl(myene, i);
+      const double ratio = exp((ej - edel) * s.beta_hb);
       rnd = uniform01(P.seed, s.key, step, 0, draw++);
-      if (rnd < ratio) del = j;
+      if (rnd < ratio) { del = j; edel = ej; }
     }
-    P.hb_status[del] = 0;
-    P.hb_ene[del] = 0.0;
+    if (lane == 0) {
+      P.hb_status[del] = 0;
+      P.hb_ene[del] = 0.0;
+      // excess -= 1 where positive; count kept as valence + excess
+      for (int k = 0; k < 6; ++k) {
+        const int st = P.hb_site[6 * del + k];
+        if (st >= 0 && site_in_excess(P, st)) atomicSub(&P.site_cnt[st], 1);
+      }
+    }
     ++ndel;
-    // excess -= 1 where positive; count kept as valence + excess
-    for (int k = 0; k < 6; ++k) {
-      int st = P.hb_site[6 * del + k];
-      if (st >= 0 && site_count(P, st) - P.site_val[st] > 0) atomicSub(&P.site_cnt[st], 1);
-    }
+    wave_sync();
   }
   return ndel;
 }
@@ -448,11 +492,6 @@ __device__ int hbond_valence_serial(const DevPtrs &P, const DSys &s, uint64_t st
 // pass.  Forces go to per-term slots gathered by k_main.  The phases of a
 // system are ordered by wave-level fences: the four waves of a block serve
 // four systems.
-__device__ inline void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-}
 
 __device__ void terms_wave(const DevPtrs &P, const double *X, const double *PX, int isys, int32_t t_local) {
   const int lane = threadIdx.x & (kTile - 1), nt = kTile;
@@ -498,9 +537,9 @@ __device__ void terms_wave(const DevPtrs &P, const double *X, const double *PX, 
   int any = 0;
   for (int k = lane; k < s.nsite; k += nt)
     if (site_count(P, s.sitoff + k) - P.site_val[s.sitoff + k] > 0) any = 1;
-  if (__any(any) && lane == 0) {
-    int nd = hbond_valence_serial(P, s, step);
-    if (nd) atomicAdd(&P.counters[2], (unsigned long long)nd);
+  if (__any(any)) {
+    const int nd = hbond_valence_wave(P, s, step, lane);
+    if (nd && lane == 0) atomicAdd(&P.counters[2], (unsigned long long)nd);
   }
   wave_sync();
   // forces of the hbonds that survive into their beads' slots

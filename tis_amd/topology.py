@@ -380,7 +380,11 @@ def helix_coords(sys: System, rng: np.random.Generator, jitter=0.5, rise=3.0,
 def compact_coords(sys: System, rng: np.random.Generator, box=40.0, step=4.2, min_dist=2.2,
                    tries=200):
     """Random self-avoiding walk per chain inside a cube: compact conformations in
-    which excluded-volume, DH and hbond terms are all active (test inputs)."""
+    which excluded-volume, DH and hbond terms are all active (test inputs).  A
+    step that leaves the cube is drawn again rather than clipped to the wall:
+    clipping puts runs of beads on one edge line, where the stack dihedral of
+    three collinear beads is undefined and the force NaN (in the reference's
+    routines as well)."""
     xyz = np.zeros((sys.nmp, 3))
     placed = []
     for i in range(sys.nmp):
@@ -393,7 +397,8 @@ def compact_coords(sys: System, rng: np.random.Generator, box=40.0, step=4.2, mi
                     ref = xyz[i - 1]
                 d = rng.normal(size=3)
                 cand = ref + step * d / np.linalg.norm(d)
-                cand = np.clip(cand, -box / 2, box / 2)
+                if np.any(np.abs(cand) > box / 2):
+                    continue
             if not placed or np.min(np.linalg.norm(np.asarray(placed) - cand, axis=1)) > min_dist:
                 break
         xyz[i] = cand
