@@ -65,6 +65,12 @@ def _lib():
     lib.oracle_hbond_valence.restype = C.c_int32
     lib.oracle_hbond_valence.argtypes = [vp, _i32p, C.c_int32, _f64p, C.c_char_p,
                                          C.c_uint64, C.c_uint32, C.c_uint64]
+    _u32p = C.POINTER(C.c_uint32)
+    lib.oracle_hbond_valence_mt.restype = C.c_int32
+    lib.oracle_hbond_valence_mt.argtypes = [vp, _i32p, C.c_int32, _f64p, C.c_char_p, _u32p,
+                                            C.c_int64, C.POINTER(C.c_int64)]
+    lib.oracle_force_sumup_mt.restype = C.c_int32
+    lib.oracle_force_sumup_mt.argtypes = [vp, _f64p, _f64p, _u32p, C.c_int64, C.POINTER(C.c_int64)]
     lib.oracle_hbond_scatter.argtypes = [vp, _i32p, C.c_int32, _f64p, C.c_char_p, _f64p]
     lib.oracle_force_coulomb.argtypes = [vp, _f64p, _i32p, _f64p, C.c_int64, _f64p]
     lib.oracle_force_ewald.argtypes = [vp, _f64p, _i32p, _f64p, C.c_int64, _f64p]
@@ -138,6 +144,40 @@ class Oracle:
         f = np.zeros_like(x)
         ncl = self.L.oracle_force_sumup(self.ws, fp(x), fp(f), seed, sys_id, step)
         return f, ncl
+
+    def force_mt(self, xyz, raw):
+        """force_sumup whose valence pass draws the Mersenne-Twister words
+        raw (uint32, in stream order).  Returns (f, words used)."""
+        x = np.ascontiguousarray(xyz, dtype=np.float64)
+        r = np.ascontiguousarray(raw, dtype=np.uint32)
+        f = np.zeros_like(x)
+        used = C.c_int64()
+        rc = self.L.oracle_force_sumup_mt(self.ws, fp(x), fp(f), r.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                          len(r), C.byref(used))
+        if rc < 0:
+            raise ValueError("raw stream too short")
+        return f, used.value
+
+    def hbond_mt(self, xyz, raw):
+        """The hbond term alone with the valence pass drawing raw (see
+        force_mt).  Returns (f, status, ene, deletions, words used)."""
+        x = np.ascontiguousarray(xyz, dtype=np.float64)
+        r = np.ascontiguousarray(raw, dtype=np.uint32)
+        cs = C.byref(self.cs)
+        hb = np.ascontiguousarray(self.lists(2), dtype=np.int32)
+        nh = max(len(self.sys.hb_mp), 1)
+        forh, ene, stat = np.zeros(18 * nh), np.zeros(nh), np.zeros(nh, np.uint8)
+        f = np.zeros_like(x)
+        self.L.oracle_hbond_eval(cs, fp(x), fp(x), ip(hb), len(hb), fp(forh), fp(ene),
+                                 stat.ctypes.data_as(C.c_char_p))
+        used = C.c_int64()
+        nd = self.L.oracle_hbond_valence_mt(cs, ip(hb), len(hb), fp(ene), stat.ctypes.data_as(C.c_char_p),
+                                            r.ctypes.data_as(C.POINTER(C.c_uint32)), len(r), C.byref(used))
+        if nd < 0:
+            raise ValueError("raw stream too short")
+        self.L.oracle_hbond_scatter(cs, ip(hb), len(hb), fp(forh), stat.ctypes.data_as(C.c_char_p), fp(f))
+        nreal = len(self.sys.hb_mp)
+        return f, stat[:nreal], ene[:nreal], nd, used.value
 
     def energy(self, xyz, velo=None):
         x = np.ascontiguousarray(xyz, dtype=np.float64)
@@ -360,6 +400,8 @@ class RefLib:
         L.tisref_bd_forces.argtypes = [C.c_int32, _f64p, C.c_int32, _i32p, _f64p, _f64p, _f64p, C.c_int32,
                                        _i32p, _f64p, _f64p, C.c_double, _f64p]
         L.tisref_flush.argtypes = []
+        L.tisref_mt_seed.argtypes = [C.c_int32]
+        L.tisref_mt_raw.argtypes = [C.c_int32, C.c_int32, _i32p]
         self.sys = None
 
     def load(self, system):
@@ -393,6 +435,16 @@ class RefLib:
         f = np.zeros((self.sys.nmp, 3))
         _on_big_stack(self.L.tisref_force_term, self.TERMS[name], fp(f))
         return f
+
+    def mt_seed(self, seed):
+        """Seed the replica stream the reference's valence pass draws from."""
+        self.L.tisref_mt_seed(seed)
+
+    def mt_raw(self, seed, n):
+        """The first n 32-bit words of a stream seeded like mt_seed(seed)."""
+        r = np.zeros(n, np.int32)
+        self.L.tisref_mt_raw(seed, n, ip(r))
+        return r.view(np.uint32)
 
     def hb_state(self):
         nh = max(len(self.sys.hb_mp), 1)

@@ -18,6 +18,7 @@ module tisref_glue
   use var_io
   use mpiconst
   use if_energy
+  use mt_stream, only: mt_state, new, init, set_mt19937, genrand_int32, delete
   implicit none
 
   type, bind(C) :: tis_params_c
@@ -457,6 +458,35 @@ contains
     f = f + fl
     deallocate(fl)
   end subroutine tisref_bd_forces
+
+  ! Seed the replica's stream mts(1,0), which force_dtrna_hbond15 draws its
+  ! valence-pass numbers from (force_dtrna_hbond15.F90:392,413,426), by the
+  ! scalar initialisation of mt_stream (as setp_random.F90:29-31 prepares
+  ! mts(0,0) before deriving the replica streams).
+  subroutine tisref_mt_seed(seed) bind(C, name="tisref_mt_seed")
+    integer(c_int32_t), value :: seed
+    if (allocated(mts)) deallocate(mts)
+    allocate(mts(0:1, 0:0))
+    call set_mt19937
+    call new(mts(1, 0))
+    call init(mts(1, 0), seed)
+  end subroutine tisref_mt_seed
+
+  ! The first n 32-bit outputs of a stream seeded as in tisref_mt_seed: the
+  ! words the valence pass consumes, two per genrand_double1/_double4 draw.
+  subroutine tisref_mt_raw(seed, n, raw) bind(C, name="tisref_mt_raw")
+    integer(c_int32_t), value :: seed, n
+    integer(c_int32_t), intent(out) :: raw(n)
+    type(mt_state) :: st
+    integer :: k
+    call set_mt19937
+    call new(st)
+    call init(st, seed)
+    do k = 1, n
+       raw(k) = genrand_int32(st)
+    end do
+    call delete(st)
+  end subroutine tisref_mt_raw
 
   subroutine tisref_hb_state(status, ene) bind(C, name="tisref_hb_state")
     integer(c_int32_t), intent(out) :: status(*)

@@ -818,11 +818,40 @@ void oracle_hbond_eval(const tis_system *s, const double *xyz, const double *pxy
   }
 }
 
-/* force_dtrna_hbond15.F90:373-510 with the counter-based generator; the
- * per-site membership lists follow neighbor_list_hb.F90:106-134. */
-int32_t oracle_hbond_valence(const tis_system *s, const int32_t *hb_idx, int32_t nlist,
-                             double *ene_hb, uint8_t *status,
-                             uint64_t seed, uint32_t sys, uint64_t step) {
+/* Uniform draws of the valence pass.  kind 0: the site pick of
+ * force_dtrna_hbond15.F90:392 (genrand_double4, (0,1]); kind 1: the shuffle
+ * and Metropolis draws of :413 and :426 (genrand_double1, [0,1]). */
+typedef double (*valence_draw_fn)(void *ctx, int kind);
+
+typedef struct { uint64_t seed; uint32_t sys; uint64_t step; uint32_t next; } philox_src;
+
+static double philox_draw(void *ctx, int kind) {
+  (void)kind;
+  philox_src *p = (philox_src *)ctx;
+  return oracle_uniform(p->seed, p->sys, p->step, 0, p->next++);
+}
+
+/* mt_genrand_double1 / _double2 / _double4 of mt_stream.F90:708-785: one
+ * draw takes two 32-bit outputs a, b of the stream; with A = a >> 5 and
+ * B = b >> 6 (logical shifts), double1 = (A*2^26 + B) / (2^53 - 1) in [0,1],
+ * double2 = (A*2^26 + B) / 2^53 in [0,1), double4 = 1 - double2 in (0,1]. */
+typedef struct { const uint32_t *raw; int64_t n, pos; int overrun; } mt_src;
+
+static double mt_draw(void *ctx, int kind) {
+  mt_src *m = (mt_src *)ctx;
+  if (m->pos + 2 > m->n) { m->overrun = 1; return 0.5; }
+  uint32_t a = m->raw[m->pos] >> 5, b = m->raw[m->pos + 1] >> 6;
+  m->pos += 2;
+  double v = (double)a * 67108864.0 + (double)b;
+  if (kind == 0) return 1.0 - v * (1.0 / 9007199254740992.0);
+  return v * (1.0 / 9007199254740991.0);
+}
+
+/* force_dtrna_hbond15.F90:373-510, drawing from `draw`; the per-site
+ * membership lists follow neighbor_list_hb.F90:106-134. */
+static int32_t hbond_valence_core(const tis_system *s, const int32_t *hb_idx, int32_t nlist,
+                                  double *ene_hb, uint8_t *status,
+                                  valence_draw_fn draw, void *dctx) {
   int ns = s->nhbsite;
   if (ns <= 0) return 0;
   int32_t *excess = (int32_t *)calloc((size_t)ns, sizeof(int32_t));
@@ -855,28 +884,30 @@ int32_t oracle_hbond_valence(const tis_system *s, const int32_t *hb_idx, int32_t
       }
   }
   for (int i = 0; i < ns; ++i) excess[i] -= s->hbsite_valence[i];
-  uint32_t draw = 0;
   int32_t ndel = 0;
   int32_t *seq = (int32_t *)malloc(sizeof(int32_t) * (size_t)(off[ns] + 1));
   for (;;) {
     int nex = 0;
     for (int i = 0; i < ns; ++i) if (excess[i] > 0) lst[nex++] = i;
     if (nex == 0) break;
-    double rnd = oracle_uniform(seed, sys, step, 0, draw++);
+    double rnd = draw(dctx, 0);
     int site = lst[(int)ceil(rnd * nex) - 1];
     int nseq = 0;
     for (int k = off[site]; k < off[site] + cnt[site]; ++k)
       if (status[mem[k]]) seq[nseq++] = mem[k];
     for (int i = 0; i < nseq; ++i) {
-      rnd = oracle_uniform(seed, sys, step, 0, draw++);
+      rnd = draw(dctx, 1);
+      /* ceiling(rnd * nhb_seq) (:414): a draw of exactly 0 gives index 0,
+       * which the reference would read as hb_seq(0); it stays at 1 here */
       int isw = (int)ceil(rnd * nseq) - 1;
+      if (isw < 0) isw = 0;
       int t = seq[i]; seq[i] = seq[isw]; seq[isw] = t;
     }
     int del = seq[0];
     for (int i = 1; i < nseq; ++i) {
       int j = seq[i];
       double ratio = exp((ene_hb[j] - ene_hb[del]) * s->p.beta_hb);
-      rnd = oracle_uniform(seed, sys, step, 0, draw++);
+      rnd = draw(dctx, 1);
       if (rnd < ratio) del = j;
     }
     status[del] = 0;
@@ -890,6 +921,22 @@ int32_t oracle_hbond_valence(const tis_system *s, const int32_t *hb_idx, int32_t
   }
   free(seq); free(mem); free(off); free(lst); free(cnt); free(excess);
   return ndel;
+}
+
+int32_t oracle_hbond_valence(const tis_system *s, const int32_t *hb_idx, int32_t nlist,
+                             double *ene_hb, uint8_t *status,
+                             uint64_t seed, uint32_t sys, uint64_t step) {
+  philox_src p = {seed, sys, step, 0u};
+  return hbond_valence_core(s, hb_idx, nlist, ene_hb, status, philox_draw, &p);
+}
+
+int32_t oracle_hbond_valence_mt(const tis_system *s, const int32_t *hb_idx, int32_t nlist,
+                                double *ene_hb, uint8_t *status,
+                                const uint32_t *raw, int64_t nraw, int64_t *used) {
+  mt_src m = {raw, nraw, 0, 0};
+  int32_t nd = hbond_valence_core(s, hb_idx, nlist, ene_hb, status, mt_draw, &m);
+  if (used) *used = m.pos;
+  return m.overrun ? -1 : nd;
 }
 
 void oracle_hbond_scatter(const tis_system *s, const int32_t *hb_idx, int32_t nlist,
@@ -1125,8 +1172,8 @@ int64_t oracle_ws_lists(oracle_ws *w, int32_t which, const int32_t **pairs, cons
  * xyz itself (pxyz_mp_rep equals xyz_mp_rep there); periodic ones read the
  * wrapped copy, which the Langevin loop advances by the same dxyz as xyz
  * (time_integral.F90:156-160). */
-int32_t oracle_force_sumup(oracle_ws *w, const double *xyz, double *f,
-                           uint64_t seed, uint32_t sys, uint64_t step) {
+static int32_t force_sumup_core(oracle_ws *w, const double *xyz, double *f,
+                                 valence_draw_fn draw, void *dctx) {
   const tis_system *s = w->s;
   memset(f, 0, sizeof(double) * 3 * (size_t)s->nmp);
   int32_t nclamp = 0;
@@ -1135,7 +1182,7 @@ int32_t oracle_force_sumup(oracle_ws *w, const double *xyz, double *f,
   oracle_force_tstack(s, xyz, w->pxyz, w->st_status, f);
   oracle_force_stack(s, xyz, w->st_status, f);
   oracle_hbond_eval(s, xyz, w->pxyz, w->hb, w->nhbl, w->for_hb, w->ene_hb, w->hb_status);
-  oracle_hbond_valence(s, w->hb, w->nhbl, w->ene_hb, w->hb_status, seed, sys, step);
+  hbond_valence_core(s, w->hb, w->nhbl, w->ene_hb, w->hb_status, draw, dctx);
   oracle_hbond_scatter(s, w->hb, w->nhbl, w->for_hb, w->hb_status, f);
   const double *px = s->p.periodic ? w->pxyz : xyz;
   nclamp += oracle_force_exv(s, px, w->exv, w->nexv, f);
@@ -1146,6 +1193,20 @@ int32_t oracle_force_sumup(oracle_ws *w, const double *xyz, double *f,
   else if (s->p.ele_form == TIS_ELE_EWALD)
     oracle_force_ewald(s, px, w->ele, w->ele_coef, w->nele, f);
   return nclamp;
+}
+
+int32_t oracle_force_sumup(oracle_ws *w, const double *xyz, double *f,
+                           uint64_t seed, uint32_t sys, uint64_t step) {
+  philox_src p = {seed, sys, step, 0u};
+  return force_sumup_core(w, xyz, f, philox_draw, &p);
+}
+
+int32_t oracle_force_sumup_mt(oracle_ws *w, const double *xyz, double *f,
+                              const uint32_t *raw, int64_t nraw, int64_t *used) {
+  mt_src m = {raw, nraw, 0, 0};
+  int32_t nc = force_sumup_core(w, xyz, f, mt_draw, &m);
+  if (used) *used = m.pos;
+  return m.overrun ? -1 : nc;
 }
 
 /* energy_sumup.F90:149-529 for a DT15 system (hbond energies reuse the

@@ -65,6 +65,15 @@ void oracle_hbond_eval(const tis_system *s, const double *xyz, const double *pxy
 int32_t oracle_hbond_valence(const tis_system *s, const int32_t *hb_idx, int32_t nlist,
                              double *ene_hb, uint8_t *status,
                              uint64_t seed, uint32_t sys, uint64_t step);
+/* The same pass drawing the reference's own numbers: raw[] holds the 32-bit
+ * outputs of the replica's Mersenne-Twister stream (mts(irep,0)) in order;
+ * each draw takes two of them and forms genrand_double4 (site pick) or
+ * genrand_double1 (shuffle, Metropolis) as mt_stream.F90:708-785 do.
+ * *used gets the number of raw words consumed; returns -1 if nraw was too
+ * short. */
+int32_t oracle_hbond_valence_mt(const tis_system *s, const int32_t *hb_idx, int32_t nlist,
+                                double *ene_hb, uint8_t *status,
+                                const uint32_t *raw, int64_t nraw, int64_t *used);
 void oracle_hbond_scatter(const tis_system *s, const int32_t *hb_idx, int32_t nlist,
                           const double *for_hb, const uint8_t *status, double *f);
 
@@ -101,6 +110,10 @@ void oracle_ws_free(oracle_ws *w);
 void oracle_neighbor(oracle_ws *w, const double *xyz);
 int32_t oracle_force_sumup(oracle_ws *w, const double *xyz, double *f,
                            uint64_t seed, uint32_t sys, uint64_t step);
+/* force_sumup with the valence pass drawing the Mersenne-Twister words raw[]
+ * (see oracle_hbond_valence_mt); returns -1 if nraw was too short. */
+int32_t oracle_force_sumup_mt(oracle_ws *w, const double *xyz, double *f,
+                              const uint32_t *raw, int64_t nraw, int64_t *used);
 void oracle_energy_sumup(oracle_ws *w, const double *xyz, const double *velo, double *energy);
 void oracle_energy_unit(oracle_ws *w, const double *xyz, const double *velo, int32_t style, double *eu);
 int64_t oracle_ws_lists(oracle_ws *w, int32_t which, const int32_t **pairs, const double **coef);

@@ -1,4 +1,6 @@
 """Seeded test systems shared by the CPU and GPU tests."""
+import os
+
 import numpy as np
 
 from tis_amd import topology as T
@@ -58,6 +60,46 @@ def coulomb_chain(seed=21, n=160, rcut=40.0, box=60.0):
     return s, x
 
 
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def bwyv19(ions=True):
+    """The reference's own DT15-format fixture: ninfo/bwyv19.ninfo (28 nt,
+    27 consecutive stacks, 108 hbonds of which 18 tertiary, one exclusive
+    tertiary stack) on pdb/bwyv19_force_check.rna.pdb coordinates, with the
+    206 Mg2+ of pdb/bwyv19_force_check.ion.pdb as a second unit, at the
+    conditions of inp/bwyv19_force_check.inp (310.15 K, I = 0.057 M,
+    i_DH_cutoff_type = 1, cutoff_ele = 50 A).  The files are kept under
+    tests/golden/bwyv19/."""
+    from tis_amd import ninfo as NI
+    d = os.path.join(GOLDEN, "bwyv19")
+    rec = NI.read_ninfo(os.path.join(d, "bwyv19.ninfo"))
+    files = [os.path.join(d, "bwyv19_force_check.rna.pdb")]
+    if ions:
+        files.append(os.path.join(d, "bwyv19_force_check.ion.pdb"))
+    beads = NI.read_pdb_cg(files)
+    s = NI.system_from_ninfo(rec, beads, tempk=310.15, ionic_strength=0.057, dh_cutoff_type=1,
+                             cutoff_ele=50.0, name="bwyv19")
+    return s, beads.xyz.copy()
+
+
+def tstack_hairpin(seed=61, n=48):
+    """A compact hairpin with tertiary stacks: exclusive ones that switch
+    off consecutive stacks on one or both sides, two that displace the same
+    consecutive stack from either side, one whose partner stack does not
+    exist (its side becomes non-exclusive), and one 12 A beyond its native
+    distance (d - d0 >= 10: it displaces nothing)."""
+    rng = np.random.default_rng(seed)
+    seq, pairs = T.hairpin_design(rng, n, 6, 5)
+    s = T.build_rna_system([(seq, pairs)], name="tstack_hairpin")
+    x = T.compact_coords(s, rng, box=34, min_dist=2.6)
+    B = lambda k: 3 * k + 1  # noqa: E731  base bead of nucleotide k (5'-S chain)
+    stacks = [(B(3), B(20), +2, -2), (B(4), B(33), -2, +1), (B(12), B(40), +1, -1),
+              (B(n - 2), B(25), +2, +2), (B(8), B(44), -2, +2)]
+    s = T.add_tertiary_stacks(s, x, stacks, dist_shift=[0.3, -0.4, 0.2, 0.1, 12.0], rng=rng)
+    return s, x
+
+
 def parity_cases(ewald=True):
     out = [("hairpin64_helix",) + hairpin64(0, "helix"),
            ("hairpin64_compact",) + hairpin64(3, "compact")]
@@ -65,6 +107,9 @@ def parity_cases(ewald=True):
         out.append((f"compact{k}",) + compact_allpairs(seed=10 + k, **kw))
     # several 64-charge chunks: pair chunks I < J of the electrostatic sweep
     out.append(("hairpin300_helix",) + hairpin_design(7, 300))
+    # tertiary stacks: the reference's bwyv19 fixture and a generated hairpin
+    out.append(("bwyv19",) + bwyv19())
+    out.append(("tstack_hairpin",) + tstack_hairpin())
     out.append(("coulomb_chain160",) + coulomb_chain())
     out.append(("coulomb_chain320",) + coulomb_chain(seed=22, n=320, box=76.0))
     if ewald:

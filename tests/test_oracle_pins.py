@@ -80,45 +80,99 @@ def test_lists_match_reference(case):
     assert np.array_equal(np.sort(o.lists(2)), np.sort(R.lists(2)))
 
 
-def _valence_free(o, s, x):
-    """True when no hbond site exceeds its valence (the reference then draws
-    no random numbers, so its hbond forces are deterministic)."""
-    f, st, _ = o.term("hbond", x)
-    used = np.zeros(len(s.hbsite_valence), int)
-    hb = o.lists(2)
-    import ctypes as C
-    nh = max(len(s.hb_mp), 1)
-    forh, ene, stat = np.zeros(18 * nh), np.zeros(nh), np.zeros(nh, np.uint8)
-    L = oracle.lib()
-    hbl = np.ascontiguousarray(hb, dtype=np.int32)
-    L.oracle_hbond_eval(C.byref(o.cs), oracle.fp(x), oracle.fp(x), oracle.ip(hbl), len(hbl),
-                        oracle.fp(forh), oracle.fp(ene), stat.ctypes.data_as(C.c_char_p))
-    for h in np.nonzero(stat)[0]:
-        for site in s.hb_site[h]:
-            if site >= 0:
-                used[site] += 1
-    return bool(np.all(used <= s.hbsite_valence))
+MT_SEED = 20261203
+MT_WORDS = 1 << 18
 
 
 @needs_ref
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
 def test_forces_match_reference(case):
+    """Every term against the reference routine.  The hbond term runs the
+    valence pass of both on the same Mersenne-Twister stream: the reference
+    draws from its replica stream mts(1,0), the oracle from the words of an
+    identically seeded stream."""
     name, s, x = case
     o = Oracle(s)
     o.neighbor(x)
     R = RefLib()
     R.load(s)
     R.neighbor(x)
-    terms = ["bond", "angle", "stack", "exv", "dh"]
-    if _valence_free(o, s, x):
-        terms.append("hbond")
-    for t in terms:
+    for t in ["bond", "angle", "stack", "exv", "dh"]:
         fr = R.force_term(t)
         fo = o.term(t, x)
         if isinstance(fo, tuple):
             fo = fo[0]
         scale = max(np.abs(fr).max(), 1e-30)
         assert np.abs(fr - fo).max() <= 1e-12 * scale, t
+    R.mt_seed(MT_SEED)
+    fr = R.force_term("hbond")
+    sr, er = R.hb_state()
+    fo, so, eo, _, _ = o.hbond_mt(x, R.mt_raw(MT_SEED, MT_WORDS))
+    assert np.array_equal(sr, so.astype(np.int32))
+    assert np.abs(er - eo).max(initial=0.0) <= 1e-12 * max(np.abs(er).max(initial=0.0), 1e-30)
+    assert np.abs(fr - fo).max() <= 1e-12 * max(np.abs(fr).max(), 1e-30)
+
+
+def test_mt_stream_known_answer():
+    """The reference's mt_stream with the scalar seed 5489 is MT19937
+    (Matsumoto & Nishimura 1998): its first outputs are the published
+    3499211612, 581869302, 3890346734, 3586334585, 545404204."""
+    if not RefLib.available():
+        pytest.skip("oracle/_ref not built (reference absent)")
+    assert RefLib().mt_raw(5489, 5).tolist() == [3499211612, 581869302, 3890346734, 3586334585, 545404204]
+
+
+def _valence_cases():
+    out = []
+    for seed in (10, 12, 41, 42, 43):
+        for n in (40, 64):
+            s, x = compact_allpairs(seed=seed, n=n)
+            out.append((f"compact{seed}_{n}", s, x))
+    return out
+
+
+@needs_ref
+@pytest.mark.parametrize("mt_seed", [1, 77, 20261203])
+def test_valence_pass_matches_reference(mt_seed):
+    """force_dtrna_hbond15.F90:373-510: with the same Mersenne-Twister words
+    the oracle deletes exactly the hbonds the reference deletes (site pick,
+    shuffle and Metropolis victim pick), over compact chains whose every
+    canonical pair is a candidate, so many sites exceed their valence."""
+    total = 0
+    for name, s, x in _valence_cases():
+        o = Oracle(s)
+        o.neighbor(x)
+        R = RefLib()
+        R.load(s)
+        R.neighbor(x)
+        R.mt_seed(mt_seed)
+        fr = R.force_term("hbond")
+        sr, er = R.hb_state()
+        fo, so, eo, nd, used = o.hbond_mt(x, R.mt_raw(mt_seed, MT_WORDS))
+        assert np.array_equal(sr, so.astype(np.int32)), name
+        assert np.abs(er - eo).max(initial=0.0) <= 1e-12 * max(np.abs(er).max(initial=0.0), 1e-30), name
+        assert np.abs(fr - fo).max() <= 1e-12 * max(np.abs(fr).max(), 1e-30), name
+        total += nd
+    assert total >= 20
+
+
+@needs_ref
+def test_valence_pass_rejects_wrong_stream():
+    """The pin is sharp: a stream shifted by one draw (two words) changes
+    the deletion set of at least one case."""
+    differ = 0
+    for name, s, x in _valence_cases():
+        o = Oracle(s)
+        o.neighbor(x)
+        R = RefLib()
+        R.load(s)
+        R.neighbor(x)
+        R.mt_seed(5)
+        R.force_term("hbond")
+        sr, _ = R.hb_state()
+        _, so, _, nd, _ = o.hbond_mt(x, R.mt_raw(5, MT_WORDS)[2:])
+        differ += int(not np.array_equal(sr, so.astype(np.int32)))
+    assert differ >= 1
 
 
 @needs_ref
@@ -127,13 +181,12 @@ def test_energies_match_reference(case):
     name, s, x = case
     o = Oracle(s)
     o.neighbor(x)
-    o.force(x)
     R = RefLib()
     R.load(s)
     R.neighbor(x)
+    o.force_mt(x, R.mt_raw(MT_SEED, MT_WORDS))
     R.force_term("stack")
-    if not _valence_free(o, s, x):
-        pytest.skip("reference draws Mersenne-Twister numbers for this configuration")
+    R.mt_seed(MT_SEED)
     R.force_term("hbond")
     v = np.random.default_rng(3).normal(size=x.shape)
     e = o.energy(x, v)
@@ -155,13 +208,12 @@ def test_energy_unit_matches_reference(case, style):
     name, s, x = case
     o = Oracle(s)
     o.neighbor(x)
-    o.force(x)
     R = RefLib()
     R.load(s)
     R.neighbor(x)
+    o.force_mt(x, R.mt_raw(MT_SEED, MT_WORDS))
     R.force_term("stack")
-    if not _valence_free(o, s, x):
-        pytest.skip("reference draws Mersenne-Twister numbers for this configuration")
+    R.mt_seed(MT_SEED)
     R.force_term("hbond")
     v = np.random.default_rng(3).normal(size=x.shape)
     eu = o.energy_unit(x, v, style)

@@ -173,6 +173,85 @@ def hairpin_pairs(n: int, stem5: int, stem_len: int, loop: int):
     return [(stem5 + k, j0 - k) for k in range(stem_len)]
 
 
+def hbsite_layout(names):
+    """Hydrogen-bond sites of the RNA beads in bead order: {bead: first site}
+    and the site valences, src/setp_dtrna15.F90:41-75,114-161."""
+    offset, valence, site = {}, [], 0
+    for i, nm in enumerate(names):
+        if nm not in HBSITE_VALENCE:
+            continue
+        offset[i] = site
+        valence.extend(HBSITE_VALENCE[nm])
+        site += len(HBSITE_VALENCE[nm])
+    return offset, valence
+
+
+def assemble_system(names, unit, unit_class, topo, tempk=310.0, ionic_strength=0.150,
+                    viscosity=1.0e-5, ele_form=ELE_DH, cutoff_ele=99.0, dh_cutoff_type=0,
+                    neigh_margin=NEIGH_MARGIN, box=None, ewld_alpha=0.0, ewld_hmax=0,
+                    phos_charge=None, seq="", name=""):
+    """A System from bead names ('P', 'S', 'A'/'C'/'G'/'U' for bases, ion
+    species 'Mg', 'K', ...), their units, the class of each unit and the
+    topology records in ``topo`` (bd_mp/bd_nat/bd_coef, ba_*, st_* with
+    st_hsTm, hb_* with hb_site/hb_tertiary, tst_* with tst2st/tst_excl).
+    Per-bead excluded-volume, mass and Stokes parameters follow
+    src/setp_dtrna15.F90:77-161 and src/setp_mass_fric.F90:33-96; the
+    phosphate charge is -Zp(T, I) (src/simu_ele_set.F90:155-185, i_charge = 1)
+    unless phos_charge is given; list radii per unit pair follow
+    src/mloop_neigh_dist.F90:25-60,149."""
+    coef, cdist, zp = ele_params(tempk, ionic_strength)
+    visc = langevin_visc(viscosity)
+    mptype, iclass, exv_rad, exv_eps, charge, mass, radius = [], [], [], [], [], [], []
+    for nm in names:
+        if nm in ION_MPTYPE:
+            mptype.append(ION_MPTYPE[nm]); iclass.append(CLASS_ION)
+            charge.append(ION_CHARGE[nm])
+        else:
+            mptype.append(MP_PHOS if nm == "P" else (MP_SUGAR if nm == "S" else MP_BASE))
+            iclass.append(CLASS_RNA)
+            charge.append((-zp if phos_charge is None else float(phos_charge)) if nm == "P" else 0.0)
+        r, e = EXV_PARAM[nm]
+        exv_rad.append(r); exv_eps.append(math.sqrt(e))
+        mass.append(MASS[nm]); radius.append(STOKES_R[nm])
+    _, hbsite_valence = hbsite_layout(names)
+    exv_rad_a = np.asarray(exv_rad)
+    unit_a = np.asarray(unit)
+    nunit = len(unit_class)
+    rl2 = np.zeros((nunit, nunit))
+    p = Params()
+    for a in range(nunit):
+        for b in range(nunit):
+            if unit_class[a] == CLASS_RNA and unit_class[b] == CLASS_RNA:
+                mc = max(0.0, p.exv_dist, p.exv_dist_PS)
+            else:
+                ra = exv_rad_a[unit_a == a]
+                rb = exv_rad_a[unit_a == b]
+                mc = max(0.0, float((ra[:, None] + rb[None, :]).max()))
+            rl2[a, b] = (mc + neigh_margin) ** 2
+    mass_a = np.asarray(mass)
+    fric = 6.0 * F_PI * visc * np.asarray(radius) / mass_a
+    params = replace(Params(), neigh_margin=neigh_margin, ele_form=ele_form,
+                     dh_cutoff_type=dh_cutoff_type, cutoff_ele=cutoff_ele,
+                     ele_coef=coef, ele_cdist=cdist, beta_hb=beta_hb(tempk),
+                     ewld_alpha=ewld_alpha, ewld_hmax=ewld_hmax,
+                     periodic=1 if box is not None else 0,
+                     box=tuple(box) if box is not None else (0.0, 0.0, 0.0))
+    nh = len(topo.get("hb_mp", []))
+    extra = {k: topo[k] for k in ("tst_mp", "tst_nat", "tst_coef", "tst2st", "tst_excl") if k in topo}
+    return System(unit=unit_a, mptype=mptype, iclass=iclass, exv_rad=exv_rad_a,
+                  exv_eps=exv_eps, charge=charge, rlist2_unit=rl2,
+                  bd_mp=topo["bd_mp"], bd_nat=topo["bd_nat"], bd_coef=topo["bd_coef"],
+                  ba_mp=topo["ba_mp"], ba_nat=topo["ba_nat"], ba_coef=topo["ba_coef"],
+                  st_mp=topo["st_mp"], st_nat=topo["st_nat"], st_coef=topo["st_coef"],
+                  st_hsTm=topo["st_hsTm"],
+                  hb_mp=topo["hb_mp"], hb_nat=topo["hb_nat"], hb_coef=topo["hb_coef"],
+                  hb_site=topo["hb_site"],
+                  hb_tertiary=topo.get("hb_tertiary", np.zeros(nh, np.int32)),
+                  hbsite_valence=hbsite_valence,
+                  mass=mass_a, fric=fric, radius=radius, tempk=tempk, params=params,
+                  seq=seq, name=name, **extra)
+
+
 def build_rna_system(chains, tempk=310.0, ionic_strength=0.150, viscosity=1.0e-5,
                      ele_form=ELE_DH, cutoff_ele=99.0, dh_cutoff_type=0,
                      neigh_margin=NEIGH_MARGIN, ions=None, box=None,
@@ -189,16 +268,11 @@ def build_rna_system(chains, tempk=310.0, ionic_strength=0.150, viscosity=1.0e-5
       simu_ele_set.F90:155-185 (i_charge = 1), else the fixed value (e.g.
       -1.0 with explicit ions).
     """
-    coef, cdist, zp = ele_params(tempk, ionic_strength)
-    visc = langevin_visc(viscosity)
-    unit, mptype, iclass, names = [], [], [], []
-    exv_rad, exv_eps, charge, mass, radius = [], [], [], [], []
+    unit, names = [], []
     bd_mp, bd_nat, bd_coef = [], [], []
     ba_mp, ba_nat, ba_coef = [], [], []
     st_mp, st_nat, st_coef, st_hsTm = [], [], [], []
     hb_mp, hb_nat, hb_coef, hb_site = [], [], [], []
-    hbsite_valence = []
-    hbsite_offset = {}
     cobjs = []
     base = 0
     for u, (seq, pairs) in enumerate(chains):
@@ -206,25 +280,9 @@ def build_rna_system(chains, tempk=310.0, ionic_strength=0.150, viscosity=1.0e-5
         cobjs.append((ch, pairs))
         for nm in ch.names:
             unit.append(u)
-            is_base = nm in "ACGU" and len(nm) == 1 and nm not in ("P", "S")
-            mptype.append(MP_PHOS if nm == "P" else (MP_SUGAR if nm == "S" else MP_BASE))
-            iclass.append(CLASS_RNA)
-            r, e = EXV_PARAM[nm]
-            exv_rad.append(r); exv_eps.append(math.sqrt(e))
-            charge.append((-zp if phos_charge is None else float(phos_charge)) if nm == "P" else 0.0)
-            mass.append(MASS[nm]); radius.append(STOKES_R[nm])
             names.append(nm)
-            del is_base
         base += len(ch.names)
-    # hbond sites, src/setp_dtrna15.F90:41-75 (bead order)
-    site = 0
-    for i, nm in enumerate(names):
-        key = nm if nm in HBSITE_VALENCE else None
-        if key is None:
-            continue
-        hbsite_offset[i] = site
-        hbsite_valence.extend(HBSITE_VALENCE[key])
-        site += len(HBSITE_VALENCE[key])
+    hbsite_offset, _ = hbsite_layout(names)
     for ch, pairs in cobjs:
         seq, n = ch.seq, len(ch.seq)
         # bonds: PS, SB, SP per nucleotide
@@ -287,49 +345,90 @@ def build_rna_system(chains, tempk=310.0, ionic_strength=0.150, viscosity=1.0e-5
                 sites[2 * q] = hbsite_offset[ch.B(i)] + _SITE_LOCAL[(bi, ai)] - 1
                 sites[2 * q + 1] = hbsite_offset[ch.B(j)] + _SITE_LOCAL[(bj, aj)] - 1
             hb_site.append(sites)
-    nunit = len(chains)
+    unit_class = [CLASS_RNA] * len(chains)
     if ions:
-        u = nunit
-        nunit += 1
+        u = len(chains)
+        unit_class.append(CLASS_ION)
         for sp in ions:
-            unit.append(u); mptype.append(ION_MPTYPE[sp]); iclass.append(CLASS_ION)
-            r, e = EXV_PARAM[sp]
-            exv_rad.append(r); exv_eps.append(math.sqrt(e))
-            charge.append(ION_CHARGE[sp]); mass.append(MASS[sp]); radius.append(STOKES_R[sp])
+            unit.append(u)
             names.append(sp)
-    exv_rad_a = np.asarray(exv_rad)
-    unit_a = np.asarray(unit)
-    cls_unit = [CLASS_RNA] * len(chains) + ([CLASS_ION] if ions else [])
-    # list radius per unit pair, src/mloop_neigh_dist.F90:25-60,149
-    rl2 = np.zeros((nunit, nunit))
-    p = Params()
-    for a in range(nunit):
-        for b in range(nunit):
-            if cls_unit[a] == CLASS_RNA and cls_unit[b] == CLASS_RNA:
-                mc = max(0.0, p.exv_dist, p.exv_dist_PS)
-            else:
-                ra = exv_rad_a[unit_a == a]
-                rb = exv_rad_a[unit_a == b]
-                mc = max(0.0, float((ra[:, None] + rb[None, :]).max()))
-            rl2[a, b] = (mc + neigh_margin) ** 2
-    mass_a = np.asarray(mass)
-    fric = 6.0 * F_PI * visc * np.asarray(radius) / mass_a
-    params = replace(Params(), neigh_margin=neigh_margin, ele_form=ele_form,
-                     dh_cutoff_type=dh_cutoff_type, cutoff_ele=cutoff_ele,
-                     ele_coef=coef, ele_cdist=cdist, beta_hb=beta_hb(tempk),
-                     ewld_alpha=ewld_alpha, ewld_hmax=ewld_hmax,
-                     periodic=1 if box is not None else 0,
-                     box=tuple(box) if box is not None else (0.0, 0.0, 0.0))
-    nh = len(hb_mp)
-    return System(unit=unit_a, mptype=mptype, iclass=iclass, exv_rad=exv_rad_a,
-                  exv_eps=exv_eps, charge=charge, rlist2_unit=rl2,
-                  bd_mp=bd_mp, bd_nat=bd_nat, bd_coef=bd_coef,
-                  ba_mp=ba_mp, ba_nat=ba_nat, ba_coef=ba_coef,
-                  st_mp=st_mp, st_nat=st_nat, st_coef=st_coef, st_hsTm=st_hsTm,
-                  hb_mp=hb_mp, hb_nat=hb_nat, hb_coef=hb_coef, hb_site=hb_site,
-                  hb_tertiary=np.zeros(nh, np.int32), hbsite_valence=hbsite_valence,
-                  mass=mass_a, fric=fric, radius=radius, tempk=tempk, params=params,
-                  seq="|".join(c[0] for c in chains), name=name)
+    topo = dict(bd_mp=bd_mp, bd_nat=bd_nat, bd_coef=bd_coef, ba_mp=ba_mp, ba_nat=ba_nat,
+                ba_coef=ba_coef, st_mp=st_mp, st_nat=st_nat, st_coef=st_coef, st_hsTm=st_hsTm,
+                hb_mp=hb_mp, hb_nat=hb_nat, hb_coef=hb_coef, hb_site=hb_site)
+    return assemble_system(names, unit, unit_class, topo, tempk=tempk,
+                           ionic_strength=ionic_strength, viscosity=viscosity, ele_form=ele_form,
+                           cutoff_ele=cutoff_ele, dh_cutoff_type=dh_cutoff_type,
+                           neigh_margin=neigh_margin, box=box, ewld_alpha=ewld_alpha,
+                           ewld_hmax=ewld_hmax, phos_charge=phos_charge,
+                           seq="|".join(c[0] for c in chains), name=name)
+
+
+# ---- para/rna.para:359-361 (st_nlocal_*), U0 of the tbs-dist records of ninfo/bwyv19.ninfo ----
+TST_DIST_K, TST_ANGL_K, TST_DIH_K, TST_U0 = 5.0, 1.5, 0.15, -5.0
+
+
+def _angle(a, b, c):
+    u, v = a - b, c - b
+    return math.acos(max(-1.0, min(1.0, float(np.dot(u, v) / (np.linalg.norm(u) * np.linalg.norm(v))))))
+
+
+def _tstack_dihedrals(x, m):
+    """The three dihedrals of a tertiary stack (B1, B2, S1, S2, P1, P2) in
+    the convention of energy_dtrna_stack_nlocal.F90:163-212, so that a
+    native taken here is a minimum of the energy."""
+    v12, v13 = x[m[0]] - x[m[1]], x[m[0]] - x[m[2]]
+    v53, v42, v46 = x[m[4]] - x[m[2]], x[m[3]] - x[m[1]], x[m[3]] - x[m[5]]
+    c4212, c1213 = np.cross(v42, v12), np.cross(v12, v13)
+    a12, a13, a42 = (float(np.linalg.norm(v)) for v in (v12, v13, v42))
+    d1 = math.atan2(float(v42 @ c1213) * a12, float(c4212 @ c1213))
+    d2 = math.atan2(float(v53 @ -c1213) * a13, float(np.cross(v53, v13) @ -c1213))
+    n = np.cross(v42, v46)
+    d3 = math.atan2(float(v12 @ n) * a42, float(-c4212 @ n))
+    return d1, d2, d3
+
+
+def add_tertiary_stacks(sys: System, xyz, stacks, u0=TST_U0, dist_shift=None, rng=None):
+    """A copy of ``sys`` with tertiary (non-local) base stacks.
+
+    stacks: [(B1, B2, type1, type2), ...] with B1, B2 0-based base beads and
+    types as in tbs-dist records (read_nativeinfo.F90:624-705): the sign picks
+    the side whose consecutive stack a base may lose (negative: the stack
+    with the base three beads before, positive: after) and 2 marks the side
+    exclusive, 1 not.  The beads of a stack are (B1, B2, S1 = B1-1,
+    S2 = B2-1, P1 = B1+1, P2 = B2+1).  Native distance, angles and dihedrals
+    are taken from ``xyz`` (minus dist_shift[k] when given, to place stack k
+    that far from its native distance; with rng, N(0, 0.2) noise on the
+    distance unless dist_shift is given, and N(0, 0.15) / N(0, 0.3) radians
+    on the angles / dihedrals),
+    coefficients from para/rna.para:359-361.  The tertiary-to-consecutive
+    stack map follows src/mloop_dtrna.F90:73-136."""
+    from .ninfo import tertiary_stack_map
+    x = np.asarray(xyz, dtype=np.float64)
+    tst_mp, tst_nat, tst_coef, side, excl = [], [], [], [], []
+    for k, (b1, b2, t1, t2) in enumerate(stacks):
+        if b1 > b2:
+            b1, b2, t1, t2 = b2, b1, t2, t1
+        m = (b1, b2, b1 - 1, b2 - 1, b1 + 1, b2 + 1)
+        if sys.mptype[b1] != MP_BASE or sys.mptype[b2] != MP_BASE:
+            raise ValueError("tertiary stacks join two base beads")
+        d = float(np.linalg.norm(x[b1] - x[b2]))
+        if dist_shift is not None:
+            d -= float(dist_shift[k])
+        elif rng is not None:
+            d += float(rng.normal(0.0, 0.2))
+        tst_mp.append(m)
+        nat = [d, _angle(x[m[2]], x[m[0]], x[m[1]]), _angle(x[m[0]], x[m[1]], x[m[3]])]
+        nat += list(_tstack_dihedrals(x, m))
+        if rng is not None:
+            nat[1:3] = [a + float(rng.normal(0.0, 0.15)) for a in nat[1:3]]
+            nat[3:] = [a + float(rng.normal(0.0, 0.3)) for a in nat[3:]]
+        tst_nat.append(tuple(nat))
+        tst_coef.append((u0, TST_DIST_K, TST_ANGL_K, TST_ANGL_K, TST_DIH_K, TST_DIH_K, TST_DIH_K))
+        side.append((1 if t1 < 0 else 2, 1 if t2 < 0 else 2))
+        excl.append((1 if abs(t1) == 2 else 0, 1 if abs(t2) == 2 else 0))
+    tst2st, excl_a = tertiary_stack_map(sys.st_mp, tst_mp, side, excl)
+    return replace(sys, tst_mp=np.asarray(tst_mp), tst_nat=np.asarray(tst_nat),
+                   tst_coef=np.asarray(tst_coef), tst2st=tst2st, tst_excl=excl_a)
 
 
 def set_temperature(sys: System, tempk: float, ionic_strength: float) -> None:
