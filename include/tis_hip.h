@@ -38,10 +38,13 @@ enum { TIS_OK = 0, TIS_ERR_ARG = -1, TIS_ERR_HIP = -2, TIS_ERR_STATE = -3,
 enum { TIS_LIST_EXV = 0, TIS_LIST_ELE = 1, TIS_LIST_HB = 2 };
 
 /* counters of tis_status_counters (TIS_CNT_REBUILDS: whole-batch rebuilds,
- * plus one per system rebuild fired by the displacement trigger) */
+ * plus one per system rebuild fired by the displacement trigger;
+ * TIS_CNT_NB_OVERFLOW and the slot after it are the run's stop flag and
+ * completed-step count, zero between calls; TIS_CNT_RESUMES: runs resumed
+ * after a list rebuild overflowed the per-bead capacity) */
 enum { TIS_CNT_CLAMP_EXV = 0, TIS_CNT_CLAMP_ELE = 1, TIS_CNT_HB_DELETED = 2,
        TIS_CNT_REBUILDS = 3, TIS_CNT_STEPS = 4, TIS_CNT_NB_OVERFLOW = 5,
-       TIS_CNT_MAX = 8 };
+       TIS_CNT_RESUMES = 7, TIS_CNT_MAX = 8 };
 
 /* Langevin launch modes for tis_langevin_steps */
 enum { TIS_MODE_GRAPH = 0,   /* batched, hipGraph-captured, one step = 2 launches */
@@ -62,6 +65,15 @@ const char *tis_last_error(const tis_ctx *ctx);
  * (its beads share Langevin constants by that pair). */
 int32_t tis_system_upload(tis_ctx *ctx, int32_t nsys, const tis_system *systems,
                           const uint32_t *keys, uint64_t seed);
+
+/* Check a batch description without a device call: every bead index of
+ * every term in [0, nmp), units in [0, nunit), hbond sites in [-1, nhbsite),
+ * tst2st in [-1, nst), tst_excl 0/1, no NULL array behind a non-zero
+ * count.  TIS_ERR_ARG with the first offence in msg (msg_cap bytes, may be
+ * NULL).  tis_system_upload runs the same check first and leaves the
+ * previous batch in place when it fails.  No reference counterpart: the
+ * reference trusts its .ninfo reader (read_nativeinfo.F90). */
+int32_t tis_system_validate(int32_t nsys, const tis_system *systems, char *msg, int64_t msg_cap);
 
 /* Refresh the temperature-dependent coefficients of system isys (tempk,
  * st_coef U0, charge, ele_coef, ele_cdist, beta_hb) from `sys`, which must
@@ -116,6 +128,13 @@ int32_t tis_energy_unit(tis_ctx *ctx, int32_t isys, int32_t style, double *energ
  * accelerations are used, if -1 per system (tis_system_origin). */
 int32_t tis_langevin_init(tis_ctx *ctx, double tstep, int32_t rebuild_every, uint64_t step0,
                           int32_t init_accel);
+/* Runs nsteps steps.  When a list rebuild inside the run needs more
+ * entries per bead than the lists hold, the run stops at the start of that
+ * step (every later kernel of the run returns at once, the state is that
+ * of the step's start), the capacity grows (entries already built are
+ * kept) and the run resumes from that step: no step ever runs on truncated
+ * lists, and the trajectory equals that of a run with the larger capacity
+ * from the start.  TIS_CNT_RESUMES counts the resumes. */
 int32_t tis_langevin_steps(tis_ctx *ctx, int64_t nsteps, int32_t mode);
 
 /* Step origin of system isys: its own step count is (global step - origin),
@@ -136,6 +155,24 @@ int32_t tis_system_origin(tis_ctx *ctx, int32_t isys, uint64_t step_origin, int3
 int32_t tis_label_define(tis_ctx *ctx, int32_t label, int32_t isys, const tis_system *sys);
 int32_t tis_label_apply(tis_ctx *ctx, int32_t n, const int32_t *isys, const int32_t *label,
                         const double *vscale);
+
+/* hbond and stack states of system isys, as a restart file records them
+ * (the DTRNA15 block of write_rst.F90:96-106): hb_status[nhb] (may be
+ * NULL) of the last force evaluation, 1 = formed and kept by the valence
+ * pass; st_status[nst] (may be NULL) at the current coordinates, 0 = the
+ * consecutive stack is switched off by an exclusive tertiary stack
+ * (force_dtrna_stack_nlocal.F90:83-92). */
+int32_t tis_term_status(tis_ctx *ctx, int32_t isys, int32_t *hb_status, int32_t *st_status);
+
+/* Entries per bead of the excluded-volume lists (the lists are stored
+ * nb[entry][bead]).  cap > 0 sets it (the entries built so far are kept;
+ * TIS_ERR_ARG if the current lists hold more than cap entries for a bead),
+ * cap <= 0 only reads it; *cap_out (may be NULL) gets the capacity in
+ * force.  List builds grow it as they need (tis_neighbor,
+ * tis_langevin_init, and the resume of tis_langevin_steps).  No reference
+ * counterpart: the reference sizes its list by MXMPNEIGHBOR per bead
+ * (const_maxsize.F90, allocate_neighbor.F90). */
+int32_t tis_list_capacity(tis_ctx *ctx, int32_t cap, int32_t *cap_out);
 
 /* Device memory held by the context's batch, in bytes (admission control). */
 int32_t tis_device_bytes(tis_ctx *ctx, int64_t *bytes);

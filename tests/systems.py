@@ -100,6 +100,46 @@ def tstack_hairpin(seed=61, n=48):
     return s, x
 
 
+def valence_crowd(nc=72, seed=71):
+    """One G whose three hbond sites each take part in nc formed hbonds (a
+    G-C candidate with each of nc cytosines placed on a shell at the G-C
+    native distance): the valence pass deletes nc - 1 of them, the first
+    few from site sequences longer than 64.  The reference keeps at most 50
+    candidates per site (MXMPHBNEIGHBOR, neighbor_list_hb.F90:124-131), so
+    this case is checked against the oracle only."""
+    import math
+    rng = np.random.default_rng(seed)
+    seq = "G" + "AAA" + "C" * nc + "A"
+    s = T.build_rna_system([(seq, [(0, 4 + k) for k in range(nc)])], name="valence_crowd")
+    x = np.zeros((s.nmp, 3))
+    # Fibonacci sphere: consecutive nucleotides on neighbouring directions
+    n = len(seq)
+    ga = math.pi * (3.0 - math.sqrt(5.0))
+    dirs = []
+    for k in range(n):
+        z = 1.0 - 2.0 * (k + 0.5) / n
+        r = math.sqrt(1.0 - z * z)
+        dirs.append((r * math.cos(ga * k), r * math.sin(ga * k), z))
+    dirs = np.asarray(dirs)
+    # sugars and phosphates off the base's radial line, so that no three
+    # beads of an hbond's angles or dihedrals are collinear
+    def perp(d, k):
+        t = np.cross(d, (0.3, 0.5, 0.81) if k % 2 else (0.7, -0.2, 0.68))
+        return t / np.linalg.norm(t)
+
+    for k in range(n):
+        sb, bb = 3 * k, 3 * k + 1
+        if k == 0:
+            x[bb] = 0.0
+            x[sb] = -4.5 * dirs[n // 2] + 1.7 * perp(dirs[n // 2], 1)
+            continue
+        x[bb] = (5.655 + rng.uniform(-0.6, 0.6)) * dirs[k]
+        x[sb] = (9.8 + rng.uniform(-0.3, 0.3)) * dirs[k] + 2.6 * perp(dirs[k], k)
+        mid = (dirs[k] + dirs[k - 1]) / np.linalg.norm(dirs[k] + dirs[k - 1])
+        x[sb - 1] = 12.5 * mid + 2.0 * perp(mid, k + 1)
+    return s, x
+
+
 def parity_cases(ewald=True):
     out = [("hairpin64_helix",) + hairpin64(0, "helix"),
            ("hairpin64_compact",) + hairpin64(3, "compact")]
@@ -140,6 +180,7 @@ def unit_cases(cases=None):
     """Systems of several units for energy_unit: two chains + ions, the
     ion-rich compact chains (one periodic) and an Ewald box."""
     cases = parity_cases(ewald=False) if cases is None else cases
-    return ([("two_chains",) + two_chain_system()] + [c for c in cases if c[0] in ("compact2", "compact3")]
+    return ([("two_chains",) + two_chain_system()]
+            + [c for c in cases if c[0] in ("compact2", "compact3", "bwyv19")]
             + [("ions_box_ewald",) + small_ions_box()])
 

@@ -64,3 +64,32 @@ def test_fortran_module_binds_every_entry_point():
     bound = set(re.findall(r'bind\(C,\s*name="(tis_[a-z0-9_]+)"\)', src))
     missing = [n for n in declared_functions() if n not in bound]
     assert not missing, missing
+
+
+def test_validate_rejects_out_of_range_indices():
+    """tis_system_validate (run first by tis_system_upload) names the first
+    index outside its bounds; a 1-based bond index, a site past nhbsite or a
+    tertiary stack mapped past nst is refused without a device call."""
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from systems import hairpin64, tstack_hairpin
+    s, _ = hairpin64()
+    engine.validate([s])
+    bad = s.bd_mp.copy()
+    bad[-1, 1] = s.nmp  # a 1-based index slip
+    s.bd_mp = bad
+    with pytest.raises(engine.TisError, match="bd_mp"):
+        engine.validate([s])
+    s, _ = hairpin64()
+    site = s.hb_site.copy()
+    site[0, 0] = len(s.hbsite_valence)
+    s.hb_site = site
+    with pytest.raises(engine.TisError, match="hb_site"):
+        engine.validate([s])
+    t, _ = tstack_hairpin()
+    engine.validate([t])
+    m = t.tst2st.copy()
+    m[0, 0] = len(t.st_mp)
+    t.tst2st = m
+    with pytest.raises(engine.TisError, match="tst2st"):
+        engine.validate([hairpin64()[0], t])

@@ -33,7 +33,7 @@ def test_tensor_fill_matches_oracle(htype):
     e.close()
 
 
-@pytest.mark.parametrize("n", [21, 64, 300])
+@pytest.mark.parametrize("n", [21, 64, 300, 1000, 3000])
 def test_factor_against_fp64(n):
     """L from FP64 diagonal blocks + FP32 matrix-core trailing updates against
     the FP64 factor; ||L L^T - D|| / ||D|| on the device equals the host value."""

@@ -7,7 +7,8 @@ from tis_amd.engine import Engine, LIST_ELE, LIST_EXV, LIST_HB, MODE_EAGER, MODE
 
 from tis_amd import topology as T
 
-from systems import compact_allpairs, force_error, hairpin_design, hairpin64, parity_cases, unit_cases
+from systems import (compact_allpairs, force_error, hairpin_design, hairpin64, parity_cases, unit_cases,
+                     valence_crowd)
 
 pytestmark = pytest.mark.gpu
 CASES = parity_cases()
@@ -154,6 +155,70 @@ def test_langevin_valence_limited_matches_oracle(seed):
     assert ndel > 0
     assert np.isfinite(xg).all()
     assert d < 1e-4
+
+
+def test_valence_long_site_sequence():
+    """One G with 72 formed hbonds on each of its sites: the valence pass
+    runs its first deletions on site sequences longer than a wave (the
+    scratch-sequence path) and deletes 71; the deletion set, forces and
+    hbond energy equal the oracle's."""
+    s, x = valence_crowd()
+    print(f"[valence crowd] nmp={s.nmp} nhb={len(s.hb_mp)}", flush=True)
+    eng = Engine([s], keys=[0], seed=0)
+    eng.set_coords(0, x)
+    eng.neighbor(0)
+    c0 = eng.counters()["hb_deleted"]
+    f = eng.force_sumup(0)
+    ndel = eng.counters()["hb_deleted"] - c0
+    e = eng.energy_sumup(0)
+    eng.close()
+    o = oracle.Oracle(s)
+    o.neighbor(x)
+    fo, _ = o.force(x, seed=0, sys_id=0, step=0)
+    eo = o.energy(x)
+    err = force_error(f, fo)
+    print(f"   {ndel} deletions, max rel err {err.max():.3e}", flush=True)
+    assert ndel == len(s.hb_mp) - 1
+    assert err.max() <= 1e-5
+    assert abs(e[16] - eo[16]) <= 1e-7 * max(abs(eo[16]), 1e-6)
+
+
+@pytest.mark.parametrize("rebuild_every", [10, 0])
+@pytest.mark.parametrize("mode", [MODE_GRAPH, MODE_EAGER, MODE_PLAIN])
+def test_list_overflow_resumes_bitwise(mode, rebuild_every):
+    """A run whose list capacity is set to the largest list of the start
+    configuration: a later rebuild needs more entries, the run stops at that
+    step, grows the capacity and resumes.  The trajectory equals, bit for bit,
+    that of a run with ample capacity from the start (so no step ran on
+    truncated lists), and the resumes are counted."""
+    systems = [compact_allpairs(seed=41 + k, n=n) for k, n in enumerate([40, 64])]
+    systems += [hairpin_design(400 + k, n) for k, n in enumerate([90, 30])]
+    res = []
+    for tight in (False, True):
+        print(f"[overflow resume] mode={mode} rebuild_every={rebuild_every} tight={tight}", flush=True)
+        eng = Engine([s for s, _ in systems], keys=list(range(len(systems))), seed=7)
+        for k, (s, x) in enumerate(systems):
+            eng.set_coords(k, x)
+        eng.langevin_init(tstep=0.05, rebuild_every=rebuild_every)
+        if tight:
+            need = 0
+            for k in range(len(systems)):
+                p = eng.pairs(k, LIST_EXV)
+                ids = np.concatenate([p[:, 0], p[:, 1]]) if len(p) else np.zeros(0, int)
+                need = max(need, int(np.bincount(ids).max()) if len(ids) else 0)
+            assert eng.list_capacity(need) == need
+        eng.langevin_steps(120, mode)
+        c = eng.counters()
+        res.append(([eng.coords(k) for k in range(len(systems))], c))
+        if tight:
+            print(f"   resumes {c['resumes']} capacity {eng.list_capacity()}", flush=True)
+            assert c["resumes"] >= 1 and eng.list_capacity() > need
+            assert c["nb_overflow"] == 0 and c["run_completed"] == 0
+        eng.close()
+    for k in range(len(systems)):
+        assert np.array_equal(res[0][0][k], res[1][0][k])
+    for key in ("steps", "rebuilds", "hb_deleted"):
+        assert res[0][1][key] == res[1][1][key], key
 
 
 def test_modes_bitwise_equal():

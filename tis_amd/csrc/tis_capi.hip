@@ -23,7 +23,8 @@ __global__ void k_force(DevPtrs, const double *, const double *, const int32_t *
                         BondedRange, int32_t);
 __global__ void k_ewald_sk(DevPtrs, const double *, const EwTask *, int);
 __global__ void k_ewald_f(DevPtrs, const double *, const EwTask *);
-__global__ void k_main(DevPtrs, const double *, const double *, double *, double *, const Tile *, int, int32_t, int, int);
+__global__ void k_main(DevPtrs, const double *, const double *, double *, double *, const Tile *, int, int32_t, int, int,
+                       int);
 __global__ void k_prep(DevPtrs, const double *, double *, double *, const Tile *, int, int, const Tile *, int, int,
                        int, int);
 __global__ void k_lists(DevPtrs, const double *, const double *, const Tile *, int, int, const Tile *, int, int, int,
@@ -34,6 +35,7 @@ __global__ void k_advance(uint64_t *, uint64_t, unsigned long long *);
 __global__ void k_count_rebuild(unsigned long long *);
 __global__ void k_energy(DevPtrs, const double *, const double *, const int32_t *, double *, int);
 __global__ void k_label_apply(DevPtrs, LabelDev, const int32_t *, const int32_t *, const double *);
+__global__ void k_stack_status(DevPtrs, const double *, const double *, int, int, int32_t *);
 }  // namespace tis
 
 using namespace tis;
@@ -86,6 +88,7 @@ struct tis_ctx {
   int32_t *d_systask = nullptr, *d_sysidx = nullptr;
   int32_t nsystask = 0;
   std::vector<int32_t> systask_of;       // index into d_systask per system or -1
+  int32_t *d_ststat = nullptr;           // st_status output of tis_term_status [nsttot]
   double *d_energy = nullptr;
   double *d_eunit = nullptr;   // raw per-unit-pair energies [nunit^2][25]
   size_t eunit_cap = 0;
@@ -103,6 +106,7 @@ struct tis_ctx {
   double tstep = 0.0;
   int32_t rebuild_every = 10;
   int64_t t_since_init = 0;
+  uint64_t step_host = 0;                // the device step counter (stepbase) as the host knows it
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   int graph_len = 0;
   // timing
@@ -334,6 +338,56 @@ int launch_lists(tis_ctx *c, const Range &r, const double *X, double *PXcur, dou
 }  // namespace
 
 // =====================================================================
+// Index arrays of one system against their bounds (0-based): bead indices
+// of every term in [0, nmp), units in [0, nunit), hbond sites in
+// [-1, nhbsite), the tertiary-to-consecutive stack map in [-1, nst).  The
+// message goes to *err.
+int vfail(std::string *err, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (err) *err = buf;
+  return TIS_ERR_ARG;
+}
+
+int check_range(std::string *err, int is, const char *what, const int32_t *a, int64_t n, int32_t lo,
+                int32_t hi) {
+  if (n > 0 && !a) return vfail(err, "system %d: %s is NULL", is, what);
+  for (int64_t k = 0; k < n; ++k)
+    if (a[k] < lo || a[k] >= hi)
+      return vfail(err, "system %d: %s[%lld] = %d outside [%d, %d)", is, what, (long long)k, a[k], lo, hi);
+  return TIS_OK;
+}
+
+int validate_system(std::string *err, int is, const tis_system &s) {
+  if (s.nmp <= 0 || s.nunit <= 0) return vfail(err, "system %d has no beads or units", is);
+  if (s.nbd < 0 || s.nba < 0 || s.nst < 0 || s.ntst < 0 || s.nhb < 0 || s.nhbsite < 0)
+    return vfail(err, "system %d: negative term count", is);
+  const void *bead_arrays[] = {s.unit, s.mptype, s.iclass, s.exv_rad, s.exv_eps, s.charge, s.mass, s.fric,
+                               s.rlist2_unit};
+  for (const void *a : bead_arrays)
+    if (!a) return vfail(err, "system %d: a per-bead array is NULL", is);
+  int rc;
+  if ((rc = check_range(err, is, "unit", s.unit, s.nmp, 0, s.nunit))) return rc;
+  if ((rc = check_range(err, is, "bd_mp", s.bd_mp, 2 * (int64_t)s.nbd, 0, s.nmp))) return rc;
+  if ((rc = check_range(err, is, "ba_mp", s.ba_mp, 3 * (int64_t)s.nba, 0, s.nmp))) return rc;
+  if ((rc = check_range(err, is, "st_mp", s.st_mp, 7 * (int64_t)s.nst, 0, s.nmp))) return rc;
+  if ((rc = check_range(err, is, "tst_mp", s.tst_mp, 6 * (int64_t)s.ntst, 0, s.nmp))) return rc;
+  if ((rc = check_range(err, is, "tst2st", s.tst2st, 2 * (int64_t)s.ntst, -1, s.nst))) return rc;
+  if ((rc = check_range(err, is, "tst_excl", s.tst_excl, 2 * (int64_t)s.ntst, 0, 2))) return rc;
+  if ((rc = check_range(err, is, "hb_mp", s.hb_mp, 6 * (int64_t)s.nhb, 0, s.nmp))) return rc;
+  if ((rc = check_range(err, is, "hb_site", s.hb_site, 6 * (int64_t)s.nhb, -1, s.nhbsite))) return rc;
+  if (s.nhbsite > 0 && !s.hbsite_valence) return vfail(err, "system %d: hbsite_valence is NULL", is);
+  const void *nb[] = {s.bd_nat, s.bd_coef}, *na[] = {s.ba_nat, s.ba_coef}, *ns[] = {s.st_nat, s.st_coef},
+             *nt[] = {s.tst_nat, s.tst_coef}, *nh[] = {s.hb_nat, s.hb_coef};
+  for (int k = 0; k < 2; ++k)
+    if ((s.nbd && !nb[k]) || (s.nba && !na[k]) || (s.nst && !ns[k]) || (s.ntst && !nt[k]) || (s.nhb && !nh[k]))
+      return vfail(err, "system %d: a term parameter array is NULL", is);
+  return TIS_OK;
+}
+
 extern "C" {
 
 int32_t tis_ctx_create(int32_t device, tis_ctx **out) {
@@ -350,6 +404,21 @@ int32_t tis_ctx_create(int32_t device, tis_ctx **out) {
   (void)hipEventCreate(&c->ev1);
   *out = c;
   return TIS_OK;
+}
+
+int32_t tis_system_validate(int32_t nsys, const tis_system *systems, char *msg, int64_t msg_cap) {
+  std::string err;
+  int rc = TIS_OK;
+  if (nsys <= 0 || !systems) {
+    err = "bad arguments to tis_system_validate";
+    rc = TIS_ERR_ARG;
+  }
+  for (int is = 0; rc == TIS_OK && is < nsys; ++is) rc = validate_system(&err, is, systems[is]);
+  if (msg && msg_cap > 0) {
+    std::strncpy(msg, err.c_str(), (size_t)msg_cap - 1);
+    msg[msg_cap - 1] = '\0';
+  }
+  return rc;
 }
 
 int32_t tis_ctx_destroy(tis_ctx *c) {
@@ -376,6 +445,10 @@ const char *tis_last_error(const tis_ctx *c) { return c ? c->err.c_str() : "null
 int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const uint32_t *keys,
                           uint64_t seed) {
   if (!c || nsys <= 0 || !sys) return fail(c, TIS_ERR_ARG, "bad arguments to tis_system_upload");
+  // every index a kernel or the incidence build dereferences, checked before
+  // anything is freed or allocated (a failed upload keeps the previous batch)
+  for (int is = 0; is < nsys; ++is)
+    if (int rc = validate_system(&c->err, is, sys[is])) return rc;
   HIPCHK(c, hipSetDevice(c->device));
   free_all(c);
   c->nsys = nsys;
@@ -721,6 +794,8 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
   rc = dalloc(c, &P.hb_ene, (size_t)nhb); if (rc) return rc;
   rc = dalloc(c, &P.for_hb, 18 * (size_t)nhb); if (rc) return rc;
   rc = dalloc(c, &P.site_cnt, (size_t)nsite); if (rc) return rc;
+  rc = dalloc(c, &P.hb_seq, (size_t)nhb); if (rc) return rc;
+  rc = dalloc(c, &c->d_ststat, (size_t)nst); if (rc) return rc;
   rc = dalloc(c, &P.nb_cnt, (size_t)off); if (rc) return rc;
   P.nb_cap = 64;
   rc = dalloc(c, &P.nb, (size_t)P.nb_cap * off); if (rc) return rc;
@@ -866,6 +941,7 @@ static int rebuild_lists(tis_ctx *c, int32_t isys) {
     for (int k = 0; k < 2; ++k)
       if (c->graph[k]) { (void)hipGraphExecDestroy(c->graph[k]); c->graph[k] = nullptr; }
   }
+  (void)hipMemsetAsync(c->d_counters + TIS_CNT_NB_OVERFLOW, 0, sizeof(unsigned long long), c->stream);
   return fail(c, TIS_ERR_OVERFLOW, "neighbour list capacity could not be grown");
 }
 
@@ -941,7 +1017,7 @@ int32_t tis_pairs_get(tis_ctx *c, int32_t isys, int32_t which, int32_t *pairs, d
 // stepbase+t+1; a stand-alone force evaluation or the initial acceleration
 // uses stepbase.
 static void launch_force_pair(tis_ctx *c, const Range &r, const double *X, const double *PX, double *XN,
-                              double *PXN, int t_local, int mode) {
+                              double *PXN, int t_local, int mode, int count_step = 0) {
   const int32_t pre_t = mode == 1 ? (int32_t)t_local : -1;  // valence draws at the force step
   // timing events on this stream: e[0] start, e[3] before k_main; e[1]/e[2]
   // and e[4]/e[5] are bound to the k_force and k_main dispatches (start and
@@ -985,10 +1061,10 @@ static void launch_force_pair(tis_ctx *c, const Range &r, const double *X, const
   const int32_t ele_on = c->any_ele ? 1 : 0;
   if (r.btn > 0 && c->timing)
     hipExtLaunchKernelGGL(k_main, dim3(r.btn), dim3(kTile * 2), 0, c->stream, e3[4], e3[5], 0, c->P, X, PX, XN,
-                          PXN, c->d_btiles + r.bt0, r.btn, (int32_t)t_local, mode, ele_on);
+                          PXN, c->d_btiles + r.bt0, r.btn, (int32_t)t_local, mode, ele_on, count_step);
   else if (r.btn > 0)
     hipLaunchKernelGGL(k_main, dim3(r.btn), dim3(kTile * 2), 0, c->stream, c->P, X, PX, XN, PXN,
-                       c->d_btiles + r.bt0, r.btn, (int32_t)t_local, mode, ele_on);
+                       c->d_btiles + r.bt0, r.btn, (int32_t)t_local, mode, ele_on, count_step);
   else if (c->timing) {
     (void)hipEventRecord(e3[4], c->stream);
     (void)hipEventRecord(e3[5], c->stream);
@@ -1227,6 +1303,7 @@ int32_t tis_langevin_init(tis_ctx *c, double tstep, int32_t rebuild_every, uint6
     HIPCHK(c, hipMemcpyAsync((void *)c->P.lcls, lc.data(), lc.size() * sizeof(LangClass), hipMemcpyHostToDevice,
                              c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_stepbase, &step0, sizeof(step0), hipMemcpyHostToDevice, c->stream));
+  c->step_host = step0;
   rc = rebuild_lists(c, -1);
   if (rc) return rc;
   // displacements since this rebuild start at zero, no rebuild pending
@@ -1255,8 +1332,10 @@ int32_t tis_langevin_init(tis_ctx *c, double tstep, int32_t rebuild_every, uint6
   return TIS_OK;
 }
 
-// one Langevin step of the batch (or of system isys) in eager launches
-static int eager_step(tis_ctx *c, int32_t isys) {
+// The list part of one Langevin step of the batch (or of system isys) in
+// eager launches: the displacement trigger's rebuilds, or the whole range
+// every rebuild_every steps.
+static void eager_lists(tis_ctx *c, int32_t isys) {
   const int cu = c->cur;
   Range r = range_of(c, isys);
   if (c->P.dyn) {
@@ -1265,7 +1344,18 @@ static int eager_step(tis_ctx *c, int32_t isys) {
     launch_lists(c, r, c->P.x[cu], c->P.px[cu], c->P.px[cu ^ 1], 1, 0);
     if (isys <= 0) hipLaunchKernelGGL(k_count_rebuild, dim3(1), dim3(1), 0, c->stream, c->d_counters);
   }
-  launch_force_pair(c, r, c->P.x[cu ^ 1], c->P.px[cu ^ 1], c->P.x[cu], c->P.px[cu], 0, 1);
+}
+
+// The force and integration part (count_step: this launch completes the step).
+static void eager_forces(tis_ctx *c, int32_t isys, int count_step) {
+  const int cu = c->cur;
+  Range r = range_of(c, isys);
+  launch_force_pair(c, r, c->P.x[cu ^ 1], c->P.px[cu ^ 1], c->P.x[cu], c->P.px[cu], 0, 1, count_step);
+}
+
+static int eager_step(tis_ctx *c, int32_t isys) {
+  eager_lists(c, isys);
+  eager_forces(c, isys, 1);
   return 0;
 }
 
@@ -1289,7 +1379,7 @@ static int capture_graph(tis_ctx *c, int parity) {
       launch_lists(c, r, c->P.x[cu], c->P.px[cu], c->P.px[cu ^ 1], 1, 0);
       hipLaunchKernelGGL(k_count_rebuild, dim3(1), dim3(1), 0, c->stream, c->d_counters);
     }
-    launch_force_pair(c, r, c->P.x[cu ^ 1], c->P.px[cu ^ 1], c->P.x[cu], c->P.px[cu], t, 1);
+    launch_force_pair(c, r, c->P.x[cu ^ 1], c->P.px[cu ^ 1], c->P.x[cu], c->P.px[cu], t, 1, 1);
     cu ^= 1;
   }
   hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, c->stream, c->d_stepbase, (uint64_t)L, c->d_counters);
@@ -1302,13 +1392,10 @@ static int capture_graph(tis_ctx *c, int parity) {
   return 0;
 }
 
-int32_t tis_langevin_steps(tis_ctx *c, int64_t nsteps, int32_t mode) {
-  if (!c || nsteps < 0) return TIS_ERR_ARG;
-  if (!c->lv_init) return fail(c, TIS_ERR_STATE, "tis_langevin_init not called");
-  HIPCHK(c, hipSetDevice(c->device));
-  HIPCHK(c, hipMemsetAsync(c->d_counters + TIS_CNT_NB_OVERFLOW, 0, sizeof(unsigned long long), c->stream));
-  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-  int64_t done = 0, nlaunch = 0;
+// The steps of one run in the given mode; a stop (list overflow) makes the
+// remaining launches return at once.
+static int run_steps(tis_ctx *c, int64_t nsteps, int32_t mode, int64_t *nlaunch) {
+  int64_t done = 0;
   const int L = steps_per_graph(c);
   if (mode == TIS_MODE_GRAPH) {
     while (done < nsteps) {
@@ -1316,11 +1403,11 @@ int32_t tis_langevin_steps(tis_ctx *c, int64_t nsteps, int32_t mode) {
         if (!c->graph[c->cur]) { int rc = capture_graph(c, c->cur); if (rc) return rc; }
         HIPCHK(c, hipGraphLaunch(c->graph[c->cur], c->stream));
         done += L; c->t_since_init += L;  // L even: parity unchanged
-        nlaunch += 1;
+        *nlaunch += 1;
       } else {
         eager_step(c, -1);
         hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, c->stream, c->d_stepbase, (uint64_t)1, c->d_counters);
-        c->cur ^= 1; ++done; ++c->t_since_init; nlaunch += 2;
+        c->cur ^= 1; ++done; ++c->t_since_init; *nlaunch += 2;
       }
     }
   } else if (mode == TIS_MODE_EAGER) {
@@ -1329,18 +1416,143 @@ int32_t tis_langevin_steps(tis_ctx *c, int64_t nsteps, int32_t mode) {
     for (; done < nsteps; ++done) {
       eager_step(c, -1);
       hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, c->stream, c->d_stepbase, (uint64_t)1, c->d_counters);
-      c->cur ^= 1; ++c->t_since_init; nlaunch += 2;
-    }
-  } else if (mode == TIS_MODE_PLAIN) {
-    for (; done < nsteps; ++done) {
-      for (int is = 0; is < c->nsys; ++is) eager_step(c, is);
-      hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, c->stream, c->d_stepbase, (uint64_t)1, c->d_counters);
-      c->cur ^= 1; ++c->t_since_init; nlaunch += 2 * c->nsys;
+      c->cur ^= 1; ++c->t_since_init; *nlaunch += 2;
     }
   } else {
-    return fail(c, TIS_ERR_ARG, "unknown mode %d", mode);
+    // one system per launch: every system's lists, then every system's
+    // force and integration launches (the last one completes the step)
+    for (; done < nsteps; ++done) {
+      for (int is = 0; is < c->nsys; ++is) eager_lists(c, is);
+      for (int is = 0; is < c->nsys; ++is) eager_forces(c, is, is == c->nsys - 1);
+      hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, c->stream, c->d_stepbase, (uint64_t)1, c->d_counters);
+      c->cur ^= 1; ++c->t_since_init; *nlaunch += 2 * c->nsys;
+    }
   }
   HIPCHK(c, hipGetLastError());
+  return TIS_OK;
+}
+
+// Larger per-bead list capacity; the entries built so far are kept (the
+// layout nb[slot][ntot] keeps its stride), so systems whose lists are not
+// rebuilt keep theirs.  Captured graphs hold the old capacity: dropped.
+static int grow_nb(tis_ctx *c, unsigned long long need) {
+  const int newcap = (int)(need + need / 2 + 16);
+  if (newcap <= c->P.nb_cap) return TIS_OK;
+  uint32_t *nb = nullptr;
+  int rc = dalloc(c, &nb, (size_t)newcap * c->ntot);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(nb, c->P.nb, sizeof(uint32_t) * (size_t)c->P.nb_cap * c->ntot, hipMemcpyDeviceToDevice,
+                           c->stream));
+  c->P.nb = nb;  // the old buffer stays in allocs until the batch is freed
+  c->P.nb_cap = newcap;
+  for (int k = 0; k < 2; ++k)
+    if (c->graph[k]) { (void)hipGraphExecDestroy(c->graph[k]); c->graph[k] = nullptr; }
+  return TIS_OK;
+}
+
+int32_t tis_term_status(tis_ctx *c, int32_t isys, int32_t *hb_status, int32_t *st_status) {
+  if (!c) return TIS_ERR_ARG;
+  int rc = check_sys(c, isys);
+  if (rc) return rc;
+  if (isys < 0) return fail(c, TIS_ERR_ARG, "tis_term_status needs one system");
+  HIPCHK(c, hipSetDevice(c->device));
+  const DSys &d = c->hs[isys].d;
+  if (hb_status && d.nhb > 0) {
+    std::vector<uint8_t> h((size_t)d.nhb);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->P.hb_status + d.hoff, h.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < d.nhb; ++k) hb_status[k] = h[k];
+  }
+  if (st_status && d.nst > 0) {
+    hipLaunchKernelGGL(k_stack_status, dim3((d.nst + 255) / 256), dim3(256), 0, c->stream, c->P, Xc(c), PXc(c),
+                       d.soff, d.nst, c->d_ststat);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(st_status, c->d_ststat, sizeof(int32_t) * (size_t)d.nst, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  return TIS_OK;
+}
+
+int32_t tis_list_capacity(tis_ctx *c, int32_t cap, int32_t *cap_out) {
+  if (!c) return TIS_ERR_ARG;
+  int rc = check_sys(c, -1);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (cap > 0 && cap != c->P.nb_cap) {
+    int need = 0;
+    if (c->lists_valid) {
+      std::vector<int32_t> cnt((size_t)c->ntot);
+      HIPCHK(c, hipMemcpy(cnt.data(), c->P.nb_cnt, sizeof(int32_t) * cnt.size(), hipMemcpyDeviceToHost));
+      for (int32_t v : cnt) need = std::max(need, v);
+    }
+    if (cap < need)
+      return fail(c, TIS_ERR_ARG, "capacity %d below the %d entries of the current lists", cap, need);
+    uint32_t *nb = nullptr;
+    rc = dalloc(c, &nb, (size_t)cap * c->ntot);
+    if (rc) return rc;
+    const int keep = std::min(cap, c->P.nb_cap);
+    HIPCHK(c, hipMemcpyAsync(nb, c->P.nb, sizeof(uint32_t) * (size_t)keep * c->ntot, hipMemcpyDeviceToDevice,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->P.nb = nb;
+    c->P.nb_cap = cap;
+    for (int k = 0; k < 2; ++k)
+      if (c->graph[k]) { (void)hipGraphExecDestroy(c->graph[k]); c->graph[k] = nullptr; }
+  }
+  if (cap_out) *cap_out = c->P.nb_cap;
+  return TIS_OK;
+}
+
+int32_t tis_langevin_steps(tis_ctx *c, int64_t nsteps, int32_t mode) {
+  if (!c || nsteps < 0) return TIS_ERR_ARG;
+  if (!c->lv_init) return fail(c, TIS_ERR_STATE, "tis_langevin_init not called");
+  if (mode != TIS_MODE_GRAPH && mode != TIS_MODE_EAGER && mode != TIS_MODE_PLAIN)
+    return fail(c, TIS_ERR_ARG, "unknown mode %d", mode);
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemsetAsync(c->d_counters + TIS_CNT_NB_OVERFLOW, 0, 2 * sizeof(unsigned long long), c->stream));
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  int64_t nlaunch = 0, remaining = nsteps;
+  for (int attempt = 0;; ++attempt) {
+    const int cur0 = c->cur;
+    const int64_t t0 = c->t_since_init;
+    const uint64_t step0 = c->step_host;
+    int rc = run_steps(c, remaining, mode, &nlaunch);
+    if (rc) return rc;
+    unsigned long long st[2] = {0ull, 0ull};  // overflow need, steps completed
+    HIPCHK(c, hipMemcpyAsync(st, c->d_counters + TIS_CNT_NB_OVERFLOW, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (st[0] == 0ull) {
+      c->step_host = step0 + (uint64_t)remaining;
+      HIPCHK(c, hipMemsetAsync(c->d_counters + TIS_CNT_NB_OVERFLOW, 0, 2 * sizeof(unsigned long long), c->stream));
+      break;
+    }
+    // a rebuild overflowed the list capacity: the run stopped at the start
+    // of the step after the st[1] completed ones; put the step counters
+    // there, grow the capacity and resume
+    if (attempt >= 8) {
+      (void)hipMemsetAsync(c->d_counters + TIS_CNT_NB_OVERFLOW, 0, 2 * sizeof(unsigned long long), c->stream);
+      return fail(c, TIS_ERR_OVERFLOW, "neighbour list capacity could not be grown");
+    }
+    const int64_t completed = (int64_t)st[1];
+    c->cur = cur0 ^ (int)(completed & 1);
+    c->t_since_init = t0 + completed;
+    c->step_host = step0 + (uint64_t)completed;
+    unsigned long long cnt[TIS_CNT_MAX];
+    uint64_t sb = 0;
+    HIPCHK(c, hipMemcpy(cnt, c->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(&sb, c->d_stepbase, sizeof(sb), hipMemcpyDeviceToHost));
+    cnt[TIS_CNT_STEPS] += c->step_host - sb;
+    cnt[TIS_CNT_NB_OVERFLOW] = 0ull;
+    cnt[TIS_CNT_NB_OVERFLOW + 1] = 0ull;
+    cnt[TIS_CNT_RESUMES] += 1ull;
+    sb = c->step_host;
+    rc = grow_nb(c, st[0]);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpy(c->d_counters, cnt, sizeof(cnt), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_stepbase, &sb, sizeof(sb), hipMemcpyHostToDevice));
+    remaining -= completed;
+  }
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
   HIPCHK(c, hipEventSynchronize(c->ev1));
   float ms = 0.0f;
@@ -1362,9 +1574,6 @@ int32_t tis_langevin_steps(tis_ctx *c, int64_t nsteps, int32_t mode) {
     c->timing = false;
     c->tev_used = 0;
   }
-  unsigned long long ovf = 0;
-  HIPCHK(c, hipMemcpy(&ovf, c->d_counters + TIS_CNT_NB_OVERFLOW, sizeof(ovf), hipMemcpyDeviceToHost));
-  if (ovf) return fail(c, TIS_ERR_OVERFLOW, "neighbour list overflow during the run (%llu > %d)", ovf, c->P.nb_cap);
   return TIS_OK;
 }
 

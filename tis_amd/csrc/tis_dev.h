@@ -141,6 +141,7 @@ struct DevPtrs {
   double *hb_ene;            // [nhb]
   double *for_hb;            // [18][nhb]
   int32_t *site_cnt;         // [nsite]
+  int32_t *hb_seq;           // [nhb] valence-pass scratch: a site's hbonds, per system at its hoff
   // lists
   int32_t *nb_cnt; uint32_t *nb; int32_t nb_cap;
   const int32_t *qbead;      // [nq] global bead of each charge

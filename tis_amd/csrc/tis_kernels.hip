@@ -391,6 +391,15 @@ __device__ inline int site_count(const DevPtrs &P, int site) {
 }
 
 // Orders the memory operations of a wave's lanes around it.
+// A list build that overflowed the per-bead capacity (TIS_CNT_NB_OVERFLOW,
+// set by the list passes) stops the run: every later kernel of the step
+// sequence returns at once, so the state stays as it was when that
+// rebuild's step began, and the host grows the capacity and resumes from
+// that step.
+__device__ inline bool run_stopped(const DevPtrs &P) {
+  return __builtin_nontemporal_load(&P.counters[5]) != 0ull;
+}
+
 __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -405,14 +414,18 @@ __device__ inline bool site_in_excess(const DevPtrs &P, int site) {
 // run by the whole wave (every lane calls it).  The scans are wave-wide:
 // sites in excess are counted and the pick-th one found by ballots over
 // chunks of 64 sites, and the listed, formed hbonds of the picked site are
-// gathered in list order by ballots over chunks of 64 candidates into a
-// lane-distributed sequence (lane i holds element i; at most 64 kept).  The
-// shuffle and the Metropolis choice then run on every lane alike, with the
-// draws of the counter-based generator taken in the same order as a serial
-// pass, so that the result equals the oracle's bit for bit.  Lane 0 applies
-// each deletion; the fence after it orders the next scan's reads.
+// gathered in list order by ballots over chunks of 64 candidates into the
+// system's scratch sequence (P.hb_seq, one entry per candidate, so any
+// length fits).  A sequence of at most 64 is moved into registers (lane i
+// holds element i) and the shuffle and the Metropolis choice run on every
+// lane alike; a longer one is shuffled and chosen from by lane 0 in the
+// scratch copy.  Either way the draws of the counter-based generator are
+// taken in the order of a serial pass, so that the result equals the
+// oracle's bit for bit.  Lane 0 applies each deletion; the fence after it
+// orders the next scan's reads.
 __device__ int hbond_valence_wave(const DevPtrs &P, const DSys &s, uint64_t step, int lane) {
   const uint64_t below = (1ull << lane) - 1ull;
+  int32_t *seq = P.hb_seq + s.hoff;
   int ndel = 0;
   uint32_t draw = 0;
   for (int iter = 0; iter < (1 << 20); ++iter) {
@@ -437,39 +450,62 @@ __device__ int hbond_valence_wave(const DevPtrs &P, const DSys &s, uint64_t step
       }
       c += n;
     }
-    // hbonds of the site in list order (at most 64), element i on lane i
-    int myh = -1, nseq = 0;
-    for (int q0 = 0; q0 < s.nhb && nseq < kTile; q0 += kTile) {
+    // hbonds of the site in list order into the scratch sequence
+    int nseq = 0;
+    for (int q0 = 0; q0 < s.nhb; q0 += kTile) {
       const int q = q0 + lane;
       const int h = s.hoff + q;
       bool m = false;
       if (q < s.nhb && P.hb_inlist[h] && P.hb_status[h])
         for (int k = 0; k < 6; ++k) m = m || P.hb_site[6 * h + k] == site;
-      uint64_t b = __ballot(m);
-      while (b && nseq < kTile) {
-        const int hv = __shfl(h, __builtin_ctzll(b));
-        b &= b - 1ull;
-        if (lane == nseq) myh = hv;
-        ++nseq;
-      }
+      const uint64_t b = __ballot(m);
+      if (m) seq[nseq + __popcll(b & below)] = h;
+      nseq += __popcll(b);
     }
     if (nseq == 0) break;
-    for (int i = 0; i < nseq; ++i) {
-      rnd = uniform01(P.seed, s.key, step, 0, draw++);
-      const int isw = (int)ceil(rnd * nseq) - 1;  // 0 <= isw < nseq
-      const int a = __shfl(myh, i), c = __shfl(myh, isw);
-      if (lane == i) myh = c;
-      if (lane == isw) myh = a;
-    }
-    const double myene = lane < nseq ? P.hb_ene[myh] : 0.0;
-    int del = __shfl(myh, 0);
-    double edel = __shfl(myene, 0);
-    for (int i = 1; i < nseq; ++i) {
-      const int j = __shfl(myh, i);
-      const double ej = __shfl(myene, i);
-      const double ratio = exp((ej - edel) * s.beta_hb);
-      rnd = uniform01(P.seed, s.key, step, 0, draw++);
-      if (rnd < ratio) { del = j; edel = ej; }
+    wave_sync();
+    int del;
+    if (nseq <= kTile) {
+      int myh = lane < nseq ? seq[lane] : -1;
+      for (int i = 0; i < nseq; ++i) {
+        rnd = uniform01(P.seed, s.key, step, 0, draw++);
+        const int isw = (int)ceil(rnd * nseq) - 1;  // 0 <= isw < nseq
+        const int a = __shfl(myh, i), c = __shfl(myh, isw);
+        if (lane == i) myh = c;
+        if (lane == isw) myh = a;
+      }
+      const double myene = lane < nseq ? P.hb_ene[myh] : 0.0;
+      del = __shfl(myh, 0);
+      double edel = __shfl(myene, 0);
+      for (int i = 1; i < nseq; ++i) {
+        const int j = __shfl(myh, i);
+        const double ej = __shfl(myene, i);
+        const double ratio = exp((ej - edel) * s.beta_hb);
+        rnd = uniform01(P.seed, s.key, step, 0, draw++);
+        if (rnd < ratio) { del = j; edel = ej; }
+      }
+    } else {
+      del = -1;
+      if (lane == 0) {
+        uint32_t d = draw;
+        for (int i = 0; i < nseq; ++i) {
+          const double r = uniform01(P.seed, s.key, step, 0, d++);
+          const int isw = (int)ceil(r * nseq) - 1;
+          const int a = seq[i];
+          seq[i] = seq[isw];
+          seq[isw] = a;
+        }
+        del = seq[0];
+        double edel = P.hb_ene[del];
+        for (int i = 1; i < nseq; ++i) {
+          const int j = seq[i];
+          const double ej = P.hb_ene[j];
+          const double ratio = exp((ej - edel) * s.beta_hb);
+          if (uniform01(P.seed, s.key, step, 0, d++) < ratio) { del = j; edel = ej; }
+        }
+      }
+      del = __shfl(del, 0);
+      draw += 2u * (uint32_t)nseq - 1u;
     }
     if (lane == 0) {
       P.hb_status[del] = 0;
@@ -726,6 +762,7 @@ __global__ __launch_bounds__(kTile * kWaves) __attribute__((amdgpu_waves_per_eu(
 void k_force(DevPtrs P, const double *X, const double *PX, const int32_t *systask, int ntask,
              const EwTask *ptasks, int npt, BondedRange br, int32_t t_local) {
   __shared__ EleLds lds;
+  if (run_stopped(P)) return;
   int b = (int)blockIdx.x;
   const int npb = (npt + kWaves - 1) / kWaves;   // blocks of four chunk pairs
   const int ntb = (ntask + kWaves - 1) / kWaves;   // blocks of four system terms passes
@@ -770,6 +807,7 @@ void k_force(DevPtrs P, const double *X, const double *PX, const int32_t *systas
 // exact = 1 (energy evaluation) takes sin/cos in FP64.
 __global__ __launch_bounds__(kTile) void k_ewald_sk(DevPtrs P, const double *PX, const EwTask *tasks, int exact) {
   __shared__ double fx[kTile], fy[kTile], fz[kTile], fq[kTile];
+  if (run_stopped(P)) return;
   const EwTask tk = tasks[blockIdx.x];
   const DSys s = P.sys[tk.sys];
   const int lane = threadIdx.x, n = P.ntot;
@@ -818,6 +856,7 @@ __global__ __launch_bounds__(kTile) void k_ewald_sk(DevPtrs P, const double *PX,
 __global__ __launch_bounds__(kTile * kWaves) void k_ewald_f(DevPtrs P, const double *PX, const EwTask *tasks) {
   __shared__ double shx[kEwKChunk], shy[kEwKChunk], shz[kEwKChunk], sfc[kEwKChunk], ssc[kEwKChunk], sss[kEwKChunk];
   __shared__ double red[3][kWaves][kTile];
+  if (run_stopped(P)) return;
   const EwTask tk = tasks[blockIdx.x];
   const DSys s = P.sys[tk.sys];
   const int tid = threadIdx.x, n = P.ntot;
@@ -1032,11 +1071,20 @@ __device__ inline double main_tail(const DevPtrs &P, const DSys &s, int g, doubl
 
 __global__ __launch_bounds__(kTile * 2) void k_main(DevPtrs P, const double *X, const double *PX,
                                                     double *XN, double *PXN, const Tile *tiles,
-                                                    int ntile, int32_t t_local, int mode, int do_dh) {
+                                                    int ntile, int32_t t_local, int mode, int do_dh,
+                                                    int count_step) {
   __shared__ double red[3][2][kTile], gsh[3][kTile];
+  if (run_stopped(P)) return;
+  // steps completed in the current run (the host resumes a stopped run
+  // after the last of them)
+  if (count_step && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&P.counters[6], 1ull);
   const int lane = threadIdx.x & (kTile - 1), w = threadIdx.x / kTile;
   const Tile tl = tiles[blockIdx.x];
   const DSys s = P.sys[tl.sys];
+  // a displacement-triggered rebuild of this step is counted here, in the
+  // step's last kernel, so that a step the run resumes from counts once
+  if (mode == 1 && P.dyn && tl.start == 0 && threadIdx.x == 0 && P.sys_flag[tl.sys])
+    atomicAdd(&P.counters[3], 1ull);
   const int li = tl.start + lane;
   const bool valid = li < s.nmp;
   const int g = s.off + (valid ? li : 0);
@@ -1525,6 +1573,7 @@ __global__ __launch_bounds__(kTile * kWaves) void k_lpair(DevPtrs P, const doubl
                                                           const EwTask *bpairs, int nbp, const EwTask *qpairs,
                                                           int nqp, int h0, int nh, int mode, int pass) {
   __shared__ ListLds lds[kWaves];
+  if (run_stopped(P)) return;
   const int w = (int)threadIdx.x / kTile, lane = (int)threadIdx.x & (kTile - 1);
   const int nbb = (nbp + kWaves - 1) / kWaves, nqb = (nqp + kWaves - 1) / kWaves;
   const int b = (int)blockIdx.x;
@@ -1599,6 +1648,7 @@ __device__ inline bool system_trigger(const DevPtrs &P, const DSys &s) {
 __global__ __launch_bounds__(kTile) void k_prep(DevPtrs P, const double *X, double *PXc, double *PXp,
                                                 const Tile *btiles, int bt0, int nbt, const Tile *qtiles, int qt0,
                                                 int nqt, int with_pred, int gate) {
+  if (run_stopped(P)) return;
   const int b = (int)blockIdx.x;
   const bool charge = b >= nbt;
   const int it = charge ? qt0 + b - nbt : bt0 + b;
@@ -1606,12 +1656,9 @@ __global__ __launch_bounds__(kTile) void k_prep(DevPtrs P, const double *X, doub
   const DSys s = P.sys[tl.sys];
   if (gate) {
     // every block of the system reaches the same decision; the first bead
-    // tile records it for k_lists and counts the rebuild
+    // tile records it for k_lists and k_main (which counts the rebuild)
     const bool fire = system_trigger(P, s);
-    if (!charge && tl.start == 0 && threadIdx.x == 0) {
-      P.sys_flag[tl.sys] = fire ? 1 : 0;
-      if (fire) atomicAdd(&P.counters[3], 1ull);
-    }
+    if (!charge && tl.start == 0 && threadIdx.x == 0) P.sys_flag[tl.sys] = fire ? 1 : 0;
     if (!fire) return;
   }
   if (!charge) {
@@ -1633,6 +1680,7 @@ __global__ __launch_bounds__(kTile) void k_lists(DevPtrs P, const double *X, con
                                                  int bt0, int nbt, const Tile *qtiles, int qt0, int nqt, int h0,
                                                  int nh, int gate) {
   __shared__ ListLds lds;
+  if (run_stopped(P)) return;
   const int b = (int)blockIdx.x;
   if (b < nbt + nqt) {
     const bool charge = b >= nbt;
@@ -1652,12 +1700,29 @@ __global__ __launch_bounds__(kTile) void k_lists(DevPtrs P, const double *X, con
   list_hb_entry(P, P.sys[P.hb_sys[h]], X, PX, h);
 }
 
+// st_status of the consecutive stacks [s0, s0 + ns) at the given
+// coordinates (force_dtrna_stack_nlocal.F90:83-92 via stack_on), for output.
+__global__ __launch_bounds__(256) void k_stack_status(DevPtrs P, const double *X, const double *PX, int s0, int ns,
+                                                      int32_t *out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < ns) out[k] = stack_on(P, X, PX, s0 + k) ? 1 : 0;
+}
+
+// The step counter: by steps at once after a captured window, one step per
+// eager step.  A stopped run does not advance it (the host reads it to know
+// where to resume).
 __global__ void k_advance(uint64_t *stepbase, uint64_t by, unsigned long long *counters) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) { *stepbase += by; counters[4] += by; }
+  if (threadIdx.x == 0 && blockIdx.x == 0 && counters[5] == 0ull) { *stepbase += by; counters[4] += by; }
+}
+
+// One step of a captured window: the step counter moves per step, so a stop
+// inside a window leaves it at the step to resume from.
+__global__ void k_advance_step(uint64_t *stepbase, unsigned long long *counters) {
+  if (threadIdx.x == 0 && blockIdx.x == 0 && counters[5] == 0ull) { *stepbase += 1ull; counters[4] += 1ull; }
 }
 
 __global__ void k_count_rebuild(unsigned long long *counters) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) counters[3] += 1;
+  if (threadIdx.x == 0 && blockIdx.x == 0 && counters[5] == 0ull) counters[3] += 1;
 }
 
 // ------------------------------------------------------------------ energies

@@ -25,7 +25,8 @@ LIBPATH = os.path.join(HERE, "libtis_hip.so")
 
 MODE_GRAPH, MODE_EAGER, MODE_PLAIN = 0, 1, 2
 LIST_EXV, LIST_ELE, LIST_HB = 0, 1, 2
-CNT_NAMES = ("clamp_exv", "clamp_ele", "hb_deleted", "rebuilds", "steps", "nb_overflow")
+CNT_NAMES = ("clamp_exv", "clamp_ele", "hb_deleted", "rebuilds", "steps", "nb_overflow", "run_completed",
+             "resumes")
 
 _f64p = C.POINTER(C.c_double)
 _i32p = C.POINTER(C.c_int32)
@@ -50,7 +51,10 @@ def load_library():
     L.tis_last_error.restype = C.c_char_p
     L.tis_last_error.argtypes = [vp]
     L.tis_system_upload.argtypes = [vp, C.c_int32, C.POINTER(CSystem), C.POINTER(C.c_uint32), C.c_uint64]
+    L.tis_system_validate.argtypes = [C.c_int32, C.POINTER(CSystem), C.c_char_p, C.c_int64]
     L.tis_set_system_T.argtypes = [vp, C.c_int32, C.POINTER(CSystem)]
+    L.tis_list_capacity.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32)]
+    L.tis_term_status.argtypes = [vp, C.c_int32, _i32p, _i32p]
     L.tis_state_set.argtypes = [vp, C.c_int32, C.c_int32, _f64p]
     L.tis_state_get.argtypes = [vp, C.c_int32, C.c_int32, _f64p]
     L.tis_neighbor.argtypes = [vp, C.c_int32]
@@ -69,7 +73,9 @@ def load_library():
     L.tis_synchronize.argtypes = [vp]
     L.tis_last_timing.argtypes = [vp, _f64p, _f64p, _f64p, C.POINTER(C.c_int64)]
     L.tis_last_force_ms.argtypes = [vp, _f64p]
-    for fn in ("tis_ctx_create", "tis_ctx_destroy", "tis_system_upload", "tis_set_system_T", "tis_state_set",
+    for fn in ("tis_ctx_create", "tis_ctx_destroy", "tis_system_upload", "tis_system_validate", "tis_set_system_T",
+               "tis_list_capacity", "tis_term_status",
+               "tis_state_set",
                "tis_state_get", "tis_neighbor", "tis_pairs_get", "tis_force_sumup", "tis_force_get",
                "tis_energy_sumup", "tis_energy_unit", "tis_langevin_init", "tis_langevin_steps", "tis_status_counters",
                "tis_system_origin", "tis_device_bytes", "tis_label_define", "tis_label_apply",
@@ -81,6 +87,17 @@ def load_library():
 
 def _fp(a):
     return a.ctypes.data_as(_f64p)
+
+
+def validate(systems) -> None:
+    """tis_system_validate: raise TisError naming the first index out of
+    range in a batch description (no device call)."""
+    L = load_library()
+    cs = (CSystem * len(systems))(*[s.cstruct() for s in systems])
+    msg = C.create_string_buffer(512)
+    rc = L.tis_system_validate(len(systems), cs, msg, len(msg))
+    if rc != 0:
+        raise TisError(f"invalid system ({rc}): {msg.value.decode()}")
 
 
 class Engine:
@@ -233,6 +250,21 @@ class Engine:
         a = (C.c_int64 * 8)()
         self._chk(self.L.tis_status_counters(self.ctx, a), "tis_status_counters")
         return dict(zip(CNT_NAMES, list(a)[:len(CNT_NAMES)]))
+
+    def term_status(self, isys):
+        """(hb_status[nhb], st_status[nst]) as bool arrays (tis_term_status)."""
+        s = self.systems[isys]
+        hb = np.zeros(max(len(s.hb_mp), 1), np.int32)
+        st = np.zeros(max(len(s.st_mp), 1), np.int32)
+        self._chk(self.L.tis_term_status(self.ctx, isys, hb.ctypes.data_as(_i32p), st.ctypes.data_as(_i32p)),
+                  "tis_term_status")
+        return hb[:len(s.hb_mp)] != 0, st[:len(s.st_mp)] != 0
+
+    def list_capacity(self, cap: int = 0) -> int:
+        """Entries per bead of the exv lists; cap > 0 sets it (tis_list_capacity)."""
+        out = C.c_int32()
+        self._chk(self.L.tis_list_capacity(self.ctx, int(cap), C.byref(out)), "tis_list_capacity")
+        return out.value
 
     def last_timing(self):
         t, m, p = C.c_double(), C.c_double(), C.c_double()

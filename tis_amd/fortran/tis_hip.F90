@@ -62,7 +62,8 @@ module tis_hip
      integer(c_int32_t) :: hydro_tensor
   end type tis_bd_system_c
 
-  public :: tis_ctx_create, tis_ctx_destroy, tis_last_error, tis_system_upload, tis_set_system_T, &
+  public :: tis_ctx_create, tis_ctx_destroy, tis_last_error, tis_system_upload, tis_system_validate, &
+       tis_set_system_T, tis_list_capacity, tis_term_status, &
        tis_state_set, tis_state_get, tis_neighbor, tis_pairs_get, tis_force_sumup, tis_force_get, &
        tis_energy_sumup, tis_energy_unit, tis_langevin_init, tis_langevin_steps, tis_system_origin, tis_device_bytes, &
        tis_status_counters, tis_synchronize, tis_last_timing, tis_last_force_ms, tis_label_define, &
@@ -91,6 +92,26 @@ module tis_hip
        type(tis_system_c), intent(in) :: systems(*)
        type(c_ptr), value :: keys
        integer(c_int64_t), value :: seed
+     end function
+     integer(c_int32_t) function tis_system_validate(nsys, systems, msg, msg_cap) &
+          bind(C, name="tis_system_validate")
+       import :: c_int32_t, c_int64_t, c_char, tis_system_c
+       integer(c_int32_t), value :: nsys
+       type(tis_system_c), intent(in) :: systems(*)
+       character(kind=c_char), intent(out) :: msg(*)
+       integer(c_int64_t), value :: msg_cap
+     end function
+     integer(c_int32_t) function tis_term_status(ctx, isys, hb_status, st_status) bind(C, name="tis_term_status")
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: isys
+       integer(c_int32_t), intent(out) :: hb_status(*), st_status(*)
+     end function
+     integer(c_int32_t) function tis_list_capacity(ctx, cap, cap_out) bind(C, name="tis_list_capacity")
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: cap
+       integer(c_int32_t), intent(out) :: cap_out
      end function
      integer(c_int32_t) function tis_set_system_T(ctx, isys, sys) bind(C, name="tis_set_system_T")
        import :: c_int32_t, c_ptr, tis_system_c
