@@ -174,6 +174,12 @@ int32_t tis_term_status(tis_ctx *ctx, int32_t isys, int32_t *hb_status, int32_t 
  * (const_maxsize.F90, allocate_neighbor.F90). */
 int32_t tis_list_capacity(tis_ctx *ctx, int32_t cap, int32_t *cap_out);
 
+/* Host-to-device and device-to-host payload bytes moved since the context
+ * was created: batch uploads (topology, parameters), tis_state_set and
+ * tis_state_get (either pointer may be NULL).  Serving reports them per
+ * admission. */
+int32_t tis_transfer_bytes(tis_ctx *ctx, int64_t *h2d, int64_t *d2h);
+
 /* Device memory held by the context's batch, in bytes (admission control). */
 int32_t tis_device_bytes(tis_ctx *ctx, int64_t *bytes);
 

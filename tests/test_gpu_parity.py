@@ -191,8 +191,14 @@ def test_list_overflow_resumes_bitwise(mode, rebuild_every):
     step, grows the capacity and resumes.  The trajectory equals, bit for bit,
     that of a run with ample capacity from the start (so no step ran on
     truncated lists), and the resumes are counted."""
-    systems = [compact_allpairs(seed=41 + k, n=n) for k, n in enumerate([40, 64])]
-    systems += [hairpin_design(400 + k, n) for k, n in enumerate([90, 30])]
+    # compact chains expanded by 25 % about their centre: the stretched bonds
+    # pull them back together, so later rebuilds hold more neighbours
+    systems = []
+    for k, n in enumerate([40, 64, 90]):
+        s, x = compact_allpairs(seed=41 + k, n=n)
+        c = x.mean(axis=0)
+        systems.append((s, c + 1.25 * (x - c)))
+    systems += [hairpin_design(400, 30)]
     res = []
     for tight in (False, True):
         print(f"[overflow resume] mode={mode} rebuild_every={rebuild_every} tight={tight}", flush=True)

@@ -161,144 +161,179 @@ __global__ void k_bd_reset(BdDev B) {
 }
 
 // ---------------------------------------------------------------- Cholesky
-// Right-looking blocked factorisation of L (holding the lower part of D) in
-// place.  Per 64-column panel starting at k: the diagonal block in FP64, the
-// panel below by an FP64 triangular solve (also storing an FP32 copy P,
-// column-major [64][mpad]), then the trailing lower triangle minus P P^T on
-// FP32 matrix cores, subtracted from the FP64 matrix.
-// Dot-product step shared by the diagonal factorisation and the panel
-// solve: a - sum_{l<c} x[l*64] y[l*64] with four partial sums.  Both
-// operands live in LDS column-major ([column][row], row = lane), so a lane's
-// own row is read conflict-free and another row is a broadcast.
-__device__ inline double chol_dot(double a, const double *x, const double *y, int c) {
-  double s0 = a, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  int l = 0;
-  for (; l + 4 <= c; l += 4) {
-    s0 -= x[l * kCb] * y[l * kCb];
-    s1 -= x[(l + 1) * kCb] * y[(l + 1) * kCb];
-    s2 -= x[(l + 2) * kCb] * y[(l + 2) * kCb];
-    s3 -= x[(l + 3) * kCb] * y[(l + 3) * kCb];
-  }
-  for (; l < c; ++l) s0 -= x[l * kCb] * y[l * kCb];
-  return (s0 + s1) + (s2 + s3);
+// Blocked right-looking factorisation of L (holding the lower part of D) in
+// place, in outer blocks of kNb = 256 columns made of four 64-column
+// panels.  Panel j of the block starting at column k0: its columns are
+// first brought up to date with the block's earlier panels (k_chol_update,
+// K = 64 j, FP32 matrix cores), then factored (k_chol_panel: the 64x64
+// diagonal block and the triangular solve of the rows below, FP64), its
+// FP32 copy going to columns [64 j, 64 j + 64) of P (P[q * mpad + r - k0]).
+// After the block's last panel the trailing lower triangle is updated with
+// all 256 columns at once (k_chol_update, K = 256), so each trailing element
+// is read and written once per 256 columns instead of once per 64.
+constexpr int kNb = 256;
+
+// LDS ordering inside one wave (writes by some lanes, reads by others)
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 }
 
-// One panel: every block (one wave) factors the 64x64 diagonal block
-// column by column in the Crout order (lane r owns row r; a column needs
-// only earlier columns, so each step is one dot product per lane and one
-// barrier); block 0 stores it, blocks 1.. each solve 64 rows of the panel
-// below against it (forward substitution, FP64) and store them with their
-// FP32 copy P.  Global loads are unrolled so that all 64 are in flight.
-__global__ __launch_bounds__(kCb) void k_chol_panel(BdDev B, int k, float *P, int mpad) {
-  __shared__ double A[kCb * kCb];   // A[c * 64 + r]
-  __shared__ double X[kCb * kCb];   // X[c * 64 + lane]
-  const int lane = threadIdx.x, n3 = B.n3;
-  const int kb = min(kCb, n3 - k);
-  {
-    // unconditional loads from clamped (in-bounds) addresses, selected after:
-    // no branch separates them, so all 64 are in flight together
-    double t[kCb];
-    const size_t row = (size_t)min(k + lane, n3 - 1);
+// One panel at column kk of the outer block at k0, 256 threads per block.
+// Wave 0 of every block factors the 64x64 diagonal block with lane r
+// holding row r in registers, right-looking: at column c each lane
+// publishes its entry of the column in LDS, reads the pivot and the entries
+// of the rows below as broadcasts and updates its own row.  Block 0 stores
+// the factor; in blocks 1.. the factor goes to LDS and each of the four
+// waves solves 64 rows of the panel below by forward substitution (row in
+// registers, the factor's entries broadcast from LDS), storing the rows and
+// their FP32 copy in P.  Full tiles take branch-free stores.
+constexpr int kPanelRows = 256;   // panel rows solved per block
+__global__ __launch_bounds__(256) void k_chol_panel(BdDev B, int kk, int k0, float *P, int mpad) {
+  __shared__ double Ls[kCb * kCb];   // Ls[c * 64 + r] = L11[r][c]
+  __shared__ double Rd[kCb];         // 1 / L11[c][c]
+  __shared__ double colb[kCb];       // column c of the block being factored
+  const int lane = threadIdx.x & (kCb - 1), w = threadIdx.x / kCb, n3 = B.n3;
+  const int kb = min(kCb, n3 - kk);
+  if (w == 0) {
+    double a[kCb];
+    {
+      // unconditional loads from clamped (in-bounds) addresses, selected after
+      const size_t row = (size_t)min(kk + lane, n3 - 1);
 #pragma unroll
-    for (int c = 0; c < kCb; ++c) t[c] = B.L[row + (size_t)min(k + c, n3 - 1) * n3];
+      for (int c = 0; c < kCb; ++c) a[c] = B.L[row + (size_t)min(kk + c, n3 - 1) * n3];
 #pragma unroll
-    for (int c = 0; c < kCb; ++c)
-      A[c * kCb + lane] = (lane < kb && c < kb && c <= lane) ? t[c] : (c == lane ? 1.0 : 0.0);
-  }
-  __syncthreads();
-  for (int c = 0; c < kCb; ++c) {
-    // column c: the pivot row first, then every row below it
-    const double v = chol_dot(A[c * kCb + lane], A + lane, A + c, c);
-    if (lane == c) {
-      if (blockIdx.x == 0 && c < kb && !(v > 0.0)) atomicCAS(B.info, 0, k + c + 1);
-      A[c * kCb + c] = sqrt(v);
+      for (int c = 0; c < kCb; ++c) a[c] = (lane < kb && c < kb) ? a[c] : (c == lane ? 1.0 : 0.0);
     }
-    __syncthreads();
-    if (lane > c) A[c * kCb + lane] = v / A[c * kCb + c];
-    __syncthreads();
+    int bad = 0;
+#pragma unroll
+    for (int c = 0; c < kCb; ++c) {
+      colb[lane] = a[c];
+      wave_lds_sync();
+      const double piv = colb[c];
+      if (c < kb && !(piv > 0.0) && bad == 0) bad = kk + c + 1;
+      const double lcc = sqrt(piv);
+      const double inv = 1.0 / lcc;
+      const double t = a[c] * inv;   // L[lane][c] for lane > c
+#pragma unroll
+      for (int j = c + 1; j < kCb; ++j) a[j] = fma(-t, colb[j] * inv, a[j]);
+      a[c] = lane == c ? lcc : (lane > c ? t : 0.0);
+      if (lane == 0) Rd[c] = inv;
+      wave_lds_sync();
+    }
+    if (blockIdx.x == 0) {
+      if (lane == 0 && bad) atomicCAS(B.info, 0, bad);
+      if (kb == kCb) {
+#pragma unroll
+        for (int c = 0; c < kCb; ++c) B.L[(size_t)(kk + lane) + (size_t)(kk + c) * n3] = c <= lane ? a[c] : 0.0;
+      } else if (lane < kb) {
+#pragma unroll
+        for (int c = 0; c < kCb; ++c)
+          if (c < kb) B.L[(size_t)(kk + lane) + (size_t)(kk + c) * n3] = c <= lane ? a[c] : 0.0;
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < kCb; ++c) Ls[c * kCb + lane] = a[c];
+    }
   }
-  if (blockIdx.x == 0) {
-    if (lane < kb)
-#pragma unroll 8
-      for (int c = 0; c < kCb; ++c)
-        if (c < kb) B.L[(size_t)(k + lane) + (size_t)(k + c) * n3] = c <= lane ? A[c * kCb + lane] : 0.0;
-    return;
-  }
-  const int rl = (blockIdx.x - 1) * kCb + lane;   // row within the trailing part
-  const int r = k + kCb + rl;
+  if (blockIdx.x == 0) return;
+  __syncthreads();
+  const int r0w = kk + kCb + (blockIdx.x - 1) * kPanelRows + w * kCb;   // first row of the wave
+  if (r0w >= n3) return;
+  const int r = r0w + lane;
   const bool live = r < n3;
+  double x[kCb];
   {
-    double t[kCb];
     const size_t row = (size_t)min(r, n3 - 1);
 #pragma unroll
-    for (int c = 0; c < kCb; ++c) t[c] = B.L[row + (size_t)(k + c) * n3];
-#pragma unroll
-    for (int c = 0; c < kCb; ++c) X[c * kCb + lane] = live ? t[c] : 0.0;
+    for (int c = 0; c < kCb; ++c) x[c] = B.L[row + (size_t)min(kk + c, n3 - 1) * n3];
   }
-  for (int c = 0; c < kCb; ++c)
-    X[c * kCb + lane] = chol_dot(X[c * kCb + lane], X + lane, A + c, c) / A[c * kCb + c];
-  if (rl < mpad)
-#pragma unroll 8
+#pragma unroll
+  for (int c = 0; c < kCb; ++c) {
+    x[c] *= Rd[c];
+#pragma unroll
+    for (int j = c + 1; j < kCb; ++j) x[j] = fma(-x[c], Ls[c * kCb + j], x[j]);
+  }
+  const int rl = r - k0;   // row of P
+  const int q0 = kk - k0;  // first column of P
+  if (kb == kCb && r0w + kCb <= n3) {
+#pragma unroll
     for (int c = 0; c < kCb; ++c) {
-      const double xv = X[c * kCb + lane];
-      if (live) B.L[(size_t)r + (size_t)(k + c) * n3] = xv;
-      P[(size_t)c * mpad + rl] = live ? (float)xv : 0.0f;
+      B.L[(size_t)r + (size_t)(kk + c) * n3] = x[c];
+      P[(size_t)(q0 + c) * mpad + rl] = (float)x[c];
     }
+  } else if (rl < mpad) {
+#pragma unroll
+    for (int c = 0; c < kCb; ++c) {
+      if (live && c < kb) B.L[(size_t)r + (size_t)(kk + c) * n3] = x[c];
+      P[(size_t)(q0 + c) * mpad + rl] = (live && c < kb) ? (float)x[c] : 0.0f;
+    }
+  }
 }
 
-// trailing update of the lower triangle by 64x64 tiles; 4 waves, each a
-// 32x32 quadrant with v_mfma_f32_32x32x2f32 over K = 64
-__global__ __launch_bounds__(256) void k_chol_syrk(BdDev B, int k, const float *P, int mpad) {
+// C -= A B^T on 64x64 tiles of the lower part of L, FP32 matrix cores
+// (v_mfma_f32_32x32x2f32, each of four waves a 32x32 quadrant), the sum
+// over K columns of P in chunks of 64 staged through LDS, subtracted from
+// the FP64 matrix (entries above the diagonal are not touched).
+// tri = 1: the lower triangle of tiles from row / column r0 (block b is the
+// tile (ti, tj), tj <= ti, in row order); tri = 0: the column of tiles at
+// column r0, rows r0 + 64 b.  Row r of A is P row r - k0.
+__global__ __launch_bounds__(256) void k_chol_update(BdDev B, const float *P, int mpad, int k0, int r0, int K,
+                                                     int tri) {
   __shared__ float Pa[kCb][kCb + 1], Pb[kCb][kCb + 1];
   __shared__ float C[kCb][kCb + 1];
   const int tid = threadIdx.x, n3 = B.n3;
   const int bidx = blockIdx.x;
-  int ti = (int)((sqrt(8.0 * (double)bidx + 1.0) - 1.0) * 0.5);
-  while ((ti + 1) * (ti + 2) / 2 <= bidx) ++ti;
-  while (ti * (ti + 1) / 2 > bidx) --ti;
-  const int tj = bidx - ti * (ti + 1) / 2;
-  {
+  int ti = bidx, tj = 0;
+  if (tri) {
+    ti = (int)((sqrt(8.0 * (double)bidx + 1.0) - 1.0) * 0.5);
+    while ((ti + 1) * (ti + 2) / 2 <= bidx) ++ti;
+    while (ti * (ti + 1) / 2 > bidx) --ti;
+    tj = bidx - ti * (ti + 1) / 2;
+  }
+  const int ra = r0 - k0 + ti * kCb, rb = r0 - k0 + tj * kCb;   // P rows of the tile's rows / columns
+  const int w = tid / 64, lane = tid & 63;
+  const int qi = w >> 1, qj = w & 1;
+  floatx16 acc;
+  for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
+  const int ar = 32 * qi + (lane & 31), br = 32 * qj + (lane & 31), kq = lane >> 5;
+  for (int q0 = 0; q0 < K; q0 += kCb) {
     float ta[kCb * kCb / 256], tb[kCb * kCb / 256];
 #pragma unroll
     for (int q = 0; q < kCb * kCb / 256; ++q) {
       const int idx = tid + 256 * q, r = idx % kCb, c = idx / kCb;
-      ta[q] = P[(size_t)c * mpad + ti * kCb + r];
-      tb[q] = P[(size_t)c * mpad + tj * kCb + r];
+      ta[q] = P[(size_t)(q0 + c) * mpad + ra + r];
+      tb[q] = P[(size_t)(q0 + c) * mpad + rb + r];
     }
+    if (q0) __syncthreads();
 #pragma unroll
     for (int q = 0; q < kCb * kCb / 256; ++q) {
       const int idx = tid + 256 * q, r = idx % kCb, c = idx / kCb;
       Pa[r][c] = ta[q];
       Pb[r][c] = tb[q];
     }
-  }
-  __syncthreads();
-  const int w = tid / 64, lane = tid & 63;
-  const int qi = w >> 1, qj = w & 1;
-  floatx16 acc;
-  for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
-  const int ar = 32 * qi + (lane & 31), br = 32 * qj + (lane & 31), kk = lane >> 5;
-  for (int s = 0; s < kCb / 2; ++s) {
-    const float a = Pa[ar][2 * s + kk];
-    const float b = Pb[br][2 * s + kk];
-    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+    __syncthreads();
+#pragma unroll 8
+    for (int s2 = 0; s2 < kCb / 2; ++s2)
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(Pa[ar][2 * s2 + kq], Pb[br][2 * s2 + kq], acc, 0, 0, 0);
   }
   for (int v = 0; v < 16; ++v) {
     const int row = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
     C[32 * qi + row][32 * qj + (lane & 31)] = acc[v];
   }
   __syncthreads();
-  const int r0 = k + kCb + ti * kCb, c0 = k + kCb + tj * kCb;
+  const int gr0 = r0 + ti * kCb, gc0 = r0 + tj * kCb;
   double old[kCb * kCb / 256];
 #pragma unroll
   for (int q = 0; q < kCb * kCb / 256; ++q) {
     const int idx = tid + 256 * q;
-    const int gr = min(r0 + idx % kCb, n3 - 1), gc = min(c0 + idx / kCb, n3 - 1);
+    const int gr = min(gr0 + idx % kCb, n3 - 1), gc = min(gc0 + idx / kCb, n3 - 1);
     old[q] = B.L[(size_t)gr + (size_t)gc * n3];   // clamped: in bounds, unconditional
   }
 #pragma unroll
   for (int q = 0; q < kCb * kCb / 256; ++q) {
-    const int idx = tid + 256 * q, r = idx % kCb, c = idx / kCb, gr = r0 + r, gc = c0 + c;
+    const int idx = tid + 256 * q, r = idx % kCb, c = idx / kCb, gr = gr0 + r, gc = gc0 + c;
     if (gr < n3 && gc < n3 && gr >= gc) B.L[(size_t)gr + (size_t)gc * n3] = old[q] - (double)C[r][c];
   }
 }
@@ -462,17 +497,32 @@ void launch_fill(tis_ctx *c, BdState *b) {
   hipLaunchKernelGGL(k_rpy_fill, dim3(gb, gb), dim3(16, 16), 0, st, d);
 }
 
-void launch_factor(tis_ctx *c, BdState *b) {
-  hipStream_t st = ctx_stream(c);
-  const BdDev &d = b->d;
-  for (int k = 0; k < d.n3; k += kCb) {
-    const int m = d.n3 - k - kCb;
-    const int mt = m > 0 ? (m + kCb - 1) / kCb : 0;
-    hipLaunchKernelGGL(k_chol_panel, dim3(1 + mt), dim3(kCb), 0, st, d, k, b->P, mt * kCb);
-    if (mt == 0) break;
-    hipLaunchKernelGGL(k_chol_syrk, dim3(mt * (mt + 1) / 2), dim3(256), 0, st, d, k, (const float *)b->P, mt * kCb);
+// The factorisation's launches, outer block by outer block: for each of its
+// panels the update of the panel's columns by the block's earlier panels,
+// then the panel; then the trailing update with the whole block.  P holds
+// kNb columns of at most n3 rows (rows from the block's start).
+void factor_launches(const BdDev &d, float *P, hipStream_t st) {
+  const int n3 = d.n3;
+  for (int k0 = 0; k0 < n3; k0 += kNb) {
+    const int mpad = (n3 - k0 + kCb - 1) / kCb * kCb;
+    for (int j = 0; j < kNb / kCb && k0 + j * kCb < n3; ++j) {
+      const int kk = k0 + j * kCb;
+      if (j > 0)
+        hipLaunchKernelGGL(k_chol_update, dim3((n3 - kk + kCb - 1) / kCb), dim3(256), 0, st, d,
+                           (const float *)P, mpad, k0, kk, j * kCb, 0);
+      const int m = n3 - kk - kCb;
+      const int nbp = m > 0 ? (m + kPanelRows - 1) / kPanelRows : 0;
+      hipLaunchKernelGGL(k_chol_panel, dim3(1 + nbp), dim3(256), 0, st, d, kk, k0, P, mpad);
+    }
+    const int kt = k0 + kNb;
+    if (kt >= n3) break;
+    const int mt = (n3 - kt + kCb - 1) / kCb;
+    hipLaunchKernelGGL(k_chol_update, dim3(mt * (mt + 1) / 2), dim3(256), 0, st, d, (const float *)P, mpad, k0, kt,
+                       kNb, 1);
   }
 }
+
+void launch_factor(tis_ctx *c, BdState *b) { factor_launches(b->d, b->P, ctx_stream(c)); }
 
 int capture(tis_ctx *c, BdState *b, void (*fn)(tis_ctx *, BdState *), hipGraphExec_t *out) {
   hipStream_t st = ctx_stream(c);
@@ -572,7 +622,7 @@ int32_t tis_bdhi_setup(tis_ctx *c, const tis_bd_system *s, uint32_t key, uint64_
   if ((rc = balloc(c, b, &d.info, 1))) return rc;
   const int mt = (int)((n3 + kCb - 1) / kCb);
   b->mpad = mt * kCb;
-  if ((rc = balloc(c, b, &b->P, (size_t)kCb * b->mpad))) return rc;
+  if ((rc = balloc(c, b, &b->P, (size_t)kNb * b->mpad))) return rc;
   // step products: ~1024+ blocks
   const int rows = (int)((n3 + kGemvRows - 1) / kGemvRows);
   b->nchunk = std::max(1, std::min((int)((n3 + 127) / 128), (1024 + rows - 1) / rows));

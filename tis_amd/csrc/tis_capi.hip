@@ -107,6 +107,7 @@ struct tis_ctx {
   int32_t rebuild_every = 10;
   int64_t t_since_init = 0;
   uint64_t step_host = 0;                // the device step counter (stepbase) as the host knows it
+  int64_t h2d_bytes = 0, d2h_bytes = 0;  // payload bytes of uploads / state downloads since creation
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   int graph_len = 0;
   // timing
@@ -177,6 +178,7 @@ int dupload(tis_ctx *c, T **p, const std::vector<T> &h) {
   int rc = dalloc(c, p, h.size());
   if (rc) return rc;
   if (!h.empty()) HIPCHK(c, hipMemcpyAsync(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  c->h2d_bytes += (int64_t)(h.size() * sizeof(T));
   return 0;
 }
 
@@ -895,6 +897,7 @@ int32_t tis_state_set(tis_ctx *c, int32_t isys, int32_t which, const double *buf
     for (int i = 0; i < h.d.nmp; ++i) comp[i] = buf[3 * i + d];
     HIPCHK(c, hipMemcpyAsync(base + (size_t)d * c->ntot + h.d.off, comp.data(), comp.size() * sizeof(double),
                              hipMemcpyHostToDevice, c->stream));
+    c->h2d_bytes += (int64_t)(comp.size() * sizeof(double));
     HIPCHK(c, hipStreamSynchronize(c->stream));
   }
   if (which == 0) { c->lists_valid = false; c->lv_init = false; }
@@ -914,6 +917,7 @@ int32_t tis_state_get(tis_ctx *c, int32_t isys, int32_t which, double *buf) {
   for (int d = 0; d < 3; ++d) {
     HIPCHK(c, hipMemcpyAsync(comp.data(), base + (size_t)d * c->ntot + h.d.off, comp.size() * sizeof(double),
                              hipMemcpyDeviceToHost, c->stream));
+    c->d2h_bytes += (int64_t)(comp.size() * sizeof(double));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     for (int i = 0; i < h.d.nmp; ++i) buf[3 * i + d] = comp[i];
   }
@@ -1269,6 +1273,13 @@ int32_t tis_label_apply(tis_ctx *c, int32_t n, const int32_t *isys, const int32_
   HIPCHK(c, hipGetLastError());
   for (int k = 0; k < n; ++k) copy_t_fields(c->hs[isys[k]].d, c->labels[label[k]].d);
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TIS_OK;
+}
+
+int32_t tis_transfer_bytes(tis_ctx *c, int64_t *h2d, int64_t *d2h) {
+  if (!c) return TIS_ERR_ARG;
+  if (h2d) *h2d = c->h2d_bytes;
+  if (d2h) *d2h = c->d2h_bytes;
   return TIS_OK;
 }
 

@@ -92,6 +92,10 @@ struct Tile { int32_t sys, start; };
 // bonds [b0, b0+nb), angles [a0, a0+na), consecutive stacks [s0, s0+ns)
 struct BondedRange { int32_t b0, nb, a0, na, s0, ns; };
 
+// XCDs of an MI355X (each with its own L2); workgroups are dealt to them
+// round-robin by workgroup index
+constexpr int kXcds = 8;
+
 // excluded-volume parameters of a species pair: D_ij (FP64 and its square)
 // and eps_i*eps_j (force_exv_dt15.F90:60-80,92)
 struct SpPair { double dij, dij2, coef; };

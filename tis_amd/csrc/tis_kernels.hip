@@ -773,7 +773,21 @@ void k_force(DevPtrs P, const double *X, const double *PX, const int32_t *systas
   }
   b -= ntb;
   if (b >= npb) {
-    bonded_lane(P, X, PX, br, (b - npb) * (kTile * kWaves) + (int)threadIdx.x);
+    // Workgroups go to the eight XCDs round-robin by workgroup index, each
+    // XCD with its own L2.  The bonded chunks (terms in bead order) are dealt
+    // so that each XCD takes one contiguous run of them: the slots a run
+    // writes (bead-ordered incidence entries) then sit in lines that one L2
+    // alone writes back, instead of lines written in parts by several.
+    const int nbb = (int)gridDim.x - ntb - npb, g0 = ntb + npb, gb = (int)blockIdx.x;
+    const int x = gb % kXcds;
+    int before = 0, first = 0;
+    for (int y = 0; y <= x; ++y) {
+      const int f = g0 + ((y - g0 % kXcds) + kXcds) % kXcds;   // first bonded workgroup on XCD y
+      if (y == x) first = f;
+      else before += f < g0 + nbb ? (g0 + nbb - 1 - f) / kXcds + 1 : 0;
+    }
+    const int chunk = before + (gb - first) / kXcds;
+    bonded_lane(P, X, PX, br, chunk * (kTile * kWaves) + (int)threadIdx.x);
     return;
   }
   const int w = (int)threadIdx.x / kTile;

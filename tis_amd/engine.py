@@ -55,6 +55,7 @@ def load_library():
     L.tis_set_system_T.argtypes = [vp, C.c_int32, C.POINTER(CSystem)]
     L.tis_list_capacity.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32)]
     L.tis_term_status.argtypes = [vp, C.c_int32, _i32p, _i32p]
+    L.tis_transfer_bytes.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.tis_state_set.argtypes = [vp, C.c_int32, C.c_int32, _f64p]
     L.tis_state_get.argtypes = [vp, C.c_int32, C.c_int32, _f64p]
     L.tis_neighbor.argtypes = [vp, C.c_int32]
@@ -74,7 +75,7 @@ def load_library():
     L.tis_last_timing.argtypes = [vp, _f64p, _f64p, _f64p, C.POINTER(C.c_int64)]
     L.tis_last_force_ms.argtypes = [vp, _f64p]
     for fn in ("tis_ctx_create", "tis_ctx_destroy", "tis_system_upload", "tis_system_validate", "tis_set_system_T",
-               "tis_list_capacity", "tis_term_status",
+               "tis_list_capacity", "tis_term_status", "tis_transfer_bytes",
                "tis_state_set",
                "tis_state_get", "tis_neighbor", "tis_pairs_get", "tis_force_sumup", "tis_force_get",
                "tis_energy_sumup", "tis_energy_unit", "tis_langevin_init", "tis_langevin_steps", "tis_status_counters",
@@ -250,6 +251,12 @@ class Engine:
         a = (C.c_int64 * 8)()
         self._chk(self.L.tis_status_counters(self.ctx, a), "tis_status_counters")
         return dict(zip(CNT_NAMES, list(a)[:len(CNT_NAMES)]))
+
+    def transfer_bytes(self):
+        """(host-to-device, device-to-host) payload bytes since creation."""
+        a, b = C.c_int64(), C.c_int64()
+        self._chk(self.L.tis_transfer_bytes(self.ctx, C.byref(a), C.byref(b)), "tis_transfer_bytes")
+        return a.value, b.value
 
     def term_status(self, isys):
         """(hb_status[nhb], st_status[nst]) as bool arrays (tis_term_status)."""
