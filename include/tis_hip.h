@@ -75,6 +75,31 @@ int32_t tis_system_upload(tis_ctx *ctx, int32_t nsys, const tis_system *systems,
  * reference trusts its .ninfo reader (read_nativeinfo.F90). */
 int32_t tis_system_validate(int32_t nsys, const tis_system *systems, char *msg, int64_t msg_cap);
 
+/* Add nadd systems to the batch without touching the device data of the
+ * systems already in it (continuous batching; no reference counterpart: the
+ * reference runs one job per process, mloop_simulator.F90).  The new
+ * systems take indices nsys .. nsys + nadd - 1 (written to isys_out, may be
+ * NULL); only their own topology and parameters are uploaded, after the
+ * last system, when they fit the batch's capacities.  Otherwise the batch is
+ * laid out again with room for twice what it then holds: retired systems'
+ * storage is dropped and the live systems' positions, velocities and
+ * accelerations are copied over on the device (their indices do not
+ * change).  Lists, captured graphs and the Langevin set-up are dropped:
+ * set the new systems' state, then tis_langevin_init (with per-system
+ * origins from tis_system_origin) before the next steps. */
+int32_t tis_system_append(tis_ctx *ctx, int32_t nadd, const tis_system *systems, const uint32_t *keys,
+                          int32_t *isys_out);
+
+/* Drop systems from every launch (their storage stays until the batch is
+ * next laid out; their indices are not reused).  Calls on a retired system
+ * fail with TIS_ERR_ARG.  As after tis_system_append, tis_langevin_init
+ * comes before the next steps. */
+int32_t tis_system_retire(tis_ctx *ctx, int32_t n, const int32_t *isys);
+
+/* stats[6]: live systems, their beads, beads stored (live and retired since
+ * the last layout), bead capacity, layouts so far, system indices issued. */
+int32_t tis_arena_stats(tis_ctx *ctx, int64_t *stats);
+
 /* Refresh the temperature-dependent coefficients of system isys (tempk,
  * st_coef U0, charge, ele_coef, ele_cdist, beta_hb) from `sys`, which must
  * have the same topology as the uploaded one.  Langevin constants follow. */
@@ -174,11 +199,15 @@ int32_t tis_term_status(tis_ctx *ctx, int32_t isys, int32_t *hb_status, int32_t 
  * (const_maxsize.F90, allocate_neighbor.F90). */
 int32_t tis_list_capacity(tis_ctx *ctx, int32_t cap, int32_t *cap_out);
 
-/* Host-to-device and device-to-host payload bytes moved since the context
- * was created: batch uploads (topology, parameters), tis_state_set and
- * tis_state_get (either pointer may be NULL).  Serving reports them per
- * admission. */
-int32_t tis_transfer_bytes(tis_ctx *ctx, int64_t *h2d, int64_t *d2h);
+/* Payload bytes moved since the context was created, bytes[4]:
+ * [0] host to device, topology, parameters and launch tables;
+ * [1] host to device, dynamic state (tis_state_set);
+ * [2] device to host, dynamic state (tis_state_get);
+ * [3] device to device, positions, velocities and accelerations carried by
+ *     a new arena layout (tis_system_append).
+ * Status words (counters, flags) are not counted.  Serving reports them per
+ * admission and completion. */
+int32_t tis_transfer_bytes(tis_ctx *ctx, int64_t *bytes);
 
 /* Device memory held by the context's batch, in bytes (admission control). */
 int32_t tis_device_bytes(tis_ctx *ctx, int64_t *bytes);

@@ -19,25 +19,43 @@ def _chain(n, seed):
 
 class RecordingBatch:
     """Stands in for Engine in the CPU tests: every step adds (key + 1) * 1e-3
-    to each coordinate of a system, so a job's final coordinates tell how
-    many steps it ran, and whether its state survived re-batching."""
+    to each coordinate of a live system, so a job's final coordinates tell
+    how many steps it ran, and whether its state survived the batch changes.
+    It records every state transfer by system index."""
 
     def __init__(self):
-        self.uploads = []
-        self.sys, self.keys = [], []
+        self.appends, self.retires = [], []
+        self.sys, self.keys, self.live = [], [], set()
+        self.x, self.v, self.a, self.origin = [], [], [], []
+        self.sets, self.gets = [], []
+        self.max_live = 0
 
-    def upload(self, systems, keys=None, seed=1):
-        self.sys, self.keys = list(systems), list(keys)
-        self.x = [np.zeros((s.nmp, 3)) for s in systems]
-        self.v = [np.zeros((s.nmp, 3)) for s in systems]
-        self.a = [np.zeros((s.nmp, 3)) for s in systems]
-        self.origin = [None] * len(systems)
-        self.uploads.append(list(keys))
+    def append(self, systems, keys=None):
+        base = len(self.sys)
+        for s, k in zip(systems, keys):
+            self.sys.append(s)
+            self.keys.append(k)
+            for arr in (self.x, self.v, self.a):
+                arr.append(np.zeros((s.nmp, 3)))
+            self.origin.append(None)
+        slots = list(range(base, len(self.sys)))
+        self.live.update(slots)
+        self.appends.append(list(keys))
+        return slots
+
+    def retire(self, slots):
+        assert all(k in self.live for k in slots)
+        self.live.difference_update(slots)
+        self.retires.append([self.keys[k] for k in slots])
 
     def set_state(self, k, which, arr):
+        assert k in self.live
+        self.sets.append((k, which))
         [self.x, self.v, self.a][which][k] = np.array(arr)
 
     def get_state(self, k, which):
+        assert k in self.live
+        self.gets.append((k, which))
         return np.array([self.x, self.v, self.a][which][k])
 
     def system_origin(self, k, origin, init_accel):
@@ -48,11 +66,18 @@ class RecordingBatch:
         self.step0 = kw["step0"]
 
     def device_bytes(self):
-        return sum(estimate_bytes(s) for s in self.sys)
+        return sum(estimate_bytes(self.sys[k]) for k in self.live)
+
+    def transfer_bytes(self):
+        return {"h2d_topology": 0, "h2d_state": 0, "d2h_state": 0, "d2d_state": 0}
+
+    def arena_stats(self):
+        return {"layouts": 0}
 
     def langevin_steps(self, n, mode):
-        for k, key in enumerate(self.keys):
-            self.x[k] = self.x[k] + n * (key + 1) * 1e-3
+        self.max_live = max(self.max_live, len(self.live))
+        for k in self.live:
+            self.x[k] = self.x[k] + n * (self.keys[k] + 1) * 1e-3
 
 
 def _jobs(nsteps, arrivals, sizes):
@@ -75,9 +100,13 @@ def test_schedule_step_clock():
         # admitted at the first round boundary at or after arrival
         assert j.origin == 20 * int(np.ceil(j.arrival / 20))
         assert j.finished >= j.origin + j.nsteps
-    # jobs 3 (arrival 40) and 4 (arrival 45, admitted at 60) change the batch
-    assert [3] in [[k for k in u if k == 3] for u in rb.uploads]
+    # jobs 3 (arrival 40) and 4 (arrival 45, admitted at 60) join a running batch
+    assert [3] in rb.appends and [4] in rb.appends
     assert st.max_inflight >= 3
+    # each job's state crossed twice: set at admission, read at completion
+    for k in range(5):
+        assert sorted(w for i, w in rb.sets if i == k) == [0, 1, 2]
+        assert sorted(w for i, w in rb.gets if i == k) == [0, 1, 2]
 
 
 def test_budget_admission_fifo():
@@ -87,7 +116,7 @@ def test_budget_admission_fifo():
     srv = Server(hbm_budget=2 * est + est // 2, round_steps=20, clock="steps", engine=rb)
     st = srv.run(jobs)
     assert st.jobs == 4
-    assert max(len(u) for u in rb.uploads) == 2          # never more than the budget allows
+    assert rb.max_live == 2                              # never more than the budget allows
     assert [j.origin for j in jobs] == [0, 0, 40, 40]    # first come, first served
     for j in jobs:
         np.testing.assert_allclose(j.x, j.x0 + 40 * (j.key + 1) * 1e-3, rtol=0, atol=1e-12)
@@ -138,9 +167,15 @@ def test_alone_equals_mix_gpu():
         if budget is None:
             budget = 2 * max(estimate_bytes(j.system) for j in jobs)
         srv = Server(hbm_budget=budget, round_steps=20, clock="steps", seed=seed)
-        srv.run(jobs)
+        st = srv.run(jobs)
         for j, ref in zip(jobs, alone):
             assert j.steps_done == j.nsteps
             for got, want in zip((j.x, j.v, j.a), ref):
                 assert np.array_equal(got, want), f"job {j.key} differs (budget {budget})"
+        # state crossed the bus only at admission and completion: x, v, a of
+        # each job once each way (continuing jobs never left the device)
+        state = sum(72 * j.system.nmp for j in jobs)
+        assert st.h2d_state == state and st.d2h_state == state
+        print(f"   budget {budget}: layouts {st.layouts}, h2d/admission "
+              f"{st.summary()['h2d_bytes_per_admission']:.0f} B, carried on device {st.d2d_state} B", flush=True)
         srv._eng.close()

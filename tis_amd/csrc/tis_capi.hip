@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -43,8 +44,70 @@ using namespace tis;
 namespace {
 
 
+struct SysCopy;
+
+// Host staging of an arena: the used prefix of every array, global indices
+// (slot positions of terms kept term-major here, [term][slot]).
+struct Stage {
+  int32_t nbead = 0, nq = 0, nbd = 0, nba = 0, nst = 0, ntst = 0, nhb = 0, nsite = 0;
+  int64_t moff = 0, mioff = 0, pp_off = 0, lp_off = 0, psk_off = 0, pf_off = 0;
+  std::vector<double> rl2, exv_rad, charge, mass;
+  std::vector<int8_t> mptype, iclass;
+  std::vector<int32_t> unit, qidx, qbead, inc_off{0};
+  std::vector<uint32_t> inc;
+  std::vector<uint8_t> lcl, species;
+  std::vector<LangClass> lcls;
+  std::vector<float> exv_eps;
+  std::vector<SpPair> sptab;
+  std::vector<int32_t> bd_mp, ba_mp, st_mp, tst_mp, tst2st, tst_excl, tst_sys, hb_mp, hb_site, hb_tert, hb_sys,
+      site_val;
+  std::vector<double> bd_nat, bd_coef, ba_nat, ba_coef, st_nat, st_coef, tst_nat, tst_coef, hb_nat, hb_coef;
+  std::vector<int32_t> bd_pos, ba_pos, st_pos, tst_pos, hb_pos;
+  std::vector<int32_t> st_exoff{0}, st_ex;
+  std::vector<double> ewhx, ewhy, ewhz, ewc;
+};
+
+// Sizes of a Stage's arrays at one moment (where an append starts).
+struct StageMarks {
+  size_t bead = 0, q = 0, bd = 0, ba = 0, st = 0, tst = 0, hb = 0, site = 0, rl2 = 0, inc = 0, lc = 0, spt = 0, k = 0,
+         ex = 0;
+};
+
+// Capacities of the arena's arrays (elements; strides of the strided ones).
+struct Caps {
+  int64_t bead = 1, q = 1, bd = 1, ba = 1, st = 1, tst = 1, hb = 1, site = 1, rl2 = 1, inc = 1, lc = 1, spt = 1,
+          k = 1, m = 1, mi = 1, pp = 1, lp = 1, psk = 1, pf = 1, ex = 1, nsys = 1;
+  bool periodic = false;
+};
+
+// Writable device pointers of the arena's uploaded arrays.
+struct ArenaBufs {
+  DSys *sys = nullptr;
+  uint8_t *lcl = nullptr, *species = nullptr;
+  LangClass *lcls = nullptr;
+  double *mass = nullptr, *exv_rad = nullptr, *charge = nullptr, *rl2 = nullptr;
+  int8_t *mptype = nullptr, *iclass = nullptr;
+  int32_t *unit = nullptr, *qidx = nullptr, *qbead = nullptr, *inc_off = nullptr;
+  uint32_t *inc = nullptr;
+  float *exv_eps = nullptr;
+  SpPair *sptab = nullptr;
+  int32_t *bd_pos = nullptr, *ba_pos = nullptr, *st_pos = nullptr, *tst_pos = nullptr, *hb_pos = nullptr;
+  int32_t *bd_mp = nullptr, *ba_mp = nullptr, *st_mp = nullptr, *tst_mp = nullptr, *hb_mp = nullptr;
+  double *bd_nat = nullptr, *bd_coef = nullptr, *ba_nat = nullptr, *ba_coef = nullptr, *st_nat = nullptr,
+         *st_coef = nullptr, *tst_nat = nullptr, *tst_coef = nullptr, *hb_nat = nullptr, *hb_coef = nullptr;
+  int32_t *tst2st = nullptr, *tst_excl = nullptr, *tst_sys = nullptr, *hb_site = nullptr, *hb_tert = nullptr,
+          *hb_sys = nullptr, *site_val = nullptr, *st_exoff = nullptr, *st_ex = nullptr, *ststat = nullptr;
+  double *ewh = nullptr, *ewc = nullptr, *energy = nullptr;
+  unsigned long long *counters = nullptr;
+  uint64_t *stepbase = nullptr;
+};
+
 struct HostSys {
   DSys d;
+  bool live = false;                    // in the launch tables
+  bool stored = false;                  // its segments are in the arena
+  int32_t live_pos = -1;                // index in the live list
+  std::shared_ptr<SysCopy> src;         // host copy (for a new layout)
   int32_t bt_begin = 0, bt_count = 0;   // bead tiles
   int32_t qt_begin = 0, qt_count = 0;   // charge tiles
   bool has_task = false;                // hbonds or tertiary stacks
@@ -77,10 +140,18 @@ struct tis_ctx {
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   std::string err;
   // batch
-  int32_t nsys = 0, ntot = 0, nqtot = 0, nhbtot = 0, ntsttot = 0, nsttot = 0;
+  int32_t nsys = 0, nbeads = 0, nqtot = 0, nhbtot = 0, ntsttot = 0, nsttot = 0;  // handles; used counts
+  int32_t nlive = 0;                     // systems in the launch tables
   std::vector<HostSys> hs;
-  std::vector<void *> allocs;
-  size_t dev_bytes = 0;                  // bytes held in allocs
+  std::vector<void *> allocs;            // the arena (and list capacity growth)
+  std::vector<void *> tab_allocs;        // the launch tables
+  size_t dev_bytes = 0;                  // bytes held in allocs and tab_allocs
+  size_t tab_bytes = 0;
+  Stage stage;
+  Caps caps;
+  ArenaBufs ab{};
+  bool have_arena = false;
+  int64_t relayouts = 0;
   DevPtrs P{};
   DSys *d_sys = nullptr;
   Tile *d_btiles = nullptr, *d_qtiles = nullptr;
@@ -107,7 +178,10 @@ struct tis_ctx {
   int32_t rebuild_every = 10;
   int64_t t_since_init = 0;
   uint64_t step_host = 0;                // the device step counter (stepbase) as the host knows it
-  int64_t h2d_bytes = 0, d2h_bytes = 0;  // payload bytes of uploads / state downloads since creation
+  // payload bytes since creation: uploads of topology and launch tables,
+  // state uploads (tis_state_set), state downloads (tis_state_get), state
+  // carried device to device by a new arena layout
+  int64_t h2d_bytes = 0, h2d_state = 0, d2h_bytes = 0, d2d_state = 0;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   int graph_len = 0;
   // timing
@@ -173,6 +247,30 @@ int dalloc(tis_ctx *c, T **p, size_t n) {
   return 0;
 }
 
+// Allocation into a given list; the bytes count in *acc (or the context's
+// total).  The memory starts zeroed.
+template <typename T>
+int dalloc_in(tis_ctx *c, std::vector<void *> &list, T **p, size_t n, size_t *acc = nullptr) {
+  *p = nullptr;
+  size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+  hipError_t e = hipMalloc((void **)p, bytes);
+  if (e != hipSuccess) return fail(c, TIS_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  list.push_back((void *)*p);
+  *(acc ? acc : &c->dev_bytes) += bytes;
+  e = hipMemsetAsync(*p, 0, bytes, c->stream);
+  if (e != hipSuccess) return fail(c, TIS_ERR_HIP, "hipMemsetAsync failed: %s", hipGetErrorString(e));
+  return 0;
+}
+
+template <typename T>
+int dupload_in(tis_ctx *c, std::vector<void *> &list, T **p, const std::vector<T> &h, size_t *acc = nullptr) {
+  int rc = dalloc_in(c, list, p, h.size(), acc);
+  if (rc) return rc;
+  if (!h.empty()) HIPCHK(c, hipMemcpyAsync(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  c->h2d_bytes += (int64_t)(h.size() * sizeof(T));
+  return 0;
+}
+
 template <typename T>
 int dupload(tis_ctx *c, T **p, const std::vector<T> &h) {
   int rc = dalloc(c, p, h.size());
@@ -187,11 +285,19 @@ void free_all(tis_ctx *c) {
   for (int k = 0; k < 2; ++k)
     if (c->graph[k]) { (void)hipGraphExecDestroy(c->graph[k]); c->graph[k] = nullptr; }
   for (void *p : c->allocs) (void)hipFree(p);
+  for (void *p : c->tab_allocs) (void)hipFree(p);
   c->allocs.clear();
-  c->dev_bytes = 0;
+  c->tab_allocs.clear();
+  c->dev_bytes = c->tab_bytes = 0;
   c->hs.clear();
   c->P = DevPtrs{};
-  c->nsys = c->ntot = 0;
+  c->ab = ArenaBufs{};
+  c->stage = Stage{};
+  c->caps = Caps{};
+  c->have_arena = false;
+  c->d_eunit = nullptr;
+  c->eunit_cap = 0;
+  c->nsys = c->nbeads = c->nlive = 0;
   c->lists_valid = false;
   c->lv_init = false;
 }
@@ -264,6 +370,8 @@ void langevin_consts(double tstep, double tempk, double fric, double mass, doubl
 int check_sys(tis_ctx *c, int32_t isys) {
   if (c->nsys == 0) return fail(c, TIS_ERR_STATE, "no systems uploaded");
   if (isys < -1 || isys >= c->nsys) return fail(c, TIS_ERR_ARG, "system index %d out of range", isys);
+  if (isys >= 0 && !c->hs[isys].live) return fail(c, TIS_ERR_ARG, "system %d is retired", isys);
+  if (c->nlive == 0) return fail(c, TIS_ERR_STATE, "no live systems");
   return 0;
 }
 
@@ -286,8 +394,8 @@ Range range_of(tis_ctx *c, int32_t isys) {
   if (isys < 0) {
     r.all = true;
     r.bt0 = 0; r.btn = c->nbtile; r.qt0 = 0; r.qtn = c->nqtile; r.task0 = 0; r.taskn = c->nsystask;
-    r.sysidx = c->d_sysidx; r.nsel = c->nsys;
-    r.b0 = 0; r.nb = c->P.nbd; r.a0 = 0; r.na = c->P.nba; r.s0 = 0; r.ns = c->P.nst;
+    r.sysidx = c->d_sysidx; r.nsel = c->nlive;
+    r.b0 = 0; r.nb = c->stage.nbd; r.a0 = 0; r.na = c->stage.nba; r.s0 = 0; r.ns = c->stage.nst;
     r.ek0 = 0; r.ekn = c->n_ewsk; r.ef0 = 0; r.efn = c->n_ewf;
     r.pt0 = 0; r.ptn = c->n_ptask;
   } else {
@@ -296,7 +404,7 @@ Range range_of(tis_ctx *c, int32_t isys) {
     r.bt0 = h.bt_begin; r.btn = h.bt_count; r.qt0 = h.qt_begin; r.qtn = h.qt_count;
     r.task0 = c->systask_of[isys] < 0 ? 0 : c->systask_of[isys];
     r.taskn = c->systask_of[isys] < 0 ? 0 : 1;
-    r.sysidx = c->d_sysidx + isys; r.nsel = 1;
+    r.sysidx = c->d_sysidx + h.live_pos; r.nsel = 1;
     r.b0 = h.d.boff; r.nb = h.d.nbd; r.a0 = h.d.aoff; r.na = h.d.nba; r.s0 = h.d.soff; r.ns = h.d.nst;
     r.ek0 = h.esk_begin; r.ekn = h.esk_count; r.ef0 = h.ef_begin; r.efn = h.ef_count;
     r.pt0 = h.pt_begin; r.ptn = h.pt_count;
@@ -390,6 +498,682 @@ int validate_system(std::string *err, int is, const tis_system &s) {
   return TIS_OK;
 }
 
+// =====================================================================
+// The batch arena.  A context's systems live in one set of device arrays:
+// every per-bead, per-charge and per-term array is a concatenation of
+// per-system segments, and a system keeps its offsets for as long as it
+// stays.  tis_system_append stages new systems after the last one and
+// uploads only their own segments when they fit the capacities;
+// tis_system_retire drops a system from the launch tables (its segments
+// stay until the arena is laid out again).  A new layout (compaction or
+// growth) stages the live systems from the host copies kept of them and
+// carries their positions, velocities and accelerations over on the
+// device.  Strided arrays (x[3][*], nb[cap][*], for_hb[18][*], the term
+// slot positions [slot][*], reciprocal vectors [3][*]) use the capacity as
+// their stride, so an append moves nothing already there.
+namespace {
+
+template <typename T>
+const T *keep(std::vector<T> &v, const T *src, int64_t n) {
+  if (!src) return nullptr;
+  v.assign(src, src + std::max<int64_t>(n, 0));
+  return v.empty() ? src : v.data();
+}
+
+struct SysCopy {
+  tis_system s{};
+  std::vector<int32_t> unit, mptype, iclass, bd_mp, ba_mp, st_mp, tst_mp, tst2st, tst_excl, hb_mp, hb_site, hb_tert,
+      valence;
+  std::vector<double> exv_rad, exv_eps, charge, rl2, bd_nat, bd_coef, ba_nat, ba_coef, st_nat, st_coef, tst_nat,
+      tst_coef, hb_nat, hb_coef, mass, fric, radius;
+};
+
+std::shared_ptr<SysCopy> copy_system(const tis_system &in) {
+  auto c = std::make_shared<SysCopy>();
+  tis_system &s = c->s;
+  s = in;
+  const int64_t n = in.nmp, nbd = in.nbd, nba = in.nba, nst = in.nst, ntst = in.ntst, nhb = in.nhb;
+  s.unit = keep(c->unit, in.unit, n);
+  s.mptype = keep(c->mptype, in.mptype, n);
+  s.iclass = keep(c->iclass, in.iclass, n);
+  s.exv_rad = keep(c->exv_rad, in.exv_rad, n);
+  s.exv_eps = keep(c->exv_eps, in.exv_eps, n);
+  s.charge = keep(c->charge, in.charge, n);
+  s.rlist2_unit = keep(c->rl2, in.rlist2_unit, (int64_t)in.nunit * in.nunit);
+  s.bd_mp = keep(c->bd_mp, in.bd_mp, 2 * nbd);
+  s.bd_nat = keep(c->bd_nat, in.bd_nat, nbd);
+  s.bd_coef = keep(c->bd_coef, in.bd_coef, nbd);
+  s.ba_mp = keep(c->ba_mp, in.ba_mp, 3 * nba);
+  s.ba_nat = keep(c->ba_nat, in.ba_nat, nba);
+  s.ba_coef = keep(c->ba_coef, in.ba_coef, nba);
+  s.st_mp = keep(c->st_mp, in.st_mp, 7 * nst);
+  s.st_nat = keep(c->st_nat, in.st_nat, 3 * nst);
+  s.st_coef = keep(c->st_coef, in.st_coef, 4 * nst);
+  s.tst_mp = keep(c->tst_mp, in.tst_mp, 6 * ntst);
+  s.tst_nat = keep(c->tst_nat, in.tst_nat, 6 * ntst);
+  s.tst_coef = keep(c->tst_coef, in.tst_coef, 7 * ntst);
+  s.tst2st = keep(c->tst2st, in.tst2st, 2 * ntst);
+  s.tst_excl = keep(c->tst_excl, in.tst_excl, 2 * ntst);
+  s.hb_mp = keep(c->hb_mp, in.hb_mp, 6 * nhb);
+  s.hb_nat = keep(c->hb_nat, in.hb_nat, 6 * nhb);
+  s.hb_coef = keep(c->hb_coef, in.hb_coef, 7 * nhb);
+  s.hb_site = keep(c->hb_site, in.hb_site, 6 * nhb);
+  s.hb_tertiary = keep(c->hb_tert, in.hb_tertiary, nhb);
+  s.hbsite_valence = keep(c->valence, in.hbsite_valence, in.nhbsite);
+  s.mass = keep(c->mass, in.mass, n);
+  s.fric = keep(c->fric, in.fric, n);
+  s.radius = keep(c->radius, in.radius, n);
+  return c;
+}
+
+Caps caps_for(const Stage &g, int nsys, bool periodic, double grow) {
+  auto c = [grow](int64_t n) { return std::max<int64_t>(1, (int64_t)std::ceil((double)n * grow)); };
+  Caps k;
+  k.bead = c(g.nbead); k.q = c(g.nq); k.bd = c(g.nbd); k.ba = c(g.nba); k.st = c(g.nst); k.tst = c(g.ntst);
+  k.hb = c(g.nhb); k.site = c(g.nsite); k.rl2 = c((int64_t)g.rl2.size()); k.inc = c((int64_t)g.inc.size());
+  k.lc = c((int64_t)g.lcls.size()); k.spt = c((int64_t)g.sptab.size()); k.k = c((int64_t)g.ewc.size());
+  k.m = c(g.moff); k.mi = c(g.mioff); k.pp = c(g.pp_off); k.lp = c(g.lp_off); k.psk = c(g.psk_off);
+  k.pf = c(g.pf_off); k.ex = c((int64_t)g.st_ex.size()); k.nsys = c(nsys);
+  k.periodic = periodic;
+  return k;
+}
+
+bool caps_fit(const Caps &k, const Stage &g, int nsys, bool periodic) {
+  return g.nbead <= k.bead && g.nq <= k.q && g.nbd <= k.bd && g.nba <= k.ba && g.nst <= k.st && g.ntst <= k.tst &&
+         g.nhb <= k.hb && g.nsite <= k.site && (int64_t)g.rl2.size() <= k.rl2 && (int64_t)g.inc.size() <= k.inc &&
+         (int64_t)g.lcls.size() <= k.lc && (int64_t)g.sptab.size() <= k.spt && (int64_t)g.ewc.size() <= k.k &&
+         g.moff <= k.m && g.mioff <= k.mi && g.pp_off <= k.pp && g.lp_off <= k.lp && g.psk_off <= k.psk &&
+         g.pf_off <= k.pf && (int64_t)g.st_ex.size() <= k.ex && nsys <= k.nsys && (!periodic || k.periodic);
+}
+
+// Stage system s (handle is) after the arena's last system: its DSys, its
+// segments of every array, with global indices.
+int stage_system(tis_ctx *c, Stage &g, HostSys &h, const tis_system &s, int is, uint32_t key) {
+  if (s.nmp <= 0 || s.nunit <= 0) return fail(c, TIS_ERR_ARG, "system %d has no beads", is);
+  if (s.p.ele_form == TIS_ELE_EWALD) {
+    if (!s.p.periodic || s.p.box[0] != s.p.box[1] || s.p.box[0] != s.p.box[2])
+      return fail(c, TIS_ERR_ARG, "system %d: Ewald needs a periodic cubic box", is);
+    if (!(s.p.ewld_alpha > 0.0) || s.p.ewld_hmax <= 0)
+      return fail(c, TIS_ERR_ARG, "system %d: Ewald needs ewld_alpha > 0 and ewld_hmax > 0", is);
+  }
+  if (s.p.ele_form != TIS_ELE_NONE && s.p.ele_form != TIS_ELE_DH && s.p.ele_form != TIS_ELE_COULOMB &&
+      s.p.ele_form != TIS_ELE_EWALD)
+    return fail(c, TIS_ERR_ARG, "system %d: unknown ele_form %d", is, s.p.ele_form);
+  if ((int64_t)g.nbead + s.nmp >= (1 << 27)) return fail(c, TIS_ERR_ARG, "too many beads in the arena");
+  DSys &d = h.d;
+  std::memset(&d, 0, sizeof(d));
+  fill_derived(d, s);
+  h.mass.clear(); h.fric.clear(); h.lc_fric.clear(); h.lc_mass.clear();
+  d.key = key;
+  const int32_t off = g.nbead;
+  d.off = off; d.nmp = s.nmp;
+  d.uoff = (int32_t)g.rl2.size(); d.nunit = s.nunit;
+  d.rl2max = 0.0;
+  for (int k = 0; k < s.nunit * s.nunit; ++k) {
+    g.rl2.push_back(s.rlist2_unit[k]);
+    d.rl2max = std::max(d.rl2max, s.rlist2_unit[k]);
+  }
+  // excluded-volume species of this system: distinct (mptype, class,
+  // radius, epsilon), numbered within the system
+  struct Spec { int8_t t, c; double r, e; };
+  std::vector<Spec> specs;
+  auto spec_of = [&](int8_t t, int8_t cl, double r, double e) -> int {
+    for (size_t k = 0; k < specs.size(); ++k)
+      if (specs[k].t == t && specs[k].c == cl && specs[k].r == r && specs[k].e == e) return (int)k;
+    specs.push_back(Spec{t, cl, r, e});
+    return (int)specs.size() - 1;
+  };
+  // beads
+  d.qoff = g.nq;
+  int nq = 0;
+  for (int i = 0; i < s.nmp; ++i) {
+    g.mptype.push_back((int8_t)s.mptype[i]);
+    g.iclass.push_back((int8_t)s.iclass[i]);
+    g.unit.push_back(s.unit[i]);
+    g.exv_rad.push_back(s.exv_rad[i]);
+    g.exv_eps.push_back((float)s.exv_eps[i]);
+    const int sp = spec_of((int8_t)s.mptype[i], (int8_t)s.iclass[i], s.exv_rad[i], s.exv_eps[i]);
+    if (sp > 255) return fail(c, TIS_ERR_ARG, "system %d: too many bead species", is);
+    g.species.push_back((uint8_t)sp);
+    g.charge.push_back(s.charge[i]);
+    g.mass.push_back(s.mass[i]);
+    h.mass.push_back(s.mass[i]);
+    h.fric.push_back(s.fric[i]);
+    // Langevin class: the system's (friction, mass) pairs in order of
+    // first appearance
+    int cl = 0;
+    while (cl < (int)h.lc_fric.size() && !(h.lc_fric[cl] == s.fric[i] && h.lc_mass[cl] == s.mass[i])) ++cl;
+    if (cl == (int)h.lc_fric.size()) {
+      if (cl == 256) return fail(c, TIS_ERR_ARG, "system %d: more than 256 (friction, mass) classes", is);
+      h.lc_fric.push_back(s.fric[i]);
+      h.lc_mass.push_back(s.mass[i]);
+    }
+    g.lcl.push_back((uint8_t)cl);
+    // charges of the ele list: |q| >= ZERO_JUDGE (neighbor_list_ele.F90:134,147)
+    if (s.p.ele_form != TIS_ELE_NONE && std::fabs(s.charge[i]) >= 1.0e-6) {
+      g.qidx.push_back(g.nq + nq);
+      g.qbead.push_back(off + i);
+      ++nq;
+    } else {
+      g.qidx.push_back(-1);
+    }
+  }
+  // species-pair table of the system (force_exv_dt15.F90:60-80): D_ij and
+  // eps_i eps_j
+  const int NS = (int)specs.size();
+  d.sptab = (int32_t)g.sptab.size();
+  d.nspecies = NS;
+  for (int a = 0; a < NS; ++a)
+    for (int b = 0; b < NS; ++b) {
+      const Spec &A = specs[a], &B = specs[b];
+      double dij;
+      if ((A.t == TIS_MP_RNA_PHOS && B.t == TIS_MP_RNA_SUGAR) || (A.t == TIS_MP_RNA_SUGAR && B.t == TIS_MP_RNA_PHOS))
+        dij = d.exv_dist_PS;
+      else if (A.c == TIS_CLASS_RNA && B.c == TIS_CLASS_RNA)
+        dij = d.exv_dist;
+      else
+        dij = A.r + B.r;
+      g.sptab.push_back(SpPair{dij, dij * dij, A.e * B.e});
+    }
+  // its Langevin classes (constants r[] set by tis_langevin_init)
+  d.lc0 = (int32_t)g.lcls.size();
+  d.nlc = (int32_t)h.lc_fric.size();
+  for (int k = 0; k < d.nlc; ++k)
+    g.lcls.push_back(LangClass{{0.0, 0.0, 0.0, 0.0}, 1.0 / h.lc_mass[k], h.lc_fric[k], h.lc_mass[k], 0.0});
+  d.nq = nq;
+  d.mwords = (nq + 31) / 32;
+  d.moff = g.moff;
+  g.moff += (int64_t)nq * d.mwords;
+  d.mioff = s.p.periodic ? g.mioff : -1;
+  if (s.p.periodic) g.mioff += (int64_t)nq * nq;
+  g.nq += nq;
+  // terms with global bead indices
+  const int32_t nbd = g.nbd, nba = g.nba, nst = g.nst, ntst = g.ntst, nhb = g.nhb, nsite = g.nsite;
+  d.boff = nbd; d.nbd = s.nbd;
+  for (int b = 0; b < s.nbd; ++b) {
+    g.bd_mp.push_back(off + s.bd_mp[2 * b]); g.bd_mp.push_back(off + s.bd_mp[2 * b + 1]);
+    g.bd_nat.push_back(s.bd_nat[b]); g.bd_coef.push_back(s.bd_coef[b]);
+  }
+  d.aoff = nba; d.nba = s.nba;
+  for (int a = 0; a < s.nba; ++a) {
+    for (int k = 0; k < 3; ++k) g.ba_mp.push_back(off + s.ba_mp[3 * a + k]);
+    g.ba_nat.push_back(s.ba_nat[a]); g.ba_coef.push_back(s.ba_coef[a]);
+  }
+  d.soff = nst; d.nst = s.nst;
+  for (int t = 0; t < s.nst; ++t) {
+    for (int k = 0; k < 7; ++k) g.st_mp.push_back(off + s.st_mp[7 * t + k]);
+    for (int k = 0; k < 3; ++k) g.st_nat.push_back(s.st_nat[3 * t + k]);
+    for (int k = 0; k < 4; ++k) g.st_coef.push_back(s.st_coef[4 * t + k]);
+  }
+  d.toff = ntst; d.ntst = s.ntst;
+  for (int t = 0; t < s.ntst; ++t) {
+    for (int k = 0; k < 6; ++k) g.tst_mp.push_back(off + s.tst_mp[6 * t + k]);
+    for (int k = 0; k < 6; ++k) g.tst_nat.push_back(s.tst_nat[6 * t + k]);
+    for (int k = 0; k < 7; ++k) g.tst_coef.push_back(s.tst_coef[7 * t + k]);
+    for (int k = 0; k < 2; ++k) {
+      g.tst2st.push_back(s.tst2st[2 * t + k] >= 0 ? nst + s.tst2st[2 * t + k] : -1);
+      g.tst_excl.push_back(s.tst_excl[2 * t + k]);
+    }
+    g.tst_sys.push_back(is);
+  }
+  // per consecutive stack, the exclusive tertiary stacks that name it
+  // (CSR st_exoff / st_ex; st_exoff keeps its end sentinel)
+  {
+    std::vector<std::vector<int32_t>> ex(s.nst);
+    for (int t = 0; t < s.ntst; ++t)
+      for (int k = 0; k < 2; ++k)
+        if (s.tst_excl[2 * t + k] && s.tst2st[2 * t + k] >= 0) ex[s.tst2st[2 * t + k]].push_back(ntst + t);
+    g.st_exoff.pop_back();
+    for (int t = 0; t < s.nst; ++t) {
+      g.st_exoff.push_back((int32_t)g.st_ex.size());
+      g.st_ex.insert(g.st_ex.end(), ex[t].begin(), ex[t].end());
+    }
+    g.st_exoff.push_back((int32_t)g.st_ex.size());
+  }
+  d.sitoff = nsite; d.nsite = s.nhbsite;
+  for (int k = 0; k < s.nhbsite; ++k) g.site_val.push_back(s.hbsite_valence[k]);
+  d.hoff = nhb; d.nhb = s.nhb;
+  std::vector<int> site_use(std::max(s.nhbsite, 1), 0);
+  for (int t = 0; t < s.nhb; ++t) {
+    for (int k = 0; k < 6; ++k) g.hb_mp.push_back(off + s.hb_mp[6 * t + k]);
+    for (int k = 0; k < 6; ++k) g.hb_nat.push_back(s.hb_nat[6 * t + k]);
+    for (int k = 0; k < 7; ++k) g.hb_coef.push_back(s.hb_coef[7 * t + k]);
+    for (int k = 0; k < 6; ++k) {
+      int st = s.hb_site[6 * t + k];
+      g.hb_site.push_back(st >= 0 ? nsite + st : -1);
+      if (st >= 0) site_use[st] += 1;
+    }
+    g.hb_tert.push_back(s.hb_tertiary ? s.hb_tertiary[t] : 0);
+    g.hb_sys.push_back(is);
+  }
+  d.hb_excess_possible = 0;
+  for (int k = 0; k < s.nhbsite; ++k)
+    if (site_use[k] > s.hbsite_valence[k]) d.hb_excess_possible = 1;
+  // incidence lists (bonds, angles, stacks, tertiary stacks, hbonds) and
+  // the incidence position of every (term, slot), kept term-major here
+  std::vector<std::vector<uint32_t>> incb(s.nmp);
+  for (int b = 0; b < s.nbd; ++b)
+    for (int k = 0; k < 2; ++k) incb[s.bd_mp[2 * b + k]].push_back(inc_pack(INC_BOND, k, nbd + b));
+  for (int a = 0; a < s.nba; ++a)
+    for (int k = 0; k < 3; ++k) incb[s.ba_mp[3 * a + k]].push_back(inc_pack(INC_ANGLE, k, nba + a));
+  for (int t = 0; t < s.nst; ++t)
+    for (int k = 0; k < 7; ++k) incb[s.st_mp[7 * t + k]].push_back(inc_pack(INC_STACK, k, nst + t));
+  for (int t = 0; t < s.ntst; ++t)
+    for (int k = 0; k < 6; ++k) incb[s.tst_mp[6 * t + k]].push_back(inc_pack(INC_TSTACK, k, ntst + t));
+  for (int t = 0; t < s.nhb; ++t)
+    for (int k = 0; k < 6; ++k) incb[s.hb_mp[6 * t + k]].push_back(inc_pack(INC_HBOND, k, nhb + t));
+  g.bd_pos.resize(2 * (size_t)(nbd + s.nbd)); g.ba_pos.resize(3 * (size_t)(nba + s.nba));
+  g.st_pos.resize(7 * (size_t)(nst + s.nst)); g.tst_pos.resize(6 * (size_t)(ntst + s.ntst));
+  g.hb_pos.resize(6 * (size_t)(nhb + s.nhb));
+  g.inc_off.pop_back();
+  for (int i = 0; i < s.nmp; ++i) {
+    g.inc_off.push_back((int32_t)g.inc.size());
+    for (uint32_t e : incb[i]) {
+      const int32_t k = (int32_t)g.inc.size();
+      const uint32_t type = e >> 29, slot = (e >> 26) & 7u, term = e & ((1u << 26) - 1u);
+      switch (type) {
+        case INC_BOND: g.bd_pos[2 * (size_t)term + slot] = k; break;
+        case INC_ANGLE: g.ba_pos[3 * (size_t)term + slot] = k; break;
+        case INC_STACK: g.st_pos[7 * (size_t)term + slot] = k; break;
+        case INC_TSTACK: g.tst_pos[6 * (size_t)term + slot] = k; break;
+        default: g.hb_pos[6 * (size_t)term + slot] = k; break;
+      }
+      g.inc.push_back(e);
+    }
+  }
+  g.inc_off.push_back((int32_t)g.inc.size());
+  g.nbd += s.nbd; g.nba += s.nba; g.nst += s.nst; g.ntst += s.ntst; g.nhb += s.nhb; g.nsite += s.nhbsite;
+  // tile counts (the tiles themselves are in the launch tables)
+  d.nbt = (s.nmp + kTile - 1) / kTile;
+  d.nqt = (nq + kTile - 1) / kTile;
+  // electrostatic chunk-pair partial forces
+  d.pp_off = g.pp_off;
+  if (s.p.ele_form != TIS_ELE_NONE && nq > 0) g.pp_off += (int64_t)d.nqt * (d.nqt + 1) / 2 * 6 * kTile;
+  // counts of the periodic tile-pair list build
+  d.lp_off = g.lp_off;
+  if (s.p.periodic) g.lp_off += (int64_t)d.nbt * d.nbt;
+  h.has_task = (s.nhb > 0 || s.ntst > 0);
+  // Ewald reciprocal vectors
+  if (s.p.ele_form == TIS_ELE_EWALD && nq > 0) {
+    d.ewk0 = (int32_t)g.ewc.size();
+    ewald_kvectors(s.p, g.ewhx, g.ewhy, g.ewhz, g.ewc);
+    d.nk = (int32_t)g.ewc.size() - d.ewk0;
+    d.nqc = (nq + kTile - 1) / kTile;
+    d.nkc = (d.nk + kEwKChunk - 1) / kEwKChunk;
+    d.psk_off = g.psk_off; g.psk_off += 2 * (int64_t)d.nqc * d.nk;
+    d.pf_off = g.pf_off; g.pf_off += 3 * (int64_t)d.nkc * nq;
+  }
+  g.nbead += s.nmp;
+  h.stored = true;
+  return 0;
+}
+
+// Device arrays of an arena at capacities k, in `list` (everything that
+// tis_system_append may later fill in place).  The DSys array, counters
+// and step base are allocated here too.
+int alloc_arena(tis_ctx *c, const Caps &k, std::vector<void *> &list, DevPtrs &P, ArenaBufs &A, int nb_cap) {
+  P = DevPtrs{};
+  A = ArenaBufs{};
+  const size_t B = (size_t)k.bead;
+#define AL(ptr, n) do { int rc_ = dalloc_in(c, list, &(ptr), (size_t)(n)); if (rc_) return rc_; } while (0)
+  AL(A.sys, k.nsys);
+  P.sys = A.sys;
+  for (int q = 0; q < 2; ++q) {
+    AL(P.x[q], 3 * B);
+    AL(P.px[q], k.periodic ? 3 * B : 1);
+  }
+  AL(P.v, 3 * B); AL(P.a, 3 * B); AL(P.f, 3 * B);
+  AL(A.lcl, B); AL(A.lcls, k.lc); AL(A.mass, B); AL(A.exv_rad, B); AL(A.charge, B); AL(A.rl2, k.rl2);
+  AL(A.mptype, B); AL(A.iclass, B); AL(A.unit, B); AL(A.qidx, B); AL(A.qbead, k.q); AL(A.inc_off, B + 1);
+  AL(A.inc, k.inc); AL(A.exv_eps, B); AL(A.species, B); AL(A.sptab, k.spt);
+  AL(A.bd_pos, 2 * k.bd); AL(A.ba_pos, 3 * k.ba); AL(A.st_pos, 7 * k.st); AL(A.tst_pos, 6 * k.tst);
+  AL(A.hb_pos, 6 * k.hb);
+  AL(P.fslot, 3 * k.inc);
+  AL(A.bd_mp, 2 * k.bd); AL(A.bd_nat, k.bd); AL(A.bd_coef, k.bd);
+  AL(A.ba_mp, 3 * k.ba); AL(A.ba_nat, k.ba); AL(A.ba_coef, k.ba);
+  AL(A.st_mp, 7 * k.st); AL(A.st_nat, 3 * k.st); AL(A.st_coef, 4 * k.st);
+  AL(A.tst_mp, 6 * k.tst); AL(A.tst_nat, 6 * k.tst); AL(A.tst_coef, 7 * k.tst); AL(A.tst2st, 2 * k.tst);
+  AL(A.tst_excl, 2 * k.tst); AL(A.tst_sys, k.tst);
+  AL(A.hb_mp, 6 * k.hb); AL(A.hb_nat, 6 * k.hb); AL(A.hb_coef, 7 * k.hb); AL(A.hb_site, 6 * k.hb);
+  AL(A.hb_tert, k.hb); AL(A.hb_sys, k.hb); AL(A.site_val, k.site);
+  AL(A.st_exoff, k.st + 1); AL(A.st_ex, k.ex);
+  AL(P.hb_inlist, k.hb); AL(P.hb_status, k.hb); AL(P.hb_ene, k.hb); AL(P.for_hb, 18 * k.hb);
+  AL(P.site_cnt, k.site); AL(P.hb_seq, k.hb); AL(A.ststat, k.st);
+  AL(P.nb_cnt, B);
+  P.nb_cap = std::max(nb_cap, 64);
+  AL(P.nb, (size_t)P.nb_cap * B);
+  AL(P.dxyz, 3 * B); AL(P.dlast, 3 * B);
+  AL(P.sys_flag, k.nsys);
+  AL(P.pp, k.pp); AL(P.qmask, k.m); AL(P.qmirror, k.mi); AL(P.lcnt, k.lp * kTile);
+  AL(A.ewh, 3 * k.k); AL(A.ewc, k.k); AL(P.psk, k.psk); AL(P.pf, k.pf);
+  AL(A.counters, TIS_CNT_MAX); AL(A.stepbase, 1); AL(A.energy, 25 * k.nsys);
+#undef AL
+  P.lcl = A.lcl; P.lcls = A.lcls; P.mass = A.mass; P.exv_rad = A.exv_rad; P.charge = A.charge; P.rl2 = A.rl2;
+  P.mptype = A.mptype; P.iclass = A.iclass; P.unit = A.unit; P.qidx = A.qidx; P.qbead = A.qbead;
+  P.inc_off = A.inc_off; P.inc = A.inc; P.exv_eps = A.exv_eps; P.species = A.species; P.sptab = A.sptab;
+  P.bd_pos = A.bd_pos; P.ba_pos = A.ba_pos; P.st_pos = A.st_pos; P.tst_pos = A.tst_pos; P.hb_pos = A.hb_pos;
+  P.bd_mp = A.bd_mp; P.bd_nat = A.bd_nat; P.bd_coef = A.bd_coef;
+  P.ba_mp = A.ba_mp; P.ba_nat = A.ba_nat; P.ba_coef = A.ba_coef;
+  P.st_mp = A.st_mp; P.st_nat = A.st_nat; P.st_coef = A.st_coef;
+  P.tst_mp = A.tst_mp; P.tst_nat = A.tst_nat; P.tst_coef = A.tst_coef; P.tst2st = A.tst2st;
+  P.tst_excl = A.tst_excl; P.tst_sys = A.tst_sys;
+  P.hb_mp = A.hb_mp; P.hb_nat = A.hb_nat; P.hb_coef = A.hb_coef; P.hb_site = A.hb_site; P.hb_tert = A.hb_tert;
+  P.hb_sys = A.hb_sys; P.site_val = A.site_val; P.st_exoff = A.st_exoff; P.st_ex = A.st_ex;
+  P.ewh = A.ewh; P.ewc = A.ewc;
+  P.counters = A.counters; P.stepbase = A.stepbase;
+  // strides
+  P.ntot = (int32_t)k.bead; P.nqtot = (int32_t)k.q;
+  P.nbd = (int32_t)k.bd; P.nba = (int32_t)k.ba; P.nst = (int32_t)k.st; P.ntst = (int32_t)k.tst;
+  P.nhb = (int32_t)k.hb; P.nktot = (int32_t)k.k;
+  return 0;
+}
+
+template <typename T>
+int put_tail(tis_ctx *c, T *dev, const std::vector<T> &h, size_t from) {
+  if (h.size() <= from) return 0;
+  HIPCHK(c, hipMemcpyAsync(dev + from, h.data() + from, (h.size() - from) * sizeof(T), hipMemcpyHostToDevice,
+                           c->stream));
+  c->h2d_bytes += (int64_t)((h.size() - from) * sizeof(T));
+  return 0;
+}
+
+// term-major staged slot positions [term][K] -> device rows [slot][stride]
+int put_pos_tail(tis_ctx *c, int32_t *dev, const std::vector<int32_t> &h, int K, size_t stride, size_t t0) {
+  const size_t nt = h.size() / K;
+  if (nt <= t0) return 0;
+  std::vector<int32_t> row(nt - t0);
+  for (int s = 0; s < K; ++s) {
+    for (size_t t = t0; t < nt; ++t) row[t - t0] = h[t * K + s];
+    HIPCHK(c, hipMemcpyAsync(dev + s * stride + t0, row.data(), row.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));   // row is reused
+    c->h2d_bytes += (int64_t)(row.size() * sizeof(int32_t));
+  }
+  return 0;
+}
+
+// Upload the staged arrays from the marks m on (m all zero: everything).
+int upload_stage(tis_ctx *c, const Stage &g, const StageMarks &m) {
+  const ArenaBufs &A = c->ab;
+  int rc = 0;
+#define PT(dev, vec, from) do { rc = put_tail(c, dev, g.vec, from); if (rc) return rc; } while (0)
+  PT(A.rl2, rl2, m.rl2);
+  PT(A.mptype, mptype, m.bead); PT(A.iclass, iclass, m.bead); PT(A.unit, unit, m.bead); PT(A.qidx, qidx, m.bead);
+  PT(A.exv_rad, exv_rad, m.bead); PT(A.exv_eps, exv_eps, m.bead); PT(A.species, species, m.bead);
+  PT(A.charge, charge, m.bead); PT(A.mass, mass, m.bead); PT(A.lcl, lcl, m.bead);
+  PT(A.inc_off, inc_off, m.bead);   // from the old end sentinel on
+  PT(A.qbead, qbead, m.q); PT(A.inc, inc, m.inc); PT(A.lcls, lcls, m.lc); PT(A.sptab, sptab, m.spt);
+  PT(A.bd_mp, bd_mp, 2 * m.bd); PT(A.bd_nat, bd_nat, m.bd); PT(A.bd_coef, bd_coef, m.bd);
+  PT(A.ba_mp, ba_mp, 3 * m.ba); PT(A.ba_nat, ba_nat, m.ba); PT(A.ba_coef, ba_coef, m.ba);
+  PT(A.st_mp, st_mp, 7 * m.st); PT(A.st_nat, st_nat, 3 * m.st); PT(A.st_coef, st_coef, 4 * m.st);
+  PT(A.tst_mp, tst_mp, 6 * m.tst); PT(A.tst_nat, tst_nat, 6 * m.tst); PT(A.tst_coef, tst_coef, 7 * m.tst);
+  PT(A.tst2st, tst2st, 2 * m.tst); PT(A.tst_excl, tst_excl, 2 * m.tst); PT(A.tst_sys, tst_sys, m.tst);
+  PT(A.hb_mp, hb_mp, 6 * m.hb); PT(A.hb_nat, hb_nat, 6 * m.hb); PT(A.hb_coef, hb_coef, 7 * m.hb);
+  PT(A.hb_site, hb_site, 6 * m.hb); PT(A.hb_tert, hb_tert, m.hb); PT(A.hb_sys, hb_sys, m.hb);
+  PT(A.site_val, site_val, m.site);
+  PT(A.st_exoff, st_exoff, m.st); PT(A.st_ex, st_ex, m.ex);
+  PT(A.ewc, ewc, m.k);
+  PT(A.ewh, ewhx, m.k); PT(A.ewh + c->P.nktot, ewhy, m.k); PT(A.ewh + 2 * (size_t)c->P.nktot, ewhz, m.k);
+#undef PT
+  const DevPtrs &P = c->P;
+  if ((rc = put_pos_tail(c, A.bd_pos, g.bd_pos, 2, P.nbd, m.bd))) return rc;
+  if ((rc = put_pos_tail(c, A.ba_pos, g.ba_pos, 3, P.nba, m.ba))) return rc;
+  if ((rc = put_pos_tail(c, A.st_pos, g.st_pos, 7, P.nst, m.st))) return rc;
+  if ((rc = put_pos_tail(c, A.tst_pos, g.tst_pos, 6, P.ntst, m.tst))) return rc;
+  if ((rc = put_pos_tail(c, A.hb_pos, g.hb_pos, 6, P.nhb, m.hb))) return rc;
+  return 0;
+}
+
+StageMarks marks_of(const Stage &g) {
+  StageMarks m;
+  m.bead = g.nbead; m.q = g.qbead.size(); m.bd = g.nbd; m.ba = g.nba; m.st = g.nst; m.tst = g.ntst; m.hb = g.nhb;
+  m.site = g.nsite; m.rl2 = g.rl2.size(); m.inc = g.inc.size(); m.lc = g.lcls.size(); m.spt = g.sptab.size();
+  m.k = g.ewc.size(); m.ex = g.st_ex.size();
+  return m;
+}
+
+// The launch tables of the live systems (tiles, chunk pairs, tile pairs,
+// Ewald tasks, hbond / tertiary-stack systems, the live list) and every
+// system's DSys; tables are rebuilt whole on each change of the live set.
+int build_tables(tis_ctx *c) {
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (void *p : c->tab_allocs) { (void)hipFree(p); }
+  c->dev_bytes -= c->tab_bytes;
+  c->tab_allocs.clear();
+  c->tab_bytes = 0;
+  std::vector<Tile> btiles, qtiles;
+  std::vector<EwTask> ptask, bpair, qpair, ewsk, ewf;
+  std::vector<int32_t> systask, sysidx;
+  c->systask_of.assign(c->nsys, -1);
+  c->any_ele = false;
+  c->any_periodic = false;
+  std::vector<DSys> dsys(c->nsys);
+  for (int is = 0; is < c->nsys; ++is) {
+    HostSys &h = c->hs[is];
+    DSys &d = h.d;
+    h.bt_begin = h.bt_count = h.qt_begin = h.qt_count = h.esk_begin = h.esk_count = 0;
+    h.ef_begin = h.ef_count = h.pt_begin = h.pt_count = h.bp_begin = h.bp_count = h.qp_begin = h.qp_count = 0;
+    if (!h.live) { std::memset(&dsys[is], 0, sizeof(DSys)); dsys[is].nmp = 0; continue; }
+    if (d.periodic) c->any_periodic = true;
+    if (d.ele_form >= 0) c->any_ele = true;
+    h.bt_begin = (int32_t)btiles.size();
+    for (int st0 = 0; st0 < d.nmp; st0 += kTile) btiles.push_back(Tile{is, st0});
+    h.bt_count = (int32_t)btiles.size() - h.bt_begin;
+    h.qt_begin = (int32_t)qtiles.size();
+    for (int st0 = 0; st0 < d.nq; st0 += kTile) qtiles.push_back(Tile{is, st0});
+    h.qt_count = (int32_t)qtiles.size() - h.qt_begin;
+    d.bt0 = h.bt_begin; d.nbt = h.bt_count; d.qt0 = h.qt_begin; d.nqt = h.qt_count;
+    // electrostatic pairs: one wave per chunk pair (I, J), I <= J
+    h.pt_begin = (int32_t)ptask.size();
+    if (d.ele_form != TIS_ELE_NONE && d.nq > 0)
+      for (int I = 0; I < d.nqt; ++I)
+        for (int J = I; J < d.nqt; ++J) ptask.push_back(EwTask{is, I, J});
+    h.pt_count = (int32_t)ptask.size() - h.pt_begin;
+    // tile pairs of the periodic list builds (k_lpair; periodic systems
+    // only: their number and the count buffer grow as nbt^2)
+    h.bp_begin = (int32_t)bpair.size();
+    h.qp_begin = (int32_t)qpair.size();
+    if (d.periodic) {
+      for (int a = 0; a < d.nbt; ++a)
+        for (int b = 0; b < d.nbt; ++b) bpair.push_back(EwTask{is, a, b});
+      for (int a = 0; a < d.nqt; ++a)
+        for (int b = 0; b < d.nqt; ++b) qpair.push_back(EwTask{is, a, b});
+    }
+    h.bp_count = (int32_t)bpair.size() - h.bp_begin;
+    h.qp_count = (int32_t)qpair.size() - h.qp_begin;
+    if (h.has_task) { c->systask_of[is] = (int32_t)systask.size(); systask.push_back(is); }
+    if (d.ele_form == TIS_ELE_EWALD && d.nq > 0) {
+      h.esk_begin = (int32_t)ewsk.size();
+      for (int kt = 0; kt < (d.nk + kTile - 1) / kTile; ++kt)
+        for (int qc = 0; qc < d.nqc; ++qc) ewsk.push_back(EwTask{is, kt, qc});
+      h.esk_count = (int32_t)ewsk.size() - h.esk_begin;
+      h.ef_begin = (int32_t)ewf.size();
+      for (int qt = 0; qt < d.nqc; ++qt)
+        for (int kc = 0; kc < d.nkc; ++kc) ewf.push_back(EwTask{is, qt, kc});
+      h.ef_count = (int32_t)ewf.size() - h.ef_begin;
+    }
+    h.live_pos = (int32_t)sysidx.size();
+    sysidx.push_back(is);
+    dsys[is] = d;
+  }
+  c->nlive = (int32_t)sysidx.size();
+  c->P.nsys = c->nsys;
+  c->nbtile = (int32_t)btiles.size(); c->nqtile = (int32_t)qtiles.size(); c->nsystask = (int32_t)systask.size();
+  c->n_ptask = (int32_t)ptask.size(); c->n_bpair = (int32_t)bpair.size(); c->n_qpair = (int32_t)qpair.size();
+  c->n_ewsk = (int32_t)ewsk.size(); c->n_ewf = (int32_t)ewf.size();
+  if (!dsys.empty()) {
+    HIPCHK(c, hipMemcpyAsync(c->ab.sys, dsys.data(), dsys.size() * sizeof(DSys), hipMemcpyHostToDevice, c->stream));
+    c->h2d_bytes += (int64_t)(dsys.size() * sizeof(DSys));
+  }
+  // whole-batch k_force launches take the off-diagonal chunk pairs (64
+  // steps) before the diagonal ones (32): the order changes no result
+  std::vector<EwTask> lpt(ptask);
+  std::stable_partition(lpt.begin(), lpt.end(), [](const EwTask &t) { return t.a != t.b; });
+  int rc = 0;
+#define UPT(ptr, vec) do { rc = dupload_in(c, c->tab_allocs, &(ptr), vec, &c->tab_bytes); if (rc) return rc; } while (0)
+  UPT(c->d_btiles, btiles); UPT(c->d_qtiles, qtiles);
+  UPT(c->d_ptask, ptask); UPT(c->d_ptask_lpt, lpt); UPT(c->d_bpair, bpair); UPT(c->d_qpair, qpair);
+  UPT(c->d_ewsk, ewsk); UPT(c->d_ewf, ewf); UPT(c->d_systask, systask); UPT(c->d_sysidx, sysidx);
+#undef UPT
+  const size_t nbt = std::max<size_t>(btiles.size(), 1), nqt = std::max<size_t>(qtiles.size(), 1);
+  if ((rc = dalloc_in(c, c->tab_allocs, &c->P.bbox, nbt, &c->tab_bytes))) return rc;
+  if ((rc = dalloc_in(c, c->tab_allocs, &c->P.qbbox, nqt, &c->tab_bytes))) return rc;
+  if ((rc = dalloc_in(c, c->tab_allocs, &c->P.sbox, 4 * nbt, &c->tab_bytes))) return rc;
+  if ((rc = dalloc_in(c, c->tab_allocs, &c->P.ttop, 2 * nbt, &c->tab_bytes))) return rc;
+  c->dev_bytes += c->tab_bytes;
+  for (int k = 0; k < 2; ++k)
+    if (c->graph[k]) { (void)hipGraphExecDestroy(c->graph[k]); c->graph[k] = nullptr; }
+  c->lists_valid = false;
+  c->lv_init = false;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// Lay the arena out again at capacities for the live systems (and `extra`
+// systems about to be staged) times `grow`: the live systems are staged
+// from their host copies in handle order and their positions (current
+// slot), velocities and accelerations are copied over on the device; the
+// counters and the step base carry over.  Retired handles keep their index
+// with no storage.
+int relayout(tis_ctx *c, const std::vector<const tis_system *> &extra, double grow) {
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  Stage g;
+  std::vector<DSys> old_d(c->nsys);
+  std::vector<HostSys> hs2(c->hs);   // committed once the new arena exists
+  bool periodic = false;
+  for (int is = 0; is < c->nsys; ++is) {
+    HostSys &h = hs2[is];
+    old_d[is] = h.d;
+    if (!h.live) { h.stored = false; continue; }
+    const uint64_t step_origin = h.d.step_origin;
+    const int32_t init_accel = h.d.init_accel;
+    int rc = stage_system(c, g, h, h.src->s, is, h.d.key);
+    if (rc) return rc;
+    h.d.step_origin = step_origin;
+    h.d.init_accel = init_accel;
+    periodic = periodic || h.d.periodic;
+  }
+  // capacity for what is about to be appended
+  Stage probe = g;
+  std::vector<HostSys> tmp(extra.size());
+  for (size_t k = 0; k < extra.size(); ++k) {
+    int rc = stage_system(c, probe, tmp[k], *extra[k], c->nsys + (int)k, 0);
+    if (rc) return rc;
+    periodic = periodic || extra[k]->p.periodic;
+  }
+  const Caps caps = caps_for(probe, c->nsys + (int)extra.size(), periodic, grow);
+  std::vector<void *> list;
+  DevPtrs P;
+  ArenaBufs A;
+  const size_t bytes0 = c->dev_bytes;
+  c->dev_bytes = c->tab_bytes;
+  int rc = alloc_arena(c, caps, list, P, A, c->P.nb_cap);
+  if (rc) {
+    for (void *p : list) (void)hipFree(p);
+    c->dev_bytes = bytes0;
+    return rc;
+  }
+  c->hs = std::move(hs2);
+  // the new layout's topology from the host copies
+  const DevPtrs O = c->P;
+  const ArenaBufs OA = c->ab;
+  const uint64_t seed = c->P.seed;
+  std::vector<void *> old = std::move(c->allocs);
+  c->allocs = list;
+  c->P = P;
+  c->P.seed = seed;
+  c->ab = A;
+  c->caps = caps;
+  c->stage = std::move(g);
+  c->d_sys = A.sys; c->d_ststat = A.ststat; c->d_energy = A.energy;
+  c->d_counters = A.counters; c->d_stepbase = A.stepbase;
+  c->d_eunit = nullptr;
+  c->eunit_cap = 0;
+  c->relayouts += 1;
+  rc = upload_stage(c, c->stage, StageMarks{});
+  // then, on the device: the dynamic state, and what temperature changes
+  // (tis_set_system_T, replica labels) have made of charges, stack U0 and
+  // the temperature fields; the counters and the step base
+  const int cu = c->cur;
+  for (int is = 0; rc == 0 && is < c->nsys; ++is) {
+    HostSys &h = c->hs[is];
+    if (!h.live) continue;
+    const size_t n = (size_t)h.d.nmp, so = (size_t)old_d[is].off, sn = (size_t)h.d.off;
+    for (int d = 0; d < 3 && rc == 0; ++d) {
+      const size_t oo = d * (size_t)O.ntot + so, no = d * (size_t)P.ntot + sn;
+      if (hipMemcpyAsync(P.x[cu] + no, O.x[cu] + oo, n * 8, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+          hipMemcpyAsync(P.v + no, O.v + oo, n * 8, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+          hipMemcpyAsync(P.a + no, O.a + oo, n * 8, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+        rc = fail(c, TIS_ERR_HIP, "state copy of a new layout failed");
+    }
+    c->d2d_state += (int64_t)(9 * n * 8);
+    if (rc == 0 && (hipMemcpyAsync(A.charge + sn, OA.charge + so, n * 8, hipMemcpyDeviceToDevice, c->stream) !=
+                        hipSuccess ||
+                    (h.d.nst > 0 && hipMemcpyAsync(A.st_coef + 4 * (size_t)h.d.soff,
+                                                   OA.st_coef + 4 * (size_t)old_d[is].soff,
+                                                   32 * (size_t)h.d.nst, hipMemcpyDeviceToDevice,
+                                                   c->stream) != hipSuccess)))
+      rc = fail(c, TIS_ERR_HIP, "parameter copy of a new layout failed");
+    copy_t_fields(h.d, old_d[is]);
+  }
+  if (rc == 0 && OA.counters &&
+      (hipMemcpyAsync(A.counters, OA.counters, TIS_CNT_MAX * sizeof(unsigned long long), hipMemcpyDeviceToDevice,
+                      c->stream) != hipSuccess ||
+       hipMemcpyAsync(A.stepbase, OA.stepbase, sizeof(uint64_t), hipMemcpyDeviceToDevice, c->stream) != hipSuccess))
+    rc = fail(c, TIS_ERR_HIP, "counter copy of a new layout failed");
+  (void)hipStreamSynchronize(c->stream);
+  for (void *p : old) (void)hipFree(p);
+  return rc;
+}
+
+// Stage systems at the arena's end and upload their segments, laying the
+// arena out again (with room to grow by `grow`) when they do not fit.
+int append_systems(tis_ctx *c, int32_t nadd, const tis_system *sys, const uint32_t *keys, double grow) {
+  for (int k = 0; k < nadd; ++k)
+    if (int rc = validate_system(&c->err, c->nsys + k, sys[k])) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  // fits as it is?
+  bool fits = false;
+  {
+    Stage t = c->stage;
+    std::vector<HostSys> tmp(nadd);
+    bool periodic = c->caps.periodic;
+    bool ok = true;
+    for (int k = 0; k < nadd && ok; ++k) {
+      if (int rc = stage_system(c, t, tmp[k], sys[k], c->nsys + k, 0)) return rc;
+      if (sys[k].p.periodic && !c->caps.periodic) ok = false;
+      periodic = periodic || sys[k].p.periodic;
+    }
+    fits = ok && c->have_arena && caps_fit(c->caps, t, c->nsys + nadd, periodic);
+  }
+  if (!fits) {
+    std::vector<const tis_system *> ex(nadd);
+    for (int k = 0; k < nadd; ++k) ex[k] = &sys[k];
+    int rc = relayout(c, ex, grow);
+    if (rc) return rc;
+    c->have_arena = true;
+  }
+  const StageMarks m = marks_of(c->stage);
+  for (int k = 0; k < nadd; ++k) {
+    const int is = c->nsys;
+    c->hs.emplace_back();
+    HostSys &h = c->hs.back();
+    h.src = copy_system(sys[k]);
+    h.live = true;
+    c->nsys += 1;
+    int rc = stage_system(c, c->stage, h, h.src->s, is, keys ? keys[k] : (uint32_t)is);
+    if (rc) { c->hs.pop_back(); c->nsys -= 1; return rc; }
+  }
+  int rc = upload_stage(c, c->stage, m);
+  if (rc) return rc;
+  c->nbeads = c->stage.nbead; c->nqtot = c->stage.nq; c->nhbtot = c->stage.nhb; c->ntsttot = c->stage.ntst;
+  c->nsttot = c->stage.nst;
+  return build_tables(c);
+}
+
+}  // namespace
+
+
 extern "C" {
 
 int32_t tis_ctx_create(int32_t device, tis_ctx **out) {
@@ -453,402 +1237,56 @@ int32_t tis_system_upload(tis_ctx *c, int32_t nsys, const tis_system *sys, const
     if (int rc = validate_system(&c->err, is, sys[is])) return rc;
   HIPCHK(c, hipSetDevice(c->device));
   free_all(c);
-  c->nsys = nsys;
-  c->hs.resize(nsys);
-  // host staging
-  std::vector<double> rl2;
-  std::vector<int8_t> mptype, iclass;
-  std::vector<int32_t> unit, qidx, qbead, inc_off;
-  std::vector<uint32_t> inc;
-  std::vector<double> exv_rad, charge, mass;
-  std::vector<uint8_t> lcl;
-  std::vector<LangClass> lcls;
-  std::vector<float> exv_eps;
-  std::vector<int32_t> bd_mp, ba_mp, st_mp, tst_mp, tst2st, tst_excl, hb_mp, hb_site, hb_tert, site_val,
-      hb_sys, tst_sys;
-  std::vector<double> bd_nat, bd_coef, ba_nat, ba_coef, st_nat, st_coef, tst_nat, tst_coef, hb_nat, hb_coef;
-  std::vector<Tile> btiles, qtiles;
-  std::vector<int32_t> systask;
-  std::vector<double> ewhx, ewhy, ewhz, ewc;
-  std::vector<EwTask> ewsk, ewf, ptask, bpair, qpair;
-  int64_t psk_off = 0, pf_off = 0;
-  // excluded-volume species: distinct (mptype, class, radius, epsilon)
-  struct Spec { int8_t t, c; double r, e; };
-  std::vector<Spec> specs;
-  std::vector<uint8_t> species;
-  auto spec_of = [&](int8_t t, int8_t cl, double r, double e) -> int {
-    for (size_t k = 0; k < specs.size(); ++k)
-      if (specs[k].t == t && specs[k].c == cl && specs[k].r == r && specs[k].e == e) return (int)k;
-    specs.push_back(Spec{t, cl, r, e});
-    return (int)specs.size() - 1;
-  };
-  c->systask_of.assign(nsys, -1);
-  int64_t moff = 0, mioff = 0, pp_off = 0, lp_off = 0;
-  int32_t off = 0, qoff = 0, nbd = 0, nba = 0, nst = 0, ntst = 0, nhb = 0, nsite = 0, uoff = 0;
-  c->any_ele = false;
-  c->any_periodic = false;
-  for (int is = 0; is < nsys; ++is) {
-    const tis_system &s = sys[is];
-    HostSys &h = c->hs[is];
-    if (s.nmp <= 0 || s.nunit <= 0) return fail(c, TIS_ERR_ARG, "system %d has no beads", is);
-    DSys &d = h.d;
-    std::memset(&d, 0, sizeof(d));
-    fill_derived(d, s);
-    d.key = keys ? keys[is] : (uint32_t)is;
-    d.off = off; d.nmp = s.nmp;
-    d.uoff = uoff; d.nunit = s.nunit;
-    d.rl2max = 0.0;
-    for (int k = 0; k < s.nunit * s.nunit; ++k) {
-      rl2.push_back(s.rlist2_unit[k]);
-      d.rl2max = std::max(d.rl2max, s.rlist2_unit[k]);
-    }
-    uoff += s.nunit * s.nunit;
-    if (s.p.periodic) c->any_periodic = true;
-    if (s.p.ele_form >= 0) c->any_ele = true;
-    if (s.p.ele_form == TIS_ELE_EWALD) {
-      if (!s.p.periodic || s.p.box[0] != s.p.box[1] || s.p.box[0] != s.p.box[2])
-        return fail(c, TIS_ERR_ARG, "system %d: Ewald needs a periodic cubic box", is);
-      if (!(s.p.ewld_alpha > 0.0) || s.p.ewld_hmax <= 0)
-        return fail(c, TIS_ERR_ARG, "system %d: Ewald needs ewld_alpha > 0 and ewld_hmax > 0", is);
-    }
-    if (s.p.ele_form != TIS_ELE_NONE && s.p.ele_form != TIS_ELE_DH && s.p.ele_form != TIS_ELE_COULOMB &&
-        s.p.ele_form != TIS_ELE_EWALD)
-      return fail(c, TIS_ERR_ARG, "system %d: unknown ele_form %d", is, s.p.ele_form);
-    // beads
-    d.qoff = qoff;
-    int nq = 0;
-    for (int i = 0; i < s.nmp; ++i) {
-      mptype.push_back((int8_t)s.mptype[i]);
-      iclass.push_back((int8_t)s.iclass[i]);
-      unit.push_back(s.unit[i]);
-      exv_rad.push_back(s.exv_rad[i]);
-      exv_eps.push_back((float)s.exv_eps[i]);
-      species.push_back((uint8_t)spec_of((int8_t)s.mptype[i], (int8_t)s.iclass[i], s.exv_rad[i], s.exv_eps[i]));
-      charge.push_back(s.charge[i]);
-      mass.push_back(s.mass[i]);
-      h.mass.push_back(s.mass[i]);
-      h.fric.push_back(s.fric[i]);
-      // Langevin class: the system's (friction, mass) pairs in order of
-      // first appearance
-      int cl = 0;
-      while (cl < (int)h.lc_fric.size() && !(h.lc_fric[cl] == s.fric[i] && h.lc_mass[cl] == s.mass[i])) ++cl;
-      if (cl == (int)h.lc_fric.size()) {
-        if (cl == 256) return fail(c, TIS_ERR_ARG, "system %d: more than 256 (friction, mass) classes", is);
-        h.lc_fric.push_back(s.fric[i]);
-        h.lc_mass.push_back(s.mass[i]);
-      }
-      lcl.push_back((uint8_t)cl);
-      // charges of the ele list: |q| >= ZERO_JUDGE (neighbor_list_ele.F90:134,147)
-      if (s.p.ele_form != TIS_ELE_NONE && std::fabs(s.charge[i]) >= 1.0e-6) {
-        qidx.push_back(qoff + nq);
-        qbead.push_back(off + i);
-        ++nq;
-      } else {
-        qidx.push_back(-1);
-      }
-    }
-    // its Langevin classes (constants r[] set by tis_langevin_init)
-    d.lc0 = (int32_t)lcls.size();
-    d.nlc = (int32_t)h.lc_fric.size();
-    for (int k = 0; k < d.nlc; ++k)
-      lcls.push_back(LangClass{{0.0, 0.0, 0.0, 0.0}, 1.0 / h.lc_mass[k], h.lc_fric[k], h.lc_mass[k], 0.0});
-    d.nq = nq;
-    d.mwords = (nq + 31) / 32;
-    d.moff = moff;
-    moff += (int64_t)nq * d.mwords;
-    d.mioff = s.p.periodic ? mioff : -1;
-    if (s.p.periodic) mioff += (int64_t)nq * nq;
-    qoff += nq;
-    // terms with global bead indices
-    d.boff = nbd; d.nbd = s.nbd;
-    for (int b = 0; b < s.nbd; ++b) {
-      bd_mp.push_back(off + s.bd_mp[2 * b]); bd_mp.push_back(off + s.bd_mp[2 * b + 1]);
-      bd_nat.push_back(s.bd_nat[b]); bd_coef.push_back(s.bd_coef[b]);
-    }
-    d.aoff = nba; d.nba = s.nba;
-    for (int a = 0; a < s.nba; ++a) {
-      for (int k = 0; k < 3; ++k) ba_mp.push_back(off + s.ba_mp[3 * a + k]);
-      ba_nat.push_back(s.ba_nat[a]); ba_coef.push_back(s.ba_coef[a]);
-    }
-    d.soff = nst; d.nst = s.nst;
-    for (int t = 0; t < s.nst; ++t) {
-      for (int k = 0; k < 7; ++k) st_mp.push_back(off + s.st_mp[7 * t + k]);
-      for (int k = 0; k < 3; ++k) st_nat.push_back(s.st_nat[3 * t + k]);
-      for (int k = 0; k < 4; ++k) st_coef.push_back(s.st_coef[4 * t + k]);
-    }
-    d.toff = ntst; d.ntst = s.ntst;
-    for (int t = 0; t < s.ntst; ++t) {
-      for (int k = 0; k < 6; ++k) tst_mp.push_back(off + s.tst_mp[6 * t + k]);
-      for (int k = 0; k < 6; ++k) tst_nat.push_back(s.tst_nat[6 * t + k]);
-      for (int k = 0; k < 7; ++k) tst_coef.push_back(s.tst_coef[7 * t + k]);
-      for (int k = 0; k < 2; ++k) {
-        tst2st.push_back(s.tst2st[2 * t + k] >= 0 ? nst + s.tst2st[2 * t + k] : -1);
-        tst_excl.push_back(s.tst_excl[2 * t + k]);
-      }
-      tst_sys.push_back(is);
-    }
-    d.sitoff = nsite; d.nsite = s.nhbsite;
-    for (int k = 0; k < s.nhbsite; ++k) site_val.push_back(s.hbsite_valence[k]);
-    d.hoff = nhb; d.nhb = s.nhb;
-    std::vector<int> site_use(std::max(s.nhbsite, 1), 0);
-    for (int t = 0; t < s.nhb; ++t) {
-      for (int k = 0; k < 6; ++k) hb_mp.push_back(off + s.hb_mp[6 * t + k]);
-      for (int k = 0; k < 6; ++k) hb_nat.push_back(s.hb_nat[6 * t + k]);
-      for (int k = 0; k < 7; ++k) hb_coef.push_back(s.hb_coef[7 * t + k]);
-      for (int k = 0; k < 6; ++k) {
-        int st = s.hb_site[6 * t + k];
-        hb_site.push_back(st >= 0 ? nsite + st : -1);
-        if (st >= 0) site_use[st] += 1;
-      }
-      hb_tert.push_back(s.hb_tertiary ? s.hb_tertiary[t] : 0);
-      hb_sys.push_back(is);
-    }
-    d.hb_excess_possible = 0;
-    for (int k = 0; k < s.nhbsite; ++k)
-      if (site_use[k] > s.hbsite_valence[k]) d.hb_excess_possible = 1;
-    // incidence lists: bonds, angles, stacks, tertiary stacks, hbonds
-    std::vector<std::vector<uint32_t>> incb(s.nmp);
-    for (int b = 0; b < s.nbd; ++b)
-      for (int k = 0; k < 2; ++k) incb[s.bd_mp[2 * b + k]].push_back(inc_pack(INC_BOND, k, nbd + b));
-    for (int a = 0; a < s.nba; ++a)
-      for (int k = 0; k < 3; ++k) incb[s.ba_mp[3 * a + k]].push_back(inc_pack(INC_ANGLE, k, nba + a));
-    for (int t = 0; t < s.nst; ++t)
-      for (int k = 0; k < 7; ++k) incb[s.st_mp[7 * t + k]].push_back(inc_pack(INC_STACK, k, nst + t));
-    for (int t = 0; t < s.ntst; ++t)
-      for (int k = 0; k < 6; ++k) incb[s.tst_mp[6 * t + k]].push_back(inc_pack(INC_TSTACK, k, ntst + t));
-    for (int t = 0; t < s.nhb; ++t)
-      for (int k = 0; k < 6; ++k) incb[s.hb_mp[6 * t + k]].push_back(inc_pack(INC_HBOND, k, nhb + t));
-    for (int i = 0; i < s.nmp; ++i) {
-      inc_off.push_back((int32_t)inc.size());
-      inc.insert(inc.end(), incb[i].begin(), incb[i].end());
-    }
-    nbd += s.nbd; nba += s.nba; nst += s.nst; ntst += s.ntst; nhb += s.nhb; nsite += s.nhbsite;
-    // tiles
-    h.bt_begin = (int32_t)btiles.size();
-    for (int st0 = 0; st0 < s.nmp; st0 += kTile) btiles.push_back(Tile{is, st0});
-    h.bt_count = (int32_t)btiles.size() - h.bt_begin;
-    h.qt_begin = (int32_t)qtiles.size();
-    for (int st0 = 0; st0 < nq; st0 += kTile) qtiles.push_back(Tile{is, st0});
-    h.qt_count = (int32_t)qtiles.size() - h.qt_begin;
-    d.bt0 = h.bt_begin; d.nbt = h.bt_count; d.qt0 = h.qt_begin; d.nqt = h.qt_count;
-    // electrostatic pairs: one wave per chunk pair (I, J), I <= J
-    d.pp_off = pp_off;
-    h.pt_begin = (int32_t)ptask.size();
-    if (s.p.ele_form != TIS_ELE_NONE && nq > 0) {
-      pp_off += (int64_t)d.nqt * (d.nqt + 1) / 2 * 6 * kTile;
-      for (int I = 0; I < d.nqt; ++I)
-        for (int J = I; J < d.nqt; ++J) ptask.push_back(EwTask{is, I, J});
-    }
-    h.pt_count = (int32_t)ptask.size() - h.pt_begin;
-    // tile pairs of the periodic list builds (k_lpair; periodic systems
-    // only: their number and the count buffer grow as nbt^2)
-    d.lp_off = lp_off;
-    h.bp_begin = (int32_t)bpair.size();
-    h.qp_begin = (int32_t)qpair.size();
-    if (s.p.periodic) {
-      lp_off += (int64_t)d.nbt * d.nbt;
-      for (int a = 0; a < d.nbt; ++a)
-        for (int b = 0; b < d.nbt; ++b) bpair.push_back(EwTask{is, a, b});
-      for (int a = 0; a < d.nqt; ++a)
-        for (int b = 0; b < d.nqt; ++b) qpair.push_back(EwTask{is, a, b});
-    }
-    h.bp_count = (int32_t)bpair.size() - h.bp_begin;
-    h.qp_count = (int32_t)qpair.size() - h.qp_begin;
-    h.has_task = (s.nhb > 0 || s.ntst > 0);
-    if (h.has_task) { c->systask_of[is] = (int32_t)systask.size(); systask.push_back(is); }
-    // Ewald reciprocal vectors and work items
-    if (s.p.ele_form == TIS_ELE_EWALD && nq > 0) {
-      d.ewk0 = (int32_t)ewc.size();
-      ewald_kvectors(s.p, ewhx, ewhy, ewhz, ewc);
-      d.nk = (int32_t)ewc.size() - d.ewk0;
-      d.nqc = (nq + kTile - 1) / kTile;
-      d.nkc = (d.nk + kEwKChunk - 1) / kEwKChunk;
-      d.psk_off = psk_off; psk_off += 2 * (int64_t)d.nqc * d.nk;
-      d.pf_off = pf_off; pf_off += 3 * (int64_t)d.nkc * nq;
-      h.esk_begin = (int32_t)ewsk.size();
-      for (int kt = 0; kt < (d.nk + kTile - 1) / kTile; ++kt)
-        for (int qc = 0; qc < d.nqc; ++qc) ewsk.push_back(EwTask{is, kt, qc});
-      h.esk_count = (int32_t)ewsk.size() - h.esk_begin;
-      h.ef_begin = (int32_t)ewf.size();
-      for (int qt = 0; qt < d.nqc; ++qt)
-        for (int kc = 0; kc < d.nkc; ++kc) ewf.push_back(EwTask{is, qt, kc});
-      h.ef_count = (int32_t)ewf.size() - h.ef_begin;
-    }
-    off += s.nmp;
-  }
-  inc_off.push_back((int32_t)inc.size());
-  if (off >= (1 << 27)) return fail(c, TIS_ERR_ARG, "too many beads (%d)", off);
-  if (specs.size() > 255) return fail(c, TIS_ERR_ARG, "too many bead species (%zu)", specs.size());
-  // incidence index of every (term, slot): the term passes write there
-  std::vector<int32_t> bd_pos(2 * (size_t)nbd), ba_pos(3 * (size_t)nba), st_pos(7 * (size_t)nst),
-      tst_pos(6 * (size_t)ntst), hb_pos(6 * (size_t)nhb);
-  for (size_t k = 0; k < inc.size(); ++k) {
-    const uint32_t e = inc[k], type = e >> 29, slot = (e >> 26) & 7u, term = e & ((1u << 26) - 1u);
-    switch (type) {
-      case INC_BOND: bd_pos[(size_t)slot * nbd + term] = (int32_t)k; break;
-      case INC_ANGLE: ba_pos[(size_t)slot * nba + term] = (int32_t)k; break;
-      case INC_STACK: st_pos[(size_t)slot * nst + term] = (int32_t)k; break;
-      case INC_TSTACK: tst_pos[(size_t)slot * ntst + term] = (int32_t)k; break;
-      default: hb_pos[(size_t)slot * nhb + term] = (int32_t)k; break;
-    }
-  }
-  // species-pair tables (force_exv_dt15.F90:60-80): D_ij and eps_i eps_j,
-  // one per distinct (exv_dist, exv_dist_PS) of the batch
-  const int NS = (int)specs.size();
-  std::vector<SpPair> sptab;
-  std::vector<std::pair<double, double>> tabkeys;
-  for (int is = 0; is < nsys; ++is) {
-    DSys &d = c->hs[is].d;
-    int found = -1;
-    for (size_t k = 0; k < tabkeys.size(); ++k)
-      if (tabkeys[k].first == d.exv_dist && tabkeys[k].second == d.exv_dist_PS) found = (int)k;
-    if (found < 0) {
-      found = (int)tabkeys.size();
-      tabkeys.push_back({d.exv_dist, d.exv_dist_PS});
-      for (int a = 0; a < NS; ++a)
-        for (int b = 0; b < NS; ++b) {
-          const Spec &A = specs[a], &B = specs[b];
-          double dij;
-          if ((A.t == TIS_MP_RNA_PHOS && B.t == TIS_MP_RNA_SUGAR) || (A.t == TIS_MP_RNA_SUGAR && B.t == TIS_MP_RNA_PHOS))
-            dij = d.exv_dist_PS;
-          else if (A.c == TIS_CLASS_RNA && B.c == TIS_CLASS_RNA)
-            dij = d.exv_dist;
-          else
-            dij = A.r + B.r;
-          sptab.push_back(SpPair{dij, dij * dij, A.e * B.e});
-        }
-    }
-    d.sptab = found * NS * NS;
-  }
-  c->ntot = off; c->nqtot = qoff; c->nhbtot = nhb; c->ntsttot = ntst; c->nsttot = nst;
-  c->nbtile = (int32_t)btiles.size(); c->nqtile = (int32_t)qtiles.size(); c->nsystask = (int32_t)systask.size();
-
-  // device buffers
-  DevPtrs &P = c->P;
-  P = DevPtrs{};
-  P.nsys = nsys; P.ntot = off; P.nqtot = qoff;
-  int rc = 0;
-  std::vector<DSys> dsys(nsys);
-  for (int is = 0; is < nsys; ++is) dsys[is] = c->hs[is].d;
-#define UP(ptr, vec) do { rc = dupload(c, &ptr, vec); if (rc) return rc; } while (0)
-  UP(c->d_sys, dsys);
-  P.sys = c->d_sys;
-  for (int k = 0; k < 2; ++k) {
-    rc = dalloc(c, &P.x[k], 3 * (size_t)off); if (rc) return rc;
-    rc = dalloc(c, &P.px[k], c->any_periodic ? 3 * (size_t)off : 1); if (rc) return rc;
-  }
-  rc = dalloc(c, &P.v, 3 * (size_t)off); if (rc) return rc;
-  rc = dalloc(c, &P.a, 3 * (size_t)off); if (rc) return rc;
-  rc = dalloc(c, &P.f, 3 * (size_t)off); if (rc) return rc;
-  uint8_t *d_lcl;
-  UP(d_lcl, lcl); P.lcl = d_lcl;
-  UP(P.lcls, lcls);
-  double *d_mass, *d_rad, *d_charge, *d_rl2;
-  int8_t *d_mpt, *d_icl;
-  int32_t *d_unit, *d_qidx, *d_qbead, *d_incoff;
-  uint32_t *d_inc;
-  float *d_eps;
-  UP(d_mass, mass); UP(d_rad, exv_rad); UP(d_charge, charge); UP(d_rl2, rl2);
-  UP(d_mpt, mptype); UP(d_icl, iclass); UP(d_unit, unit); UP(d_qidx, qidx); UP(d_qbead, qbead);
-  UP(d_incoff, inc_off); UP(d_inc, inc); UP(d_eps, exv_eps);
-  int32_t *d_pos;
-  UP(d_pos, bd_pos); P.bd_pos = d_pos; UP(d_pos, ba_pos); P.ba_pos = d_pos; UP(d_pos, st_pos); P.st_pos = d_pos;
-  UP(d_pos, tst_pos); P.tst_pos = d_pos; UP(d_pos, hb_pos); P.hb_pos = d_pos;
-  P.ninc = (int32_t)inc.size();
-  rc = dalloc(c, &P.fslot, 3 * std::max<size_t>(inc.size(), 1)); if (rc) return rc;
-  uint8_t *d_species;
-  SpPair *d_sptab;
-  UP(d_species, species); UP(d_sptab, sptab);
-  P.species = d_species; P.sptab = d_sptab; P.nspecies = NS;
-  rc = dalloc(c, &P.bbox, (size_t)c->nbtile); if (rc) return rc;
-  rc = dalloc(c, &P.qbbox, (size_t)c->nqtile); if (rc) return rc;
-  rc = dalloc(c, &P.sbox, 4 * (size_t)c->nbtile); if (rc) return rc;
-  rc = dalloc(c, &P.lcnt, (size_t)std::max<int64_t>(lp_off, 1) * kTile); if (rc) return rc;
-  P.mass = d_mass; P.exv_rad = d_rad; P.charge = d_charge; P.rl2 = d_rl2;
-  P.mptype = d_mpt; P.iclass = d_icl; P.unit = d_unit; P.qidx = d_qidx; P.qbead = d_qbead;
-  P.inc_off = d_incoff; P.inc = d_inc; P.exv_eps = d_eps;
-  int32_t *i32; double *f64;
-  UP(i32, bd_mp); P.bd_mp = i32; UP(f64, bd_nat); P.bd_nat = f64; UP(f64, bd_coef); P.bd_coef = f64; P.nbd = nbd;
-  UP(i32, ba_mp); P.ba_mp = i32; UP(f64, ba_nat); P.ba_nat = f64; UP(f64, ba_coef); P.ba_coef = f64; P.nba = nba;
-  UP(i32, st_mp); P.st_mp = i32; UP(f64, st_nat); P.st_nat = f64; UP(f64, st_coef); P.st_coef = f64; P.nst = nst;
-  UP(i32, tst_mp); P.tst_mp = i32; UP(f64, tst_nat); P.tst_nat = f64; UP(f64, tst_coef); P.tst_coef = f64;
-  UP(i32, tst2st); P.tst2st = i32; UP(i32, tst_excl); P.tst_excl = i32; P.ntst = ntst;
-  UP(i32, hb_mp); P.hb_mp = i32; UP(f64, hb_nat); P.hb_nat = f64; UP(f64, hb_coef); P.hb_coef = f64;
-  UP(i32, hb_site); P.hb_site = i32; UP(i32, hb_tert); P.hb_tert = i32; P.nhb = nhb;
-  UP(i32, hb_sys); P.hb_sys = i32; UP(i32, tst_sys); P.tst_sys = i32;
-  UP(i32, site_val); P.site_val = i32;
-  {
-    // per consecutive stack, the exclusive tertiary stacks that name it
-    std::vector<int32_t> cnt(nst + 1, 0), exoff(nst + 1, 0), ex;
-    for (int t = 0; t < ntst; ++t)
-      for (int k = 0; k < 2; ++k)
-        if (tst_excl[2 * t + k] && tst2st[2 * t + k] >= 0) ++cnt[tst2st[2 * t + k]];
-    for (int st = 0; st < nst; ++st) exoff[st + 1] = exoff[st] + cnt[st];
-    ex.assign(std::max(exoff[nst], 1), 0);
-    std::vector<int32_t> fill(exoff.begin(), exoff.end() - 1);
-    for (int t = 0; t < ntst; ++t)
-      for (int k = 0; k < 2; ++k)
-        if (tst_excl[2 * t + k] && tst2st[2 * t + k] >= 0) ex[fill[tst2st[2 * t + k]]++] = t;
-    UP(i32, exoff); P.st_exoff = i32; UP(i32, ex); P.st_ex = i32;
-  }
-  rc = dalloc(c, &P.hb_inlist, (size_t)nhb); if (rc) return rc;
-  rc = dalloc(c, &P.hb_status, (size_t)nhb); if (rc) return rc;
-  rc = dalloc(c, &P.hb_ene, (size_t)nhb); if (rc) return rc;
-  rc = dalloc(c, &P.for_hb, 18 * (size_t)nhb); if (rc) return rc;
-  rc = dalloc(c, &P.site_cnt, (size_t)nsite); if (rc) return rc;
-  rc = dalloc(c, &P.hb_seq, (size_t)nhb); if (rc) return rc;
-  rc = dalloc(c, &c->d_ststat, (size_t)nst); if (rc) return rc;
-  rc = dalloc(c, &P.nb_cnt, (size_t)off); if (rc) return rc;
-  P.nb_cap = 64;
-  rc = dalloc(c, &P.nb, (size_t)P.nb_cap * off); if (rc) return rc;
-  rc = dalloc(c, &P.dxyz, 3 * (size_t)off); if (rc) return rc;
-  rc = dalloc(c, &P.dlast, 3 * (size_t)off); if (rc) return rc;
-  rc = dalloc(c, &P.ttop, 2 * btiles.size()); if (rc) return rc;
-  rc = dalloc(c, &P.sys_flag, (size_t)nsys); if (rc) return rc;
-  rc = dalloc(c, &P.pp, (size_t)std::max<int64_t>(pp_off, 1)); if (rc) return rc;
-  rc = dalloc(c, &P.qmask, (size_t)moff); if (rc) return rc;
-  rc = dalloc(c, &P.qmirror, (size_t)std::max<int64_t>(mioff, 1)); if (rc) return rc;
-  rc = dalloc(c, &c->d_counters, TIS_CNT_MAX); if (rc) return rc;
-  rc = dalloc(c, &c->d_stepbase, 1); if (rc) return rc;
-  P.counters = c->d_counters;
-  P.stepbase = c->d_stepbase;
-  P.seed = seed;
-  UP(c->d_btiles, btiles);
-  UP(c->d_qtiles, qtiles);
-  {
-    std::vector<double> h3;
-    h3.insert(h3.end(), ewhx.begin(), ewhx.end());
-    h3.insert(h3.end(), ewhy.begin(), ewhy.end());
-    h3.insert(h3.end(), ewhz.begin(), ewhz.end());
-    double *dh = nullptr, *dc = nullptr;
-    UP(dh, h3); UP(dc, ewc);
-    P.ewh = dh; P.ewc = dc; P.nktot = (int32_t)ewc.size();
-    rc = dalloc(c, &P.psk, (size_t)std::max<int64_t>(psk_off, 1)); if (rc) return rc;
-    rc = dalloc(c, &P.pf, (size_t)std::max<int64_t>(pf_off, 1)); if (rc) return rc;
-    UP(c->d_ewsk, ewsk);
-    UP(c->d_ewf, ewf);
-    UP(c->d_ptask, ptask);
-    c->n_ptask = (int32_t)ptask.size();
-    // whole-batch k_force launches take the off-diagonal chunk pairs (64
-    // steps) before the diagonal ones (32): the order changes no result
-    std::vector<EwTask> lpt(ptask);
-    std::stable_partition(lpt.begin(), lpt.end(), [](const EwTask &t) { return t.a != t.b; });
-    UP(c->d_ptask_lpt, lpt);
-    UP(c->d_bpair, bpair);
-    c->n_bpair = (int32_t)bpair.size();
-    UP(c->d_qpair, qpair);
-    c->n_qpair = (int32_t)qpair.size();
-    c->n_ewsk = (int32_t)ewsk.size();
-    c->n_ewf = (int32_t)ewf.size();
-  }
-  UP(c->d_systask, systask);
-  std::vector<int32_t> sysidx(nsys);
-  for (int is = 0; is < nsys; ++is) sysidx[is] = is;
-  UP(c->d_sysidx, sysidx);
-  rc = dalloc(c, &c->d_energy, 25 * (size_t)nsys); if (rc) return rc;
-#undef UP
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  // an arena that fits the batch exactly
+  int rc = append_systems(c, nsys, sys, keys, 1.0);
+  if (rc) return rc;
+  c->P.seed = seed;
   c->cur = 0;
+  return TIS_OK;
+}
+
+int32_t tis_system_append(tis_ctx *c, int32_t nadd, const tis_system *sys, const uint32_t *keys,
+                          int32_t *isys_out) {
+  if (!c || nadd <= 0 || !sys) return fail(c, TIS_ERR_ARG, "bad arguments to tis_system_append");
+  if (!c->have_arena) return fail(c, TIS_ERR_STATE, "no batch: tis_system_upload first");
+  const int first = c->nsys;
+  // room to grow: a new layout reserves twice what it holds
+  int rc = append_systems(c, nadd, sys, keys, 2.0);
+  if (rc) return rc;
+  if (isys_out)
+    for (int k = 0; k < nadd; ++k) isys_out[k] = first + k;
+  return TIS_OK;
+}
+
+int32_t tis_system_retire(tis_ctx *c, int32_t n, const int32_t *isys) {
+  if (!c || n < 0 || (n > 0 && !isys)) return TIS_ERR_ARG;
+  for (int k = 0; k < n; ++k) {
+    if (isys[k] < 0 || isys[k] >= c->nsys || !c->hs[isys[k]].live)
+      return fail(c, TIS_ERR_ARG, "system %d is not live", isys[k]);
+    for (int j = 0; j < k; ++j)
+      if (isys[j] == isys[k]) return fail(c, TIS_ERR_ARG, "system %d listed twice", isys[k]);
+  }
+  if (n == 0) return TIS_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  for (int k = 0; k < n; ++k) {
+    HostSys &h = c->hs[isys[k]];
+    h.live = false;
+    h.src.reset();
+  }
+  return build_tables(c);
+}
+
+int32_t tis_arena_stats(tis_ctx *c, int64_t *stats) {
+  if (!c || !stats) return TIS_ERR_ARG;
+  int64_t live = 0;
+  for (const HostSys &h : c->hs)
+    if (h.live) live += h.d.nmp;
+  stats[0] = c->nlive;
+  stats[1] = live;
+  stats[2] = c->stage.nbead;
+  stats[3] = c->have_arena ? c->caps.bead : 0;
+  stats[4] = c->relayouts;
+  stats[5] = c->nsys;
   return TIS_OK;
 }
 
@@ -895,9 +1333,9 @@ int32_t tis_state_set(tis_ctx *c, int32_t isys, int32_t which, const double *buf
   std::vector<double> comp(h.d.nmp);
   for (int d = 0; d < 3; ++d) {
     for (int i = 0; i < h.d.nmp; ++i) comp[i] = buf[3 * i + d];
-    HIPCHK(c, hipMemcpyAsync(base + (size_t)d * c->ntot + h.d.off, comp.data(), comp.size() * sizeof(double),
+    HIPCHK(c, hipMemcpyAsync(base + (size_t)d * c->P.ntot + h.d.off, comp.data(), comp.size() * sizeof(double),
                              hipMemcpyHostToDevice, c->stream));
-    c->h2d_bytes += (int64_t)(comp.size() * sizeof(double));
+    c->h2d_state += (int64_t)(comp.size() * sizeof(double));
     HIPCHK(c, hipStreamSynchronize(c->stream));
   }
   if (which == 0) { c->lists_valid = false; c->lv_init = false; }
@@ -915,7 +1353,7 @@ int32_t tis_state_get(tis_ctx *c, int32_t isys, int32_t which, double *buf) {
   const double *base = which == 0 ? Xc(c) : (which == 1 ? c->P.v : (which == 2 ? c->P.a : c->P.f));
   std::vector<double> comp(h.d.nmp);
   for (int d = 0; d < 3; ++d) {
-    HIPCHK(c, hipMemcpyAsync(comp.data(), base + (size_t)d * c->ntot + h.d.off, comp.size() * sizeof(double),
+    HIPCHK(c, hipMemcpyAsync(comp.data(), base + (size_t)d * c->P.ntot + h.d.off, comp.size() * sizeof(double),
                              hipMemcpyDeviceToHost, c->stream));
     c->d2h_bytes += (int64_t)(comp.size() * sizeof(double));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -938,7 +1376,7 @@ static int rebuild_lists(tis_ctx *c, int32_t isys) {
     // grow the per-bead capacity and rebuild
     int newcap = (int)(ovf + ovf / 2 + 16);
     uint32_t *nb = nullptr;
-    int rc = dalloc(c, &nb, (size_t)newcap * c->ntot);
+    int rc = dalloc(c, &nb, (size_t)newcap * c->P.ntot);
     if (rc) return rc;
     c->P.nb = nb;  // the old buffer stays in allocs until the batch is freed
     c->P.nb_cap = newcap;
@@ -973,7 +1411,7 @@ int32_t tis_pairs_get(tis_ctx *c, int32_t isys, int32_t which, int32_t *pairs, d
     std::vector<uint32_t> nb((size_t)c->P.nb_cap * d.nmp);
     HIPCHK(c, hipMemcpy(cnt.data(), c->P.nb_cnt + d.off, d.nmp * sizeof(int32_t), hipMemcpyDeviceToHost));
     for (int m = 0; m < c->P.nb_cap; ++m)
-      HIPCHK(c, hipMemcpy(nb.data() + (size_t)m * d.nmp, c->P.nb + (size_t)m * c->ntot + d.off,
+      HIPCHK(c, hipMemcpy(nb.data() + (size_t)m * d.nmp, c->P.nb + (size_t)m * c->P.ntot + d.off,
                           d.nmp * sizeof(uint32_t), hipMemcpyDeviceToHost));
     for (int i = 0; i < d.nmp; ++i)
       for (int m = 0; m < std::min(cnt[i], c->P.nb_cap); ++m) {
@@ -1105,13 +1543,23 @@ int32_t tis_energy_sumup(tis_ctx *c, int32_t isys, double *energy_out) {
   const int ek0 = isys < 0 ? 0 : c->hs[isys].esk_begin, ekn = isys < 0 ? c->n_ewsk : c->hs[isys].esk_count;
   if (ekn > 0)
     hipLaunchKernelGGL(k_ewald_sk, dim3(ekn), dim3(kTile), 0, c->stream, c->P, PXc(c), c->d_ewsk + ek0, 1);
-  const int nsel = isys < 0 ? c->nsys : 1;
+  // one block per live system (in live-list order); the output is by
+  // system index, zeros for retired systems
+  const int nsel = isys < 0 ? c->nlive : 1;
   hipLaunchKernelGGL(k_energy, dim3(nsel), dim3(256), 0, c->stream, c->P, Xc(c), PXc(c),
-                     c->d_sysidx + (isys < 0 ? 0 : isys), c->d_energy, 0);
+                     c->d_sysidx + (isys < 0 ? 0 : c->hs[isys].live_pos), c->d_energy, 0);
   HIPCHK(c, hipGetLastError());
-  HIPCHK(c, hipMemcpyAsync(energy_out, c->d_energy, 25 * sizeof(double) * (size_t)nsel, hipMemcpyDeviceToHost,
-                           c->stream));
+  std::vector<double> e(25 * (size_t)nsel);
+  HIPCHK(c, hipMemcpyAsync(e.data(), c->d_energy, e.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (isys >= 0) {
+    std::memcpy(energy_out, e.data(), 25 * sizeof(double));
+  } else {
+    for (int is = 0; is < c->nsys; ++is) {
+      const HostSys &h = c->hs[is];
+      for (int k = 0; k < 25; ++k) energy_out[25 * (size_t)is + k] = h.live ? e[25 * (size_t)h.live_pos + k] : 0.0;
+    }
+  }
   return TIS_OK;
 }
 
@@ -1137,7 +1585,7 @@ int32_t tis_energy_unit(tis_ctx *c, int32_t isys, int32_t style, double *energy_
     c->eunit_cap = nn * 25;
   }
   hipLaunchKernelGGL(k_energy, dim3(1, (unsigned)nn), dim3(256), 0, c->stream, c->P, Xc(c), PXc(c),
-                     c->d_sysidx + isys, c->d_eunit, 1);
+                     c->d_sysidx + c->hs[isys].live_pos, c->d_eunit, 1);
   HIPCHK(c, hipGetLastError());
   std::vector<double> raw(nn * 25);
   HIPCHK(c, hipMemcpyAsync(raw.data(), c->d_eunit, raw.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1187,6 +1635,7 @@ int32_t tis_system_origin(tis_ctx *c, int32_t isys, uint64_t step_origin, int32_
   h.d.init_accel = init_accel ? 1 : 0;
   // kernels (and captured graphs) read DSys from device memory
   HIPCHK(c, hipMemcpyAsync(c->d_sys + isys, &h.d, sizeof(DSys), hipMemcpyHostToDevice, c->stream));
+  c->h2d_bytes += (int64_t)sizeof(DSys);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return TIS_OK;
 }
@@ -1218,7 +1667,8 @@ int32_t tis_label_apply(tis_ctx *c, int32_t n, const int32_t *isys, const int32_
   int rc = check_sys(c, -1);
   if (rc) return rc;
   for (int k = 0; k < n; ++k) {
-    if (isys[k] < 0 || isys[k] >= c->nsys) return fail(c, TIS_ERR_ARG, "system %d out of range", isys[k]);
+    if (isys[k] < 0 || isys[k] >= c->nsys || !c->hs[isys[k]].live)
+      return fail(c, TIS_ERR_ARG, "system %d out of range or retired", isys[k]);
     if (label[k] < 0 || label[k] >= (int)c->labels.size() || !c->labels[label[k]].set)
       return fail(c, TIS_ERR_ARG, "label %d not defined", label[k]);
     const tis_ctx::Label &L = c->labels[label[k]];
@@ -1276,10 +1726,12 @@ int32_t tis_label_apply(tis_ctx *c, int32_t n, const int32_t *isys, const int32_
   return TIS_OK;
 }
 
-int32_t tis_transfer_bytes(tis_ctx *c, int64_t *h2d, int64_t *d2h) {
-  if (!c) return TIS_ERR_ARG;
-  if (h2d) *h2d = c->h2d_bytes;
-  if (d2h) *d2h = c->d2h_bytes;
+int32_t tis_transfer_bytes(tis_ctx *c, int64_t *bytes) {
+  if (!c || !bytes) return TIS_ERR_ARG;
+  bytes[0] = c->h2d_bytes;
+  bytes[1] = c->h2d_state;
+  bytes[2] = c->d2h_bytes;
+  bytes[3] = c->d2d_state;
   return TIS_OK;
 }
 
@@ -1302,14 +1754,14 @@ int32_t tis_langevin_init(tis_ctx *c, double tstep, int32_t rebuild_every, uint6
   for (int k = 0; k < 2; ++k)
     if (c->graph[k]) { (void)hipGraphExecDestroy(c->graph[k]); c->graph[k] = nullptr; }
   // Langevin constants of every system's (friction, mass) classes
-  std::vector<LangClass> lc;
+  std::vector<LangClass> lc(c->stage.lcls);
   for (int is = 0; is < c->nsys; ++is) {
     const HostSys &h = c->hs[is];
-    for (int k = 0; k < h.d.nlc; ++k) {
-      lc.push_back(LangClass{{0.0, 0.0, 0.0, 0.0}, 1.0 / h.lc_mass[k], h.lc_fric[k], h.lc_mass[k], 0.0});
-      langevin_consts(tstep, h.d.tempk, h.lc_fric[k], h.lc_mass[k], lc.back().r);
-    }
+    if (!h.live) continue;
+    for (int k = 0; k < h.d.nlc; ++k)
+      langevin_consts(tstep, h.d.tempk, h.lc_fric[k], h.lc_mass[k], lc[h.d.lc0 + k].r);
   }
+  c->h2d_bytes += (int64_t)(lc.size() * sizeof(LangClass));
   if (!lc.empty())
     HIPCHK(c, hipMemcpyAsync((void *)c->P.lcls, lc.data(), lc.size() * sizeof(LangClass), hipMemcpyHostToDevice,
                              c->stream));
@@ -1318,8 +1770,8 @@ int32_t tis_langevin_init(tis_ctx *c, double tstep, int32_t rebuild_every, uint6
   rc = rebuild_lists(c, -1);
   if (rc) return rc;
   // displacements since this rebuild start at zero, no rebuild pending
-  HIPCHK(c, hipMemsetAsync(c->P.dxyz, 0, 3 * sizeof(double) * (size_t)std::max(c->ntot, 1), c->stream));
-  HIPCHK(c, hipMemsetAsync(c->P.sys_flag, 0, (size_t)std::max(c->nsys, 1), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->P.dxyz, 0, 3 * sizeof(double) * (size_t)std::max(c->P.ntot, 1), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->P.sys_flag, 0, (size_t)c->caps.nsys, c->stream));
   HIPCHK(c, hipMemsetAsync(c->P.ttop, 0, 2 * sizeof(double) * (size_t)std::max(c->nbtile, 1), c->stream));
   Range r = range_of(c, -1);
   const int cu = c->cur;
@@ -1432,11 +1884,16 @@ static int run_steps(tis_ctx *c, int64_t nsteps, int32_t mode, int64_t *nlaunch)
   } else {
     // one system per launch: every system's lists, then every system's
     // force and integration launches (the last one completes the step)
+    int last = -1;
+    for (int is = 0; is < c->nsys; ++is)
+      if (c->hs[is].live) last = is;
     for (; done < nsteps; ++done) {
-      for (int is = 0; is < c->nsys; ++is) eager_lists(c, is);
-      for (int is = 0; is < c->nsys; ++is) eager_forces(c, is, is == c->nsys - 1);
+      for (int is = 0; is < c->nsys; ++is)
+        if (c->hs[is].live) eager_lists(c, is);
+      for (int is = 0; is < c->nsys; ++is)
+        if (c->hs[is].live) eager_forces(c, is, is == last);
       hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, c->stream, c->d_stepbase, (uint64_t)1, c->d_counters);
-      c->cur ^= 1; ++c->t_since_init; *nlaunch += 2 * c->nsys;
+      c->cur ^= 1; ++c->t_since_init; *nlaunch += 2 * c->nlive;
     }
   }
   HIPCHK(c, hipGetLastError());
@@ -1450,9 +1907,9 @@ static int grow_nb(tis_ctx *c, unsigned long long need) {
   const int newcap = (int)(need + need / 2 + 16);
   if (newcap <= c->P.nb_cap) return TIS_OK;
   uint32_t *nb = nullptr;
-  int rc = dalloc(c, &nb, (size_t)newcap * c->ntot);
+  int rc = dalloc(c, &nb, (size_t)newcap * c->P.ntot);
   if (rc) return rc;
-  HIPCHK(c, hipMemcpyAsync(nb, c->P.nb, sizeof(uint32_t) * (size_t)c->P.nb_cap * c->ntot, hipMemcpyDeviceToDevice,
+  HIPCHK(c, hipMemcpyAsync(nb, c->P.nb, sizeof(uint32_t) * (size_t)c->P.nb_cap * c->P.ntot, hipMemcpyDeviceToDevice,
                            c->stream));
   c->P.nb = nb;  // the old buffer stays in allocs until the batch is freed
   c->P.nb_cap = newcap;
@@ -1493,17 +1950,17 @@ int32_t tis_list_capacity(tis_ctx *c, int32_t cap, int32_t *cap_out) {
   if (cap > 0 && cap != c->P.nb_cap) {
     int need = 0;
     if (c->lists_valid) {
-      std::vector<int32_t> cnt((size_t)c->ntot);
+      std::vector<int32_t> cnt((size_t)c->P.ntot);
       HIPCHK(c, hipMemcpy(cnt.data(), c->P.nb_cnt, sizeof(int32_t) * cnt.size(), hipMemcpyDeviceToHost));
       for (int32_t v : cnt) need = std::max(need, v);
     }
     if (cap < need)
       return fail(c, TIS_ERR_ARG, "capacity %d below the %d entries of the current lists", cap, need);
     uint32_t *nb = nullptr;
-    rc = dalloc(c, &nb, (size_t)cap * c->ntot);
+    rc = dalloc(c, &nb, (size_t)cap * c->P.ntot);
     if (rc) return rc;
     const int keep = std::min(cap, c->P.nb_cap);
-    HIPCHK(c, hipMemcpyAsync(nb, c->P.nb, sizeof(uint32_t) * (size_t)keep * c->ntot, hipMemcpyDeviceToDevice,
+    HIPCHK(c, hipMemcpyAsync(nb, c->P.nb, sizeof(uint32_t) * (size_t)keep * c->P.ntot, hipMemcpyDeviceToDevice,
                              c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->P.nb = nb;

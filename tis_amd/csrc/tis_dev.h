@@ -45,7 +45,7 @@ struct DSys {
   int32_t aoff, nba;         // angles
   int32_t sitoff, nsite;     // hbond sites
   int32_t hb_excess_possible; // some site has more candidates than its valence
-  int32_t sptab;             // offset of this system's species-pair table
+  int32_t sptab, nspecies;   // offset and order of this system's species-pair table
   int32_t bt0, nbt, qt0, nqt; // its bead tiles and charge tiles
   double rl2max;             // largest exv list radius^2 over its unit pairs
   int32_t uoff, nunit;       // unit-pair list radius table rl2[uoff + a*nunit + b]
@@ -115,8 +115,8 @@ struct DevPtrs {
   const double *mass;
   const int8_t *mptype, *iclass;
   const int32_t *unit, *qidx;   // qidx: charge index (global) or -1
-  const uint8_t *species;       // excluded-volume species of each bead
-  const SpPair *sptab; int32_t nspecies;
+  const uint8_t *species;       // excluded-volume species of each bead (within its system)
+  const SpPair *sptab;
   BBox *bbox, *qbbox;          // per bead tile / per charge tile
   BBox *sbox;                  // per 16-bead group of each bead tile [4 * nbtile]
   int *lcnt;                   // exv entry counts per (bead tile pair, lane) of the tile-pair build

@@ -1135,7 +1135,7 @@ __global__ __launch_bounds__(kTile * 2) void k_main(DevPtrs P, const double *X, 
     // the two waves take alternate groups of four neighbours
     const double *Y = s.periodic ? PX : X;
     const double xi0 = Y[g], xi1 = Y[n + g], xi2 = Y[2 * n + g];
-    const SpPair *tab = P.sptab + s.sptab + (int)P.species[g] * P.nspecies;
+    const SpPair *tab = P.sptab + s.sptab + (int)P.species[g] * s.nspecies;
     const int cnt = min(P.nb_cnt[g], P.nb_cap);
     unsigned nclamp = 0;
     for (int m0 = 4 * w; m0 < cnt; m0 += 8) exv_group(P, s, Y, g, m0, cnt, xi0, xi1, xi2, tab, F, nclamp);

@@ -55,7 +55,10 @@ def load_library():
     L.tis_set_system_T.argtypes = [vp, C.c_int32, C.POINTER(CSystem)]
     L.tis_list_capacity.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32)]
     L.tis_term_status.argtypes = [vp, C.c_int32, _i32p, _i32p]
-    L.tis_transfer_bytes.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.tis_transfer_bytes.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.tis_system_append.argtypes = [vp, C.c_int32, C.POINTER(CSystem), C.POINTER(C.c_uint32), _i32p]
+    L.tis_system_retire.argtypes = [vp, C.c_int32, _i32p]
+    L.tis_arena_stats.argtypes = [vp, C.POINTER(C.c_int64)]
     L.tis_state_set.argtypes = [vp, C.c_int32, C.c_int32, _f64p]
     L.tis_state_get.argtypes = [vp, C.c_int32, C.c_int32, _f64p]
     L.tis_neighbor.argtypes = [vp, C.c_int32]
@@ -75,7 +78,8 @@ def load_library():
     L.tis_last_timing.argtypes = [vp, _f64p, _f64p, _f64p, C.POINTER(C.c_int64)]
     L.tis_last_force_ms.argtypes = [vp, _f64p]
     for fn in ("tis_ctx_create", "tis_ctx_destroy", "tis_system_upload", "tis_system_validate", "tis_set_system_T",
-               "tis_list_capacity", "tis_term_status", "tis_transfer_bytes",
+               "tis_list_capacity", "tis_term_status", "tis_transfer_bytes", "tis_system_append",
+               "tis_system_retire", "tis_arena_stats",
                "tis_state_set",
                "tis_state_get", "tis_neighbor", "tis_pairs_get", "tis_force_sumup", "tis_force_get",
                "tis_energy_sumup", "tis_energy_unit", "tis_langevin_init", "tis_langevin_steps", "tis_status_counters",
@@ -121,6 +125,31 @@ class Engine:
             self._keys = (C.c_uint32 * len(keys))(*[int(v) for v in keys])
             k = self._keys
         self._chk(self.L.tis_system_upload(self.ctx, len(self.systems), self._cs, k, seed), "upload")
+        self.live = set(range(len(self.systems)))
+
+    def append(self, systems: list[System], keys=None) -> list[int]:
+        """Add systems to the batch (tis_system_append): only their own
+        topology is uploaded; returns their indices."""
+        n = len(systems)
+        cs = (CSystem * n)(*[s.cstruct() for s in systems])
+        k = None if keys is None else (C.c_uint32 * n)(*[int(v) for v in keys])
+        out = (C.c_int32 * n)()
+        self._chk(self.L.tis_system_append(self.ctx, n, cs, k, out), "tis_system_append")
+        self.systems.extend(systems)
+        self.live.update(int(v) for v in out)
+        return [int(v) for v in out]
+
+    def retire(self, isys):
+        """Drop systems from every launch (tis_system_retire)."""
+        a = np.ascontiguousarray(isys, dtype=np.int32)
+        self._chk(self.L.tis_system_retire(self.ctx, len(a), a.ctypes.data_as(_i32p)), "tis_system_retire")
+        self.live.difference_update(int(v) for v in a)
+
+    def arena_stats(self) -> dict:
+        a = (C.c_int64 * 6)()
+        self._chk(self.L.tis_arena_stats(self.ctx, a), "tis_arena_stats")
+        return dict(zip(("live_systems", "live_beads", "stored_beads", "bead_capacity", "layouts", "indices"),
+                        list(a)))
 
     def _chk(self, rc, what):
         if rc != 0:
@@ -187,8 +216,9 @@ class Engine:
         return f
 
     def force_all(self):
+        """Forces of every live system (None for retired ones)."""
         self._chk(self.L.tis_force_sumup(self.ctx, -1, None), "tis_force_sumup")
-        return [self.get_state(i, 3) for i in range(len(self.systems))]
+        return [self.get_state(i, 3) if i in self.live else None for i in range(len(self.systems))]
 
     def energy_sumup(self, isys):
         e = np.zeros(E_MAX)
@@ -252,11 +282,13 @@ class Engine:
         self._chk(self.L.tis_status_counters(self.ctx, a), "tis_status_counters")
         return dict(zip(CNT_NAMES, list(a)[:len(CNT_NAMES)]))
 
-    def transfer_bytes(self):
-        """(host-to-device, device-to-host) payload bytes since creation."""
-        a, b = C.c_int64(), C.c_int64()
-        self._chk(self.L.tis_transfer_bytes(self.ctx, C.byref(a), C.byref(b)), "tis_transfer_bytes")
-        return a.value, b.value
+    def transfer_bytes(self) -> dict:
+        """Payload bytes since creation (tis_transfer_bytes): topology and
+        tables uploaded, state uploaded, state downloaded, state carried on
+        the device by a new layout."""
+        a = (C.c_int64 * 4)()
+        self._chk(self.L.tis_transfer_bytes(self.ctx, a), "tis_transfer_bytes")
+        return dict(zip(("h2d_topology", "h2d_state", "d2h_state", "d2d_state"), list(a)))
 
     def term_status(self, isys):
         """(hb_status[nhb], st_status[nst]) as bool arrays (tis_term_status)."""

@@ -64,6 +64,7 @@ module tis_hip
 
   public :: tis_ctx_create, tis_ctx_destroy, tis_last_error, tis_system_upload, tis_system_validate, &
        tis_set_system_T, tis_list_capacity, tis_term_status, tis_transfer_bytes, &
+       tis_system_append, tis_system_retire, tis_arena_stats, &
        tis_state_set, tis_state_get, tis_neighbor, tis_pairs_get, tis_force_sumup, tis_force_get, &
        tis_energy_sumup, tis_energy_unit, tis_langevin_init, tis_langevin_steps, tis_system_origin, tis_device_bytes, &
        tis_status_counters, tis_synchronize, tis_last_timing, tis_last_force_ms, tis_label_define, &
@@ -101,10 +102,27 @@ module tis_hip
        character(kind=c_char), intent(out) :: msg(*)
        integer(c_int64_t), value :: msg_cap
      end function
-     integer(c_int32_t) function tis_transfer_bytes(ctx, h2d, d2h) bind(C, name="tis_transfer_bytes")
+     integer(c_int32_t) function tis_transfer_bytes(ctx, bytes) bind(C, name="tis_transfer_bytes")
        import :: c_int32_t, c_int64_t, c_ptr
        type(c_ptr), value :: ctx
-       integer(c_int64_t), intent(out) :: h2d, d2h
+       integer(c_int64_t), intent(out) :: bytes(4)
+     end function
+     integer(c_int32_t) function tis_system_append(ctx, nadd, systems, keys, isys_out) &
+          bind(C, name="tis_system_append")
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: ctx, systems, keys, isys_out
+       integer(c_int32_t), value :: nadd
+     end function
+     integer(c_int32_t) function tis_system_retire(ctx, n, isys) bind(C, name="tis_system_retire")
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: n
+       integer(c_int32_t), intent(in) :: isys(*)
+     end function
+     integer(c_int32_t) function tis_arena_stats(ctx, stats) bind(C, name="tis_arena_stats")
+       import :: c_int32_t, c_int64_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int64_t), intent(out) :: stats(6)
      end function
      integer(c_int32_t) function tis_term_status(ctx, isys, hb_status, st_status) bind(C, name="tis_term_status")
        import :: c_int32_t, c_ptr

@@ -4,11 +4,14 @@ A job is one system (a chain, or any molecule set) with starting coordinates
 and a number of Langevin steps.  Jobs arrive over time; the server runs
 rounds of at most ``round_steps`` steps over every admitted job in one batch
 (one hipGraph-captured launch sequence per step), and between rounds admits
-queued jobs as HBM budget frees up and retires finished ones.  A job keeps
-its state (coordinates, velocities, accelerations) and its own step count
-across the re-batching: its random streams are keyed by (seed, job key,
-job-local step, bead), so its trajectory is bitwise the one it would follow
-alone on the GPU.
+queued jobs as HBM budget frees up and retires finished ones.
+
+The batch lives in the library's arena (tis_system_append /
+tis_system_retire): an admission uploads only the new jobs' topology and
+starting state, a completion downloads only the finished jobs' state, and
+the jobs that continue never leave the device.  A job keeps its own step
+count: its random streams are keyed by (seed, job key, job-local step,
+bead), so its trajectory is bitwise the one it would follow alone.
 
 The reference runs one job per process (mloop_simulator.F90 over its
 replicas); this loop is what replaces many such processes on one device.
@@ -72,6 +75,8 @@ class Job:
     v: np.ndarray | None = None
     a: np.ndarray | None = None
     bytes: int = 0
+    slot: int = -1                 # system index in the device batch
+    fresh: bool = True             # its accelerations are still to be initialised
 
     @property
     def latency(self) -> float:
@@ -89,13 +94,24 @@ class ServeStats:
     max_inflight: int = 0
     max_bytes: int = 0
     latencies: list = field(default_factory=list)
+    admitted: int = 0
+    # payload bytes (tis_transfer_bytes) of admissions and completions
+    h2d_topology: int = 0
+    h2d_state: int = 0
+    d2h_state: int = 0
+    d2d_state: int = 0
+    layouts: int = 0
 
     def summary(self) -> dict:
         lat = np.asarray(self.latencies, dtype=float)
         out = {"jobs": self.jobs, "rounds": self.rounds, "rebatches": self.rebatches,
                "bead_steps": self.bead_steps, "busy_s": self.busy_s, "span": self.span_s,
                "max_inflight": self.max_inflight, "max_device_bytes": self.max_bytes,
-               "bead_steps_per_s_busy": self.bead_steps / self.busy_s if self.busy_s > 0 else None}
+               "bead_steps_per_s_busy": self.bead_steps / self.busy_s if self.busy_s > 0 else None,
+               "h2d_bytes_per_admission": (self.h2d_topology + self.h2d_state) / max(self.admitted, 1),
+               "h2d_state_bytes": self.h2d_state, "d2h_state_bytes": self.d2h_state,
+               "d2h_bytes_per_completion": self.d2h_state / max(self.jobs, 1),
+               "d2d_state_bytes": self.d2d_state, "layouts": self.layouts}
         if lat.size:
             out["p50_latency"] = float(np.percentile(lat, 50))
             out["p99_latency"] = float(np.percentile(lat, 99))
@@ -134,8 +150,6 @@ class Server:
         self._eng = engine
         self.queue: list[Job] = []
         self.active: list[Job] = []
-        self.batch: list[Job] = []     # jobs in the device batch, in slot order
-        self._pulled = False           # batch state copied to the jobs since the last round
         self.done: list[Job] = []
         self.gstep = 0
         self.held_bytes = 0
@@ -160,13 +174,6 @@ class Server:
             raise ValueError(f"job {job.key} needs {job.bytes} B, over the HBM budget {self.budget} B")
         self.queue.append(job)
 
-    def _engine(self, systems, keys):
-        if self._eng is None:
-            self._eng = Engine(systems, keys=keys, seed=self.seed, device=self.device)
-        else:
-            self._eng.upload(systems, keys=keys, seed=self.seed)
-        return self._eng
-
     def _admit(self, now) -> list[Job]:
         new = []
         held = sum(j.bytes for j in self.active)
@@ -180,41 +187,60 @@ class Server:
             self.queue.remove(job)
         return new
 
-    def _rebatch(self, new: list[Job]):
-        """Upload the active set; continuing jobs carry their state over."""
+    def _append(self, new: list[Job]):
+        """Append the new jobs to the device batch: their topology and
+        starting state are the only data that cross to the device."""
         for j in new:
             j.origin = self.gstep
             j.admitted = self.now()
+        systems, keys = [j.system for j in new], [j.key for j in new]
+        if self._eng is None:
+            self._eng = Engine(systems, keys=keys, seed=self.seed, device=self.device)
+            slots = list(range(len(new)))
+        else:
+            slots = self._eng.append(systems, keys=keys)
+        eng = self._eng
+        for j, h in zip(new, slots):
             n = j.system.nmp
-            j.x = np.array(j.x0, dtype=np.float64)
-            j.v = np.zeros((n, 3)) if j.v0 is None else np.array(j.v0, dtype=np.float64)
-            j.a = np.zeros((n, 3))
+            j.slot = h
+            j.fresh = True
+            eng.set_state(h, 0, j.x0)
+            eng.set_state(h, 1, np.zeros((n, 3)) if j.v0 is None else j.v0)
+            eng.set_state(h, 2, np.zeros((n, 3)))
+            eng.system_origin(h, j.origin, True)
         self.active.extend(new)
-        self.batch = list(self.active)
-        self._pulled = False
-        if not self.batch:
-            return
-        eng = self._engine([j.system for j in self.batch], [j.key for j in self.batch])
-        for k, j in enumerate(self.batch):
-            eng.set_state(k, 0, j.x)
-            eng.set_state(k, 1, j.v)
-            eng.set_state(k, 2, j.a)
-            eng.system_origin(k, j.origin, j.steps_done == 0)
+        self.stats.admitted += len(new)
+
+    def _retire(self, finished: list[Job]):
+        """Download the finished jobs' state and drop them from the batch."""
+        eng = self._eng
+        for j in finished:
+            j.x = eng.get_state(j.slot, 0)
+            j.v = eng.get_state(j.slot, 1)
+            j.a = eng.get_state(j.slot, 2)
+        eng.retire([j.slot for j in finished])
+
+    def _restart(self):
+        """Langevin set-up after a change of the batch: new jobs get new
+        accelerations, continuing ones keep theirs (mode 3)."""
+        eng = self._eng
+        for j in self.active:
+            if j.fresh and j.steps_done > 0:
+                eng.system_origin(j.slot, j.origin, False)
+                j.fresh = False
         eng.langevin_init(tstep=self.tstep, rebuild_every=self.rebuild_every, step0=self.gstep, init_accel=None)
         self.held_bytes = eng.device_bytes()
         self.stats.rebatches += 1
         self.stats.max_bytes = max(self.stats.max_bytes, self.held_bytes)
         self.stats.max_inflight = max(self.stats.max_inflight, len(self.active))
 
-    def _pull_state(self):
-        if self._pulled:
-            return
-        eng = self._eng
-        for k, j in enumerate(self.batch):
-            j.x = eng.get_state(k, 0)
-            j.v = eng.get_state(k, 1)
-            j.a = eng.get_state(k, 2)
-        self._pulled = True
+    def _count_bytes(self, before: dict):
+        after = self._eng.transfer_bytes()
+        self.stats.h2d_topology += after["h2d_topology"] - before.get("h2d_topology", 0)
+        self.stats.h2d_state += after["h2d_state"] - before.get("h2d_state", 0)
+        self.stats.d2h_state += after["d2h_state"] - before.get("d2h_state", 0)
+        self.stats.d2d_state += after["d2d_state"] - before.get("d2d_state", 0)
+        self.stats.layouts = self._eng.arena_stats()["layouts"]
 
     def run(self, jobs: list[Job] | None = None) -> ServeStats:
         """Serve until every submitted job has finished."""
@@ -223,6 +249,7 @@ class Server:
         self.queue.sort(key=lambda j: j.arrival)
         self._t0 = time.perf_counter()
         first = self.queue[0].arrival if self.queue else 0.0
+        dirty = False
         while self.queue or self.active:
             now = self.now()
             if not self.active and self.queue and self.queue[0].arrival > now:
@@ -235,16 +262,19 @@ class Server:
                 continue
             tb = time.perf_counter()
             new = self._admit(self.now())
-            if new or [id(j) for j in self.batch] != [id(j) for j in self.active]:
-                if self.batch:
-                    self._pull_state()
-                self._rebatch(new)
+            if new:
+                before = self._eng.transfer_bytes() if self._eng is not None else {}
+                self._append(new)
+                self._count_bytes(before)
+                dirty = True
             if not self.active:
                 self.stats.busy_s += time.perf_counter() - tb
                 continue
+            if dirty:
+                self._restart()
+                dirty = False
             nrun = min(self.round_steps, min(j.nsteps - j.steps_done for j in self.active))
             self._eng.langevin_steps(nrun, MODE_GRAPH)
-            self._pulled = False
             self.gstep += nrun
             self.stats.rounds += 1
             self.stats.bead_steps += nrun * sum(j.system.nmp for j in self.active)
@@ -252,7 +282,10 @@ class Server:
                 j.steps_done += nrun
             finished = [j for j in self.active if j.steps_done >= j.nsteps]
             if finished:
-                self._pull_state()
+                before = self._eng.transfer_bytes()
+                self._retire(finished)
+                self._count_bytes(before)
+                dirty = True
                 t = self.now()
                 for j in finished:
                     j.finished = t
