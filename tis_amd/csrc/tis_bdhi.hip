@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -338,6 +339,22 @@ __global__ __launch_bounds__(256) void k_chol_update(BdDev B, const float *P, in
   }
 }
 
+// L = the lower triangle of D (diagonal included), zero above: the input of
+// the in-place factorisation.  Block (x, y): rows 256 x .. of column y.
+__global__ __launch_bounds__(256) void k_copy_lower(const double *D, double *L, int n3) {
+  const int r = blockIdx.x * 256 + threadIdx.x, c = blockIdx.y;
+  if (r < n3) L[(size_t)r + (size_t)c * n3] = r >= c ? D[(size_t)r + (size_t)c * n3] : 0.0;
+}
+
+// ---------------------------------------------------------------- no-alloc scratch
+// Scratch of tis_bdhi_factor_noalloc: the FP32 copy P of an outer block's
+// panels (256 columns of at most kNoAllocMaxN3 rows) followed by the pivot
+// flag.  Each device holds its own copy until the process ends; every call
+// on a device shares it.
+constexpr int kNoAllocMaxN3 = 9216;
+constexpr size_t kScratchP = (size_t)kNb * kNoAllocMaxN3;
+static __device__ __align__(256) float scratch[kScratchP + 64];
+
 // ---------------------------------------------------------------- step
 // Partial products over a column chunk: yD = D[:, chunk] F[chunk],
 // yL = L[:, chunk] xi[chunk] (lower part only); one thread per row.
@@ -559,12 +576,56 @@ int check_info(tis_ctx *c, BdState *b) {
   return 0;
 }
 
+// device address of `scratch` per device, taken once per device with that
+// device current (the array's name is a host-side stand-in)
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_ptr(int device, void **out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return TIS_ERR_HIP;
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return TIS_OK;
+}
+
 }  // namespace
 }  // namespace tis
 
 using namespace tis;
 
 extern "C" {
+
+int32_t tis_bdhi_factor_noalloc(int32_t device, const double *D, double *L, int32_t n3, void *stream,
+                                int32_t *info) {
+  if (!D || !L || n3 <= 0 || n3 > kNoAllocMaxN3 || device < 0 || device >= kMaxDevices) return TIS_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return TIS_ERR_HIP;
+  void *ptr = nullptr;
+  if (int rc = scratch_ptr(device, &ptr)) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  BdDev d{};
+  d.n3 = n3;
+  d.n = n3 / 3;
+  d.L = L;
+  d.D = const_cast<double *>(D);
+  d.info = (int *)((float *)ptr + kScratchP);
+  hipLaunchKernelGGL(k_bd_reset, dim3(1), dim3(64), 0, st, d);
+  hipLaunchKernelGGL(k_copy_lower, dim3((n3 + 255) / 256, n3), dim3(256), 0, st, D, L, n3);
+  factor_launches(d, (float *)ptr, st);
+  if (hipGetLastError() != hipSuccess) return TIS_ERR_HIP;
+  if (info) {
+    int v = 0;
+    if (hipMemcpyAsync(&v, d.info, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess) return TIS_ERR_HIP;
+    if (hipStreamSynchronize(st) != hipSuccess) return TIS_ERR_HIP;
+    *info = v;
+    if (v) return TIS_ERR_STATE;
+  }
+  return TIS_OK;
+}
 
 int32_t tis_bdhi_setup(tis_ctx *c, const tis_bd_system *s, uint32_t key, uint64_t seed) {
   if (!c || !s || s->nmp <= 1 || !s->radius) return TIS_ERR_ARG;

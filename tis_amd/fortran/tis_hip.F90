@@ -69,7 +69,8 @@ module tis_hip
        tis_energy_sumup, tis_energy_unit, tis_langevin_init, tis_langevin_steps, tis_system_origin, tis_device_bytes, &
        tis_status_counters, tis_synchronize, tis_last_timing, tis_last_force_ms, tis_label_define, &
        tis_label_apply, tis_bdhi_setup, tis_bdhi_state_set, tis_bdhi_state_get, tis_bdhi_forces, &
-       tis_bdhi_tensors, tis_bdhi_tensor_get, tis_bdhi_factor_error, tis_bdhi_steps, tis_bdhi_timing
+       tis_bdhi_tensors, tis_bdhi_tensor_get, tis_bdhi_factor_error, tis_bdhi_steps, tis_bdhi_timing, &
+       tis_bdhi_factor_noalloc
   public :: hip_check, hip_neighbor, hip_force_sumup, hip_energy_sumup
 
   interface
@@ -297,6 +298,12 @@ module tis_hip
        import :: c_int32_t, c_ptr, c_double
        type(c_ptr), value :: ctx
        real(c_double), intent(out) :: total_ms, fill_ms, factor_ms
+     end function
+     integer(c_int32_t) function tis_bdhi_factor_noalloc(device, D, L, n3, stream, info) &
+          bind(C, name="tis_bdhi_factor_noalloc")
+       import :: c_int32_t, c_ptr
+       integer(c_int32_t), value :: device, n3
+       type(c_ptr), value :: D, L, stream, info
      end function
      integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
        import :: c_size_t, c_ptr
