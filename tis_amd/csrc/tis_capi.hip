@@ -62,7 +62,6 @@ struct Stage {
   std::vector<int32_t> bd_mp, ba_mp, st_mp, tst_mp, tst2st, tst_excl, tst_sys, hb_mp, hb_site, hb_tert, hb_sys,
       site_val;
   std::vector<double> bd_nat, bd_coef, ba_nat, ba_coef, st_nat, st_coef, tst_nat, tst_coef, hb_nat, hb_coef;
-  std::vector<int32_t> bd_pos, ba_pos, st_pos, tst_pos, hb_pos;
   std::vector<int32_t> st_exoff{0}, st_ex;
   std::vector<double> ewhx, ewhy, ewhz, ewc;
 };
@@ -88,10 +87,9 @@ struct ArenaBufs {
   double *mass = nullptr, *exv_rad = nullptr, *charge = nullptr, *rl2 = nullptr;
   int8_t *mptype = nullptr, *iclass = nullptr;
   int32_t *unit = nullptr, *qidx = nullptr, *qbead = nullptr, *inc_off = nullptr;
-  uint32_t *inc = nullptr;
+  int32_t *inc_slot = nullptr;
   float *exv_eps = nullptr;
   SpPair *sptab = nullptr;
-  int32_t *bd_pos = nullptr, *ba_pos = nullptr, *st_pos = nullptr, *tst_pos = nullptr, *hb_pos = nullptr;
   int32_t *bd_mp = nullptr, *ba_mp = nullptr, *st_mp = nullptr, *tst_mp = nullptr, *hb_mp = nullptr;
   double *bd_nat = nullptr, *bd_coef = nullptr, *ba_nat = nullptr, *ba_coef = nullptr, *st_nat = nullptr,
          *st_coef = nullptr, *tst_nat = nullptr, *tst_coef = nullptr, *hb_nat = nullptr, *hb_coef = nullptr;
@@ -762,24 +760,10 @@ int stage_system(tis_ctx *c, Stage &g, HostSys &h, const tis_system &s, int is, 
     for (int k = 0; k < 6; ++k) incb[s.tst_mp[6 * t + k]].push_back(inc_pack(INC_TSTACK, k, ntst + t));
   for (int t = 0; t < s.nhb; ++t)
     for (int k = 0; k < 6; ++k) incb[s.hb_mp[6 * t + k]].push_back(inc_pack(INC_HBOND, k, nhb + t));
-  g.bd_pos.resize(2 * (size_t)(nbd + s.nbd)); g.ba_pos.resize(3 * (size_t)(nba + s.nba));
-  g.st_pos.resize(7 * (size_t)(nst + s.nst)); g.tst_pos.resize(6 * (size_t)(ntst + s.ntst));
-  g.hb_pos.resize(6 * (size_t)(nhb + s.nhb));
   g.inc_off.pop_back();
   for (int i = 0; i < s.nmp; ++i) {
     g.inc_off.push_back((int32_t)g.inc.size());
-    for (uint32_t e : incb[i]) {
-      const int32_t k = (int32_t)g.inc.size();
-      const uint32_t type = e >> 29, slot = (e >> 26) & 7u, term = e & ((1u << 26) - 1u);
-      switch (type) {
-        case INC_BOND: g.bd_pos[2 * (size_t)term + slot] = k; break;
-        case INC_ANGLE: g.ba_pos[3 * (size_t)term + slot] = k; break;
-        case INC_STACK: g.st_pos[7 * (size_t)term + slot] = k; break;
-        case INC_TSTACK: g.tst_pos[6 * (size_t)term + slot] = k; break;
-        default: g.hb_pos[6 * (size_t)term + slot] = k; break;
-      }
-      g.inc.push_back(e);
-    }
+    g.inc.insert(g.inc.end(), incb[i].begin(), incb[i].end());
   }
   g.inc_off.push_back((int32_t)g.inc.size());
   g.nbd += s.nbd; g.nba += s.nba; g.nst += s.nst; g.ntst += s.ntst; g.nhb += s.nhb; g.nsite += s.nhbsite;
@@ -825,10 +809,14 @@ int alloc_arena(tis_ctx *c, const Caps &k, std::vector<void *> &list, DevPtrs &P
   AL(P.v, 3 * B); AL(P.a, 3 * B); AL(P.f, 3 * B);
   AL(A.lcl, B); AL(A.lcls, k.lc); AL(A.mass, B); AL(A.exv_rad, B); AL(A.charge, B); AL(A.rl2, k.rl2);
   AL(A.mptype, B); AL(A.iclass, B); AL(A.unit, B); AL(A.qidx, B); AL(A.qbead, k.q); AL(A.inc_off, B + 1);
-  AL(A.inc, k.inc); AL(A.exv_eps, B); AL(A.species, B); AL(A.sptab, k.spt);
-  AL(A.bd_pos, 2 * k.bd); AL(A.ba_pos, 3 * k.ba); AL(A.st_pos, 7 * k.st); AL(A.tst_pos, 6 * k.tst);
-  AL(A.hb_pos, 6 * k.hb);
-  AL(P.fslot, 3 * k.inc);
+  AL(A.inc_slot, k.inc); AL(A.exv_eps, B); AL(A.species, B); AL(A.sptab, k.spt);
+  // slot regions, term-major per type
+  P.sb_bd = 0;
+  P.sb_ba = P.sb_bd + 2 * k.bd;
+  P.sb_st = P.sb_ba + 3 * k.ba;
+  P.sb_tst = P.sb_st + 7 * k.st;
+  P.sb_hb = P.sb_tst + 6 * k.tst;
+  AL(P.fslot, 3 * (P.sb_hb + 6 * k.hb));
   AL(A.bd_mp, 2 * k.bd); AL(A.bd_nat, k.bd); AL(A.bd_coef, k.bd);
   AL(A.ba_mp, 3 * k.ba); AL(A.ba_nat, k.ba); AL(A.ba_coef, k.ba);
   AL(A.st_mp, 7 * k.st); AL(A.st_nat, 3 * k.st); AL(A.st_coef, 4 * k.st);
@@ -850,8 +838,7 @@ int alloc_arena(tis_ctx *c, const Caps &k, std::vector<void *> &list, DevPtrs &P
 #undef AL
   P.lcl = A.lcl; P.lcls = A.lcls; P.mass = A.mass; P.exv_rad = A.exv_rad; P.charge = A.charge; P.rl2 = A.rl2;
   P.mptype = A.mptype; P.iclass = A.iclass; P.unit = A.unit; P.qidx = A.qidx; P.qbead = A.qbead;
-  P.inc_off = A.inc_off; P.inc = A.inc; P.exv_eps = A.exv_eps; P.species = A.species; P.sptab = A.sptab;
-  P.bd_pos = A.bd_pos; P.ba_pos = A.ba_pos; P.st_pos = A.st_pos; P.tst_pos = A.tst_pos; P.hb_pos = A.hb_pos;
+  P.inc_off = A.inc_off; P.inc_slot = A.inc_slot; P.exv_eps = A.exv_eps; P.species = A.species; P.sptab = A.sptab;
   P.bd_mp = A.bd_mp; P.bd_nat = A.bd_nat; P.bd_coef = A.bd_coef;
   P.ba_mp = A.ba_mp; P.ba_nat = A.ba_nat; P.ba_coef = A.ba_coef;
   P.st_mp = A.st_mp; P.st_nat = A.st_nat; P.st_coef = A.st_coef;
@@ -877,21 +864,6 @@ int put_tail(tis_ctx *c, T *dev, const std::vector<T> &h, size_t from) {
   return 0;
 }
 
-// term-major staged slot positions [term][K] -> device rows [slot][stride]
-int put_pos_tail(tis_ctx *c, int32_t *dev, const std::vector<int32_t> &h, int K, size_t stride, size_t t0) {
-  const size_t nt = h.size() / K;
-  if (nt <= t0) return 0;
-  std::vector<int32_t> row(nt - t0);
-  for (int s = 0; s < K; ++s) {
-    for (size_t t = t0; t < nt; ++t) row[t - t0] = h[t * K + s];
-    HIPCHK(c, hipMemcpyAsync(dev + s * stride + t0, row.data(), row.size() * sizeof(int32_t), hipMemcpyHostToDevice,
-                             c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));   // row is reused
-    c->h2d_bytes += (int64_t)(row.size() * sizeof(int32_t));
-  }
-  return 0;
-}
-
 // Upload the staged arrays from the marks m on (m all zero: everything).
 int upload_stage(tis_ctx *c, const Stage &g, const StageMarks &m) {
   const ArenaBufs &A = c->ab;
@@ -902,7 +874,7 @@ int upload_stage(tis_ctx *c, const Stage &g, const StageMarks &m) {
   PT(A.exv_rad, exv_rad, m.bead); PT(A.exv_eps, exv_eps, m.bead); PT(A.species, species, m.bead);
   PT(A.charge, charge, m.bead); PT(A.mass, mass, m.bead); PT(A.lcl, lcl, m.bead);
   PT(A.inc_off, inc_off, m.bead);   // from the old end sentinel on
-  PT(A.qbead, qbead, m.q); PT(A.inc, inc, m.inc); PT(A.lcls, lcls, m.lc); PT(A.sptab, sptab, m.spt);
+  PT(A.qbead, qbead, m.q); PT(A.lcls, lcls, m.lc); PT(A.sptab, sptab, m.spt);
   PT(A.bd_mp, bd_mp, 2 * m.bd); PT(A.bd_nat, bd_nat, m.bd); PT(A.bd_coef, bd_coef, m.bd);
   PT(A.ba_mp, ba_mp, 3 * m.ba); PT(A.ba_nat, ba_nat, m.ba); PT(A.ba_coef, ba_coef, m.ba);
   PT(A.st_mp, st_mp, 7 * m.st); PT(A.st_nat, st_nat, 3 * m.st); PT(A.st_coef, st_coef, 4 * m.st);
@@ -915,12 +887,21 @@ int upload_stage(tis_ctx *c, const Stage &g, const StageMarks &m) {
   PT(A.ewc, ewc, m.k);
   PT(A.ewh, ewhx, m.k); PT(A.ewh + c->P.nktot, ewhy, m.k); PT(A.ewh + 2 * (size_t)c->P.nktot, ewhz, m.k);
 #undef PT
+  // incidence entries (type | slot | term) -> slot positions of the layout
   const DevPtrs &P = c->P;
-  if ((rc = put_pos_tail(c, A.bd_pos, g.bd_pos, 2, P.nbd, m.bd))) return rc;
-  if ((rc = put_pos_tail(c, A.ba_pos, g.ba_pos, 3, P.nba, m.ba))) return rc;
-  if ((rc = put_pos_tail(c, A.st_pos, g.st_pos, 7, P.nst, m.st))) return rc;
-  if ((rc = put_pos_tail(c, A.tst_pos, g.tst_pos, 6, P.ntst, m.tst))) return rc;
-  if ((rc = put_pos_tail(c, A.hb_pos, g.hb_pos, 6, P.nhb, m.hb))) return rc;
+  if (g.inc.size() > m.inc) {
+    std::vector<int32_t> pos(g.inc.size() - m.inc);
+    const int64_t base[5] = {P.sb_bd, P.sb_ba, P.sb_st, P.sb_tst, P.sb_hb};
+    const int K[5] = {2, 3, 7, 6, 6};
+    for (size_t k = m.inc; k < g.inc.size(); ++k) {
+      const uint32_t e = g.inc[k], type = e >> 29, slot = (e >> 26) & 7u, term = e & ((1u << 26) - 1u);
+      pos[k - m.inc] = (int32_t)(base[type] + (int64_t)K[type] * term + slot);
+    }
+    HIPCHK(c, hipMemcpyAsync(A.inc_slot + m.inc, pos.data(), pos.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));   // pos is local
+    c->h2d_bytes += (int64_t)(pos.size() * sizeof(int32_t));
+  }
   return 0;
 }
 

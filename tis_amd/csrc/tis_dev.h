@@ -92,9 +92,6 @@ struct Tile { int32_t sys, start; };
 // bonds [b0, b0+nb), angles [a0, a0+na), consecutive stacks [s0, s0+ns)
 struct BondedRange { int32_t b0, nb, a0, na, s0, ns; };
 
-// XCDs of an MI355X (each with its own L2); workgroups are dealt to them
-// round-robin by workgroup index
-constexpr int kXcds = 8;
 
 // excluded-volume parameters of a species pair: D_ij (FP64 and its square)
 // and eps_i*eps_j (force_exv_dt15.F90:60-80,92)
@@ -124,12 +121,15 @@ struct DevPtrs {
   const float *exv_eps;
   const double *charge;
   const double *rl2;
-  // incidence: per bead the slots of the bonded terms it takes part in
+  // incidence: per bead the slots of the bonded terms it takes part in,
+  // inc_slot[inc_off[i] .. inc_off[i+1]) in a fixed order.  Slots are stored
+  // term-major by type, slot k of term t of a type at sb_<type> + K t + k
+  // (K = 2, 3, 7, 6, 6 for bonds, angles, stacks, tertiary stacks, hbonds),
+  // so that a term's lane writes its slots to consecutive addresses.
   const int32_t *inc_off;
-  const uint32_t *inc;
-  int32_t ninc;
-  float *fslot;              // [ninc][3] slot forces written by the term passes (summed in FP64)
-  const int32_t *bd_pos, *ba_pos, *st_pos, *tst_pos, *hb_pos;  // slot k of term t -> incidence index
+  const int32_t *inc_slot;
+  float *fslot;              // [slots][3] slot forces written by the term passes (summed in FP64)
+  int64_t sb_bd, sb_ba, sb_st, sb_tst, sb_hb;
   // terms (global bead indices)
   const int32_t *bd_mp; const double *bd_nat, *bd_coef; int32_t nbd;
   const int32_t *ba_mp; const double *ba_nat, *ba_coef; int32_t nba;

@@ -127,13 +127,9 @@ __device__ inline void stack_eval(const DevPtrs &P, const double *X, int st, dou
   for (int k = 0; k < 21; ++k) fo[k] *= sc;
 }
 
-__device__ inline void put_slots(const DevPtrs &P, const int32_t *pos, int nterm, int term, int nslot,
-                                 const double *fo) {
-  for (int k = 0; k < nslot; ++k) {
-    const int q = pos[(size_t)k * nterm + term];
-    float *o = P.fslot + 3 * (size_t)q;   // one 12-byte slot
-    o[0] = (float)fo[3 * k]; o[1] = (float)fo[3 * k + 1]; o[2] = (float)fo[3 * k + 2];
-  }
+__device__ inline void put_slots(const DevPtrs &P, int64_t base, int term, int nslot, const double *fo) {
+  float *o = P.fslot + 3 * (base + (int64_t)nslot * term);   // the term's nslot 12-byte slots, contiguous
+  for (int k = 0; k < 3 * nslot; ++k) o[k] = (float)fo[k];
 }
 
 // st_status of force_dtrna_stack_nlocal.F90:83-92 evaluated where it is
@@ -167,14 +163,14 @@ __device__ inline void bonded_lane(const DevPtrs &P, const double *X, const doub
   if (t < nb) {
     double fo[6];
     bond_eval(P, X, b0 + t, fo);
-    put_slots(P, P.bd_pos, P.nbd, b0 + t, 2, fo);
+    put_slots(P, P.sb_bd, b0 + t, 2, fo);
     return;
   }
   t -= nb;
   if (t < na) {
     double fo[9];
     angle_eval(P, X, a0 + t, fo);
-    put_slots(P, P.ba_pos, P.nba, a0 + t, 3, fo);
+    put_slots(P, P.sb_ba, a0 + t, 3, fo);
     return;
   }
   t -= na;
@@ -183,7 +179,7 @@ __device__ inline void bonded_lane(const DevPtrs &P, const double *X, const doub
     double fo[21];
     if (stack_on(P, X, PX, st)) stack_eval(P, X, st, fo);
     else for (int k = 0; k < 21; ++k) fo[k] = 0.0;
-    put_slots(P, P.st_pos, P.nst, st, 7, fo);
+    put_slots(P, P.sb_st, st, 7, fo);
   }
 }
 
@@ -536,7 +532,7 @@ __device__ void terms_wave(const DevPtrs &P, const double *X, const double *PX, 
   for (int k = lane; k < s.ntst; k += nt) {
     double fo[18];
     tstack_eval(P, s, X, PX, s.toff + k, fo);
-    put_slots(P, P.tst_pos, P.ntst, s.toff + k, 6, fo);
+    put_slots(P, P.sb_tst, s.toff + k, 6, fo);
   }
   if (!s.hb_excess_possible) {
     // no site can exceed its valence: every hbond within the cutoff stays
@@ -548,7 +544,7 @@ __device__ void terms_wave(const DevPtrs &P, const double *X, const double *PX, 
       if (!st) for (int q = 0; q < 18; ++q) fo[q] = 0.0;
       P.hb_status[h] = (uint8_t)st;
       P.hb_ene[h] = e;
-      put_slots(P, P.hb_pos, P.nhb, h, 6, fo);
+      put_slots(P, P.sb_hb, h, 6, fo);
     }
     return;
   }
@@ -584,7 +580,7 @@ __device__ void terms_wave(const DevPtrs &P, const double *X, const double *PX, 
     double fo[18];
     const bool on = P.hb_status[h] != 0;
     for (int q = 0; q < 18; ++q) fo[q] = on ? P.for_hb[(size_t)q * P.nhb + h] : 0.0;
-    put_slots(P, P.hb_pos, P.nhb, h, 6, fo);
+    put_slots(P, P.sb_hb, h, 6, fo);
   }
 }
 
@@ -773,21 +769,7 @@ void k_force(DevPtrs P, const double *X, const double *PX, const int32_t *systas
   }
   b -= ntb;
   if (b >= npb) {
-    // Workgroups go to the eight XCDs round-robin by workgroup index, each
-    // XCD with its own L2.  The bonded chunks (terms in bead order) are dealt
-    // so that each XCD takes one contiguous run of them: the slots a run
-    // writes (bead-ordered incidence entries) then sit in lines that one L2
-    // alone writes back, instead of lines written in parts by several.
-    const int nbb = (int)gridDim.x - ntb - npb, g0 = ntb + npb, gb = (int)blockIdx.x;
-    const int x = gb % kXcds;
-    int before = 0, first = 0;
-    for (int y = 0; y <= x; ++y) {
-      const int f = g0 + ((y - g0 % kXcds) + kXcds) % kXcds;   // first bonded workgroup on XCD y
-      if (y == x) first = f;
-      else before += f < g0 + nbb ? (g0 + nbb - 1 - f) / kXcds + 1 : 0;
-    }
-    const int chunk = before + (gb - first) / kXcds;
-    bonded_lane(P, X, PX, br, chunk * (kTile * kWaves) + (int)threadIdx.x);
+    bonded_lane(P, X, PX, br, (b - npb) * (kTile * kWaves) + (int)threadIdx.x);
     return;
   }
   const int w = (int)threadIdx.x / kTile;
@@ -1121,9 +1103,12 @@ __global__ __launch_bounds__(kTile * 2) void k_main(DevPtrs P, const double *X, 
       const int ib = P.inc_off[g], ie = P.inc_off[g + 1];
       for (int k0 = ib; k0 < ie; k0 += 8) {
         float sl[8][3];
+        int64_t at[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) at[u] = P.inc_slot[min(k0 + u, ie - 1)];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-          const float *q = P.fslot + 3 * (size_t)min(k0 + u, ie - 1);
+          const float *q = P.fslot + 3 * at[u];
           sl[u][0] = q[0]; sl[u][1] = q[1]; sl[u][2] = q[2];
         }
 #pragma unroll
