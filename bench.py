@@ -368,6 +368,7 @@ def main():
     ap.add_argument("--no-ewald", action="store_true")
     ap.add_argument("--no-valence", action="store_true")
     ap.add_argument("--no-bdhi", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--swap-every", type=int, default=1000)
     ap.add_argument("--trigger-steps", type=int, default=1000,
                     help="steps of the displacement-trigger leg (0 = skip)")
@@ -409,6 +410,12 @@ def main():
     rb0 = eng.counters()["rebuilds"]
     print(f"[bench] {len(systems)} systems, {nbeads} beads, setup {time.perf_counter() - t_build:.1f} s",
           file=sys.stderr, flush=True)
+    # launch shape of the force pass for this batch (no result depends on it)
+    tuner = None if args.no_tune else eng.tune(reps=20)
+    if tuner:
+        print(f"[bench] tuner: pairs/wave {tuner['pairs_per_wave']}, terms/lane {tuner['terms_per_lane']}, "
+              f"off-diagonal first {tuner['offdiag_first']}; k_force ms {tuner['candidates_ms']}",
+              file=sys.stderr, flush=True)
 
     # the 64-replica ladder: temperature labels swap every swap_every steps
     from tis_amd.remd import ReplicaLadder
@@ -579,6 +586,7 @@ def main():
         "bdhi": bdhi,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "tuner": tuner,
     }
     print(json.dumps(out))
 

@@ -209,6 +209,22 @@ int32_t tis_list_capacity(tis_ctx *ctx, int32_t cap, int32_t *cap_out);
  * admission and completion. */
 int32_t tis_transfer_bytes(tis_ctx *ctx, int64_t *bytes);
 
+/* Launch-shape tuner of the force pass (no reference counterpart): times
+ * the force pass of the whole batch at the current coordinates under 8
+ * launch shapes -- electrostatic chunk pairs per wave {1, 2} x bonded terms
+ * per lane {1, 2} x off-diagonal chunk pairs first or in system order --
+ * in reps rounds that time each candidate once (after warm-up passes), and
+ * keeps the one with the smallest median for this batch.  No result
+ * depends on the shape (forces and trajectories are bitwise the same) and
+ * no state changes (the counters the passes bump are restored).
+ * shape_out[3] = (pairs per wave, terms per lane, off-diagonal first),
+ * ms_out[8] = median force-pass ms per candidate in the order above, pairs
+ * per wave varying fastest (either may be NULL). */
+int32_t tis_tune(tis_ctx *ctx, int32_t reps, int32_t *shape_out, double *ms_out);
+/* Set (set != NULL) and/or read (get != NULL) the launch shape, 3 values as
+ * in tis_tune. */
+int32_t tis_launch_shape(tis_ctx *ctx, const int32_t *set, int32_t *get);
+
 /* Device memory held by the context's batch, in bytes (admission control). */
 int32_t tis_device_bytes(tis_ctx *ctx, int64_t *bytes);
 

@@ -179,3 +179,32 @@ def test_alone_equals_mix_gpu():
         print(f"   budget {budget}: layouts {st.layouts}, h2d/admission "
               f"{st.summary()['h2d_bytes_per_admission']:.0f} B, carried on device {st.d2d_state} B", flush=True)
         srv._eng.close()
+
+
+@pytest.mark.gpu
+def test_alone_equals_mix_long_jobs_gpu():
+    """Serving-size jobs: a 1000-nt and a 1200-nt job (2,999 and 3,599 beads)
+    join a batch of short ones at different rounds; each job's final x, v,
+    a equal its run alone, bitwise, and only admissions and completions
+    move state across the bus."""
+    from tis_amd.engine import Engine, MODE_GRAPH
+    seed = 91
+    spec = ([60, 80, 40, 60], [0, 20, 20, 40], [1000, 40, 1200, 64])
+    alone = []
+    for j in _jobs(*spec):
+        print(f"[long jobs] alone: key {j.key}, {j.system.nmp} beads", flush=True)
+        e = Engine([j.system], keys=[j.key], seed=seed)
+        e.set_coords(0, j.x0)
+        e.langevin_init(tstep=0.05, rebuild_every=10, step0=0, init_accel=True)
+        e.langevin_steps(j.nsteps, MODE_GRAPH)
+        alone.append([e.get_state(0, w) for w in range(3)])
+        e.close()
+    jobs = _jobs(*spec)
+    srv = Server(round_steps=20, clock="steps", seed=seed)
+    st = srv.run(jobs)
+    for j, ref in zip(jobs, alone):
+        for got, want in zip((j.x, j.v, j.a), ref):
+            assert np.array_equal(got, want), f"job {j.key} differs"
+    state = sum(72 * j.system.nmp for j in jobs)
+    assert st.h2d_state == state and st.d2h_state == state
+    srv._eng.close()

@@ -20,6 +20,7 @@
 #include "tis_internal.h"
 
 namespace tis {
+template <int PPW, int TPL>
 __global__ void k_force(DevPtrs, const double *, const double *, const int32_t *, int, const EwTask *, int,
                         BondedRange, int32_t);
 __global__ void k_ewald_sk(DevPtrs, const double *, const EwTask *, int);
@@ -180,6 +181,10 @@ struct tis_ctx {
   // state uploads (tis_state_set), state downloads (tis_state_get), state
   // carried device to device by a new arena layout
   int64_t h2d_bytes = 0, h2d_state = 0, d2h_bytes = 0, d2d_state = 0;
+  // launch shape of the force pass (tis_tune): chunk pairs per wave, bonded
+  // terms per lane, off-diagonal chunk pairs first (1) or system order (0)
+  struct Shape { int ppw = 1, tpl = 1, lpt = 1; } shape;
+  std::vector<double> tune_ms;           // k_force ms per candidate of the last tis_tune
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   int graph_len = 0;
   // timing
@@ -1458,20 +1463,25 @@ static void launch_force_pair(tis_ctx *c, const Range &r, const double *X, const
   }
   // k_force: per-system tertiary stacks and hbonds, the electrostatic chunk
   // pairs, then the bonded terms in blocks of 256 lanes
+  // (launch shape: chunk pairs per wave, bonded terms per lane and the
+  // order of the chunk pairs; tis_tune chooses it, no result depends on it)
   const int nterm = r.nb + r.na + r.ns;
   const int npt = c->any_ele ? r.ptn : 0;
-  const int nblk = (r.taskn + kWaves - 1) / kWaves + (npt + kWaves - 1) / kWaves +
-                   (nterm + kTile * kWaves - 1) / (kTile * kWaves);
+  const int ppw = c->shape.ppw, tpl = c->shape.tpl;
+  const int nblk = (r.taskn + kWaves - 1) / kWaves + (npt + kWaves * ppw - 1) / (kWaves * ppw) +
+                   (nterm + kTile * kWaves * tpl - 1) / (kTile * kWaves * tpl);
   if (nblk > 0) {
     const BondedRange br{r.b0, r.nb, r.a0, r.na, r.s0, r.ns};
     const int32_t *task = c->d_systask + r.task0;
-    const EwTask *pt = r.all ? c->d_ptask_lpt : c->d_ptask + r.pt0;
+    const EwTask *pt = r.all ? (c->shape.lpt ? c->d_ptask_lpt : c->d_ptask) : c->d_ptask + r.pt0;
+    // one kernel per launch shape (each with its own register allocation)
+    auto kf = ppw == 1 ? (tpl == 1 ? k_force<1, 1> : k_force<1, 2>) : (tpl == 1 ? k_force<2, 1> : k_force<2, 2>);
     if (c->timing)
-      hipExtLaunchKernelGGL(k_force, dim3(nblk), dim3(kTile * kWaves), 0, c->stream, e3[1], e3[2], 0, c->P, X, PX,
+      hipExtLaunchKernelGGL(kf, dim3(nblk), dim3(kTile * kWaves), 0, c->stream, e3[1], e3[2], 0, c->P, X, PX,
                             task, r.taskn, pt, npt, br, pre_t);
     else
-      hipLaunchKernelGGL(k_force, dim3(nblk), dim3(kTile * kWaves), 0, c->stream, c->P, X, PX, task, r.taskn, pt,
-                         npt, br, pre_t);
+      hipLaunchKernelGGL(kf, dim3(nblk), dim3(kTile * kWaves), 0, c->stream, c->P, X, PX, task, r.taskn, pt, npt,
+                         br, pre_t);
   } else if (c->timing) {
     (void)hipEventRecord(e3[1], c->stream);
     (void)hipEventRecord(e3[2], c->stream);
@@ -1896,6 +1906,86 @@ static int grow_nb(tis_ctx *c, unsigned long long need) {
   c->P.nb_cap = newcap;
   for (int k = 0; k < 2; ++k)
     if (c->graph[k]) { (void)hipGraphExecDestroy(c->graph[k]); c->graph[k] = nullptr; }
+  return TIS_OK;
+}
+
+// ---------------------------------------------------------------- launch-shape tuner
+// The force pass of the whole batch at the current coordinates (mode 0:
+// forces only, no state changes) under each candidate launch shape: after
+// warm-up passes, reps rounds that time every candidate once (events bound
+// to the k_force dispatch), so that clock ramps and drifts fall on all
+// candidates alike; the candidate with the smallest median is kept.  The
+// counters the passes bump are put back.
+int32_t tis_tune(tis_ctx *c, int32_t reps, int32_t *shape_out, double *ms_out) {
+  if (!c || reps < 1 || reps > 1000) return TIS_ERR_ARG;
+  int rc = check_sys(c, -1);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (!c->lists_valid) { rc = rebuild_lists(c, -1); if (rc) return rc; }
+  unsigned long long cnt[TIS_CNT_MAX];
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(cnt, c->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
+  constexpr int kCand = 8;
+  const tis_ctx::Shape cand[kCand] = {{1, 1, 1}, {2, 1, 1}, {1, 2, 1}, {2, 2, 1},
+                                      {1, 1, 0}, {2, 1, 0}, {1, 2, 0}, {2, 2, 0}};
+  const tis_ctx::Shape keep = c->shape;
+  const Range r = range_of(c, -1);
+  for (int w = 0; w < 4; ++w)
+    for (int k = 0; k < kCand; ++k) {
+      c->shape = cand[k];
+      launch_force_pair(c, r, Xc(c), PXc(c), nullptr, nullptr, 0, 0);
+    }
+  std::vector<std::vector<double>> t(kCand);
+  for (int rep = 0; rep < reps; ++rep) {
+    // an untimed pass after the host's wait, then every candidate once,
+    // starting from a different one each round
+    launch_force_pair(c, r, Xc(c), PXc(c), nullptr, nullptr, 0, 0);
+    c->timing = true;
+    c->tev_used = 0;
+    for (int i = 0; i < kCand; ++i) {
+      c->shape = cand[(rep + i) % kCand];
+      launch_force_pair(c, r, Xc(c), PXc(c), nullptr, nullptr, 0, 0);
+    }
+    c->timing = false;
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) {
+      c->shape = keep;
+      return fail(c, TIS_ERR_HIP, "tuning pass failed");
+    }
+    for (int i = 0; i < kCand; ++i) {
+      float e = 0.0f;
+      if (hipEventElapsedTime(&e, c->tev[kTev * i + 1], c->tev[kTev * i + 2]) != hipSuccess) e = 0.0f;
+      t[(rep + i) % kCand].push_back(e);
+    }
+    c->tev_used = 0;
+  }
+  c->tune_ms.assign(kCand, 0.0);
+  int best = 0;
+  for (int k = 0; k < kCand; ++k) {
+    std::sort(t[k].begin(), t[k].end());
+    const size_t m = t[k].size();
+    c->tune_ms[k] = m % 2 ? t[k][m / 2] : 0.5 * (t[k][m / 2 - 1] + t[k][m / 2]);
+    if (c->tune_ms[k] < c->tune_ms[best]) best = k;
+  }
+  c->shape = cand[best];
+  HIPCHK(c, hipMemcpy(c->d_counters, cnt, sizeof(cnt), hipMemcpyHostToDevice));
+  for (int k = 0; k < 2; ++k)
+    if (c->graph[k]) { (void)hipGraphExecDestroy(c->graph[k]); c->graph[k] = nullptr; }
+  if (shape_out) { shape_out[0] = c->shape.ppw; shape_out[1] = c->shape.tpl; shape_out[2] = c->shape.lpt; }
+  if (ms_out)
+    for (int k = 0; k < kCand; ++k) ms_out[k] = c->tune_ms[k];
+  return TIS_OK;
+}
+
+int32_t tis_launch_shape(tis_ctx *c, const int32_t *set, int32_t *get) {
+  if (!c) return TIS_ERR_ARG;
+  if (set) {
+    if ((set[0] != 1 && set[0] != 2) || (set[1] != 1 && set[1] != 2) || (set[2] != 0 && set[2] != 1))
+      return fail(c, TIS_ERR_ARG, "launch shape (%d, %d, %d) not offered", set[0], set[1], set[2]);
+    c->shape = tis_ctx::Shape{set[0], set[1], set[2]};
+    for (int k = 0; k < 2; ++k)
+      if (c->graph[k]) { (void)hipGraphExecDestroy(c->graph[k]); c->graph[k] = nullptr; }
+  }
+  if (get) { get[0] = c->shape.ppw; get[1] = c->shape.tpl; get[2] = c->shape.lpt; }
   return TIS_OK;
 }
 

@@ -754,13 +754,15 @@ __device__ inline void ele_pair_task(const DevPtrs &P, const DSys &s, const doub
 // electrostatic chunk pairs per block (ele_pair_task, one wave each), then
 // the bonded terms, 256 lanes per block (bonded_lane).  k_main gathers all of
 // their outputs.
+template <int PPW, int TPL>
 __global__ __launch_bounds__(kTile * kWaves) __attribute__((amdgpu_waves_per_eu(4)))
 void k_force(DevPtrs P, const double *X, const double *PX, const int32_t *systask, int ntask,
              const EwTask *ptasks, int npt, BondedRange br, int32_t t_local) {
+  constexpr int ppw = PPW, tpl = TPL;
   __shared__ EleLds lds;
   if (run_stopped(P)) return;
   int b = (int)blockIdx.x;
-  const int npb = (npt + kWaves - 1) / kWaves;   // blocks of four chunk pairs
+  const int npb = (npt + kWaves * ppw - 1) / (kWaves * ppw);   // blocks of 4 ppw chunk pairs
   const int ntb = (ntask + kWaves - 1) / kWaves;   // blocks of four system terms passes
   if (b < ntb) {
     const int task = b * kWaves + (int)threadIdx.x / kTile;
@@ -769,28 +771,50 @@ void k_force(DevPtrs P, const double *X, const double *PX, const int32_t *systas
   }
   b -= ntb;
   if (b >= npb) {
-    bonded_lane(P, X, PX, br, (b - npb) * (kTile * kWaves) + (int)threadIdx.x);
+    // tpl bonded terms per lane, a grid-wide stride apart
+    const int nbb = (int)gridDim.x - ntb - npb;
+    for (int i = 0; i < tpl; ++i)
+      bonded_lane(P, X, PX, br, (b - npb + i * nbb) * (kTile * kWaves) + (int)threadIdx.x);
     return;
   }
   const int w = (int)threadIdx.x / kTile;
-  const int task = b * kWaves + w;
   const int dir = wave_dir();
-  if (task >= npt) return;
-  const EwTask tk = ptasks[task];
-  const DSys s = P.sys[tk.sys];
   double *SX = lds.sx[w], *SY = lds.sy[w], *SZ = lds.sz[w];
   float *SQ = lds.sq[w];
-  const double *Y = s.periodic ? PX : X;
-  if (dir != 0) {
-    if (s.ele_form == 0) ele_pair_task<0, true>(P, s, Y, tk.a, tk.b, dir, SX, SY, SZ, SQ);
-    else if (s.ele_form == 1) ele_pair_task<1, true>(P, s, Y, tk.a, tk.b, dir, SX, SY, SZ, SQ);
-    else ele_pair_task<2, true>(P, s, Y, tk.a, tk.b, dir, SX, SY, SZ, SQ);
-  } else {
-    if (s.ele_form == 0) ele_pair_task<0, false>(P, s, Y, tk.a, tk.b, dir, SX, SY, SZ, SQ);
-    else if (s.ele_form == 1) ele_pair_task<1, false>(P, s, Y, tk.a, tk.b, dir, SX, SY, SZ, SQ);
-    else ele_pair_task<2, false>(P, s, Y, tk.a, tk.b, dir, SX, SY, SZ, SQ);
+  // ppw consecutive chunk pairs per wave, one after the other through the
+  // wave's LDS slice
+#pragma unroll 1
+  for (int i = 0; i < ppw; ++i) {
+    const int task = (b * kWaves + w) * ppw + i;
+    if (task >= npt) return;
+    if (i) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    const EwTask tk = ptasks[task];
+    const DSys s = P.sys[tk.sys];
+    const double *Y = s.periodic ? PX : X;
+    if (dir != 0) {
+      if (s.ele_form == 0) ele_pair_task<0, true>(P, s, Y, tk.a, tk.b, dir, SX, SY, SZ, SQ);
+      else if (s.ele_form == 1) ele_pair_task<1, true>(P, s, Y, tk.a, tk.b, dir, SX, SY, SZ, SQ);
+      else ele_pair_task<2, true>(P, s, Y, tk.a, tk.b, dir, SX, SY, SZ, SQ);
+    } else {
+      if (s.ele_form == 0) ele_pair_task<0, false>(P, s, Y, tk.a, tk.b, dir, SX, SY, SZ, SQ);
+      else if (s.ele_form == 1) ele_pair_task<1, false>(P, s, Y, tk.a, tk.b, dir, SX, SY, SZ, SQ);
+      else ele_pair_task<2, false>(P, s, Y, tk.a, tk.b, dir, SX, SY, SZ, SQ);
+    }
   }
 }
+
+template __global__ void k_force<1, 1>(DevPtrs, const double *, const double *, const int32_t *, int, const EwTask *,
+                                       int, BondedRange, int32_t);
+template __global__ void k_force<2, 1>(DevPtrs, const double *, const double *, const int32_t *, int, const EwTask *,
+                                       int, BondedRange, int32_t);
+template __global__ void k_force<1, 2>(DevPtrs, const double *, const double *, const int32_t *, int, const EwTask *,
+                                       int, BondedRange, int32_t);
+template __global__ void k_force<2, 2>(DevPtrs, const double *, const double *, const int32_t *, int, const EwTask *,
+                                       int, BondedRange, int32_t);
 
 // ------------------------------------------------------------------ Ewald reciprocal space
 // force_ele_coulomb_ewld.F90:62-84.  The phase k.r = 2 pi (h . r/L) is

@@ -59,6 +59,8 @@ def load_library():
     L.tis_system_append.argtypes = [vp, C.c_int32, C.POINTER(CSystem), C.POINTER(C.c_uint32), _i32p]
     L.tis_system_retire.argtypes = [vp, C.c_int32, _i32p]
     L.tis_arena_stats.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.tis_tune.argtypes = [vp, C.c_int32, _i32p, _f64p]
+    L.tis_launch_shape.argtypes = [vp, _i32p, _i32p]
     L.tis_state_set.argtypes = [vp, C.c_int32, C.c_int32, _f64p]
     L.tis_state_get.argtypes = [vp, C.c_int32, C.c_int32, _f64p]
     L.tis_neighbor.argtypes = [vp, C.c_int32]
@@ -79,7 +81,7 @@ def load_library():
     L.tis_last_force_ms.argtypes = [vp, _f64p]
     for fn in ("tis_ctx_create", "tis_ctx_destroy", "tis_system_upload", "tis_system_validate", "tis_set_system_T",
                "tis_list_capacity", "tis_term_status", "tis_transfer_bytes", "tis_system_append",
-               "tis_system_retire", "tis_arena_stats",
+               "tis_system_retire", "tis_arena_stats", "tis_tune", "tis_launch_shape",
                "tis_state_set",
                "tis_state_get", "tis_neighbor", "tis_pairs_get", "tis_force_sumup", "tis_force_get",
                "tis_energy_sumup", "tis_energy_unit", "tis_langevin_init", "tis_langevin_steps", "tis_status_counters",
@@ -144,6 +146,24 @@ class Engine:
         a = np.ascontiguousarray(isys, dtype=np.int32)
         self._chk(self.L.tis_system_retire(self.ctx, len(a), a.ctypes.data_as(_i32p)), "tis_system_retire")
         self.live.difference_update(int(v) for v in a)
+
+    def tune(self, reps: int = 20) -> dict:
+        """tis_tune: time the force pass under each launch shape, keep the
+        fastest; returns the choice and the per-candidate times."""
+        shape = (C.c_int32 * 3)()
+        ms = (C.c_double * 8)()
+        self._chk(self.L.tis_tune(self.ctx, int(reps), shape, ms), "tis_tune")
+        names = [f"ppw{p}_tpl{t}_{'offdiag_first' if o else 'system_order'}"
+                 for o in (1, 0) for t in (1, 2) for p in (1, 2)]
+        return {"pairs_per_wave": shape[0], "terms_per_lane": shape[1], "offdiag_first": bool(shape[2]),
+                "candidates_ms": dict(zip(names, [round(v, 5) for v in ms]))}
+
+    def launch_shape(self, shape=None):
+        """Set (ppw, tpl, offdiag_first) when given; return the current one."""
+        get = (C.c_int32 * 3)()
+        st = None if shape is None else (C.c_int32 * 3)(*[int(v) for v in shape])
+        self._chk(self.L.tis_launch_shape(self.ctx, st, get), "tis_launch_shape")
+        return tuple(get)
 
     def arena_stats(self) -> dict:
         a = (C.c_int64 * 6)()

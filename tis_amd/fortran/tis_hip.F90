@@ -64,7 +64,7 @@ module tis_hip
 
   public :: tis_ctx_create, tis_ctx_destroy, tis_last_error, tis_system_upload, tis_system_validate, &
        tis_set_system_T, tis_list_capacity, tis_term_status, tis_transfer_bytes, &
-       tis_system_append, tis_system_retire, tis_arena_stats, &
+       tis_system_append, tis_system_retire, tis_arena_stats, tis_tune, tis_launch_shape, &
        tis_state_set, tis_state_get, tis_neighbor, tis_pairs_get, tis_force_sumup, tis_force_get, &
        tis_energy_sumup, tis_energy_unit, tis_langevin_init, tis_langevin_steps, tis_system_origin, tis_device_bytes, &
        tis_status_counters, tis_synchronize, tis_last_timing, tis_last_force_ms, tis_label_define, &
@@ -119,6 +119,17 @@ module tis_hip
        type(c_ptr), value :: ctx
        integer(c_int32_t), value :: n
        integer(c_int32_t), intent(in) :: isys(*)
+     end function
+     integer(c_int32_t) function tis_tune(ctx, reps, shape_out, ms_out) bind(C, name="tis_tune")
+       import :: c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: reps
+       integer(c_int32_t), intent(out) :: shape_out(3)
+       real(c_double), intent(out) :: ms_out(8)
+     end function
+     integer(c_int32_t) function tis_launch_shape(ctx, set, get) bind(C, name="tis_launch_shape")
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: ctx, set, get
      end function
      integer(c_int32_t) function tis_arena_stats(ctx, stats) bind(C, name="tis_arena_stats")
        import :: c_int32_t, c_int64_t, c_ptr
