@@ -527,7 +527,10 @@ def main():
     dom_ms = main_ms if dom == "k_main" else ele_ms
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
     roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(dom),
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": pmc_traffic(dom if dom == "k_main" or tuner is None or
+                                   (tuner["pairs_per_wave"], tuner["terms_per_lane"]) == (1, 1)
+                                   else f"k_force<{tuner['pairs_per_wave']}, {tuner['terms_per_lane']}>"),
             "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT),
             "avg_launch_ms": round(dom_ms, 5), "bytes_per_launch": dom_bytes,
             "k_main_ms": round(main_ms, 5), "k_force_ms": round(ele_ms, 5), "pre_main_ms": round(pre_ms, 5)}
