@@ -20,6 +20,7 @@ import argparse
 import json
 import math
 import os
+import subprocess
 import sys
 import time
 
@@ -272,21 +273,22 @@ def bdhi_leg(sizes=(300, 1000, 3000), steps=50, refactor=10):
     return out
 
 
-def cpu_baseline_reference(systems, coords, budget_s=15.0):
+def cpu_baseline_reference(systems, coords, budget_s=15.0, share=(0, 1)):
     """The reference's own Fortran routines (oracle/_ref, compiled from its
     sources) on this host, one thread, over chains of the same workload until
     budget_s: per step the terms force_sumup evaluates for these chains
     (force_bond, force_bangle, force_dtrna_stack_nlocal + force_dtrna_stack,
     force_dtrna_hbond15, force_exv_dt15, force_ele_DH) and neighbor() every
     10 steps, at the sample's coordinates (the O(N) Langevin update is not
-    counted).  None when the library is absent."""
+    counted).  share = (k, m): this process takes every m-th system of the
+    sample from the k-th.  None when the library is absent."""
     import oracle
     if not oracle.RefLib.available():
         return None
     R = oracle.RefLib()
     steps = 300
     terms = [R.TERMS[t] for t in ("bond", "angle", "stack", "hbond", "exv", "dh")]
-    order = list(range(0, len(systems), max(1, len(systems) // 64)))
+    order = list(range(0, len(systems), max(1, len(systems) // 64)))[share[0]::share[1]]
     res = {}
 
     def run():
@@ -325,8 +327,31 @@ def cpu_baseline_reference(systems, coords, budget_s=15.0):
         os.close(saved)
         os.close(devnull)
     return {"value": res["bead_steps"] / res["dt"], "unit": "bead-steps/s", "cores": 1, "kind": "reference",
+            "bead_steps": res["bead_steps"], "seconds": res["dt"], "systems": res["nch"],
             "sample": f"{res['nch']} systems of the workload x {steps} steps of the reference's force routines "
                       f"and neighbor() every 10 steps (oracle/_ref, 1 thread, fixed coordinates), {res['dt']:.1f} s"}
+
+
+def cpu_baseline_reference_procs(nproc, budget_s):
+    """cpu_baseline_reference in nproc worker processes at once (the
+    reference's routines keep their state in module globals, so one process
+    per core), each over its share of the 64-system sample for budget_s;
+    throughput = all bead-steps / the longest worker's time."""
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-worker", "", "--cpu-budget", str(budget_s)]
+    procs = []
+    for k in range(nproc):
+        c = list(cmd)
+        c[3] = f"{k},{nproc}"
+        procs.append(subprocess.Popen(c, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True))
+    outs = [json.loads(p.communicate(timeout=budget_s * 4 + 120)[0].strip().splitlines()[-1]) for p in procs]
+    if any(o is None for o in outs):
+        return None
+    bead_steps = sum(o["bead_steps"] for o in outs)
+    dt = max(o["seconds"] for o in outs)
+    return {"value": bead_steps / dt, "unit": "bead-steps/s", "cores": nproc, "kind": "reference",
+            "sample": f"{sum(o['systems'] for o in outs)} systems of the workload shared by {nproc} processes "
+                      f"(one per core), 300 steps each of the reference's force routines and neighbor() every "
+                      f"10 steps (oracle/_ref, fixed coordinates), {dt:.1f} s"}
 
 
 def cpu_baseline(systems, coords, budget_s=15.0):
@@ -360,6 +385,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=10)
     ap.add_argument("--eager-steps", type=int, default=200)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-procs", type=int, default=8, help="processes of the multi-core CPU baseline")
+    ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=100)
@@ -374,10 +401,17 @@ def main():
                     help="steps of the displacement-trigger leg (0 = skip)")
     ap.add_argument("--rebuild-every", type=int, default=10,
                     help="neighbour-list rebuild interval in steps; 0 = displacement trigger (margin 8 A)")
-    ap.add_argument("--only", choices=["ewald", "bdhi", "copy"], default=None,
+    ap.add_argument("--only", choices=["ewald", "bdhi", "copy", "serve"], default=None,
                     help="run one side leg alone (profiling); 'copy' is a plain 1 GiB device copy "
-                         "that calibrates the HBM byte counters")
+                         "that calibrates the HBM byte counters; 'serve' runs the serving stream with "
+                         "--serve-jobs and --serve-scale (1 = the configured U[1e4, 1e6] steps per job)")
     args = ap.parse_args()
+    if args.cpu_worker is not None:
+        # one worker of the multi-core CPU baseline: no GPU work in this process
+        k, m = (int(v) for v in args.cpu_worker.split(","))
+        systems, coords, _ = build_workload()
+        print(json.dumps(cpu_baseline_reference(systems, coords, args.cpu_budget, share=(k, m))))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: the systems of this workload are independent and would be "
               "sharded by bead budget across GPUs, one process per GPU, with no per-step exchange.",
@@ -385,6 +419,9 @@ def main():
         sys.exit(2)
     if args.only == "ewald":
         print(json.dumps({"ewald": ewald_leg()}))
+        return
+    if args.only == "serve":
+        print(json.dumps({"serving": serving_stream(njobs=args.serve_jobs, scale=args.serve_scale)}))
         return
     if args.only == "bdhi":
         print(json.dumps({"bdhi": bdhi_leg(sizes=(300, 3000), steps=20)}))
@@ -553,7 +590,14 @@ def main():
     bdhi = None if args.no_bdhi else bdhi_leg()
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline_reference(systems, coords, args.cpu_budget) or cpu_baseline(systems, coords,
+        cpu = None
+        if args.cpu_procs > 1:
+            one = cpu_baseline_reference(systems, coords, args.cpu_budget / 2)
+            if one is not None:
+                cpu = cpu_baseline_reference_procs(args.cpu_procs, args.cpu_budget / 2)
+                if cpu is not None:
+                    cpu["single_core_value"] = one["value"]
+        cpu = cpu or cpu_baseline_reference(systems, coords, args.cpu_budget) or cpu_baseline(systems, coords,
                                                                                        args.cpu_budget)
     out = {
         "metric": "batched bead-steps/s vs plain GPU @ force err <=1e-5 vs FP64; p50/p99 job latency",

@@ -257,8 +257,8 @@ def _small_margin_systems(margin=1.5):
 @pytest.mark.parametrize("which", [0, 2])
 def test_displacement_trigger_matches_oracle(which):
     """Displacement-triggered rebuilds (time_integral.F90:57-93): the GPU
-    trajectory and rebuild count against the oracle's trigger loop.  The
-    150-nt chain (8 bead tiles) rebuilds its lists by tile pairs."""
+    trajectory and rebuild count against the oracle's trigger loop (a
+    64-nt and a 150-nt chain with a small margin)."""
     s, x = _small_margin_systems()[which]
     print(f"[trigger] system {which} nmp={s.nmp}", flush=True)
     v = np.zeros_like(x)
@@ -277,6 +277,34 @@ def test_displacement_trigger_matches_oracle(which):
     assert nr >= 4
     assert ng == nr
     assert d < 1e-4
+
+
+def test_triggered_rebuild_lists_match_oracle():
+    """The lists of a displacement-triggered rebuild (only the fired
+    system's tiles working in the gated build) are the oracle's lists at the
+    coordinates of that step, pair for pair."""
+    s, x = _small_margin_systems()[2]
+    eng = Engine([s], keys=[4], seed=17)
+    eng.set_coords(0, x)
+    eng.langevin_init(tstep=0.05, rebuild_every=0, step0=0, init_accel=True)
+    checked = 0
+    for step in range(80):
+        xb = eng.coords(0)
+        r0 = eng.counters()["rebuilds"]
+        eng.langevin_steps(1, MODE_EAGER)
+        if eng.counters()["rebuilds"] == r0:
+            continue
+        o = oracle.Oracle(s)
+        o.neighbor(xb)
+        pe, oe = eng.pairs(0, LIST_EXV), o.lists(0)
+        print(f"[triggered lists] step {step + 1}: {len(pe)} exv pairs", flush=True)
+        assert len(pe) == len(oe)
+        assert np.array_equal(pe[np.lexsort(pe.T[::-1])], oe[np.lexsort(oe.T[::-1])])
+        checked += 1
+        if checked == 3:
+            break
+    eng.close()
+    assert checked >= 2
 
 
 def test_displacement_trigger_modes_and_batch_bitwise():

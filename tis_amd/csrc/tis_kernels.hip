@@ -1284,6 +1284,32 @@ __device__ inline double point_box_d2(const double p[3], const BBox &b) {
 // (one lane per tile), the group boxes are staged with the tile, and the
 // unit-pair radii sit in LDS, so neither the culling nor the per-pair tests
 // wait on dependent global loads.
+// One candidate pair of the exv list (neighbor_list.F90 distance test with
+// the unit-pair radius, neighbor_assign.F90 exclusions): this lane's bead
+// li against bead lj2 of the same system (staged coordinates, unit,
+// type | class << 8).  On true, entry = mirror << 27 | global index.
+template <bool RL2_LDS>
+__device__ inline bool exv_candidate(const DevPtrs &P, const DSys &s, const double xi[3], int li, int ui, int ti,
+                                     int ci, int lj2, double xj0, double xj1, double xj2, int uj, int tcj,
+                                     const double *rl2l, uint32_t &entry) {
+  if (lj2 == li) return false;
+  const bool own_lo = li < lj2;
+  double v[3];
+  pair_vec(s, xj0, xj1, xj2, xi, own_lo, v);
+  const int code = s.periodic ? pbneighbor(s, v) : 0;
+  const double d2 = sq3_exact(v[0], v[1], v[2]);
+  if (!(d2 < s.rl2max)) return false;   // rl2max bounds every unit-pair radius
+  const int ulo = own_lo ? ui : uj, uhi = own_lo ? uj : ui;
+  const double r2 = RL2_LDS ? rl2l[ulo * s.nunit + uhi] : P.rl2[s.uoff + ulo * s.nunit + uhi];
+  if (!(d2 < r2)) return false;
+  const int tj = tcj & 0xFF, cj = tcj >> 8;
+  const bool keep = own_lo ? exv_keep(s, ui, uj, ci, ti, li, lj2) : exv_keep(s, uj, ui, cj, tj, lj2, li);
+  if (!keep) return false;
+  const int oc = own_lo ? code : mirror_flip(code);
+  entry = ((uint32_t)oc << kMirrorShift) | (uint32_t)(s.off + lj2);
+  return true;
+}
+
 template <bool RL2_LDS>
 __device__ inline void list_exv_tile_t(const DevPtrs &P, const DSys &s, const double *X, const double *PX,
                                        const Tile tl, int it, ListLds &L) {
@@ -1329,23 +1355,11 @@ __device__ inline void list_exv_tile_t(const DevPtrs &P, const DSys &s, const do
         const int q1 = min(q0 + 16, kend);
         if (valid)
           for (int k = q0; k < q1; ++k) {
-            const int lj2 = jt + k;
-            if (lj2 == li) continue;
-            const bool own_lo = li < lj2;
-            double v[3];
-            pair_vec(s, sx[k], sy[k], sz[k], xi, own_lo, v);
-            const int code = s.periodic ? pbneighbor(s, v) : 0;
-            const double d2 = sq3_exact(v[0], v[1], v[2]);
-            if (!(d2 < s.rl2max)) continue;   // rl2max bounds every unit-pair radius
-            const int uj = su[k];
-            const int ulo = own_lo ? ui : uj, uhi = own_lo ? uj : ui;
-            const double r2 = RL2_LDS ? L.rl2[ulo * s.nunit + uhi] : P.rl2[s.uoff + ulo * s.nunit + uhi];
-            if (!(d2 < r2)) continue;
-            const int tj = stc[k] & 0xFF, cj = stc[k] >> 8;
-            const bool keep = own_lo ? exv_keep(s, ui, uj, ci, ti, li, lj2) : exv_keep(s, uj, ui, cj, tj, lj2, li);
-            if (!keep) continue;
-            const int oc = own_lo ? code : mirror_flip(code);
-            if (cnt < P.nb_cap) P.nb[(size_t)cnt * n + g] = ((uint32_t)oc << kMirrorShift) | (uint32_t)(s.off + lj2);
+            uint32_t e;
+            if (!exv_candidate<RL2_LDS>(P, s, xi, li, ui, ti, ci, jt + k, sx[k], sy[k], sz[k], su[k], stc[k], L.rl2,
+                                        e))
+              continue;
+            if (cnt < P.nb_cap) P.nb[(size_t)cnt * n + g] = e;
             ++cnt;
           }
       }
