@@ -191,10 +191,22 @@ __device__ inline void wave_lds_sync() {
 // registers, the factor's entries broadcast from LDS), storing the rows and
 // their FP32 copy in P.  Full tiles take branch-free stores.
 constexpr int kPanelRows = 256;   // panel rows solved per block
+// 1/sqrt(x) for x > 0: the hardware estimate refined by two Newton steps
+// y <- y + y (1/2 - (x/2) y^2), each doubling the correct bits (to about
+// 1 ulp in FP64); x <= 0 gives NaN or infinity, as 1/sqrt(x) would.
+__device__ inline double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  y = fma(y, fma(-h * y, y, 0.5), y);
+  y = fma(y, fma(-h * y, y, 0.5), y);
+  return y;
+}
+
 __global__ __launch_bounds__(256) void k_chol_panel(BdDev B, int kk, int k0, float *P, int mpad) {
   __shared__ double Ls[kCb * kCb];   // Ls[c * 64 + r] = L11[r][c]
   __shared__ double Rd[kCb];         // 1 / L11[c][c]
-  __shared__ double colb[kCb];       // column c of the block being factored
+  __shared__ double colb[2][kCb];    // the column whose pivot comes next (two buffers)
+  __shared__ double colt[kCb];       // column c of the factor
   const int lane = threadIdx.x & (kCb - 1), w = threadIdx.x / kCb, n3 = B.n3;
   const int kb = min(kCb, n3 - kk);
   if (w == 0) {
@@ -207,18 +219,28 @@ __global__ __launch_bounds__(256) void k_chol_panel(BdDev B, int kk, int k0, flo
 #pragma unroll
       for (int c = 0; c < kCb; ++c) a[c] = (lane < kb && c < kb) ? a[c] : (c == lane ? 1.0 : 0.0);
     }
+    // Column c: every lane reads the pivot from the published column, scales
+    // its own entry (t = L[lane][c]) and publishes it; the lanes then update
+    // column c + 1 first and publish it for the next pivot before updating
+    // the rest, so that this store's latency hides behind the updates.
     int bad = 0;
+    colb[0][lane] = a[0];
+    wave_lds_sync();
 #pragma unroll
     for (int c = 0; c < kCb; ++c) {
-      colb[lane] = a[c];
-      wave_lds_sync();
-      const double piv = colb[c];
+      const double piv = colb[c & 1][c];
       if (c < kb && !(piv > 0.0) && bad == 0) bad = kk + c + 1;
-      const double lcc = sqrt(piv);
-      const double inv = 1.0 / lcc;
+      const double inv = rsqrt_nr(piv);
+      const double lcc = piv * inv;
       const double t = a[c] * inv;   // L[lane][c] for lane > c
+      colt[lane] = t;
+      wave_lds_sync();
+      if (c + 1 < kCb) {
+        a[c + 1] = fma(-t, colt[c + 1], a[c + 1]);
+        colb[(c + 1) & 1][lane] = a[c + 1];
+      }
 #pragma unroll
-      for (int j = c + 1; j < kCb; ++j) a[j] = fma(-t, colb[j] * inv, a[j]);
+      for (int j = c + 2; j < kCb; ++j) a[j] = fma(-t, colt[j], a[j]);
       a[c] = lane == c ? lcc : (lane > c ? t : 0.0);
       if (lane == 0) Rd[c] = inv;
       wave_lds_sync();
