@@ -44,6 +44,7 @@ struct BdDev {
   uint64_t seed;
   uint32_t key;
   int *info;                     // first failing pivot + 1, or 0
+  double *stash;                 // two 64x64 diagonal factors awaiting their store
 };
 
 // ---------------------------------------------------------------- forces
@@ -181,16 +182,22 @@ __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 }
 
-// One panel at column kk of the outer block at k0, 256 threads per block.
-// Wave 0 of every block factors the 64x64 diagonal block with lane r
-// holding row r in registers, right-looking: at column c each lane
-// publishes its entry of the column in LDS, reads the pivot and the entries
-// of the rows below as broadcasts and updates its own row.  Block 0 stores
-// the factor; in blocks 1.. the factor goes to LDS and each of the four
-// waves solves 64 rows of the panel below by forward substitution (row in
-// registers, the factor's entries broadcast from LDS), storing the rows and
-// their FP32 copy in P.  Full tiles take branch-free stores.
-constexpr int kPanelRows = 256;   // panel rows solved per block
+// The value of v at lane l of the wave, to every lane (two 32-bit reads).
+__device__ inline double lane_value(double v, int l) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// 1/x: the hardware estimate refined by two Newton steps y <- y + y (1 - x y).
+__device__ inline double rcp_nr(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  y = fma(y, fma(-x, y, 1.0), y);
+  y = fma(y, fma(-x, y, 1.0), y);
+  return y;
+}
+
 // 1/sqrt(x) for x > 0: the hardware estimate refined by two Newton steps
 // y <- y + y (1/2 - (x/2) y^2), each doubling the correct bits (to about
 // 1 ulp in FP64); x <= 0 gives NaN or infinity, as 1/sqrt(x) would.
@@ -202,14 +209,90 @@ __device__ inline double rsqrt_nr(double x) {
   return y;
 }
 
-__global__ __launch_bounds__(256) void k_chol_panel(BdDev B, int kk, int k0, float *P, int mpad) {
+// Factors the 64x64 diagonal block held by one wave, lane r holding row r in
+// a[] (rows past kb are unit rows), as L D L^T, right-looking: column c's
+// pivot d = A[c][c] and the entry A[c+1][c] come from lanes c and c + 1 by
+// lane reads, each lane updates its entry of column c + 1 at once (the chain
+// from one pivot to the next has no LDS in it), and the rest of the update,
+// A[r][j] -= A[r][c] A[j][c] / d for j > c + 1, follows one column later from
+// column c as published in LDS (two buffers, one wave barrier per column).
+// Column c is scaled to the Cholesky factor as soon as it is final:
+// L[r][c] = A[r][c] / sqrt(d), L[c][c] = sqrt(d), zero above the diagonal;
+// Rd[c] = 1 / L[c][c].  Returns kk + c + 1 for the first column c < kb with
+// a pivot that is not positive, else 0.
+__device__ inline int diag_factor(double (&a)[kCb], double (*colt)[kCb], double *Rd, int kk, int kb) {
+  const int lane = threadIdx.x & (kCb - 1);
+  int bad = 0;
+  double sp = 0.0;   // A[lane][c - 1] / d_{c - 1}
+#pragma unroll
+  for (int c = 0; c < kCb; ++c) {
+    // column c - 1's update of column c + 1 first: column c + 1 is next on the chain
+    if (c >= 1 && c + 1 < kCb) a[c + 1] = fma(-sp, colt[(c - 1) & 1][c + 1], a[c + 1]);
+    const double d = lane_value(a[c], c);
+    if (c < kb && !(d > 0.0) && bad == 0) bad = kk + c + 1;
+    const double u = a[c];
+    colt[c & 1][lane] = u;
+    const double s = u * rcp_nr(d);
+    if (c + 1 < kCb) a[c + 1] = fma(-s, lane_value(u, c + 1), a[c + 1]);
+    if (c >= 1) {
+#pragma unroll
+      for (int j = c + 2; j < kCb; ++j) a[j] = fma(-sp, colt[(c - 1) & 1][j], a[j]);
+    }
+    const double rs = rsqrt_nr(d);
+    a[c] = lane == c ? d * rs : (lane > c ? u * rs : 0.0);
+    if (lane == 0) Rd[c] = rs;
+    sp = s;
+    wave_lds_sync();
+  }
+  return bad;
+}
+
+// One panel at column kk of the outer block at k0.  Wave 0 of every block
+// factors the 64x64 diagonal block (diag_factor).  Block 0 keeps the factor
+// in a stash (two buffers, by panel parity) and the next panel's block 0
+// stores it over the diagonal block, so no block of this panel can read a
+// partly overwritten block; a panel of one block (the last) stores its own
+// factor, and the kernels between two panels do not touch the diagonal
+// blocks of earlier panels.  In blocks 1.. the factor goes to LDS
+// and each solving wave solves 64 rows of the panel below by forward
+// substitution (row in registers, the factor's entries broadcast from LDS),
+// storing the rows and their FP32 copy in P.  Full tiles take branch-free
+// stores.  One solving wave per block keeps the LDS broadcasts of the solve
+// from queueing behind each other (a CU per block: the rows below the
+// diagonal block are at most n3 - 64), and with a wave of its own for the
+// diagonal block the solving wave's loads overlap the factorisation.
+constexpr int kPanelWaves = 1;                      // solving waves per panel block
+constexpr int kPanelDiagWave = 1;                   // 1: a wave of its own for the diagonal block
+constexpr int kPanelRowsPerBlock = kCb * kPanelWaves;  // rows of the panel solved per block
+
+// S = 1: wave 0 only factors the diagonal block and the W solving waves
+// load their rows meanwhile; S = 0: wave 0 factors, then solves too.
+template <int W, int S>
+__global__ __launch_bounds__(64 * (W + S)) void k_chol_panel(BdDev B, int kk, int k0, float *P, int mpad) {
   __shared__ double Ls[kCb * kCb];   // Ls[c * 64 + r] = L11[r][c]
   __shared__ double Rd[kCb];         // 1 / L11[c][c]
-  __shared__ double colb[2][kCb];    // the column whose pivot comes next (two buffers)
-  __shared__ double colt[kCb];       // column c of the factor
+  __shared__ double colt[2][kCb];    // the last two columns, as published
   const int lane = threadIdx.x & (kCb - 1), w = threadIdx.x / kCb, n3 = B.n3;
   const int kb = min(kCb, n3 - kk);
+  const int r0w = kk + kCb + ((int)blockIdx.x - 1) * (kCb * W) + (w - S) * kCb;   // first row of the wave
+  const bool solver = blockIdx.x > 0 && w >= S && r0w < n3;
+  const int r = r0w + lane;
+  double x[kCb];
+  const auto load_rows = [&]() {
+    const size_t row = (size_t)min(r, n3 - 1);
+#pragma unroll
+    for (int c = 0; c < kCb; ++c) x[c] = B.L[row + (size_t)min(kk + c, n3 - 1) * n3];
+  };
   if (w == 0) {
+    const int p = kk / kCb;
+    if (blockIdx.x == 0 && p > 0) {
+      // the previous panel's diagonal factor, kept aside until no block of
+      // that panel reads the diagonal block (a full block: not the last panel)
+      const double *sp = B.stash + (size_t)((p - 1) & 1) * kCb * kCb;
+      const int kp = kk - kCb;
+#pragma unroll 8
+      for (int c = 0; c < kCb; ++c) B.L[(size_t)(kp + lane) + (size_t)(kp + c) * n3] = sp[c * kCb + lane];
+    }
     double a[kCb];
     {
       // unconditional loads from clamped (in-bounds) addresses, selected after
@@ -219,59 +302,36 @@ __global__ __launch_bounds__(256) void k_chol_panel(BdDev B, int kk, int k0, flo
 #pragma unroll
       for (int c = 0; c < kCb; ++c) a[c] = (lane < kb && c < kb) ? a[c] : (c == lane ? 1.0 : 0.0);
     }
-    // Column c: every lane reads the pivot from the published column, scales
-    // its own entry (t = L[lane][c]) and publishes it; the lanes then update
-    // column c + 1 first and publish it for the next pivot before updating
-    // the rest, so that this store's latency hides behind the updates.
-    int bad = 0;
-    colb[0][lane] = a[0];
-    wave_lds_sync();
+    const int bad = diag_factor(a, colt, Rd, kk, kb);
+    if (blockIdx.x > 0) {
 #pragma unroll
-    for (int c = 0; c < kCb; ++c) {
-      const double piv = colb[c & 1][c];
-      if (c < kb && !(piv > 0.0) && bad == 0) bad = kk + c + 1;
-      const double inv = rsqrt_nr(piv);
-      const double lcc = piv * inv;
-      const double t = a[c] * inv;   // L[lane][c] for lane > c
-      colt[lane] = t;
-      wave_lds_sync();
-      if (c + 1 < kCb) {
-        a[c + 1] = fma(-t, colt[c + 1], a[c + 1]);
-        colb[(c + 1) & 1][lane] = a[c + 1];
-      }
+      for (int c = 0; c < kCb; ++c) Ls[c * kCb + lane] = a[c];
+    } else if (gridDim.x > 1) {
+      // other blocks may still be reading the diagonal block: the factor
+      // waits in the stash for the next panel's launch
+      if (lane == 0 && bad) atomicCAS(B.info, 0, bad);
+      double *sp = B.stash + (size_t)(p & 1) * kCb * kCb;
 #pragma unroll
-      for (int j = c + 2; j < kCb; ++j) a[j] = fma(-t, colt[j], a[j]);
-      a[c] = lane == c ? lcc : (lane > c ? t : 0.0);
-      if (lane == 0) Rd[c] = inv;
-      wave_lds_sync();
-    }
-    if (blockIdx.x == 0) {
+      for (int c = 0; c < kCb; ++c) sp[c * kCb + lane] = a[c];
+    } else {
       if (lane == 0 && bad) atomicCAS(B.info, 0, bad);
       if (kb == kCb) {
 #pragma unroll
-        for (int c = 0; c < kCb; ++c) B.L[(size_t)(kk + lane) + (size_t)(kk + c) * n3] = c <= lane ? a[c] : 0.0;
+        for (int c = 0; c < kCb; ++c) B.L[(size_t)(kk + lane) + (size_t)(kk + c) * n3] = a[c];
       } else if (lane < kb) {
 #pragma unroll
         for (int c = 0; c < kCb; ++c)
-          if (c < kb) B.L[(size_t)(kk + lane) + (size_t)(kk + c) * n3] = c <= lane ? a[c] : 0.0;
+          if (c < kb) B.L[(size_t)(kk + lane) + (size_t)(kk + c) * n3] = a[c];
       }
-    } else {
-#pragma unroll
-      for (int c = 0; c < kCb; ++c) Ls[c * kCb + lane] = a[c];
     }
+  } else if (S == 1 && solver) {
+    load_rows();
   }
   if (blockIdx.x == 0) return;
   __syncthreads();
-  const int r0w = kk + kCb + (blockIdx.x - 1) * kPanelRows + w * kCb;   // first row of the wave
-  if (r0w >= n3) return;
-  const int r = r0w + lane;
+  if (!solver) return;
+  if (S == 0) load_rows();
   const bool live = r < n3;
-  double x[kCb];
-  {
-    const size_t row = (size_t)min(r, n3 - 1);
-#pragma unroll
-    for (int c = 0; c < kCb; ++c) x[c] = B.L[row + (size_t)min(kk + c, n3 - 1) * n3];
-  }
 #pragma unroll
   for (int c = 0; c < kCb; ++c) {
     x[c] *= Rd[c];
@@ -302,10 +362,15 @@ __global__ __launch_bounds__(256) void k_chol_panel(BdDev B, int kk, int k0, flo
 // tri = 1: the lower triangle of tiles from row / column r0 (block b is the
 // tile (ti, tj), tj <= ti, in row order); tri = 0: the column of tiles at
 // column r0, rows r0 + 64 b.  Row r of A is P row r - k0.
+// The products are formed transposed (the tile's columns as the first
+// operand), so that each accumulator register holds 32 consecutive rows of
+// one column: the tile's old values are loaded straight into that layout
+// before the products (their latency hides behind the matrix work) and the
+// results are stored from it, both as 256-byte column pieces.  The next K
+// chunk's loads are issued before the current chunk's products.
 __global__ __launch_bounds__(256) void k_chol_update(BdDev B, const float *P, int mpad, int k0, int r0, int K,
                                                      int tri) {
   __shared__ float Pa[kCb][kCb + 1], Pb[kCb][kCb + 1];
-  __shared__ float C[kCb][kCb + 1];
   const int tid = threadIdx.x, n3 = B.n3;
   const int bidx = blockIdx.x;
   int ti = bidx, tj = 0;
@@ -318,18 +383,28 @@ __global__ __launch_bounds__(256) void k_chol_update(BdDev B, const float *P, in
   const int ra = r0 - k0 + ti * kCb, rb = r0 - k0 + tj * kCb;   // P rows of the tile's rows / columns
   const int w = tid / 64, lane = tid & 63;
   const int qi = w >> 1, qj = w & 1;
+  // accumulator v of this lane: row gr, column gc0 + 32 qj + 8 (v / 4) + 4 (lane / 32) + v % 4
+  const int gr = r0 + ti * kCb + 32 * qi + (lane & 31);
+  const int gcl = r0 + tj * kCb + 32 * qj + 4 * (lane >> 5);
+  double old[16];
+  {
+    const size_t grc = (size_t)min(gr, n3 - 1);   // clamped: in bounds, unconditional
+#pragma unroll
+    for (int v = 0; v < 16; ++v)
+      old[v] = B.L[grc + (size_t)min(gcl + 8 * (v >> 2) + (v & 3), n3 - 1) * n3];
+  }
   floatx16 acc;
   for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
   const int ar = 32 * qi + (lane & 31), br = 32 * qj + (lane & 31), kq = lane >> 5;
-  for (int q0 = 0; q0 < K; q0 += kCb) {
-    float ta[kCb * kCb / 256], tb[kCb * kCb / 256];
+  float ta[kCb * kCb / 256], tb[kCb * kCb / 256];
 #pragma unroll
-    for (int q = 0; q < kCb * kCb / 256; ++q) {
-      const int idx = tid + 256 * q, r = idx % kCb, c = idx / kCb;
-      ta[q] = P[(size_t)(q0 + c) * mpad + ra + r];
-      tb[q] = P[(size_t)(q0 + c) * mpad + rb + r];
-    }
-    if (q0) __syncthreads();
+  for (int q = 0; q < kCb * kCb / 256; ++q) {
+    const int idx = tid + 256 * q, r = idx % kCb, c = idx / kCb;
+    ta[q] = P[(size_t)c * mpad + ra + r];
+    tb[q] = P[(size_t)c * mpad + rb + r];
+  }
+  for (int q0 = 0; q0 < K; q0 += kCb) {
+    if (q0) __syncthreads();   // the last chunk's products have read the LDS tiles
 #pragma unroll
     for (int q = 0; q < kCb * kCb / 256; ++q) {
       const int idx = tid + 256 * q, r = idx % kCb, c = idx / kCb;
@@ -337,32 +412,25 @@ __global__ __launch_bounds__(256) void k_chol_update(BdDev B, const float *P, in
       Pb[r][c] = tb[q];
     }
     __syncthreads();
+    if (q0 + kCb < K) {
+#pragma unroll
+      for (int q = 0; q < kCb * kCb / 256; ++q) {
+        const int idx = tid + 256 * q, r = idx % kCb, c = idx / kCb;
+        ta[q] = P[(size_t)(q0 + kCb + c) * mpad + ra + r];
+        tb[q] = P[(size_t)(q0 + kCb + c) * mpad + rb + r];
+      }
+    }
 #pragma unroll 8
     for (int s2 = 0; s2 < kCb / 2; ++s2)
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(Pa[ar][2 * s2 + kq], Pb[br][2 * s2 + kq], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(Pb[br][2 * s2 + kq], Pa[ar][2 * s2 + kq], acc, 0, 0, 0);
   }
+#pragma unroll
   for (int v = 0; v < 16; ++v) {
-    const int row = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-    C[32 * qi + row][32 * qj + (lane & 31)] = acc[v];
-  }
-  __syncthreads();
-  const int gr0 = r0 + ti * kCb, gc0 = r0 + tj * kCb;
-  double old[kCb * kCb / 256];
-#pragma unroll
-  for (int q = 0; q < kCb * kCb / 256; ++q) {
-    const int idx = tid + 256 * q;
-    const int gr = min(gr0 + idx % kCb, n3 - 1), gc = min(gc0 + idx / kCb, n3 - 1);
-    old[q] = B.L[(size_t)gr + (size_t)gc * n3];   // clamped: in bounds, unconditional
-  }
-#pragma unroll
-  for (int q = 0; q < kCb * kCb / 256; ++q) {
-    const int idx = tid + 256 * q, r = idx % kCb, c = idx / kCb, gr = gr0 + r, gc = gc0 + c;
-    if (gr < n3 && gc < n3 && gr >= gc) B.L[(size_t)gr + (size_t)gc * n3] = old[q] - (double)C[r][c];
+    const int gc = gcl + 8 * (v >> 2) + (v & 3);
+    if (gr < n3 && gc < n3 && gr >= gc) B.L[(size_t)gr + (size_t)gc * n3] = old[v] - (double)acc[v];
   }
 }
 
-// L = the lower triangle of D (diagonal included), zero above: the input of
-// the in-place factorisation.  Block (x, y): rows 256 x .. of column y.
 __global__ __launch_bounds__(256) void k_copy_lower(const double *D, double *L, int n3) {
   const int r = blockIdx.x * 256 + threadIdx.x, c = blockIdx.y;
   if (r < n3) L[(size_t)r + (size_t)c * n3] = r >= c ? D[(size_t)r + (size_t)c * n3] : 0.0;
@@ -371,11 +439,12 @@ __global__ __launch_bounds__(256) void k_copy_lower(const double *D, double *L, 
 // ---------------------------------------------------------------- no-alloc scratch
 // Scratch of tis_bdhi_factor_noalloc: the FP32 copy P of an outer block's
 // panels (256 columns of at most kNoAllocMaxN3 rows) followed by the pivot
-// flag.  Each device holds its own copy until the process ends; every call
+// flag and the stash of two diagonal factors.  Each device holds its own copy until the process ends; every call
 // on a device shares it.
 constexpr int kNoAllocMaxN3 = 9216;
 constexpr size_t kScratchP = (size_t)kNb * kNoAllocMaxN3;
-static __device__ __align__(256) float scratch[kScratchP + 64];
+constexpr size_t kScratchStash = kScratchP + 64;   // two 64x64 FP64 blocks
+static __device__ __align__(256) float scratch[kScratchStash + 4 * kCb * kCb];   // P, info, stash
 
 // ---------------------------------------------------------------- step
 // Partial products over a column chunk: yD = D[:, chunk] F[chunk],
@@ -550,8 +619,9 @@ void factor_launches(const BdDev &d, float *P, hipStream_t st) {
         hipLaunchKernelGGL(k_chol_update, dim3((n3 - kk + kCb - 1) / kCb), dim3(256), 0, st, d,
                            (const float *)P, mpad, k0, kk, j * kCb, 0);
       const int m = n3 - kk - kCb;
-      const int nbp = m > 0 ? (m + kPanelRows - 1) / kPanelRows : 0;
-      hipLaunchKernelGGL(k_chol_panel, dim3(1 + nbp), dim3(256), 0, st, d, kk, k0, P, mpad);
+      const int nbp = m > 0 ? (m + kPanelRowsPerBlock - 1) / kPanelRowsPerBlock : 0;
+      hipLaunchKernelGGL((k_chol_panel<kPanelWaves, kPanelDiagWave>), dim3(1 + nbp),
+                         dim3(kCb * (kPanelWaves + kPanelDiagWave)), 0, st, d, kk, k0, P, mpad);
     }
     const int kt = k0 + kNb;
     if (kt >= n3) break;
@@ -635,6 +705,7 @@ int32_t tis_bdhi_factor_noalloc(int32_t device, const double *D, double *L, int3
   d.L = L;
   d.D = const_cast<double *>(D);
   d.info = (int *)((float *)ptr + kScratchP);
+  d.stash = (double *)((float *)ptr + kScratchStash);
   hipLaunchKernelGGL(k_bd_reset, dim3(1), dim3(64), 0, st, d);
   hipLaunchKernelGGL(k_copy_lower, dim3((n3 + 255) / 256, n3), dim3(256), 0, st, D, L, n3);
   factor_launches(d, (float *)ptr, st);
@@ -702,8 +773,9 @@ int32_t tis_bdhi_setup(tis_ctx *c, const tis_bd_system *s, uint32_t key, uint64_
   if ((rc = balloc(c, b, &d.xi, (size_t)n3))) return rc;
   if ((rc = balloc(c, b, &d.D, (size_t)n3 * n3))) return rc;
   if ((rc = balloc(c, b, &d.L, (size_t)n3 * n3))) return rc;
-  if ((rc = balloc(c, b, &d.info, 1))) return rc;
   const int mt = (int)((n3 + kCb - 1) / kCb);
+  if ((rc = balloc(c, b, &d.info, 1))) return rc;
+  if ((rc = balloc(c, b, &d.stash, (size_t)2 * kCb * kCb))) return rc;
   b->mpad = mt * kCb;
   if ((rc = balloc(c, b, &b->P, (size_t)kNb * b->mpad))) return rc;
   // step products: ~1024+ blocks
