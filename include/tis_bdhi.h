@@ -73,9 +73,10 @@ int32_t tis_bdhi_timing(tis_ctx *ctx, double *total_ms, double *fill_ms, double 
  * reference.  D and L are the caller's device buffers, (n3,n3) column-major
  * FP64; L gets the lower Cholesky factor of D with a zero upper part, from
  * the same kernels (and the same bits) as tis_bdhi_tensors.  The FP32 copy
- * of one outer block's panels (256 columns x n3 rows) and the pivot flag
- * live in a static device array of the library (9.4 MB per device, held
- * until the process ends): n3 <= 9216 (3,072 beads), TIS_ERR_ARG beyond.  Work is queued on `stream` (a hipStream_t;
+ * of one outer block's panels (256 columns x n3 rows), the pivot flag and
+ * two 64x64 diagonal factors awaiting their store live in a static device
+ * array of the library (9.5 MB per device, held until the process ends):
+ * n3 <= 9216 (3,072 beads), TIS_ERR_ARG beyond.  Work is queued on `stream` (a hipStream_t;
  * NULL: the default stream).  With info != NULL the call waits for the
  * stream and stores the first failing pivot + 1 (0: none; TIS_ERR_STATE is
  * then returned); with info == NULL it returns at once.  Every call on a
