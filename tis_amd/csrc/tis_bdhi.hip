@@ -265,6 +265,54 @@ constexpr int kPanelWaves = 1;                      // solving waves per panel b
 constexpr int kPanelDiagWave = 1;                   // 1: a wave of its own for the diagonal block
 constexpr int kPanelRowsPerBlock = kCb * kPanelWaves;  // rows of the panel solved per block
 
+// The value of v at lane n of each row of 16 lanes, to the whole row
+// (DPP row_newbcast, 64-bit).
+template <int N>
+__device__ __forceinline__ double row_bcast(double v) {
+  return __longlong_as_double(__builtin_amdgcn_mov_dpp(__double_as_longlong(v), 0x150 + N, 0xf, 0xf, true));
+}
+
+// acc += (the value of r at lane n of each row of 16 lanes) * nx, as one
+// v_fmac_f64 with a DPP row_newbcast operand (the same rounding as fma).
+// The memory clobber keeps loads where the source puts them (ahead of use).
+template <int N>
+__device__ __forceinline__ void fmac_row_bcast(double &acc, double r, double nx) {
+  asm("v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+      : "+v"(acc)
+      : "v"(r), "v"(nx), "i"(N)
+      : "memory");
+}
+
+// Forward substitution of one row per lane, x <- x L11^-T, column by column.
+// Column C's factor entries reach the lanes as four registers, R[k] holding
+// L11[16 k + (lane % 16)][C] (the same 16 values in each row of 16 lanes),
+// from which row_newbcast takes entry J for every lane: four LDS reads per
+// column instead of one broadcast read per entry.
+template <int J>
+__device__ __forceinline__ void solve_rest(double (&x)[kCb], const double (&R)[4], double nx) {
+  if constexpr (J < kCb) {
+    fmac_row_bcast<J & 15>(x[J], R[J >> 4], nx);
+    solve_rest<J + 1>(x, R, nx);
+  }
+}
+template <int C>
+__device__ __forceinline__ void solve_cols(double (&x)[kCb], const double *Ls, const double *Rd, int lane,
+                                           const double (&R)[4], double rd) {
+  if constexpr (C < kCb) {
+    // the next column's entries are loaded a column ahead (the fmacs keep
+    // the compiler from moving the loads down to their use)
+    double Rn[4], rdn = 0.0;
+    if constexpr (C + 1 < kCb) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) Rn[k] = Ls[(C + 1) * kCb + 16 * k + (lane & 15)];
+      rdn = Rd[C + 1];
+    }
+    x[C] *= rd;
+    solve_rest<C + 1>(x, R, -x[C]);
+    solve_cols<C + 1>(x, Ls, Rd, lane, Rn, rdn);
+  }
+}
+
 // S = 1: wave 0 only factors the diagonal block and the W solving waves
 // load their rows meanwhile; S = 0: wave 0 factors, then solves too.
 template <int W, int S>
@@ -332,11 +380,11 @@ __global__ __launch_bounds__(64 * (W + S)) void k_chol_panel(BdDev B, int kk, in
   if (!solver) return;
   if (S == 0) load_rows();
   const bool live = r < n3;
+  {
+    double R0[4];
 #pragma unroll
-  for (int c = 0; c < kCb; ++c) {
-    x[c] *= Rd[c];
-#pragma unroll
-    for (int j = c + 1; j < kCb; ++j) x[j] = fma(-x[c], Ls[c * kCb + j], x[j]);
+    for (int k = 0; k < 4; ++k) R0[k] = Ls[16 * k + (lane & 15)];
+    solve_cols<0>(x, Ls, Rd, lane, R0, Rd[0]);
   }
   const int rl = r - k0;   // row of P
   const int q0 = kk - k0;  // first column of P
