@@ -209,6 +209,28 @@ __device__ inline double rsqrt_nr(double x) {
   return y;
 }
 
+// acc += (the value of r at lane n of each row of 16 lanes) * nx, as one
+// v_fmac_f64 with a DPP row_newbcast operand (the same rounding as fma).
+// The memory clobber keeps loads where the source puts them (ahead of use).
+template <int N>
+__device__ __forceinline__ void fmac_row_bcast(double &acc, double r, double nx) {
+  asm("v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+      : "+v"(acc)
+      : "v"(r), "v"(nx), "i"(N)
+      : "memory");
+}
+
+// x[j] += v_j nx for j = J .. 63, where the column v reaches the lanes as
+// four registers, R[k] holding v[16 k + (lane % 16)] (the same 16 values in
+// each row of 16 lanes), from which row_newbcast takes entry j for every
+// lane: four LDS reads per column instead of one broadcast read per entry.
+template <int J>
+__device__ __forceinline__ void bcast_update(double (&x)[kCb], const double (&R)[4], double nx) {
+  if constexpr (J < kCb) {
+    fmac_row_bcast<J & 15>(x[J], R[J >> 4], nx);
+    bcast_update<J + 1>(x, R, nx);
+  }
+}
 // Factors the 64x64 diagonal block held by one wave, lane r holding row r in
 // a[] (rows past kb are unit rows), as L D L^T, right-looking: column c's
 // pivot d = A[c][c] and the entry A[c+1][c] come from lanes c and c + 1 by
@@ -220,30 +242,34 @@ __device__ inline double rsqrt_nr(double x) {
 // L[r][c] = A[r][c] / sqrt(d), L[c][c] = sqrt(d), zero above the diagonal;
 // Rd[c] = 1 / L[c][c].  Returns kk + c + 1 for the first column c < kb with
 // a pivot that is not positive, else 0.
-__device__ inline int diag_factor(double (&a)[kCb], double (*colt)[kCb], double *Rd, int kk, int kb) {
-  const int lane = threadIdx.x & (kCb - 1);
-  int bad = 0;
-  double sp = 0.0;   // A[lane][c - 1] / d_{c - 1}
+template <int C>
+__device__ __forceinline__ void diag_step(double (&a)[kCb], double (*colt)[kCb], double *Rd, int kk, int kb,
+                                          int lane, int &bad, double sp) {
+  if constexpr (C < kCb) {
+    double R[4];   // column C - 1 as published (sp = A[lane][C - 1] / d_{C - 1})
+    if constexpr (C >= 1) {
 #pragma unroll
-  for (int c = 0; c < kCb; ++c) {
-    // column c - 1's update of column c + 1 first: column c + 1 is next on the chain
-    if (c >= 1 && c + 1 < kCb) a[c + 1] = fma(-sp, colt[(c - 1) & 1][c + 1], a[c + 1]);
-    const double d = lane_value(a[c], c);
-    if (c < kb && !(d > 0.0) && bad == 0) bad = kk + c + 1;
-    const double u = a[c];
-    colt[c & 1][lane] = u;
-    const double s = u * rcp_nr(d);
-    if (c + 1 < kCb) a[c + 1] = fma(-s, lane_value(u, c + 1), a[c + 1]);
-    if (c >= 1) {
-#pragma unroll
-      for (int j = c + 2; j < kCb; ++j) a[j] = fma(-sp, colt[(c - 1) & 1][j], a[j]);
+      for (int k = 0; k < 4; ++k) R[k] = colt[(C - 1) & 1][16 * k + (lane & 15)];
+      // column C - 1's update of column C + 1 first: column C + 1 is next on the chain
+      if constexpr (C + 1 < kCb) fmac_row_bcast<(C + 1) & 15>(a[C + 1], R[(C + 1) >> 4], -sp);
     }
+    const double d = lane_value(a[C], C);
+    if (C < kb && !(d > 0.0) && bad == 0) bad = kk + C + 1;
+    const double u = a[C];
+    colt[C & 1][lane] = u;
+    const double s = u * rcp_nr(d);
+    if constexpr (C + 1 < kCb) a[C + 1] = fma(-s, lane_value(u, C + 1), a[C + 1]);
+    if constexpr (C >= 1) bcast_update<C + 2>(a, R, -sp);
     const double rs = rsqrt_nr(d);
-    a[c] = lane == c ? d * rs : (lane > c ? u * rs : 0.0);
-    if (lane == 0) Rd[c] = rs;
-    sp = s;
+    a[C] = lane == C ? d * rs : (lane > C ? u * rs : 0.0);
+    if (lane == 0) Rd[C] = rs;
     wave_lds_sync();
+    diag_step<C + 1>(a, colt, Rd, kk, kb, lane, bad, s);
   }
+}
+__device__ inline int diag_factor(double (&a)[kCb], double (*colt)[kCb], double *Rd, int kk, int kb) {
+  int bad = 0;
+  diag_step<0>(a, colt, Rd, kk, kb, threadIdx.x & (kCb - 1), bad, 0.0);
   return bad;
 }
 
@@ -265,36 +291,6 @@ constexpr int kPanelWaves = 1;                      // solving waves per panel b
 constexpr int kPanelDiagWave = 1;                   // 1: a wave of its own for the diagonal block
 constexpr int kPanelRowsPerBlock = kCb * kPanelWaves;  // rows of the panel solved per block
 
-// The value of v at lane n of each row of 16 lanes, to the whole row
-// (DPP row_newbcast, 64-bit).
-template <int N>
-__device__ __forceinline__ double row_bcast(double v) {
-  return __longlong_as_double(__builtin_amdgcn_mov_dpp(__double_as_longlong(v), 0x150 + N, 0xf, 0xf, true));
-}
-
-// acc += (the value of r at lane n of each row of 16 lanes) * nx, as one
-// v_fmac_f64 with a DPP row_newbcast operand (the same rounding as fma).
-// The memory clobber keeps loads where the source puts them (ahead of use).
-template <int N>
-__device__ __forceinline__ void fmac_row_bcast(double &acc, double r, double nx) {
-  asm("v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf bound_ctrl:1"
-      : "+v"(acc)
-      : "v"(r), "v"(nx), "i"(N)
-      : "memory");
-}
-
-// Forward substitution of one row per lane, x <- x L11^-T, column by column.
-// Column C's factor entries reach the lanes as four registers, R[k] holding
-// L11[16 k + (lane % 16)][C] (the same 16 values in each row of 16 lanes),
-// from which row_newbcast takes entry J for every lane: four LDS reads per
-// column instead of one broadcast read per entry.
-template <int J>
-__device__ __forceinline__ void solve_rest(double (&x)[kCb], const double (&R)[4], double nx) {
-  if constexpr (J < kCb) {
-    fmac_row_bcast<J & 15>(x[J], R[J >> 4], nx);
-    solve_rest<J + 1>(x, R, nx);
-  }
-}
 template <int C>
 __device__ __forceinline__ void solve_cols(double (&x)[kCb], const double *Ls, const double *Rd, int lane,
                                            const double (&R)[4], double rd) {
@@ -308,7 +304,7 @@ __device__ __forceinline__ void solve_cols(double (&x)[kCb], const double *Ls, c
       rdn = Rd[C + 1];
     }
     x[C] *= rd;
-    solve_rest<C + 1>(x, R, -x[C]);
+    bcast_update<C + 1>(x, R, -x[C]);
     solve_cols<C + 1>(x, Ls, Rd, lane, Rn, rdn);
   }
 }
