@@ -165,11 +165,11 @@ __global__ void k_bd_reset(BdDev B) {
 // ---------------------------------------------------------------- Cholesky
 // Blocked right-looking factorisation of L (holding the lower part of D) in
 // place, in outer blocks of kNb = 256 columns made of four 64-column
-// panels.  Panel j of the block starting at column k0: its columns are
-// first brought up to date with the block's earlier panels (k_chol_update,
-// K = 64 j, FP32 matrix cores), then factored (k_chol_panel: the 64x64
-// diagonal block and the triangular solve of the rows below, FP64), its
-// FP32 copy going to columns [64 j, 64 j + 64) of P (P[q * mpad + r - k0]).
+// panels.  Panel j of the block starting at column k0 is one launch of
+// k_chol_panel: its columns are brought up to date with the block's earlier
+// panels (K = 64 j, FP32 matrix cores), then factored (the 64x64 diagonal
+// block and the triangular solve of the rows below, FP64), its FP32 copy
+// going to columns [64 j, 64 j + 64) of P (P[q * mpad + r - k0]).
 // After the block's last panel the trailing lower triangle is updated with
 // all 256 columns at once (k_chol_update, K = 256), so each trailing element
 // is read and written once per 256 columns instead of once per 64.
@@ -273,24 +273,6 @@ __device__ inline int diag_factor(double (&a)[kCb], double (*colt)[kCb], double 
   return bad;
 }
 
-// One panel at column kk of the outer block at k0.  Wave 0 of every block
-// factors the 64x64 diagonal block (diag_factor).  Block 0 keeps the factor
-// in a stash (two buffers, by panel parity) and the next panel's block 0
-// stores it over the diagonal block, so no block of this panel can read a
-// partly overwritten block; a panel of one block (the last) stores its own
-// factor, and the kernels between two panels do not touch the diagonal
-// blocks of earlier panels.  In blocks 1.. the factor goes to LDS
-// and each solving wave solves 64 rows of the panel below by forward
-// substitution (row in registers, the factor's entries broadcast from LDS),
-// storing the rows and their FP32 copy in P.  Full tiles take branch-free
-// stores.  One solving wave per block keeps the LDS broadcasts of the solve
-// from queueing behind each other (a CU per block: the rows below the
-// diagonal block are at most n3 - 64), and with a wave of its own for the
-// diagonal block the solving wave's loads overlap the factorisation.
-constexpr int kPanelWaves = 1;                      // solving waves per panel block
-constexpr int kPanelDiagWave = 1;                   // 1: a wave of its own for the diagonal block
-constexpr int kPanelRowsPerBlock = kCb * kPanelWaves;  // rows of the panel solved per block
-
 template <int C>
 __device__ __forceinline__ void solve_cols(double (&x)[kCb], const double *Ls, const double *Rd, int lane,
                                            const double (&R)[4], double rd) {
@@ -309,92 +291,174 @@ __device__ __forceinline__ void solve_cols(double (&x)[kCb], const double *Ls, c
   }
 }
 
-// S = 1: wave 0 only factors the diagonal block and the W solving waves
-// load their rows meanwhile; S = 0: wave 0 factors, then solves too.
-template <int W, int S>
-__global__ __launch_bounds__(64 * (W + S)) void k_chol_panel(BdDev B, int kk, int k0, float *P, int mpad) {
-  __shared__ double Ls[kCb * kCb];   // Ls[c * 64 + r] = L11[r][c]
+// Sum over k < K of P[k][ra + i] P[k][rb + j] for the 32x32 quadrant i, j <
+// 32 (P column k at P + k * mpad), in the accumulator layout of
+// v_mfma_f32_32x32x2f32 with the columns' entries as the first operand: the
+// operands and the order of the sums are those of k_chol_update.  Operands
+// come straight from memory (L2), eight k pairs ahead.
+__device__ inline floatx16 quad_product(const float *P, int mpad, int ra, int rb, int K, int lane) {
+  floatx16 acc;
+  for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+  constexpr int kU = 8;   // k pairs per batch of loads
+  const int l = lane & 31;
+  const float *pa = P + (size_t)(lane >> 5) * mpad + ra + l;
+  const float *pb = P + (size_t)(lane >> 5) * mpad + rb + l;
+  float fa[kU], fb[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    fa[u] = pa[(size_t)(2 * u) * mpad];
+    fb[u] = pb[(size_t)(2 * u) * mpad];
+  }
+  for (int s0 = 0; s0 < K / 2; s0 += kU) {
+    float na[kU], nb[kU];
+    const bool more = s0 + kU < K / 2;
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const size_t off = (size_t)(2 * (more ? s0 + kU + u : u)) * mpad;
+      na[u] = pa[off];
+      nb[u] = pb[off];
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(fb[u], fa[u], acc, 0, 0, 0);
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      fa[u] = na[u];
+      fb[u] = nb[u];
+    }
+  }
+  return acc;
+}
+
+// One panel at column kk of the outer block at k0, in blocks of four waves;
+// block b > 0 owns the 64 rows from kk + 64 b.  First the panel's columns
+// take the update by the outer block's earlier panels (K = kk - k0 columns
+// of P, FP32 matrix cores, quad_product; the same bits as with a separate
+// k_chol_update launch): the four waves form the product for the 64x64
+// diagonal block, a quadrant each, which passes through LDS into the
+// row-per-lane layout and is subtracted from the FP64 values that wave 0
+// loaded at the start; wave 0 then factors the diagonal block (diag_factor)
+// while waves 2-3 form the product for the block's rows, which wave 2 (its
+// rows loaded at the start) subtracts before its solve.
+// Block 0 keeps the factor in a stash (two buffers, by panel parity) and the
+// next panel's block 0 stores it over the diagonal block, so no block of
+// this panel can read a partly overwritten block; a panel of one block (the
+// last) stores its own factor, and the kernels between two panels do not
+// touch the diagonal blocks of earlier panels.  In blocks 1.. the factor
+// goes to LDS and wave 2 solves the block's rows by forward substitution
+// (row per lane; see solve_cols), storing the rows and their FP32 copy in P.
+// Full tiles take branch-free stores.  (Four waves leave the diagonal
+// factorisation the registers of one wave per SIMD.)
+constexpr int kPanelThreads = 256;
+constexpr int kPanelRowsPerBlock = kCb;   // rows of the panel solved per block
+
+__global__ __launch_bounds__(kPanelThreads) void k_chol_panel(BdDev B, int kk, int k0, float *P, int mpad) {
+  // LDS: the diagonal block's product (64 x 65 floats), later the diagonal
+  // factor Ls[c * 64 + r] = L11[r][c]; the product for the block's rows
+  __shared__ __align__(16) double smem[kCb * kCb];
+  __shared__ float Tr[kCb][kCb + 1];
   __shared__ double Rd[kCb];         // 1 / L11[c][c]
   __shared__ double colt[2][kCb];    // the last two columns, as published
+  static_assert(sizeof(smem) >= kCb * (kCb + 1) * sizeof(float), "the product does not fit");
+  double *Ls = smem;
+  float (*Td)[kCb + 1] = reinterpret_cast<float (*)[kCb + 1]>(smem);
   const int lane = threadIdx.x & (kCb - 1), w = threadIdx.x / kCb, n3 = B.n3;
   const int kb = min(kCb, n3 - kk);
-  const int r0w = kk + kCb + ((int)blockIdx.x - 1) * (kCb * W) + (w - S) * kCb;   // first row of the wave
-  const bool solver = blockIdx.x > 0 && w >= S && r0w < n3;
+  const int K = kk - k0;
+  const int r0w = kk + kCb * (int)blockIdx.x;   // the block's first row (blocks 1..)
+  const bool rows = blockIdx.x > 0 && r0w < n3;
   const int r = r0w + lane;
-  double x[kCb];
-  const auto load_rows = [&]() {
-    const size_t row = (size_t)min(r, n3 - 1);
+  const int p = kk / kCb;
+  // wave 0: the diagonal block's rows; wave 2: the block's rows (unconditional
+  // loads from clamped, in-bounds addresses)
+  double v[kCb];
+  if (w == 0 || w == 2) {
+    const size_t row = (size_t)min(w == 0 ? kk + lane : r, n3 - 1);
 #pragma unroll
-    for (int c = 0; c < kCb; ++c) x[c] = B.L[row + (size_t)min(kk + c, n3 - 1) * n3];
-  };
-  if (w == 0) {
-    const int p = kk / kCb;
-    if (blockIdx.x == 0 && p > 0) {
-      // the previous panel's diagonal factor, kept aside until no block of
-      // that panel reads the diagonal block (a full block: not the last panel)
-      const double *sp = B.stash + (size_t)((p - 1) & 1) * kCb * kCb;
-      const int kp = kk - kCb;
+    for (int c = 0; c < kCb; ++c) v[c] = B.L[row + (size_t)min(kk + c, n3 - 1) * n3];
+  }
+  if (w == 0 && blockIdx.x == 0 && p > 0) {
+    // the previous panel's diagonal factor, kept aside until no block of
+    // that panel reads the diagonal block (a full block: not the last panel)
+    const double *sp = B.stash + (size_t)((p - 1) & 1) * kCb * kCb;
+    const int kp = kk - kCb;
 #pragma unroll 8
-      for (int c = 0; c < kCb; ++c) B.L[(size_t)(kp + lane) + (size_t)(kp + c) * n3] = sp[c * kCb + lane];
-    }
-    double a[kCb];
-    {
-      // unconditional loads from clamped (in-bounds) addresses, selected after
-      const size_t row = (size_t)min(kk + lane, n3 - 1);
+    for (int c = 0; c < kCb; ++c) B.L[(size_t)(kp + lane) + (size_t)(kp + c) * n3] = sp[c * kCb + lane];
+  }
+  const int l32 = lane & 31, c32 = 4 * (lane >> 5);   // accumulator e: row l32, column c32 + 8 (e / 4) + e % 4
+  if (K > 0) {
+    const int qi = w >> 1, qj = w & 1;
+    const floatx16 acc = quad_product(P, mpad, kk - k0 + 32 * qi, kk - k0 + 32 * qj, K, lane);
 #pragma unroll
-      for (int c = 0; c < kCb; ++c) a[c] = B.L[row + (size_t)min(kk + c, n3 - 1) * n3];
+    for (int e = 0; e < 16; ++e) Td[32 * qi + l32][32 * qj + c32 + 8 * (e >> 2) + (e & 3)] = acc[e];
+    __syncthreads();
+    if (w == 0) {
 #pragma unroll
-      for (int c = 0; c < kCb; ++c) a[c] = (lane < kb && c < kb) ? a[c] : (c == lane ? 1.0 : 0.0);
+      for (int c = 0; c < kCb; ++c) v[c] -= (double)Td[lane][c];
     }
-    const int bad = diag_factor(a, colt, Rd, kk, kb);
+  }
+  if (w == 0) {
+#pragma unroll
+    for (int c = 0; c < kCb; ++c) v[c] = (lane < kb && c < kb) ? v[c] : (c == lane ? 1.0 : 0.0);
+    const int bad = diag_factor(v, colt, Rd, kk, kb);
     if (blockIdx.x > 0) {
 #pragma unroll
-      for (int c = 0; c < kCb; ++c) Ls[c * kCb + lane] = a[c];
+      for (int c = 0; c < kCb; ++c) Ls[c * kCb + lane] = v[c];
     } else if (gridDim.x > 1) {
       // other blocks may still be reading the diagonal block: the factor
       // waits in the stash for the next panel's launch
       if (lane == 0 && bad) atomicCAS(B.info, 0, bad);
       double *sp = B.stash + (size_t)(p & 1) * kCb * kCb;
 #pragma unroll
-      for (int c = 0; c < kCb; ++c) sp[c * kCb + lane] = a[c];
+      for (int c = 0; c < kCb; ++c) sp[c * kCb + lane] = v[c];
     } else {
       if (lane == 0 && bad) atomicCAS(B.info, 0, bad);
       if (kb == kCb) {
 #pragma unroll
-        for (int c = 0; c < kCb; ++c) B.L[(size_t)(kk + lane) + (size_t)(kk + c) * n3] = a[c];
+        for (int c = 0; c < kCb; ++c) B.L[(size_t)(kk + lane) + (size_t)(kk + c) * n3] = v[c];
       } else if (lane < kb) {
 #pragma unroll
         for (int c = 0; c < kCb; ++c)
-          if (c < kb) B.L[(size_t)(kk + lane) + (size_t)(kk + c) * n3] = a[c];
+          if (c < kb) B.L[(size_t)(kk + lane) + (size_t)(kk + c) * n3] = v[c];
       }
     }
-  } else if (S == 1 && solver) {
-    load_rows();
   }
   if (blockIdx.x == 0) return;
+  if (K > 0 && w >= 2 && rows) {
+    // the product for the block's rows, beside the diagonal factorisation
+    const int qi = w & 1;
+#pragma unroll
+    for (int qj = 0; qj < 2; ++qj) {
+      const floatx16 acc = quad_product(P, mpad, r0w - k0 + 32 * qi, kk - k0 + 32 * qj, K, lane);
+#pragma unroll
+      for (int e = 0; e < 16; ++e) Tr[32 * qi + l32][32 * qj + c32 + 8 * (e >> 2) + (e & 3)] = acc[e];
+    }
+  }
   __syncthreads();
-  if (!solver) return;
-  if (S == 0) load_rows();
+  if (w != 2 || !rows) return;
+  if (K > 0) {
+#pragma unroll
+    for (int c = 0; c < kCb; ++c) v[c] -= (double)Tr[lane][c];
+  }
   const bool live = r < n3;
   {
     double R0[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) R0[k] = Ls[16 * k + (lane & 15)];
-    solve_cols<0>(x, Ls, Rd, lane, R0, Rd[0]);
+    solve_cols<0>(v, Ls, Rd, lane, R0, Rd[0]);
   }
   const int rl = r - k0;   // row of P
   const int q0 = kk - k0;  // first column of P
   if (kb == kCb && r0w + kCb <= n3) {
 #pragma unroll
     for (int c = 0; c < kCb; ++c) {
-      B.L[(size_t)r + (size_t)(kk + c) * n3] = x[c];
-      P[(size_t)(q0 + c) * mpad + rl] = (float)x[c];
+      B.L[(size_t)r + (size_t)(kk + c) * n3] = v[c];
+      P[(size_t)(q0 + c) * mpad + rl] = (float)v[c];
     }
   } else if (rl < mpad) {
 #pragma unroll
     for (int c = 0; c < kCb; ++c) {
-      if (live && c < kb) B.L[(size_t)r + (size_t)(kk + c) * n3] = x[c];
-      P[(size_t)(q0 + c) * mpad + rl] = (live && c < kb) ? (float)x[c] : 0.0f;
+      if (live && c < kb) B.L[(size_t)r + (size_t)(kk + c) * n3] = v[c];
+      P[(size_t)(q0 + c) * mpad + rl] = (live && c < kb) ? (float)v[c] : 0.0f;
     }
   }
 }
@@ -403,27 +467,22 @@ __global__ __launch_bounds__(64 * (W + S)) void k_chol_panel(BdDev B, int kk, in
 // (v_mfma_f32_32x32x2f32, each of four waves a 32x32 quadrant), the sum
 // over K columns of P in chunks of 64 staged through LDS, subtracted from
 // the FP64 matrix (entries above the diagonal are not touched).
-// tri = 1: the lower triangle of tiles from row / column r0 (block b is the
-// tile (ti, tj), tj <= ti, in row order); tri = 0: the column of tiles at
-// column r0, rows r0 + 64 b.  Row r of A is P row r - k0.
+// The tiles are the lower triangle of tiles from row / column r0 (block b is
+// the tile (ti, tj), tj <= ti, in row order).  Row r of A is P row r - k0.
 // The products are formed transposed (the tile's columns as the first
 // operand), so that each accumulator register holds 32 consecutive rows of
 // one column: the tile's old values are loaded straight into that layout
 // before the products (their latency hides behind the matrix work) and the
 // results are stored from it, both as 256-byte column pieces.  The next K
 // chunk's loads are issued before the current chunk's products.
-__global__ __launch_bounds__(256) void k_chol_update(BdDev B, const float *P, int mpad, int k0, int r0, int K,
-                                                     int tri) {
+__global__ __launch_bounds__(256) void k_chol_update(BdDev B, const float *P, int mpad, int k0, int r0, int K) {
   __shared__ float Pa[kCb][kCb + 1], Pb[kCb][kCb + 1];
   const int tid = threadIdx.x, n3 = B.n3;
   const int bidx = blockIdx.x;
-  int ti = bidx, tj = 0;
-  if (tri) {
-    ti = (int)((sqrt(8.0 * (double)bidx + 1.0) - 1.0) * 0.5);
-    while ((ti + 1) * (ti + 2) / 2 <= bidx) ++ti;
-    while (ti * (ti + 1) / 2 > bidx) --ti;
-    tj = bidx - ti * (ti + 1) / 2;
-  }
+  int ti = (int)((sqrt(8.0 * (double)bidx + 1.0) - 1.0) * 0.5);
+  while ((ti + 1) * (ti + 2) / 2 <= bidx) ++ti;
+  while (ti * (ti + 1) / 2 > bidx) --ti;
+  const int tj = bidx - ti * (ti + 1) / 2;
   const int ra = r0 - k0 + ti * kCb, rb = r0 - k0 + tj * kCb;   // P rows of the tile's rows / columns
   const int w = tid / 64, lane = tid & 63;
   const int qi = w >> 1, qj = w & 1;
@@ -649,29 +708,25 @@ void launch_fill(tis_ctx *c, BdState *b) {
   hipLaunchKernelGGL(k_rpy_fill, dim3(gb, gb), dim3(16, 16), 0, st, d);
 }
 
-// The factorisation's launches, outer block by outer block: for each of its
-// panels the update of the panel's columns by the block's earlier panels,
-// then the panel; then the trailing update with the whole block.  P holds
-// kNb columns of at most n3 rows (rows from the block's start).
+// The factorisation's launches, outer block by outer block: its panels
+// (each first updating its columns with the block's earlier panels), then
+// the trailing update with the whole block.  P holds kNb columns of at most
+// n3 rows (rows from the block's start).
 void factor_launches(const BdDev &d, float *P, hipStream_t st) {
   const int n3 = d.n3;
   for (int k0 = 0; k0 < n3; k0 += kNb) {
     const int mpad = (n3 - k0 + kCb - 1) / kCb * kCb;
     for (int j = 0; j < kNb / kCb && k0 + j * kCb < n3; ++j) {
       const int kk = k0 + j * kCb;
-      if (j > 0)
-        hipLaunchKernelGGL(k_chol_update, dim3((n3 - kk + kCb - 1) / kCb), dim3(256), 0, st, d,
-                           (const float *)P, mpad, k0, kk, j * kCb, 0);
       const int m = n3 - kk - kCb;
       const int nbp = m > 0 ? (m + kPanelRowsPerBlock - 1) / kPanelRowsPerBlock : 0;
-      hipLaunchKernelGGL((k_chol_panel<kPanelWaves, kPanelDiagWave>), dim3(1 + nbp),
-                         dim3(kCb * (kPanelWaves + kPanelDiagWave)), 0, st, d, kk, k0, P, mpad);
+      hipLaunchKernelGGL(k_chol_panel, dim3(1 + nbp), dim3(kPanelThreads), 0, st, d, kk, k0, P, mpad);
     }
     const int kt = k0 + kNb;
     if (kt >= n3) break;
     const int mt = (n3 - kt + kCb - 1) / kCb;
     hipLaunchKernelGGL(k_chol_update, dim3(mt * (mt + 1) / 2), dim3(256), 0, st, d, (const float *)P, mpad, k0, kt,
-                       kNb, 1);
+                       kNb);
   }
 }
 
