@@ -240,6 +240,12 @@ __device__ inline double uniform01(uint64_t seed, uint32_t sys, uint64_t step, u
 __device__ inline double sq3_exact(double a, double b, double c) {
   return __dadd_rn(__dadd_rn(__dmul_rn(a, a), __dmul_rn(b, b)), __dmul_rn(c, c));
 }
+// (a^2 + b^2) + c^2 with each product and sum rounded on its own (no fused
+// multiply-add), the reference's evaluation of |v|^2
+__device__ inline double sq3_separate(double a, double b, double c) {
+#pragma clang fp contract(off)
+  return (a * a + b * b) + c * c;
+}
 
 // util_pbneighbor.F90:22-35; returns mirror code 9*i1+3*i2+i3 (0-based)
 __device__ inline int pbneighbor(const DSys &s, double v[3]) {

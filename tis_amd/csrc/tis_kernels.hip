@@ -1353,15 +1353,33 @@ __device__ inline void list_exv_tile_t(const DevPtrs &P, const DSys &s, const do
           if (!__any(need)) continue;
         }
         const int q1 = min(q0 + 16, kend);
-        if (valid)
-          for (int k = q0; k < q1; ++k) {
-            uint32_t e;
-            if (!exv_candidate<RL2_LDS>(P, s, xi, li, ui, ti, ci, jt + k, sx[k], sy[k], sz[k], su[k], stc[k], L.rl2,
-                                        e))
-              continue;
-            if (cnt < P.nb_cap) P.nb[(size_t)cnt * n + g] = e;
-            ++cnt;
+        if (valid) {
+          // exv_candidate for a non-periodic system, every test evaluated
+          // and combined (no branch per test), the group's staged values
+          // read together
+#pragma unroll
+          for (int dk = 0; dk < 16; ++dk) {
+            const int k = q0 + dk;
+            if (k >= q1) continue;   // (the same for every lane)
+            const int lj2 = jt + k;
+            const double d2 = sq3_separate(sx[k] - xi[0], sy[k] - xi[1], sz[k] - xi[2]);
+            const bool own_lo = li < lj2;
+            const int uj = su[k], tcj = stc[k];
+            const int ulo = own_lo ? ui : uj, uhi = own_lo ? uj : ui;
+            const double r2 = RL2_LDS ? L.rl2[ulo * s.nunit + uhi] : P.rl2[s.uoff + ulo * s.nunit + uhi];
+            const int tlo = own_lo ? ti : (tcj & 0xFF), clo = own_lo ? ci : (tcj >> 8);
+            const int lo = own_lo ? li : lj2, hi = own_lo ? lj2 : li;
+            int sep = -1;   // exv_keep's separation by the lower bead's type, as selects
+            sep = tlo == 2 ? s.nsep[0] : sep;
+            sep = tlo == 4 ? s.nsep[1] : sep;
+            sep = tlo == 3 ? s.nsep[2] : sep;
+            const bool keep = (ulo != uhi) | (clo != 2) | ((sep >= 0) & (hi >= lo + sep));
+            if ((lj2 != li) & (d2 < s.rl2max) & (d2 < r2) & keep) {
+              if (cnt < P.nb_cap) P.nb[(size_t)cnt * n + g] = (uint32_t)(s.off + lj2);
+              ++cnt;
+            }
           }
+        }
       }
       __syncthreads();
     }
@@ -1373,7 +1391,8 @@ __device__ inline void list_exv_tile_t(const DevPtrs &P, const DSys &s, const do
 }
 
 // the unit-pair radii from LDS when the table fits there (a compile-time
-// choice: a run-time one makes every radius read a generic-pointer load)
+// choice: a run-time one makes every radius read a generic-pointer load).
+// Non-periodic systems only (k_lpair builds the periodic ones).
 __device__ inline void list_exv_tile(const DevPtrs &P, const DSys &s, const double *X, const double *PX,
                                      const Tile tl, int it, ListLds &L) {
   if (s.nunit <= kRl2Lds) list_exv_tile_t<true>(P, s, X, PX, tl, it, L);
