@@ -1310,7 +1310,7 @@ __device__ inline bool exv_candidate(const DevPtrs &P, const DSys &s, const doub
   return true;
 }
 
-template <bool RL2_LDS>
+template <bool RL2_LDS, bool ONE_UNIT>
 __device__ inline void list_exv_tile_t(const DevPtrs &P, const DSys &s, const double *X, const double *PX,
                                        const Tile tl, int it, ListLds &L) {
   auto &sx = L.sx; auto &sy = L.sy; auto &sz = L.sz; auto &su = L.su; auto &stc = L.stc;
@@ -1364,9 +1364,11 @@ __device__ inline void list_exv_tile_t(const DevPtrs &P, const DSys &s, const do
             const int lj2 = jt + k;
             const double d2 = sq3_separate(sx[k] - xi[0], sy[k] - xi[1], sz[k] - xi[2]);
             const bool own_lo = li < lj2;
-            const int uj = su[k], tcj = stc[k];
+            // one unit: its radius is rl2max and both beads are in it
+            const int uj = ONE_UNIT ? ui : su[k], tcj = stc[k];
             const int ulo = own_lo ? ui : uj, uhi = own_lo ? uj : ui;
-            const double r2 = RL2_LDS ? L.rl2[ulo * s.nunit + uhi] : P.rl2[s.uoff + ulo * s.nunit + uhi];
+            const double r2 = ONE_UNIT ? s.rl2max
+                                       : (RL2_LDS ? L.rl2[ulo * s.nunit + uhi] : P.rl2[s.uoff + ulo * s.nunit + uhi]);
             const int tlo = own_lo ? ti : (tcj & 0xFF), clo = own_lo ? ci : (tcj >> 8);
             const int lo = own_lo ? li : lj2, hi = own_lo ? lj2 : li;
             int sep = -1;   // exv_keep's separation by the lower bead's type, as selects
@@ -1395,8 +1397,9 @@ __device__ inline void list_exv_tile_t(const DevPtrs &P, const DSys &s, const do
 // Non-periodic systems only (k_lpair builds the periodic ones).
 __device__ inline void list_exv_tile(const DevPtrs &P, const DSys &s, const double *X, const double *PX,
                                      const Tile tl, int it, ListLds &L) {
-  if (s.nunit <= kRl2Lds) list_exv_tile_t<true>(P, s, X, PX, tl, it, L);
-  else list_exv_tile_t<false>(P, s, X, PX, tl, it, L);
+  if (s.nunit == 1) list_exv_tile_t<true, true>(P, s, X, PX, tl, it, L);
+  else if (s.nunit <= kRl2Lds) list_exv_tile_t<true, false>(P, s, X, PX, tl, it, L);
+  else list_exv_tile_t<false, false>(P, s, X, PX, tl, it, L);
 }
 
 // ele bit matrix (neighbor_list_ele.F90:123-210): bit (qi, qj) set when the
