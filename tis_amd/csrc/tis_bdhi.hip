@@ -231,46 +231,127 @@ __device__ __forceinline__ void bcast_update(double (&x)[kCb], const double (&R)
     bcast_update<J + 1>(x, R, nx);
   }
 }
-// Factors the 64x64 diagonal block held by one wave, lane r holding row r in
-// a[] (rows past kb are unit rows), as L D L^T, right-looking: column c's
-// pivot d = A[c][c] and the entry A[c+1][c] come from lanes c and c + 1 by
-// lane reads, each lane updates its entry of column c + 1 at once (the chain
-// from one pivot to the next has no LDS in it), and the rest of the update,
-// A[r][j] -= A[r][c] A[j][c] / d for j > c + 1, follows one column later from
-// column c as published in LDS (two buffers, one wave barrier per column).
-// Column c is scaled to the Cholesky factor as soon as it is final:
-// L[r][c] = A[r][c] / sqrt(d), L[c][c] = sqrt(d), zero above the diagonal;
-// Rd[c] = 1 / L[c][c].  Returns kk + c + 1 for the first column c < kb with
-// a pivot that is not positive, else 0.
+// x[j] += v_j nx for the local columns j = J .. 31 of a half of the block,
+// v reaching the lanes as two registers (R[k] = v[16 k + lane % 16] of the
+// half's rows) as in bcast_update.
+template <int J>
+__device__ __forceinline__ void bcast_update_half(double (&x)[kCb], const double (&R)[2], double nx) {
+  if constexpr (J < 32) {
+    fmac_row_bcast<J & 15>(x[J], R[J >> 4], nx);
+    bcast_update_half<J + 1>(x, R, nx);
+  }
+}
+
+// The 64x64 diagonal block as L D L^T, right-looking, by two waves: lane r
+// holds row r, wave 0 its columns 0-31 and wave 1 its columns 32-63 (local
+// index j = column - 32 w, in a[0..31]; rows past kb are unit rows).
+// Column c's pivot d = A[c][c] and the entry A[c+1][c] come from lanes c and
+// c + 1 by lane reads and each lane updates its entry of column c + 1 at
+// once; the rest of the update, A[r][j] -= A[r][c] A[j][c] / d for j > c + 1,
+// follows one column later from column c as published in LDS (U[c][r]),
+// taken as registers and DPP row broadcasts.  For c < 32 the chain runs in
+// wave 0, which also publishes the multipliers S[c][r] = A[r][c] / d and a
+// count of published columns; wave 1 meanwhile applies each published
+// column to its 32 columns (in column order, as one wave would), then runs
+// the chain for c >= 32.  Column c is scaled to the Cholesky factor once
+// final: L[r][c] = A[r][c] / sqrt(d), L[c][c] = sqrt(d), zero above the
+// diagonal; Rd[c] = 1 / L[c][c].  Each wave's pivot check gives kk + c + 1
+// for its first column c < kb with a pivot that is not positive, else 0.
+struct DiagLds {
+  double U[kCb][kCb];
+  double S[32][kCb];
+  int pub;    // columns of wave 0 published
+  int bad0;   // wave 0's pivot check
+};
+
 template <int C>
-__device__ __forceinline__ void diag_step(double (&a)[kCb], double (*colt)[kCb], double *Rd, int kk, int kb,
-                                          int lane, int &bad, double sp) {
-  if constexpr (C < kCb) {
-    double R[4];   // column C - 1 as published (sp = A[lane][C - 1] / d_{C - 1})
+__device__ __forceinline__ void diag_lo_step(double (&a)[kCb], DiagLds &D, double *Rd, int kk, int kb, int lane,
+                                             int &bad, double sp) {
+  if constexpr (C < 32) {
+    double R[2];   // column C - 1 as published, rows 0-31 (sp = A[lane][C - 1] / d_{C - 1})
     if constexpr (C >= 1) {
 #pragma unroll
-      for (int k = 0; k < 4; ++k) R[k] = colt[(C - 1) & 1][16 * k + (lane & 15)];
+      for (int k = 0; k < 2; ++k) R[k] = D.U[C - 1][16 * k + (lane & 15)];
       // column C - 1's update of column C + 1 first: column C + 1 is next on the chain
-      if constexpr (C + 1 < kCb) fmac_row_bcast<(C + 1) & 15>(a[C + 1], R[(C + 1) >> 4], -sp);
+      if constexpr (C + 1 < 32) fmac_row_bcast<(C + 1) & 15>(a[C + 1], R[(C + 1) >> 4], -sp);
     }
     const double d = lane_value(a[C], C);
     if (C < kb && !(d > 0.0) && bad == 0) bad = kk + C + 1;
     const double u = a[C];
-    colt[C & 1][lane] = u;
+    D.U[C][lane] = u;
     const double s = u * rcp_nr(d);
-    if constexpr (C + 1 < kCb) a[C + 1] = fma(-s, lane_value(u, C + 1), a[C + 1]);
-    if constexpr (C >= 1) bcast_update<C + 2>(a, R, -sp);
+    D.S[C][lane] = s;
+    if constexpr (C == 31) {
+      if (lane == 0) D.bad0 = bad;
+    }
+    // the column goes to wave 1: its LDS writes complete before the count
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if (lane == 0) __hip_atomic_store(&D.pub, C + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if constexpr (C + 1 < 32) a[C + 1] = fma(-s, lane_value(u, C + 1), a[C + 1]);
+    if constexpr (C >= 1) bcast_update_half<C + 2>(a, R, -sp);
     const double rs = rsqrt_nr(d);
     a[C] = lane == C ? d * rs : (lane > C ? u * rs : 0.0);
     if (lane == 0) Rd[C] = rs;
     wave_lds_sync();
-    diag_step<C + 1>(a, colt, Rd, kk, kb, lane, bad, s);
+    diag_lo_step<C + 1>(a, D, Rd, kk, kb, lane, bad, s);
   }
 }
-__device__ inline int diag_factor(double (&a)[kCb], double (*colt)[kCb], double *Rd, int kk, int kb) {
+
+// wave 1: wave 0's column C applied to columns 32-63 once published
+template <int C>
+__device__ __forceinline__ void diag_hi_lag(double (&a)[kCb], DiagLds &D, int lane) {
+  if constexpr (C < 32) {
+    // wave 0 publishes all 32 columns unconditionally, so this wait ends
+    while (__hip_atomic_load(&D.pub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) <= C)
+      __builtin_amdgcn_s_sleep(1);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    const double s = D.S[C][lane];
+    double R[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) R[k] = D.U[C][32 + 16 * k + (lane & 15)];
+    bcast_update_half<0>(a, R, -s);
+    diag_hi_lag<C + 1>(a, D, lane);
+  }
+}
+
+template <int C>
+__device__ __forceinline__ void diag_hi_step(double (&a)[kCb], DiagLds &D, double *Rd, int kk, int kb, int lane,
+                                             int &bad, double sp) {
+  if constexpr (C < kCb) {
+    constexpr int c = C - 32;   // local column
+    double R[2];   // column C - 1 as published, rows 32-63
+    if constexpr (C >= 33) {
+#pragma unroll
+      for (int k = 0; k < 2; ++k) R[k] = D.U[C - 1][32 + 16 * k + (lane & 15)];
+      if constexpr (C + 1 < kCb) fmac_row_bcast<(c + 1) & 15>(a[c + 1], R[(c + 1) >> 4], -sp);
+    }
+    const double d = lane_value(a[c], C);
+    if (C < kb && !(d > 0.0) && bad == 0) bad = kk + C + 1;
+    const double u = a[c];
+    D.U[C][lane] = u;
+    const double s = u * rcp_nr(d);
+    if constexpr (C + 1 < kCb) a[c + 1] = fma(-s, lane_value(u, C + 1), a[c + 1]);
+    if constexpr (C >= 33) bcast_update_half<c + 2>(a, R, -sp);
+    const double rs = rsqrt_nr(d);
+    a[c] = lane == C ? d * rs : (lane > C ? u * rs : 0.0);
+    if (lane == 0) Rd[C] = rs;
+    wave_lds_sync();
+    diag_hi_step<C + 1>(a, D, Rd, kk, kb, lane, bad, s);
+  }
+}
+
+// wave w (0 or 1) of the two-wave factorisation; wave 1 returns the pivot
+// check of the whole block, wave 0 its own
+__device__ inline int diag_factor2(double (&a)[kCb], DiagLds &D, double *Rd, int kk, int kb, int w) {
+  const int lane = threadIdx.x & (kCb - 1);
   int bad = 0;
-  diag_step<0>(a, colt, Rd, kk, kb, threadIdx.x & (kCb - 1), bad, 0.0);
-  return bad;
+  if (w == 0) {
+    diag_lo_step<0>(a, D, Rd, kk, kb, lane, bad, 0.0);
+    return bad;
+  }
+  diag_hi_lag<0>(a, D, lane);
+  diag_hi_step<32>(a, D, Rd, kk, kb, lane, bad, 0.0);
+  return D.bad0 ? D.bad0 : bad;
 }
 
 template <int C>
@@ -357,7 +438,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_chol_panel(BdDev B, int kk, i
   __shared__ __align__(16) double smem[kCb * kCb];
   __shared__ float Tr[kCb][kCb + 1];
   __shared__ double Rd[kCb];         // 1 / L11[c][c]
-  __shared__ double colt[2][kCb];    // the last two columns, as published
+  __shared__ DiagLds D;              // the diagonal factorisation's published columns
   static_assert(sizeof(smem) >= kCb * (kCb + 1) * sizeof(float), "the product does not fit");
   double *Ls = smem;
   float (*Td)[kCb + 1] = reinterpret_cast<float (*)[kCb + 1]>(smem);
@@ -368,14 +449,20 @@ __global__ __launch_bounds__(kPanelThreads) void k_chol_panel(BdDev B, int kk, i
   const bool rows = blockIdx.x > 0 && r0w < n3;
   const int r = r0w + lane;
   const int p = kk / kCb;
-  // wave 0: the diagonal block's rows; wave 2: the block's rows (unconditional
-  // loads from clamped, in-bounds addresses)
+  // waves 0 and 1: the diagonal block's rows, columns [32 w, 32 w + 32);
+  // wave 2: the block's rows (unconditional loads from clamped, in-bounds
+  // addresses)
   double v[kCb];
-  if (w == 0 || w == 2) {
-    const size_t row = (size_t)min(w == 0 ? kk + lane : r, n3 - 1);
+  if (w < 2) {
+    const size_t row = (size_t)min(kk + lane, n3 - 1);
+#pragma unroll
+    for (int c = 0; c < 32; ++c) v[c] = B.L[row + (size_t)min(kk + 32 * w + c, n3 - 1) * n3];
+  } else if (w == 2) {
+    const size_t row = (size_t)min(r, n3 - 1);
 #pragma unroll
     for (int c = 0; c < kCb; ++c) v[c] = B.L[row + (size_t)min(kk + c, n3 - 1) * n3];
   }
+  if (threadIdx.x == 0) D.pub = 0;
   if (w == 0 && blockIdx.x == 0 && p > 0) {
     // the previous panel's diagonal factor, kept aside until no block of
     // that panel reads the diagonal block (a full block: not the last panel)
@@ -390,35 +477,36 @@ __global__ __launch_bounds__(kPanelThreads) void k_chol_panel(BdDev B, int kk, i
     const floatx16 acc = quad_product(P, mpad, kk - k0 + 32 * qi, kk - k0 + 32 * qj, K, lane);
 #pragma unroll
     for (int e = 0; e < 16; ++e) Td[32 * qi + l32][32 * qj + c32 + 8 * (e >> 2) + (e & 3)] = acc[e];
-    __syncthreads();
-    if (w == 0) {
-#pragma unroll
-      for (int c = 0; c < kCb; ++c) v[c] -= (double)Td[lane][c];
-    }
   }
-  if (w == 0) {
+  __syncthreads();   // the diagonal block's product; D.pub cleared
+  if (w < 2) {
+    const int c0 = 32 * w;
+    if (K > 0) {
 #pragma unroll
-    for (int c = 0; c < kCb; ++c) v[c] = (lane < kb && c < kb) ? v[c] : (c == lane ? 1.0 : 0.0);
-    const int bad = diag_factor(v, colt, Rd, kk, kb);
+      for (int c = 0; c < 32; ++c) v[c] -= (double)Td[lane][c0 + c];
+    }
+#pragma unroll
+    for (int c = 0; c < 32; ++c) v[c] = (lane < kb && c0 + c < kb) ? v[c] : (c0 + c == lane ? 1.0 : 0.0);
+    const int bad = diag_factor2(v, D, Rd, kk, kb, w);
     if (blockIdx.x > 0) {
 #pragma unroll
-      for (int c = 0; c < kCb; ++c) Ls[c * kCb + lane] = v[c];
+      for (int c = 0; c < 32; ++c) Ls[(c0 + c) * kCb + lane] = v[c];
     } else if (gridDim.x > 1) {
       // other blocks may still be reading the diagonal block: the factor
       // waits in the stash for the next panel's launch
-      if (lane == 0 && bad) atomicCAS(B.info, 0, bad);
+      if (w == 1 && lane == 0 && bad) atomicCAS(B.info, 0, bad);
       double *sp = B.stash + (size_t)(p & 1) * kCb * kCb;
 #pragma unroll
-      for (int c = 0; c < kCb; ++c) sp[c * kCb + lane] = v[c];
+      for (int c = 0; c < 32; ++c) sp[(c0 + c) * kCb + lane] = v[c];
     } else {
-      if (lane == 0 && bad) atomicCAS(B.info, 0, bad);
+      if (w == 1 && lane == 0 && bad) atomicCAS(B.info, 0, bad);
       if (kb == kCb) {
 #pragma unroll
-        for (int c = 0; c < kCb; ++c) B.L[(size_t)(kk + lane) + (size_t)(kk + c) * n3] = v[c];
+        for (int c = 0; c < 32; ++c) B.L[(size_t)(kk + lane) + (size_t)(kk + c0 + c) * n3] = v[c];
       } else if (lane < kb) {
 #pragma unroll
-        for (int c = 0; c < kCb; ++c)
-          if (c < kb) B.L[(size_t)(kk + lane) + (size_t)(kk + c) * n3] = v[c];
+        for (int c = 0; c < 32; ++c)
+          if (c0 + c < kb) B.L[(size_t)(kk + lane) + (size_t)(kk + c0 + c) * n3] = v[c];
       }
     }
   }
