@@ -11,7 +11,6 @@ cross-compiles the device code to assembly.
 """
 import os
 import re
-import shutil
 import subprocess
 
 import pytest
