@@ -437,6 +437,16 @@ int launch_lists(tis_ctx *c, const Range &r, const double *X, double *PXcur, dou
                  int gate) {
   const int h0 = r.all ? 0 : c->hs[r.isys].d.hoff, nh = r.all ? c->nhbtot : c->hs[r.isys].d.nhb;
   const bool periodic = r.all ? c->any_periodic : c->hs[r.isys].d.periodic != 0;
+  if (gate && !periodic) {
+    // displacement-triggered rebuilds of non-periodic systems: k_lists takes
+    // the decision and the displacement reset itself and culls by the boxes
+    // k_main leaves at the end of each step (gate 2), so k_prep is skipped;
+    // each system's bead tiles also rebuild its hbond candidates
+    if (r.btn + r.qtn > 0)
+      hipLaunchKernelGGL(k_lists, dim3(r.btn + r.qtn), dim3(kTile), 0, c->stream, c->P, X, PXcur, c->d_btiles, r.bt0,
+                         r.btn, c->d_qtiles, r.qt0, r.qtn, h0, nh, 2);
+    return 0;
+  }
   if (r.btn + r.qtn > 0)
     hipLaunchKernelGGL(k_prep, dim3(r.btn + r.qtn), dim3(kTile), 0, c->stream, c->P, X, PXcur, PXpred, c->d_btiles,
                        r.bt0, r.btn, c->d_qtiles, r.qt0, r.qtn, with_pred, gate);

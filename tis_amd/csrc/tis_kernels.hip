@@ -965,6 +965,32 @@ __device__ inline void integrate(const DevPtrs &P, const DSys &s, int g, const B
   }
 }
 
+// The boxes of a tile of 64 positions (one per lane of the wave): the tile's
+// box to *tile and, for bead tiles (groups != nullptr), the boxes of its four
+// 16-bead groups.
+__device__ inline void tile_boxes(double p0, double p1, double p2, BBox *tile, BBox *groups) {
+  const int lane = threadIdx.x & (kTile - 1);
+  double lo[3] = {p0, p1, p2}, hi[3] = {p0, p1, p2};
+#pragma unroll
+  for (int off = 1; off < kTile; off <<= 1) {
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = fmin(lo[d], __shfl_xor(lo[d], off));
+      hi[d] = fmax(hi[d], __shfl_xor(hi[d], off));
+    }
+    // after the offsets 1..8 each group of 16 lanes holds its group's box
+    if (off == 8 && groups && (lane & 15) == 0) {
+      BBox b;
+      for (int d = 0; d < 3; ++d) { b.lo[d] = lo[d]; b.hi[d] = hi[d]; }
+      groups[lane / 16] = b;
+    }
+  }
+  if (lane == 0) {
+    BBox b;
+    for (int d = 0; d < 3; ++d) { b.lo[d] = lo[d]; b.hi[d] = hi[d]; }
+    *tile = b;
+  }
+}
+
 // Two waves share a tile of 64 beads: wave 0 sums each bead's bonded slots,
 // wave 1 draws its Langevin noise and adds its electrostatic sums, and the
 // two waves take alternate groups of four excluded-volume neighbours; the
@@ -1160,7 +1186,26 @@ __global__ __launch_bounds__(kTile * 2) void k_main(DevPtrs P, const double *X, 
     const double gs[3] = {gsh[0][lane], gsh[1][lane], gsh[2][lane]};
     d2 = main_tail(P, s, g, F, red, bs, gs, lane, XN, PXN, mode);
   }
-  if (P.dyn && mode != 0) tile_top2(P, s.bt0 + tl.start / kTile, d2);
+  if (P.dyn && mode != 0) {
+    const int it = s.bt0 + tl.start / kTile;
+    tile_top2(P, it, d2);
+    // the tile's boxes at the new positions, which a displacement-triggered
+    // rebuild of the next step culls by (k_lists then runs without k_prep;
+    // lanes past the last bead take the tile's first bead, as bbox_tile)
+    if (!s.periodic) {
+      double p[3] = {0.0, 0.0, 0.0};
+      if (valid) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) p[d] = XN[d * n + g];   // this lane's own store
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double p0 = __shfl(p[d], 0);
+        p[d] = valid ? p[d] : p0;
+      }
+      tile_boxes(p[0], p[1], p[2], &P.bbox[it], &P.sbox[4 * it]);
+    }
+  }
 }
 
 // ------------------------------------------------------------------ lists
@@ -1205,25 +1250,7 @@ __device__ inline void bbox_tile(const DevPtrs &P, const DSys &s, const double *
   const int first = charge ? P.qbead[s.qoff + tl.start] : s.off + tl.start;
   int g = first;
   if (li < lim) g = charge ? P.qbead[s.qoff + li] : s.off + li;
-  double lo[3] = {Y[g], Y[n + g], Y[2 * n + g]}, hi[3] = {lo[0], lo[1], lo[2]};
-#pragma unroll
-  for (int off = 1; off < kTile; off <<= 1) {
-    for (int d = 0; d < 3; ++d) {
-      lo[d] = fmin(lo[d], __shfl_xor(lo[d], off));
-      hi[d] = fmax(hi[d], __shfl_xor(hi[d], off));
-    }
-    // after the offsets 1..8 each group of 16 lanes holds its group's box
-    if (off == 8 && !charge && (lane & 15) == 0) {
-      BBox b;
-      for (int d = 0; d < 3; ++d) { b.lo[d] = lo[d]; b.hi[d] = hi[d]; }
-      P.sbox[4 * it + lane / 16] = b;
-    }
-  }
-  if (lane == 0) {
-    BBox b;
-    for (int d = 0; d < 3; ++d) { b.lo[d] = lo[d]; b.hi[d] = hi[d]; }
-    if (charge) P.qbbox[it] = b; else P.bbox[it] = b;
-  }
+  tile_boxes(Y[g], Y[n + g], Y[2 * n + g], charge ? &P.qbbox[it] : &P.bbox[it], charge ? nullptr : &P.sbox[4 * it]);
 }
 
 // squared distance bounds between two boxes (FP64); conservative culling
@@ -1406,8 +1433,24 @@ __device__ inline void list_exv_tile(const DevPtrs &P, const DSys &s, const doub
 // pair is in the list; radius (margin + cutoff)^2 per i_DH_cutoff_type.  Tile
 // pairs whose boxes lie entirely inside (outside) the radius are filled with
 // ones (zeros) without per-pair tests; the rest are tested pair by pair.
+// A box of charge tile t (charges [64 t, 64 t + 64) of system s): the union
+// of the boxes of the bead tiles that hold them (charges follow bead order).
+__device__ inline BBox charge_tile_union_box(const DevPtrs &P, const DSys &s, int t) {
+  const int q0 = t * kTile, q1 = min(q0 + kTile, s.nq) - 1;
+  const int t0 = (P.qbead[s.qoff + q0] - s.off) / kTile, t1 = (P.qbead[s.qoff + q1] - s.off) / kTile;
+  BBox b = P.bbox[s.bt0 + t0];
+  for (int k = t0 + 1; k <= t1; ++k) {
+    const BBox c = P.bbox[s.bt0 + k];
+    for (int d = 0; d < 3; ++d) { b.lo[d] = fmin(b.lo[d], c.lo[d]); b.hi[d] = fmax(b.hi[d], c.hi[d]); }
+  }
+  return b;
+}
+
+// (union_boxes: the charge tiles' boxes from the bead tiles' boxes instead of
+// qbbox, which only k_prep computes; conservative, so the decisions below
+// stay exact)
 __device__ inline void list_ele_tile(const DevPtrs &P, const DSys &s, const double *X, const double *PX,
-                                     const Tile tl, int it, ListLds &L) {
+                                     const Tile tl, int it, ListLds &L, bool union_boxes) {
   auto &sx = L.sx; auto &sy = L.sy; auto &sz = L.sz;
   const int lane = threadIdx.x;
   const int qi = tl.start + lane;
@@ -1417,13 +1460,13 @@ __device__ inline void list_ele_tile(const DevPtrs &P, const DSys &s, const doub
   double xi[3] = {0, 0, 0};
   if (valid) ld3(Y, n, P.qbead[s.qoff + qi], xi);
   uint32_t *row = P.qmask + s.moff + (int64_t)(valid ? qi : 0) * s.mwords;
-  const BBox bi = P.qbbox[it];
+  const BBox bi = union_boxes ? charge_tile_union_box(P, s, tl.start / kTile) : P.qbbox[it];
   for (int j0 = 0; j0 < s.nqt; j0 += kTile) {
     // mode of the tile pairs (it, j0 + lane), decided at once: 0 none, 1 all,
     // 2 test
     int lmode = 2;
     if (!s.periodic && j0 + lane < s.nqt) {
-      const BBox bj = P.qbbox[s.qt0 + j0 + lane];
+      const BBox bj = union_boxes ? charge_tile_union_box(P, s, j0 + lane) : P.qbbox[s.qt0 + j0 + lane];
       if (box_dmin2(bi, bj) > s.ele_rl2 * (1.0 + 1e-9)) lmode = 0;
       else if (box_dmax2(bi, bj) < s.ele_rl2 * (1.0 - 1e-9)) lmode = 1;
     }
@@ -1745,10 +1788,25 @@ __global__ __launch_bounds__(kTile) void k_lists(DevPtrs P, const double *X, con
     const bool charge = b >= nbt;
     const int it = charge ? qt0 + b - nbt : bt0 + b;
     const Tile tl = charge ? qtiles[it] : btiles[it];
-    if (gate && !P.sys_flag[tl.sys]) return;
+    if (gate == 1 && !P.sys_flag[tl.sys]) return;
     const DSys s = P.sys[tl.sys];
+    if (gate == 2) {
+      // without k_prep (non-periodic batch): k_prep's decision, taken by
+      // every block of the system, and its displacement reset; the bead
+      // tiles' boxes are k_main's of this step's coordinates
+      const bool fire = system_trigger(P, s);
+      if (!charge && tl.start == 0 && threadIdx.x == 0) P.sys_flag[tl.sys] = fire ? 1 : 0;
+      if (!fire) return;
+      if (!charge) {
+        const int li = tl.start + (int)threadIdx.x, n = P.ntot;
+        if (li < s.nmp)
+          for (int d = 0; d < 3; ++d) P.dxyz[d * n + s.off + li] = P.dlast[d * n + s.off + li];
+        // the system's hbond candidates, shared out over its bead tiles
+        for (int k = tl.start + (int)threadIdx.x; k < s.nhb; k += s.nbt * kTile) list_hb_entry(P, s, X, PX, s.hoff + k);
+      }
+    }
     if (s.periodic) return;   // built by tile pairs (k_lpair)
-    if (charge) list_ele_tile(P, s, X, PX, tl, it, lds);
+    if (charge) list_ele_tile(P, s, X, PX, tl, it, lds, gate == 2);
     else list_exv_tile(P, s, X, PX, tl, it, lds);
     return;
   }
