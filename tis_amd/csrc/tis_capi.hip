@@ -109,7 +109,8 @@ struct HostSys {
   std::shared_ptr<SysCopy> src;         // host copy (for a new layout)
   int32_t bt_begin = 0, bt_count = 0;   // bead tiles
   int32_t qt_begin = 0, qt_count = 0;   // charge tiles
-  bool has_task = false;                // hbonds or tertiary stacks
+  bool has_task = false;                // tertiary stacks or valence-limited hbonds (terms_wave)
+  int32_t hd_begin = 0, hd_count = 0;   // its entries of the hbond-lane list
   int32_t esk_begin = 0, esk_count = 0;  // Ewald structure-factor tasks
   int32_t ef_begin = 0, ef_count = 0;    // Ewald force tasks
   int32_t pt_begin = 0, pt_count = 0;    // periodic electrostatic chunk-pair tasks
@@ -157,6 +158,8 @@ struct tis_ctx {
   int32_t nbtile = 0, nqtile = 0;
   int32_t *d_systask = nullptr, *d_sysidx = nullptr;
   int32_t nsystask = 0;
+  int32_t *d_hbdense = nullptr;          // hbond lanes (DevPtrs::hb_dense)
+  int32_t nhbdense = 0;
   std::vector<int32_t> systask_of;       // index into d_systask per system or -1
   int32_t *d_ststat = nullptr;           // st_status output of tis_term_status [nsttot]
   double *d_energy = nullptr;
@@ -386,6 +389,7 @@ struct Range {
   bool all;                 // the whole batch
   int isys;                 // else this system
   int bt0, btn, qt0, qtn, task0, taskn;
+  int hd0, hdn;             // hbond lanes
   const int32_t *sysidx; int nsel;
   int b0, nb, a0, na, s0, ns;
   int ek0, ekn, ef0, efn;   // Ewald tasks
@@ -397,6 +401,7 @@ Range range_of(tis_ctx *c, int32_t isys) {
   if (isys < 0) {
     r.all = true;
     r.bt0 = 0; r.btn = c->nbtile; r.qt0 = 0; r.qtn = c->nqtile; r.task0 = 0; r.taskn = c->nsystask;
+    r.hd0 = 0; r.hdn = c->nhbdense;
     r.sysidx = c->d_sysidx; r.nsel = c->nlive;
     r.b0 = 0; r.nb = c->stage.nbd; r.a0 = 0; r.na = c->stage.nba; r.s0 = 0; r.ns = c->stage.nst;
     r.ek0 = 0; r.ekn = c->n_ewsk; r.ef0 = 0; r.efn = c->n_ewf;
@@ -407,6 +412,7 @@ Range range_of(tis_ctx *c, int32_t isys) {
     r.bt0 = h.bt_begin; r.btn = h.bt_count; r.qt0 = h.qt_begin; r.qtn = h.qt_count;
     r.task0 = c->systask_of[isys] < 0 ? 0 : c->systask_of[isys];
     r.taskn = c->systask_of[isys] < 0 ? 0 : 1;
+    r.hd0 = h.hd_begin; r.hdn = h.hd_count;
     r.sysidx = c->d_sysidx + h.live_pos; r.nsel = 1;
     r.b0 = h.d.boff; r.nb = h.d.nbd; r.a0 = h.d.aoff; r.na = h.d.nba; r.s0 = h.d.soff; r.ns = h.d.nst;
     r.ek0 = h.esk_begin; r.ekn = h.esk_count; r.ef0 = h.ef_begin; r.efn = h.ef_count;
@@ -791,7 +797,7 @@ int stage_system(tis_ctx *c, Stage &g, HostSys &h, const tis_system &s, int is, 
   // counts of the periodic tile-pair list build
   d.lp_off = g.lp_off;
   if (s.p.periodic) g.lp_off += (int64_t)d.nbt * d.nbt;
-  h.has_task = (s.nhb > 0 || s.ntst > 0);
+  h.has_task = (s.ntst > 0 || (s.nhb > 0 && d.hb_excess_possible));
   // Ewald reciprocal vectors
   if (s.p.ele_form == TIS_ELE_EWALD && nq > 0) {
     d.ewk0 = (int32_t)g.ewc.size();
@@ -939,7 +945,7 @@ int build_tables(tis_ctx *c) {
   c->tab_bytes = 0;
   std::vector<Tile> btiles, qtiles;
   std::vector<EwTask> ptask, bpair, qpair, ewsk, ewf;
-  std::vector<int32_t> systask, sysidx;
+  std::vector<int32_t> systask, sysidx, hbdense;
   c->systask_of.assign(c->nsys, -1);
   c->any_ele = false;
   c->any_periodic = false;
@@ -949,6 +955,7 @@ int build_tables(tis_ctx *c) {
     DSys &d = h.d;
     h.bt_begin = h.bt_count = h.qt_begin = h.qt_count = h.esk_begin = h.esk_count = 0;
     h.ef_begin = h.ef_count = h.pt_begin = h.pt_count = h.bp_begin = h.bp_count = h.qp_begin = h.qp_count = 0;
+    h.hd_begin = h.hd_count = 0;
     if (!h.live) { std::memset(&dsys[is], 0, sizeof(DSys)); dsys[is].nmp = 0; continue; }
     if (d.periodic) c->any_periodic = true;
     if (d.ele_form >= 0) c->any_ele = true;
@@ -978,6 +985,10 @@ int build_tables(tis_ctx *c) {
     h.bp_count = (int32_t)bpair.size() - h.bp_begin;
     h.qp_count = (int32_t)qpair.size() - h.qp_begin;
     if (h.has_task) { c->systask_of[is] = (int32_t)systask.size(); systask.push_back(is); }
+    h.hd_begin = (int32_t)hbdense.size();
+    if (!d.hb_excess_possible)
+      for (int k = 0; k < d.nhb; ++k) hbdense.push_back(d.hoff + k);
+    h.hd_count = (int32_t)hbdense.size() - h.hd_begin;
     if (d.ele_form == TIS_ELE_EWALD && d.nq > 0) {
       h.esk_begin = (int32_t)ewsk.size();
       for (int kt = 0; kt < (d.nk + kTile - 1) / kTile; ++kt)
@@ -995,6 +1006,7 @@ int build_tables(tis_ctx *c) {
   c->nlive = (int32_t)sysidx.size();
   c->P.nsys = c->nsys;
   c->nbtile = (int32_t)btiles.size(); c->nqtile = (int32_t)qtiles.size(); c->nsystask = (int32_t)systask.size();
+  c->nhbdense = (int32_t)hbdense.size();
   c->n_ptask = (int32_t)ptask.size(); c->n_bpair = (int32_t)bpair.size(); c->n_qpair = (int32_t)qpair.size();
   c->n_ewsk = (int32_t)ewsk.size(); c->n_ewf = (int32_t)ewf.size();
   if (!dsys.empty()) {
@@ -1010,7 +1022,9 @@ int build_tables(tis_ctx *c) {
   UPT(c->d_btiles, btiles); UPT(c->d_qtiles, qtiles);
   UPT(c->d_ptask, ptask); UPT(c->d_ptask_lpt, lpt); UPT(c->d_bpair, bpair); UPT(c->d_qpair, qpair);
   UPT(c->d_ewsk, ewsk); UPT(c->d_ewf, ewf); UPT(c->d_systask, systask); UPT(c->d_sysidx, sysidx);
+  UPT(c->d_hbdense, hbdense);
 #undef UPT
+  c->P.hb_dense = c->d_hbdense;
   const size_t nbt = std::max<size_t>(btiles.size(), 1), nqt = std::max<size_t>(qtiles.size(), 1);
   if ((rc = dalloc_in(c, c->tab_allocs, &c->P.bbox, nbt, &c->tab_bytes))) return rc;
   if ((rc = dalloc_in(c, c->tab_allocs, &c->P.qbbox, nqt, &c->tab_bytes))) return rc;
@@ -1471,17 +1485,18 @@ static void launch_force_pair(tis_ctx *c, const Range &r, const double *X, const
     c->tev_used += kTev;
     (void)hipEventRecord(e3[0], c->stream);
   }
-  // k_force: per-system tertiary stacks and hbonds, the electrostatic chunk
-  // pairs, then the bonded terms in blocks of 256 lanes
+  // k_force: hbond lanes, per-system tertiary stacks and valence-limited
+  // hbonds, the electrostatic chunk pairs, then the bonded terms in blocks of
+  // 256 lanes
   // (launch shape: chunk pairs per wave, bonded terms per lane and the
   // order of the chunk pairs; tis_tune chooses it, no result depends on it)
   const int nterm = r.nb + r.na + r.ns;
   const int npt = c->any_ele ? r.ptn : 0;
   const int ppw = c->shape.ppw, tpl = c->shape.tpl;
-  const int nblk = (r.taskn + kWaves - 1) / kWaves + (npt + kWaves * ppw - 1) / (kWaves * ppw) +
-                   (nterm + kTile * kWaves * tpl - 1) / (kTile * kWaves * tpl);
+  const int nblk = (r.hdn + kTile * kWaves - 1) / (kTile * kWaves) + (r.taskn + kWaves - 1) / kWaves +
+                   (npt + kWaves * ppw - 1) / (kWaves * ppw) + (nterm + kTile * kWaves * tpl - 1) / (kTile * kWaves * tpl);
   if (nblk > 0) {
-    const BondedRange br{r.b0, r.nb, r.a0, r.na, r.s0, r.ns};
+    const BondedRange br{r.b0, r.nb, r.a0, r.na, r.s0, r.ns, r.hd0, r.hdn};
     const int32_t *task = c->d_systask + r.task0;
     const EwTask *pt = r.all ? (c->shape.lpt ? c->d_ptask_lpt : c->d_ptask) : c->d_ptask + r.pt0;
     // one kernel per launch shape (each with its own register allocation)

@@ -89,8 +89,9 @@ constexpr int kEwKChunk = 256;   // reciprocal vectors per block of the force pa
 
 struct Tile { int32_t sys, start; };
 
-// bonds [b0, b0+nb), angles [a0, a0+na), consecutive stacks [s0, s0+ns)
-struct BondedRange { int32_t b0, nb, a0, na, s0, ns; };
+// bonds [b0, b0+nb), angles [a0, a0+na), consecutive stacks [s0, s0+ns);
+// hbond lanes: entries [hd0, hd0+nhd) of DevPtrs::hb_dense
+struct BondedRange { int32_t b0, nb, a0, na, s0, ns, hd0, nhd; };
 
 
 // excluded-volume parameters of a species pair: D_ij (FP64 and its square)
@@ -146,6 +147,7 @@ struct DevPtrs {
   double *for_hb;            // [18][nhb]
   int32_t *site_cnt;         // [nsite]
   int32_t *hb_seq;           // [nhb] valence-pass scratch: a site's hbonds, per system at its hoff
+  const int32_t *hb_dense;   // hbonds of the systems none of whose sites can exceed its valence, system order
   // lists
   int32_t *nb_cnt; uint32_t *nb; int32_t nb_cap;
   const int32_t *qbead;      // [nq] global bead of each charge

@@ -518,10 +518,24 @@ __device__ int hbond_valence_wave(const DevPtrs &P, const DSys &s, uint64_t step
   return ndel;
 }
 
+// One hbond of a system none of whose sites can exceed its valence: every
+// listed hbond within the cutoff stays (force_dtrna_hbond15.F90:91-353 without
+// the valence pass), so the hbonds of all such systems are independent lanes.
+__device__ inline void hbond_lane(const DevPtrs &P, const double *X, const double *PX, int h) {
+  const DSys &s = P.sys[P.hb_sys[h]];
+  int st = 0;
+  double e = 0.0, fo[18];
+  if (P.hb_inlist[h]) st = hbond_eval(P, s, X, PX, h, fo, &e);
+  if (!st) for (int q = 0; q < 18; ++q) fo[q] = 0.0;
+  P.hb_status[h] = (uint8_t)st;
+  P.hb_ene[h] = e;
+  put_slots(P, P.sb_hb, h, 6, fo);
+}
+
 // ------------------------------------------------------------------ terms pass
-// One wave per system that has tertiary stacks or hbond candidates:
-// tertiary stack forces, hbond geometry of listed candidates and the valence
-// pass.  Forces go to per-term slots gathered by k_main.  The phases of a
+// One wave per system that has tertiary stacks or hbond candidates some of
+// whose sites can exceed their valence: tertiary stack forces, and for the
+// latter the hbond geometry of listed candidates and the valence pass.  Forces go to per-term slots gathered by k_main.  The phases of a
 // system are ordered by wave-level fences: the four waves of a block serve
 // four systems.
 
@@ -534,20 +548,9 @@ __device__ void terms_wave(const DevPtrs &P, const double *X, const double *PX, 
     tstack_eval(P, s, X, PX, s.toff + k, fo);
     put_slots(P, P.sb_tst, s.toff + k, 6, fo);
   }
-  if (!s.hb_excess_possible) {
-    // no site can exceed its valence: every hbond within the cutoff stays
-    for (int k = lane; k < s.nhb; k += nt) {
-      const int h = s.hoff + k;
-      int st = 0;
-      double e = 0.0, fo[18];
-      if (P.hb_inlist[h]) st = hbond_eval(P, s, X, PX, h, fo, &e);
-      if (!st) for (int q = 0; q < 18; ++q) fo[q] = 0.0;
-      P.hb_status[h] = (uint8_t)st;
-      P.hb_ene[h] = e;
-      put_slots(P, P.sb_hb, h, 6, fo);
-    }
-    return;
-  }
+  // no site can exceed its valence: the hbonds run as lanes of their own
+  // (hbond_lane)
+  if (!s.hb_excess_possible) return;
   for (int k = lane; k < s.nsite; k += nt) P.site_cnt[s.sitoff + k] = 0;
   wave_sync();
   for (int k = lane; k < s.nhb; k += nt) {
@@ -749,11 +752,13 @@ __device__ inline void ele_pair_task(const DevPtrs &P, const DSys &s, const doub
   if (nclamp) atomicAdd(&P.counters[1], (unsigned long long)nclamp);
 }
 
-// The force pass, one launch with three kinds of blocks: four systems'
-// tertiary stacks and hbonds per block (terms_wave, one wave each), four
-// electrostatic chunk pairs per block (ele_pair_task, one wave each), then
-// the bonded terms, 256 lanes per block (bonded_lane).  k_main gathers all of
-// their outputs.
+// The force pass, one launch with four kinds of blocks: 256 hbond lanes per
+// block (hbond_lane: the hbonds of systems without valence-limited sites, one
+// lane each, first because each lane is a long FP64 chain), four systems'
+// tertiary stacks and valence-limited hbonds per block (terms_wave, one wave
+// each), four electrostatic chunk pairs per block (ele_pair_task, one wave
+// each), then the bonded terms, 256 lanes per block (bonded_lane).  k_main
+// gathers all of their outputs.
 template <int PPW, int TPL>
 __global__ __launch_bounds__(kTile * kWaves) __attribute__((amdgpu_waves_per_eu(4)))
 void k_force(DevPtrs P, const double *X, const double *PX, const int32_t *systask, int ntask,
@@ -762,8 +767,15 @@ void k_force(DevPtrs P, const double *X, const double *PX, const int32_t *systas
   __shared__ EleLds lds;
   if (run_stopped(P)) return;
   int b = (int)blockIdx.x;
+  const int nhdb = (br.nhd + kTile * kWaves - 1) / (kTile * kWaves);   // blocks of 256 hbond lanes
   const int npb = (npt + kWaves * ppw - 1) / (kWaves * ppw);   // blocks of 4 ppw chunk pairs
   const int ntb = (ntask + kWaves - 1) / kWaves;   // blocks of four system terms passes
+  if (b < nhdb) {
+    const int t = b * (kTile * kWaves) + (int)threadIdx.x;
+    if (t < br.nhd) hbond_lane(P, X, PX, P.hb_dense[br.hd0 + t]);
+    return;
+  }
+  b -= nhdb;
   if (b < ntb) {
     const int task = b * kWaves + (int)threadIdx.x / kTile;
     if (task < ntask) terms_wave(P, X, PX, systask[task], t_local);
@@ -772,7 +784,7 @@ void k_force(DevPtrs P, const double *X, const double *PX, const int32_t *systas
   b -= ntb;
   if (b >= npb) {
     // tpl bonded terms per lane, a grid-wide stride apart
-    const int nbb = (int)gridDim.x - ntb - npb;
+    const int nbb = (int)gridDim.x - nhdb - ntb - npb;
     for (int i = 0; i < tpl; ++i)
       bonded_lane(P, X, PX, br, (b - npb + i * nbb) * (kTile * kWaves) + (int)threadIdx.x);
     return;
