@@ -616,9 +616,13 @@ __device__ inline float ele_dv(const DSys &s, float qq, float d2, unsigned &ncla
 }
 
 // Staging of the J chunk of each wave's chunk pair.
+// The J chunk of each wave's chunk pair: ele_pair_task uses the first 64
+// entries of each array, ele_pair_unrolled keeps the chunk twice over (so
+// that its reads need no wrap-around).  Arrays, not records, so that the 64
+// lanes' reads of consecutive entries fall in distinct LDS banks.
+struct EleWave { double x[2 * kTile], y[2 * kTile], z[2 * kTile]; float q[2 * kTile]; };
 struct EleLds {
-  double sx[kWaves][kTile], sy[kWaves][kTile], sz[kWaves][kTile];
-  float sq[kWaves][kTile];
+  EleWave w[kWaves];
 };
 
 // Whole-wave rotation by one lane (DPP wave_rol:1): each lane receives the
@@ -752,6 +756,116 @@ __device__ inline void ele_pair_task(const DevPtrs &P, const DSys &s, const doub
   if (nclamp) atomicAdd(&P.counters[1], (unsigned long long)nclamp);
 }
 
+// ele_pair_task for a non-periodic system with the whole-wave DPP rotation
+// (dir = 1 or 63), with the same pairs, the same operations in the same order
+// and so the same results, in fewer instructions per step: the J chunk is
+// staged twice over in an order where lane l meets, at step k, entry c + k
+// (c = l for dir = 1, 63 - l for dir = 63), so the reads of eight
+// consecutive steps take immediate offsets from one address; the list bits
+// are turned into the same order once (bit k of mr is the pair of step k); a
+// pair that is not on gets charge product 0, so its force is exactly zero
+// and adds nothing, with no branch; the DE_MAX clamps are counted per wave.
+template <int FORM>
+__device__ inline void ele_pair_unrolled(const DevPtrs &P, const DSys &s, const double *Y, int I, int J, int dir,
+                                         EleWave *JE) {
+  // the chunk pair is the wave's: uniform, so the step loops below are scalar
+  I = __builtin_amdgcn_readfirstlane(I);
+  J = __builtin_amdgcn_readfirstlane(J);
+  const int lane = threadIdx.x & (kTile - 1);
+  const int c = dir == 1 ? lane : kTile - 1 - lane;
+  const int n = P.ntot;
+  const int qi = I * kTile + lane;
+  const bool valid = qi < s.nq;
+  double xi0 = 0.0, xi1 = 0.0, xi2 = 0.0, chi = 0.0;
+  if (valid) {
+    const int gi = P.qbead[s.qoff + qi];
+    xi0 = Y[gi]; xi1 = Y[n + gi]; xi2 = Y[2 * n + gi];
+    chi = P.charge[gi];
+  }
+  const int jt = J * kTile;
+  {
+    const bool vj = jt + lane < s.nq;
+    const int gj = vj ? P.qbead[s.qoff + jt + lane] : 0;
+    const double x = vj ? Y[gj] : 0.0, y = vj ? Y[n + gj] : 0.0, z = vj ? Y[2 * n + gj] : 0.0;
+    const float q = vj ? (float)P.charge[gj] : 0.0f;
+    JE->x[c] = x; JE->y[c] = y; JE->z[c] = z; JE->q[c] = q;
+    JE->x[c + kTile] = x; JE->y[c + kTile] = y; JE->z[c + kTile] = z; JE->q[c + kTile] = q;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const float qiC = (float)(chi * s.ele_coef);
+  const double cut2 = FORM == 0 ? s.dh_cut2 : s.cl_cut2;
+  const uint32_t *row = P.qmask + s.moff + (int64_t)(valid ? qi : 0) * s.mwords;
+  uint64_t m64 = 0;
+  if (valid) {
+    const int wb = jt >> 5;
+    m64 = row[wb];
+    if (wb + 1 < s.mwords) m64 |= (uint64_t)row[wb + 1] << 32;
+  }
+  const bool diag = I == J;
+  // bit k of mr: the pair of step k, j = (lane + dir k) mod 64
+  uint64_t mr = __builtin_rotateright64(dir == 1 ? m64 : __builtin_bitreverse64(m64), (uint64_t)c);
+  if (diag && lane >= 32) mr &= ~(1ull << 32);   // step 32 of the diagonal chunk: lanes < 32 only
+  double F0 = 0.0, F1 = 0.0, F2 = 0.0, G0 = 0.0, G1 = 0.0, G2 = 0.0;
+  uint32_t nclamp = 0;
+  // eight steps from k0; BITS false: every pair of the chunk pair is listed,
+  // so only the cutoff decides (a charge beyond nq has charge 0 and adds 0)
+  auto steps8 = [&](const int k0, const uint32_t b8, auto bits_t) {
+    constexpr bool BITS = decltype(bits_t)::value;
+    const double *px = JE->x + c + k0, *py = JE->y + c + k0, *pz = JE->z + c + k0;
+    const float *pq = JE->q + c + k0;
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) {
+      const double v0 = px[kk] - xi0, v1 = py[kk] - xi1, v2 = pz[kk] - xi2;
+      const double d2 = v0 * v0 + v1 * v1 + v2 * v2;
+      const bool on = (!BITS || ((b8 >> kk) & 1u)) && d2 <= cut2;
+      float sq = pq[kk];
+      asm volatile("" : "+v"(sq));   // read for every lane: a select below, not a branch
+      const float qq = on ? qiC * sq : 0.0f;
+      const float rinv = hw_rsq(on ? (float)d2 : 1.0f);
+      float dv;
+      if (FORM == 0) {
+        const float rcd = (float)s.rcdist;
+        const float d1 = (on ? (float)d2 : 1.0f) * rinv;
+        dv = qq * rinv * rinv * (rinv + rcd) * hw_exp(-d1 * rcd);
+        const bool cl = dv > (float)kDeMax;
+        nclamp += (uint32_t)__popcll(__ballot(cl));
+        dv = cl ? (float)kDeMax : dv;
+      } else {
+        dv = qq * rinv * rinv * rinv;
+      }
+      const double dd = (double)dv;
+      F0 -= dd * v0; F1 -= dd * v1; F2 -= dd * v2;
+      G0 += dd * v0; G1 += dd * v1; G2 += dd * v2;
+      G0 = wave_rot(G0); G1 = wave_rot(G1); G2 = wave_rot(G2);
+    }
+  };
+  if (__any(m64 != 0)) {
+    // every valid j listed for every valid lane (off-diagonal chunk pairs)
+    const int nj = s.nq - jt < kTile ? s.nq - jt : kTile;
+    const uint64_t vjm = nj >= kTile ? ~0ull : (1ull << nj) - 1ull;
+    const bool all_listed = !diag && __all(!valid || (m64 & vjm) == vjm);
+    if (all_listed) {
+#pragma unroll 1
+      for (int k0 = 0; k0 < kTile; k0 += 8) steps8(k0, 0u, std::false_type{});
+    } else {
+      const int k_begin = diag ? 1 : 0, k_end = diag ? 33 : kTile;
+#pragma unroll 1
+      for (int k0 = k_begin; k0 < k_end; k0 += 8) steps8(k0, (uint32_t)(mr >> k0), std::true_type{});
+    }
+  }
+  double *pp = P.pp + s.pp_off + ele_task_index(I, J, s.nqt) * (6 * kTile);
+  if (diag) {
+    const int src = (lane - dir * 33) & (kTile - 1);
+    F0 += __shfl(G0, src); F1 += __shfl(G1, src); F2 += __shfl(G2, src);
+  } else {
+    pp[3 * kTile + lane] = G0; pp[4 * kTile + lane] = G1; pp[5 * kTile + lane] = G2;
+  }
+  pp[lane] = F0; pp[kTile + lane] = F1; pp[2 * kTile + lane] = F2;
+  if (lane == 0 && nclamp) atomicAdd(&P.counters[1], (unsigned long long)nclamp);
+}
+
 // The force pass, one launch with four kinds of blocks: 256 hbond lanes per
 // block (hbond_lane: the hbonds of systems without valence-limited sites, one
 // lane each, first because each lane is a long FP64 chain), four systems'
@@ -791,8 +905,9 @@ void k_force(DevPtrs P, const double *X, const double *PX, const int32_t *systas
   }
   const int w = (int)threadIdx.x / kTile;
   const int dir = wave_dir();
-  double *SX = lds.sx[w], *SY = lds.sy[w], *SZ = lds.sz[w];
-  float *SQ = lds.sq[w];
+  EleWave *JE = &lds.w[w];
+  double *SX = JE->x, *SY = JE->y, *SZ = JE->z;
+  float *SQ = JE->q;
   // ppw consecutive chunk pairs per wave, one after the other through the
   // wave's LDS slice
 #pragma unroll 1
@@ -807,7 +922,11 @@ void k_force(DevPtrs P, const double *X, const double *PX, const int32_t *systas
     const EwTask tk = ptasks[task];
     const DSys s = P.sys[tk.sys];
     const double *Y = s.periodic ? PX : X;
-    if (dir != 0) {
+    if (dir != 0 && !s.periodic && s.ele_form == 0) {
+      ele_pair_unrolled<0>(P, s, Y, tk.a, tk.b, dir, JE);
+    } else if (dir != 0 && !s.periodic && s.ele_form == 1) {
+      ele_pair_unrolled<1>(P, s, Y, tk.a, tk.b, dir, JE);
+    } else if (dir != 0) {
       if (s.ele_form == 0) ele_pair_task<0, true>(P, s, Y, tk.a, tk.b, dir, SX, SY, SZ, SQ);
       else if (s.ele_form == 1) ele_pair_task<1, true>(P, s, Y, tk.a, tk.b, dir, SX, SY, SZ, SQ);
       else ele_pair_task<2, true>(P, s, Y, tk.a, tk.b, dir, SX, SY, SZ, SQ);
