@@ -155,6 +155,7 @@ struct tis_ctx {
   DevPtrs P{};
   DSys *d_sys = nullptr;
   Tile *d_btiles = nullptr, *d_qtiles = nullptr;
+  Tile *d_btiles_xcd = nullptr;          // the bead tiles in k_main's whole-batch launch order
   int32_t nbtile = 0, nqtile = 0;
   int32_t *d_systask = nullptr, *d_sysidx = nullptr;
   int32_t nsystask = 0;
@@ -395,6 +396,46 @@ struct Range {
   int ek0, ekn, ef0, efn;   // Ewald tasks
   int pt0, ptn;             // periodic electrostatic chunk-pair tasks
 };
+
+// Whole-batch launch order of the bead tiles for k_main: workgroup b runs on
+// XCD b mod 8, each with its own L2, and a tile reads the coordinates of its
+// system's beads (the exv neighbours), so every system's tiles go to one XCD
+// (systems dealt largest first to the XCD with the fewest tiles) and that
+// L2 holds one eighth of the coordinates rather than all of them.  The
+// order changes no result: a tile's work depends on the tile only.
+static std::vector<Tile> xcd_order(const std::vector<Tile> &tiles, int nsys) {
+  constexpr int kXcd = 8;
+  std::vector<int> ntile(nsys, 0);
+  for (const Tile &t : tiles) ++ntile[t.sys];
+  std::vector<int> sys_order(nsys);
+  for (int i = 0; i < nsys; ++i) sys_order[i] = i;
+  std::stable_sort(sys_order.begin(), sys_order.end(), [&](int a, int b) { return ntile[a] > ntile[b]; });
+  std::vector<int> xcd_of(nsys, 0), load(kXcd, 0);
+  for (int sy : sys_order) {
+    if (!ntile[sy]) continue;
+    const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    xcd_of[sy] = x;
+    load[x] += ntile[sy];
+  }
+  std::vector<std::vector<Tile>> bin(kXcd);
+  for (const Tile &t : tiles) bin[xcd_of[t.sys]].push_back(t);
+  const size_t nt = tiles.size();
+  std::vector<Tile> out(nt);
+  std::vector<char> used(nt, 0);
+  std::vector<Tile> rest;
+  std::vector<size_t> next(kXcd, 0);
+  for (size_t p = 0; p < nt; ++p) {
+    std::vector<Tile> &b = bin[p % kXcd];
+    size_t &k = next[p % kXcd];
+    if (k < b.size()) { out[p] = b[k++]; used[p] = 1; }
+  }
+  for (int x = 0; x < kXcd; ++x)
+    for (size_t k = next[x]; k < bin[x].size(); ++k) rest.push_back(bin[x][k]);
+  size_t r = 0;
+  for (size_t p = 0; p < nt; ++p)
+    if (!used[p]) out[p] = rest[r++];
+  return out;
+}
 
 Range range_of(tis_ctx *c, int32_t isys) {
   Range r{};
@@ -1020,6 +1061,10 @@ int build_tables(tis_ctx *c) {
   int rc = 0;
 #define UPT(ptr, vec) do { rc = dupload_in(c, c->tab_allocs, &(ptr), vec, &c->tab_bytes); if (rc) return rc; } while (0)
   UPT(c->d_btiles, btiles); UPT(c->d_qtiles, qtiles);
+  {
+    const std::vector<Tile> xt = xcd_order(btiles, c->nsys);
+    UPT(c->d_btiles_xcd, xt);
+  }
   UPT(c->d_ptask, ptask); UPT(c->d_ptask_lpt, lpt); UPT(c->d_bpair, bpair); UPT(c->d_qpair, qpair);
   UPT(c->d_ewsk, ewsk); UPT(c->d_ewf, ewf); UPT(c->d_systask, systask); UPT(c->d_sysidx, sysidx);
   UPT(c->d_hbdense, hbdense);
@@ -1519,10 +1564,12 @@ static void launch_force_pair(tis_ctx *c, const Range &r, const double *X, const
   const int32_t ele_on = c->any_ele ? 1 : 0;
   if (r.btn > 0 && c->timing)
     hipExtLaunchKernelGGL(k_main, dim3(r.btn), dim3(kTile * 2), 0, c->stream, e3[4], e3[5], 0, c->P, X, PX, XN,
-                          PXN, c->d_btiles + r.bt0, r.btn, (int32_t)t_local, mode, ele_on, count_step);
+                          PXN, r.all ? c->d_btiles_xcd : c->d_btiles + r.bt0, r.btn, (int32_t)t_local, mode, ele_on,
+                          count_step);
   else if (r.btn > 0)
     hipLaunchKernelGGL(k_main, dim3(r.btn), dim3(kTile * 2), 0, c->stream, c->P, X, PX, XN, PXN,
-                       c->d_btiles + r.bt0, r.btn, (int32_t)t_local, mode, ele_on, count_step);
+                       r.all ? c->d_btiles_xcd : c->d_btiles + r.bt0, r.btn, (int32_t)t_local, mode, ele_on,
+                       count_step);
   else if (c->timing) {
     (void)hipEventRecord(e3[4], c->stream);
     (void)hipEventRecord(e3[5], c->stream);
