@@ -1127,23 +1127,28 @@ __device__ inline void tile_boxes(double p0, double p1, double p2, BBox *tile, B
 // two waves take alternate groups of four excluded-volume neighbours; the
 // partial sums are added in wave order (a fixed order: results are bitwise
 // reproducible), then wave 0 integrates.
+__device__ inline void exv_entries(const DevPtrs &P, int g, int m0, int cnt, uint32_t e[4]) {
+  // entries past cnt re-read entry cnt - 1 (m0 < cnt) and are skipped
+#pragma unroll
+  for (int u = 0; u < 4; ++u) e[u] = P.nb[(size_t)min(m0 + u, cnt - 1) * P.ntot + g];
+}
+
+// e: the group's list entries (exv_entries); the entries of the wave's next
+// group (m0 + 8) are loaded into en while this group's neighbours are read
 __device__ inline void exv_group(const DevPtrs &P, const DSys &s, const double *Y, int g, int m0, int cnt,
                                  double xi0, double xi1, double xi2, const SpPair *tab, double F[3],
-                                 unsigned &nclamp) {
+                                 unsigned &nclamp, const uint32_t e[4], uint32_t en[4]) {
   const int n = P.ntot;
-  // every load of the group is issued before the arithmetic: entries past
-  // cnt re-read entry cnt - 1 (m0 < cnt) and are skipped below
-  uint32_t e[4];
+  // every load of the group is issued before the arithmetic
   double y0[4], y1[4], y2[4];
   SpPair tp[4];
-#pragma unroll
-  for (int u = 0; u < 4; ++u) e[u] = P.nb[(size_t)min(m0 + u, cnt - 1) * n + g];
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     const int j = (int)(e[u] & kIdxMask);
     y0[u] = Y[j]; y1[u] = Y[n + j]; y2[u] = Y[2 * n + j];
     tp[u] = tab[P.species[j]];
   }
+  if (m0 + 8 < cnt) exv_entries(P, g, m0 + 8, cnt, en);
   const double a = s.exv_a;
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
@@ -1304,7 +1309,13 @@ __global__ __launch_bounds__(kTile * 2) void k_main(DevPtrs P, const double *X, 
     const SpPair *tab = P.sptab + s.sptab + (int)P.species[g] * s.nspecies;
     const int cnt = min(P.nb_cnt[g], P.nb_cap);
     unsigned nclamp = 0;
-    for (int m0 = 4 * w; m0 < cnt; m0 += 8) exv_group(P, s, Y, g, m0, cnt, xi0, xi1, xi2, tab, F, nclamp);
+    uint32_t e[4] = {0u, 0u, 0u, 0u}, en[4] = {0u, 0u, 0u, 0u};
+    if (4 * w < cnt) exv_entries(P, g, 4 * w, cnt, e);
+    for (int m0 = 4 * w; m0 < cnt; m0 += 8) {
+      exv_group(P, s, Y, g, m0, cnt, xi0, xi1, xi2, tab, F, nclamp, e, en);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) e[u] = en[u];
+    }
     if (nclamp) atomicAdd(&P.counters[0], (unsigned long long)nclamp);
   }
   for (int d = 0; d < 3; ++d) red[d][w][lane] = F[d];
