@@ -776,7 +776,10 @@ __device__ inline void ele_pair_unrolled(const DevPtrs &P, const DSys &s, const 
   const int n = P.ntot;
   const int qi = I * kTile + lane;
   const bool valid = qi < s.nq;
-  double xi0 = 0.0, xi1 = 0.0, xi2 = 0.0, chi = 0.0;
+  // absent charges sit far apart (+-1e15, charge 0): their pairs stay
+  // finite and off
+  constexpr double kFar = 1e15;
+  double xi0 = -kFar, xi1 = -kFar, xi2 = -kFar, chi = 0.0;
   if (valid) {
     const int gi = P.qbead[s.qoff + qi];
     xi0 = Y[gi]; xi1 = Y[n + gi]; xi2 = Y[2 * n + gi];
@@ -786,7 +789,7 @@ __device__ inline void ele_pair_unrolled(const DevPtrs &P, const DSys &s, const 
   {
     const bool vj = jt + lane < s.nq;
     const int gj = vj ? P.qbead[s.qoff + jt + lane] : 0;
-    const double x = vj ? Y[gj] : 0.0, y = vj ? Y[n + gj] : 0.0, z = vj ? Y[2 * n + gj] : 0.0;
+    const double x = vj ? Y[gj] : kFar, y = vj ? Y[n + gj] : kFar, z = vj ? Y[2 * n + gj] : kFar;
     const float q = vj ? (float)P.charge[gj] : 0.0f;
     JE->x[c] = x; JE->y[c] = y; JE->z[c] = z; JE->q[c] = q;
     JE->x[c + kTile] = x; JE->y[c + kTile] = y; JE->z[c + kTile] = z; JE->q[c + kTile] = q;
@@ -809,6 +812,7 @@ __device__ inline void ele_pair_unrolled(const DevPtrs &P, const DSys &s, const 
   if (diag && lane >= 32) mr &= ~(1ull << 32);   // step 32 of the diagonal chunk: lanes < 32 only
   double F0 = 0.0, F1 = 0.0, F2 = 0.0, G0 = 0.0, G1 = 0.0, G2 = 0.0;
   uint32_t nclamp = 0;
+  const float rcd = (float)s.rcdist, nrl = -rcd * 1.44269504088896341f;   // exp(-r rcd) = 2^(r nrl)
   // eight steps from k0; BITS false: every pair of the chunk pair is listed,
   // so only the cutoff decides (a charge beyond nq has charge 0 and adds 0)
   auto steps8 = [&](const int k0, const uint32_t b8, auto bits_t) {
@@ -823,12 +827,13 @@ __device__ inline void ele_pair_unrolled(const DevPtrs &P, const DSys &s, const 
       float sq = pq[kk];
       asm volatile("" : "+v"(sq));   // read for every lane: a select below, not a branch
       const float qq = on ? qiC * sq : 0.0f;
-      const float rinv = hw_rsq(on ? (float)d2 : 1.0f);
+      // every pair listed: an off pair is beyond the cutoff (d2 > 0)
+      const float d2f = BITS && !on ? 1.0f : (float)d2;
+      const float rinv = hw_rsq(d2f);
       float dv;
       if (FORM == 0) {
-        const float rcd = (float)s.rcdist;
-        const float d1 = (on ? (float)d2 : 1.0f) * rinv;
-        dv = qq * rinv * rinv * (rinv + rcd) * hw_exp(-d1 * rcd);
+        const float d1 = d2f * rinv;
+        dv = qq * rinv * rinv * (rinv + rcd) * __builtin_amdgcn_exp2f(d1 * nrl);
         const bool cl = dv > (float)kDeMax;
         nclamp += (uint32_t)__popcll(__ballot(cl));
         dv = cl ? (float)kDeMax : dv;
