@@ -1291,20 +1291,29 @@ __global__ __launch_bounds__(kTile * 2) void k_main(DevPtrs P, const double *X, 
     } else {
       // bonded slots in incidence order, eight loads in flight at a time
       // (slots past ie re-read slot ie - 1 and are not added)
+      // (the next eight slot indices load while these slots are read)
       const int ib = P.inc_off[g], ie = P.inc_off[g + 1];
+      int64_t at[8] = {}, atn[8] = {};
+      if (ib < ie) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) at[u] = P.inc_slot[min(ib + u, ie - 1)];
+      }
       for (int k0 = ib; k0 < ie; k0 += 8) {
         float sl[8][3];
-        int64_t at[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) at[u] = P.inc_slot[min(k0 + u, ie - 1)];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
           const float *q = P.fslot + 3 * at[u];
           sl[u][0] = q[0]; sl[u][1] = q[1]; sl[u][2] = q[2];
         }
+        if (k0 + 8 < ie) {
+#pragma unroll
+          for (int u = 0; u < 8; ++u) atn[u] = P.inc_slot[min(k0 + 8 + u, ie - 1)];
+        }
 #pragma unroll
         for (int u = 0; u < 8; ++u)
           if (k0 + u < ie) { F[0] += (double)sl[u][0]; F[1] += (double)sl[u][1]; F[2] += (double)sl[u][2]; }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) at[u] = atn[u];
       }
     }
     // excluded volume (force_exv_dt15.F90:53-115), owner view of each pair;
