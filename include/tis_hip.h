@@ -133,6 +133,11 @@ int32_t tis_force_get(tis_ctx *ctx, int32_t isys, double *force_out);
  * launch, energy_out[nsys * TIS_E_MAX] (the replica loop of
  * energy_allrep.F90:123-125). */
 int32_t tis_energy_sumup(tis_ctx *ctx, int32_t isys, double *energy_out);
+/* energy_sumup of the n systems isys[0..n) in one launch (the replicas of
+ * an exchange, energy_allrep.F90:123-216, when the batch holds other
+ * systems too): energy_out[n * TIS_E_MAX], in the order of isys.  Repeats
+ * are allowed; TIS_ERR_ARG for n < 1 or an index out of range. */
+int32_t tis_energy_sumup_set(tis_ctx *ctx, int32_t n, const int32_t *isys, double *energy_out);
 /* energy_unit of energy_sumup (energy_sumup.F90:17-36, 76-508) for one
  * system: energy_unit[(iunit, junit, type)] column-major (nunit, nunit,
  * TIS_E_MAX), each term added at the unit pair the reference's term routine

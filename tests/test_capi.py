@@ -48,6 +48,7 @@ def test_null_arguments_are_rejected():
     assert lib.tis_bdhi_tensors(None) < 0
     assert lib.tis_bdhi_steps(None, C.c_double(0.05), 1, 1, 0) < 0
     assert lib.tis_system_origin(None, 0, 0, 0) < 0
+    assert lib.tis_energy_sumup_set(None, 1, None, None) < 0
 
 
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):

@@ -44,6 +44,9 @@ class FixedEnergies:
         e[:, E_TOTAL] = self.E
         return e
 
+    def energy_systems(self, isys):
+        return self.energy_all()[list(isys)]
+
     def get_state(self, k, which):
         assert which == 1
         return self.v[k].copy()
@@ -120,6 +123,30 @@ def test_swapped_temperature_energies_gpu():
             f = math.sqrt(temps[lj] / temps[li])
             assert np.allclose(eng.get_state(ri, 1), v_before[ri] * f, rtol=1e-15)
     eng.langevin_steps(20, MODE_GRAPH)   # the batch keeps running after a swap
+
+
+@pytest.mark.gpu
+def test_energy_of_a_set_equals_the_batch_energies_gpu():
+    """tis_energy_sumup_set (the replicas' energies of an exchange) gives
+    the bits of the whole-batch energy_sumup for the systems it names, in
+    its order and with repeats."""
+    from tis_amd.engine import Engine, MODE_GRAPH
+    systems = []
+    for t in (300.0, 330.0, 360.0):
+        s = T.build_rna_system([T.config0_chain()])
+        T.set_temperature(s, t, 0.150)
+        systems.append(s)
+    eng = Engine(systems, keys=[0, 1, 2], seed=3)
+    rng_ = np.random.default_rng(4)
+    for k, s in enumerate(systems):
+        eng.set_coords(k, T.helix_coords(s, rng_))
+    eng.langevin_init(tstep=0.05, rebuild_every=10)
+    eng.langevin_steps(30, MODE_GRAPH)
+    eng.force_all()
+    eall = eng.energy_all()
+    es = eng.energy_systems([2, 0, 2, 1])
+    assert np.array_equal(es, eall[[2, 0, 2, 1]])
+    assert np.array_equal(eng.energy_systems([1]), eall[[1]])
 
 
 @pytest.mark.gpu

@@ -164,6 +164,9 @@ struct tis_ctx {
   std::vector<int32_t> systask_of;       // index into d_systask per system or -1
   int32_t *d_ststat = nullptr;           // st_status output of tis_term_status [nsttot]
   double *d_energy = nullptr;
+  int32_t *d_eset = nullptr;             // tis_energy_sumup_set: system list and energies
+  double *d_eset_out = nullptr;
+  size_t eset_cap = 0, eset_out_cap = 0;
   double *d_eunit = nullptr;   // raw per-unit-pair energies [nunit^2][25]
   size_t eunit_cap = 0;
   unsigned long long *d_counters = nullptr;
@@ -1270,6 +1273,8 @@ int32_t tis_ctx_destroy(tis_ctx *c) {
   for (void *p : c->label_bufs) (void)hipFree(p);
   if (c->d_lab_args) (void)hipFree(c->d_lab_args);
   if (c->d_lab_scale) (void)hipFree(c->d_lab_scale);
+  if (c->d_eset) (void)hipFree(c->d_eset);
+  if (c->d_eset_out) (void)hipFree(c->d_eset_out);
   for (hipEvent_t e : c->tev) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -1623,6 +1628,48 @@ int32_t tis_energy_sumup(tis_ctx *c, int32_t isys, double *energy_out) {
       for (int k = 0; k < 25; ++k) energy_out[25 * (size_t)is + k] = h.live ? e[25 * (size_t)h.live_pos + k] : 0.0;
     }
   }
+  return TIS_OK;
+}
+
+int32_t tis_energy_sumup_set(tis_ctx *c, int32_t n, const int32_t *isys, double *energy_out) {
+  if (!c || !isys || !energy_out || n < 1) return TIS_ERR_ARG;
+  for (int k = 0; k < n; ++k) {
+    if (isys[k] < 0) return fail(c, TIS_ERR_ARG, "tis_energy_sumup_set: system index %d", isys[k]);
+    const int rc = check_sys(c, isys[k]);
+    if (rc) return rc;
+  }
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = 0;
+  if (!c->lists_valid) { rc = rebuild_lists(c, -1); if (rc) return rc; }
+  // structure factors of the selected Ewald systems
+  for (int k = 0; k < n; ++k) {
+    const HostSys &h = c->hs[isys[k]];
+    if (h.esk_count > 0)
+      hipLaunchKernelGGL(k_ewald_sk, dim3(h.esk_count), dim3(kTile), 0, c->stream, c->P, PXc(c),
+                         c->d_ewsk + h.esk_begin, 1);
+  }
+  // k_energy takes a list of system indices, one block each (the same
+  // kernel as the whole-batch launch)
+  if ((size_t)n > c->eset_cap) {
+    if (c->d_eset) { (void)hipFree(c->d_eset); c->d_eset = nullptr; c->dev_bytes -= c->eset_cap * sizeof(int32_t); }
+    c->eset_cap = 0;
+    HIPCHK(c, hipMalloc(&c->d_eset, (size_t)n * sizeof(int32_t)));
+    c->eset_cap = (size_t)n;
+    c->dev_bytes += c->eset_cap * sizeof(int32_t);
+  }
+  if ((size_t)n * 25 > c->eset_out_cap) {
+    if (c->d_eset_out) { (void)hipFree(c->d_eset_out); c->d_eset_out = nullptr; c->dev_bytes -= c->eset_out_cap * sizeof(double); }
+    c->eset_out_cap = 0;
+    HIPCHK(c, hipMalloc(&c->d_eset_out, (size_t)n * 25 * sizeof(double)));
+    c->eset_out_cap = (size_t)n * 25;
+    c->dev_bytes += c->eset_out_cap * sizeof(double);
+  }
+  HIPCHK(c, hipMemcpyAsync(c->d_eset, isys, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_energy, dim3(n), dim3(256), 0, c->stream, c->P, Xc(c), PXc(c), c->d_eset, c->d_eset_out, 0);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(energy_out, c->d_eset_out, (size_t)n * 25 * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return TIS_OK;
 }
 

@@ -68,6 +68,7 @@ def load_library():
     L.tis_force_sumup.argtypes = [vp, C.c_int32, _f64p]
     L.tis_force_get.argtypes = [vp, C.c_int32, _f64p]
     L.tis_energy_sumup.argtypes = [vp, C.c_int32, _f64p]
+    L.tis_energy_sumup_set.argtypes = [vp, C.c_int32, _i32p, _f64p]
     L.tis_energy_unit.argtypes = [vp, C.c_int32, C.c_int32, _f64p]
     L.tis_langevin_init.argtypes = [vp, C.c_double, C.c_int32, C.c_uint64, C.c_int32]
     L.tis_langevin_steps.argtypes = [vp, C.c_int64, C.c_int32]
@@ -84,7 +85,7 @@ def load_library():
                "tis_system_retire", "tis_arena_stats", "tis_tune", "tis_launch_shape",
                "tis_state_set",
                "tis_state_get", "tis_neighbor", "tis_pairs_get", "tis_force_sumup", "tis_force_get",
-               "tis_energy_sumup", "tis_energy_unit", "tis_langevin_init", "tis_langevin_steps", "tis_status_counters",
+               "tis_energy_sumup", "tis_energy_sumup_set", "tis_energy_unit", "tis_langevin_init", "tis_langevin_steps", "tis_status_counters",
                "tis_system_origin", "tis_device_bytes", "tis_label_define", "tis_label_apply",
                "tis_synchronize", "tis_last_timing", "tis_last_force_ms"):
         getattr(L, fn).restype = C.c_int32
@@ -256,6 +257,15 @@ class Engine:
         """(nsys, E_MAX) energies of every system in one launch."""
         e = np.zeros((len(self.systems), E_MAX))
         self._chk(self.L.tis_energy_sumup(self.ctx, -1, _fp(e)), "tis_energy_sumup")
+        return e
+
+    def energy_systems(self, isys):
+        """(len(isys), E_MAX) energies of the systems isys, in that order, in
+        one launch."""
+        a = np.ascontiguousarray(isys, dtype=np.int32)
+        e = np.zeros((len(a), E_MAX))
+        self._chk(self.L.tis_energy_sumup_set(self.ctx, len(a), a.ctypes.data_as(_i32p), _fp(e)),
+                  "tis_energy_sumup_set")
         return e
 
     # ---- Langevin

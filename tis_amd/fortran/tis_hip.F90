@@ -66,7 +66,7 @@ module tis_hip
        tis_set_system_T, tis_list_capacity, tis_term_status, tis_transfer_bytes, &
        tis_system_append, tis_system_retire, tis_arena_stats, tis_tune, tis_launch_shape, &
        tis_state_set, tis_state_get, tis_neighbor, tis_pairs_get, tis_force_sumup, tis_force_get, &
-       tis_energy_sumup, tis_energy_unit, tis_langevin_init, tis_langevin_steps, tis_system_origin, tis_device_bytes, &
+       tis_energy_sumup, tis_energy_sumup_set, tis_energy_unit, tis_langevin_init, tis_langevin_steps, tis_system_origin, tis_device_bytes, &
        tis_status_counters, tis_synchronize, tis_last_timing, tis_last_force_ms, tis_label_define, &
        tis_label_apply, tis_bdhi_setup, tis_bdhi_state_set, tis_bdhi_state_get, tis_bdhi_forces, &
        tis_bdhi_tensors, tis_bdhi_tensor_get, tis_bdhi_factor_error, tis_bdhi_steps, tis_bdhi_timing, &
@@ -194,6 +194,13 @@ module tis_hip
        import :: c_int32_t, c_ptr, c_double
        type(c_ptr), value :: ctx
        integer(c_int32_t), value :: isys
+       real(c_double), intent(out) :: energy_out(*)
+     end function
+     integer(c_int32_t) function tis_energy_sumup_set(ctx, n, isys, energy_out) bind(C, name="tis_energy_sumup_set")
+       import :: c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: n
+       integer(c_int32_t), intent(in) :: isys(*)
        real(c_double), intent(out) :: energy_out(*)
      end function
      integer(c_int32_t) function tis_energy_unit(ctx, isys, style, energy_unit) bind(C, name="tis_energy_unit")

@@ -58,15 +58,14 @@ class ReplicaLadder:
 
     def energies(self):
         """(E at own T, E at partner T) per replica (energy_allrep.F90:123-216)."""
-        eall = self.eng.energy_all()
-        own = np.array([eall[s, E_TOTAL] for s in self.slots])
+        own = self.eng.energy_systems(self.slots)[:, E_TOTAL].copy()
         part = np.zeros_like(own)
         moved = [r for r, lab in enumerate(self.rep2lab) if self.partner(lab) >= 0]
         if moved:
             self.eng.label_apply([self.slots[r] for r in moved], [self.partner(self.rep2lab[r]) for r in moved])
-            eall = self.eng.energy_all()
-            for r in moved:
-                part[r] = eall[self.slots[r], E_TOTAL]
+            e = self.eng.energy_systems([self.slots[r] for r in moved])
+            for k, r in enumerate(moved):
+                part[r] = e[k, E_TOTAL]
             self.eng.label_apply([self.slots[r] for r in moved], [self.rep2lab[r] for r in moved])
         return own, part
 
