@@ -757,8 +757,10 @@ __device__ inline void ele_pair_task(const DevPtrs &P, const DSys &s, const doub
 }
 
 // ele_pair_task for a non-periodic system with the whole-wave DPP rotation
-// (dir = 1 or 63), with the same pairs, the same operations in the same order
-// and so the same results, in fewer instructions per step: the J chunk is
+// (dir = 1 or 63): the same pairs in the same order and the same FP64
+// distances, cutoff tests and accumulation (the DH exponent takes its scale
+// as one constant, -rcd log2(e), so its FP32 rounding can differ by an ulp),
+// in fewer instructions per step: the J chunk is
 // staged twice over in an order where lane l meets, at step k, entry c + k
 // (c = l for dir = 1, 63 - l for dir = 63), so the reads of eight
 // consecutive steps take immediate offsets from one address; the list bits
