@@ -20,7 +20,7 @@
 #include "tis_internal.h"
 
 namespace tis {
-template <int PPW, int TPL, int OCC>
+template <int PPW, int TPL>
 __global__ void k_force(DevPtrs, const double *, const double *, const int32_t *, int, const EwTask *, int,
                         BondedRange, int32_t);
 __global__ void k_ewald_sk(DevPtrs, const double *, const EwTask *, int);
@@ -1550,15 +1550,7 @@ static void launch_force_pair(tis_ctx *c, const Range &r, const double *X, const
     const int32_t *task = c->d_systask + r.task0;
     const EwTask *pt = r.all ? (c->shape.lpt ? c->d_ptask_lpt : c->d_ptask) : c->d_ptask + r.pt0;
     // one kernel per launch shape (each with its own register allocation)
-    // and per register budget: six waves per SIMD for a whole-batch launch
-    // without terms waves, where the pair waves hide the hbond lanes' longer
-    // (more spilled) chains; one system's launch is as long as its hbond
-    // chain and keeps four (results do not depend on it)
-    auto kf = (r.taskn > 0 || !r.all)
-                  ? (ppw == 1 ? (tpl == 1 ? k_force<1, 1, 4> : k_force<1, 2, 4>)
-                              : (tpl == 1 ? k_force<2, 1, 4> : k_force<2, 2, 4>))
-                  : (ppw == 1 ? (tpl == 1 ? k_force<1, 1, 6> : k_force<1, 2, 6>)
-                              : (tpl == 1 ? k_force<2, 1, 6> : k_force<2, 2, 6>));
+    auto kf = ppw == 1 ? (tpl == 1 ? k_force<1, 1> : k_force<1, 2>) : (tpl == 1 ? k_force<2, 1> : k_force<2, 2>);
     if (c->timing)
       hipExtLaunchKernelGGL(kf, dim3(nblk), dim3(kTile * kWaves), 0, c->stream, e3[1], e3[2], 0, c->P, X, PX,
                             task, r.taskn, pt, npt, br, pre_t);

@@ -880,14 +880,8 @@ __device__ inline void ele_pair_unrolled(const DevPtrs &P, const DSys &s, const 
 // each), four electrostatic chunk pairs per block (ele_pair_task, one wave
 // each), then the bonded terms, 256 lanes per block (bonded_lane).  k_main
 // gathers all of their outputs.
-// OCC: waves per SIMD the register allocation is made for.  Four (128
-// VGPRs) for launches with terms waves, whose valence pass spills under a
-// smaller budget, and for one system's launch, whose length is its hbond
-// chain; six (80 VGPRs, no spills in the pair loops) for whole-batch
-// launches without terms waves, so that bonded blocks run beside the pair
-// waves.
-template <int PPW, int TPL, int OCC>
-__global__ __launch_bounds__(kTile * kWaves) __attribute__((amdgpu_waves_per_eu(OCC)))
+template <int PPW, int TPL>
+__global__ __launch_bounds__(kTile * kWaves) __attribute__((amdgpu_waves_per_eu(4)))
 void k_force(DevPtrs P, const double *X, const double *PX, const int32_t *systask, int ntask,
              const EwTask *ptasks, int npt, BondedRange br, int32_t t_local) {
   constexpr int ppw = PPW, tpl = TPL;
@@ -951,12 +945,14 @@ void k_force(DevPtrs P, const double *X, const double *PX, const int32_t *systas
   }
 }
 
-#define TIS_K_FORCE(P_, T_, O_)                                                                          \
-  template __global__ void k_force<P_, T_, O_>(DevPtrs, const double *, const double *, const int32_t *, int, \
-                                               const EwTask *, int, BondedRange, int32_t);
-TIS_K_FORCE(1, 1, 4) TIS_K_FORCE(2, 1, 4) TIS_K_FORCE(1, 2, 4) TIS_K_FORCE(2, 2, 4)
-TIS_K_FORCE(1, 1, 6) TIS_K_FORCE(2, 1, 6) TIS_K_FORCE(1, 2, 6) TIS_K_FORCE(2, 2, 6)
-#undef TIS_K_FORCE
+template __global__ void k_force<1, 1>(DevPtrs, const double *, const double *, const int32_t *, int, const EwTask *,
+                                       int, BondedRange, int32_t);
+template __global__ void k_force<2, 1>(DevPtrs, const double *, const double *, const int32_t *, int, const EwTask *,
+                                       int, BondedRange, int32_t);
+template __global__ void k_force<1, 2>(DevPtrs, const double *, const double *, const int32_t *, int, const EwTask *,
+                                       int, BondedRange, int32_t);
+template __global__ void k_force<2, 2>(DevPtrs, const double *, const double *, const int32_t *, int, const EwTask *,
+                                       int, BondedRange, int32_t);
 
 // ------------------------------------------------------------------ Ewald reciprocal space
 // force_ele_coulomb_ewld.F90:62-84.  The phase k.r = 2 pi (h . r/L) is
